@@ -1,0 +1,335 @@
+"""Pins the FP64 CPU restatement (oracle/imps_oracle.py) against closed forms, exact
+diagonalisation, brute force and the invariants the reference's own tests assert.
+
+The reference stores no golden-vector files (its tests only assert thresholds), so each
+check below names the reference assertion it mirrors or the mathematics that fixes it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def _open_chain_matrix(W, L):
+    """Dense open-chain Hamiltonian from a Schur MPO: boundary vectors e_1 and e_w."""
+    w = W.shape[0]
+    cur = np.transpose(W[0], (1, 0, 2))
+    for _ in range(1, L):
+        new = np.einsum("aUD,aubd->bUuDd", cur, W)
+        s = new.shape
+        cur = new.reshape(s[0], s[1] * s[2], s[3] * s[4])
+    return cur[w - 1]
+
+
+def _kron_all(ms):
+    out = np.array([[1.0]])
+    for m in ms:
+        out = np.kron(out, m)
+    return out
+
+
+# ---------------------------------------------------------------------------- tolerances
+
+
+def test_updatetol_clamping(oracle):
+    # test/algorithms/api.jl:453-463
+    assert oracle.updatetol(0.1, 1, 5.0e-4) == pytest.approx(5.0e-5)
+    assert oracle.updatetol(0.1, 1, 1.0e-20) == pytest.approx(1.0e-14)
+    assert oracle.updatetol(0.1, 1, 1.0e20) == pytest.approx(1.0e-4)
+    assert oracle.updatetol(0.1, 0, 5.0e-4) == oracle.updatetol(0.1, 1, 5.0e-4)
+    assert oracle.updatetol(1.0, 4, 1e-6) == pytest.approx(5e-7)
+
+
+# ---------------------------------------------------------------------------- models
+
+
+def test_tfim_and_xxz_mpo_match_pauli_construction(oracle):
+    L = 5
+    X, Z = oracle.sigma_x(), oracle.sigma_z()
+    I2 = np.eye(2)
+
+    def site(op, i):
+        return _kron_all([op if k == i else I2 for k in range(L)])
+
+    J, h = 1.0, 0.7
+    H = sum(-J * site(X, i) @ site(X, i + 1) for i in range(L - 1)) + sum(-h * site(Z, i) for i in range(L))
+    assert np.abs(_open_chain_matrix(oracle.tfim_mpo(J, h), L) - H).max() < 1e-14
+
+    # XXZ with the complex Sy: SxSx + SySy is real
+    Sx, Sz = X / 2, Z / 2
+    Sy = np.array([[0, -1j], [1j, 0]]) / 2
+    Jx, Dz, hz = 1.0, 0.8, 0.3
+    Hc = sum(Jx * (site(Sx, i) @ site(Sx, i + 1) + (_kron_all([Sy if k == i else I2 for k in range(L)])
+                                                   @ _kron_all([Sy if k == i + 1 else I2 for k in range(L)])))
+             + Jx * Dz * site(Sz, i) @ site(Sz, i + 1) for i in range(L - 1)) - sum(hz * site(Sz, i) for i in range(L))
+    assert np.abs(Hc.imag).max() < 1e-15
+    assert np.abs(_open_chain_matrix(oracle.xxz_mpo(Jx, Dz, hz), L) - Hc.real).max() < 1e-14
+
+
+@pytest.mark.parametrize("U", [0.0, 4.0])
+def test_hubbard_mpo_matches_fermion_operators(oracle, U):
+    L, t, mu = 4, 1.0, U / 2
+    c_up, c_dn, F, _, _ = oracle.hubbard_local_ops()
+    I4 = np.eye(4)
+
+    def fop(op, i):
+        return _kron_all([F] * i + [op] + [I4] * (L - i - 1))
+
+    cu = [fop(c_up, i) for i in range(L)]
+    cd = [fop(c_dn, i) for i in range(L)]
+    # canonical anticommutation relations of the Jordan-Wigner operators
+    for a in cu + cd:
+        for b in cu + cd:
+            ac = a @ b.T + b.T @ a
+            assert np.abs(ac - (np.eye(4 ** L) if a is b else 0)).max() < 1e-14
+    H = np.zeros((4 ** L, 4 ** L))
+    for i in range(L - 1):
+        for c in (cu, cd):
+            hop = c[i].T @ c[i + 1]
+            H += -t * (hop + hop.T)
+    for i in range(L):
+        nu, nd = cu[i].T @ cu[i], cd[i].T @ cd[i]
+        H += U * nu @ nd - mu * (nu + nd)
+    Hm = _open_chain_matrix(oracle.hubbard_mpo(t, U, mu), L)
+    assert np.abs(Hm - H).max() < 1e-14
+    if U == 0.0:
+        ek = -2 * t * np.cos(np.arange(1, L + 1) * np.pi / (L + 1))
+        assert np.linalg.eigvalsh(Hm)[0] == pytest.approx(2 * ek[ek < 0].sum(), abs=1e-12)
+
+
+def test_schur_levels(oracle):
+    # test/operators/mpo.jl:50-56: unit corners, identity and empty levels
+    for W in (oracle.tfim_mpo(), oracle.xxz_mpo(), oracle.hubbard_mpo()):
+        w, d = W.shape[0], W.shape[1]
+        assert np.array_equal(W[0, :, 0, :], np.eye(d))
+        assert np.array_equal(W[w - 1, :, w - 1, :], np.eye(d))
+        assert oracle.isidentitylevel([W], 0) and oracle.isidentitylevel([W], w - 1)
+        for i in range(1, w - 1):
+            assert not oracle.isidentitylevel([W], i)
+            assert not oracle.isemptylevel([W], i)
+    assert oracle.tfim_mpo().shape == (3, 2, 3, 2)
+    assert oracle.xxz_mpo().shape == (5, 2, 5, 2)
+    assert oracle.hubbard_mpo().shape == (6, 4, 6, 4)
+    assert len(oracle.nonzero_blocks(oracle.hubbard_mpo())) == 11
+
+
+# ---------------------------------------------------------------------------- H_AC / H_C
+
+
+def _rand_env(rng, D, w, sym=True):
+    G = rng.standard_normal((D, w, D))
+    if sym:
+        G = 0.5 * (G + G.transpose(2, 1, 0))
+    return G
+
+
+def test_hac_block_loop_equals_dense_and_brute_force(oracle):
+    rng = np.random.default_rng(5)
+    D, W = 3, oracle.hubbard_mpo()
+    w, d = W.shape[0], W.shape[1]
+    GL, GR = _rand_env(rng, D, w, False), _rand_env(rng, D, w, False)
+    x = rng.standard_normal((D, d, D))
+    y1 = oracle.hac_apply(GL, W, GR, x)
+    y2 = oracle.hac_apply_dense(GL, W, GR, x)
+    assert np.abs(y1 - y2).max() < 1e-12 * np.abs(y2).max()
+    # brute force: explicit (dD^2) x (dD^2) matrix from nested loops
+    n = D * d * D
+    M = np.zeros((n, n))
+    idx = lambda a, s, b: a + D * (s + d * b)  # column-major vec
+    for b in range(D):
+        for u in range(d):
+            for c in range(D):
+                for a in range(D):
+                    for s in range(d):
+                        for bb in range(D):
+                            M[idx(b, u, c), idx(a, s, bb)] = sum(
+                                GL[b, i, a] * W[i, u, j, s] * GR[bb, j, c] for i in range(w) for j in range(w))
+    y3 = (M @ x.reshape(-1, order="F")).reshape(D, d, D, order="F")
+    assert np.abs(y1 - y3).max() < 1e-12 * np.abs(y3).max()
+
+
+def test_hac_identity_mpo_and_hermiticity(oracle):
+    rng = np.random.default_rng(24)
+    D, d = 4, 2
+    # test/algorithms/api.jl:226-231: identity MPO tensor + identity environments => x
+    I4 = np.eye(D).reshape(D, 1, D)
+    Wid = np.eye(d).reshape(1, d, 1, d)
+    x = rng.standard_normal((D, d, D))
+    assert np.allclose(oracle.hac_apply(I4, Wid, I4, x), x, atol=1e-14)
+    # api.jl:233-240: <x, H y> = <H x, y> with environments from a DMRGCache
+    psi = oracle.randomimps(D, [d], seed=23)
+    Ws = [oracle.tfim_mpo()]
+    envs = oracle.dmrg_cache(psi, Ws)
+    GL, GR = envs.leftenv(0), envs.rightenv(0)
+    y = rng.standard_normal((D, d, D))
+    lhs = np.vdot(x, oracle.hac_apply(GL, Ws[0], GR, y))
+    rhs = np.vdot(oracle.hac_apply(GL, Ws[0], GR, x), y)
+    assert abs(lhs - rhs) < 1e-9
+    c1, c2 = rng.standard_normal((D, D)), rng.standard_normal((D, D))
+    GLc = envs.leftenv(1)
+    assert abs(np.vdot(c1, oracle.hc_apply(GLc, GR, c2)) - np.vdot(oracle.hc_apply(GLc, GR, c1), c2)) < 1e-9
+
+
+def test_hc_matches_explicit_sum(oracle):
+    rng = np.random.default_rng(3)
+    D, w = 5, 4
+    GL, GR = _rand_env(rng, D, w, False), _rand_env(rng, D, w, False)
+    x = rng.standard_normal((D, D))
+    ref = sum(GL[:, k, :] @ x @ GR[:, k, :] for k in range(w))
+    assert np.abs(oracle.hc_apply(GL, GR, x) - ref).max() < 1e-13
+
+
+# ---------------------------------------------------------------------------- primitives
+
+
+def test_regularize_traceless(oracle):
+    # test/algorithms/api.jl:242-246
+    rng = np.random.default_rng(1)
+    M = rng.standard_normal((3, 3))
+    oracle.regularize(M, np.eye(3), np.eye(3) / 3)
+    assert abs(np.trace(M)) < 1e-12
+    # no conjugation/transposition mix-up: c = sum l[a,b] v[b,a] = tr(l v)
+    l, v = rng.standard_normal((4, 4)), rng.standard_normal((4, 4))
+    v0 = v.copy()
+    oracle.regularize(v, l, np.eye(4))
+    assert np.allclose(v, v0 - np.trace(l @ v0) * np.eye(4))
+
+
+def test_gmres(oracle):
+    # test/algorithms/api.jl:248-251: (a0 + a1 A) x = b with A = 2I, a0 = a1 = 1 => b/3
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(5)
+    x, info = oracle.gmres_linsolve(lambda v: 2 * v, b, np.zeros(5), 1e-12, a0=1.0, a1=1.0)
+    assert np.allclose(x, b / 3) and info.converged
+    # random well-conditioned operator, restarts exercised (krylovdim 6 < n)
+    n = 40
+    A = rng.standard_normal((n, n)) / (3 * math.sqrt(n))
+    b = rng.standard_normal(n)
+    x, info = oracle.gmres_linsolve(lambda v: A @ v, b, np.zeros(n), 1e-12, krylovdim=6, a0=1.0, a1=-1.0)
+    assert info.converged and info.numiter > 1
+    assert np.allclose(x, np.linalg.solve(np.eye(n) - A, b), atol=1e-10)
+
+
+def test_lanczos_matches_eigh(oracle):
+    rng = np.random.default_rng(2)
+    n = 200
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    x0 = rng.standard_normal(n)
+    lam, v, info = oracle.lanczos_eigsolve(lambda x: A @ x, x0, 1e-10, krylovdim=20)
+    ev, U = np.linalg.eigh(A)
+    assert info.converged and info.numiter > 1  # thick restarts exercised
+    assert lam == pytest.approx(ev[0], abs=1e-9)
+    assert abs(abs(v @ U[:, 0]) - 1) < 1e-9
+    assert v @ x0 >= 0
+
+
+def test_arnoldi_dominant(oracle):
+    rng = np.random.default_rng(4)
+    n = 60
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lamd = np.concatenate([[1.5], 0.9 * rng.uniform(-1, 1, n - 1)])
+    A = Q @ np.diag(lamd) @ np.linalg.inv(Q) + 0.0
+    lam, v, info = oracle.arnoldi_dominant(lambda x: A @ x, rng.standard_normal(n), 1e-12, krylovdim=12)
+    assert info.converged
+    assert lam == pytest.approx(1.5, abs=1e-10)
+    assert np.linalg.norm(A @ v - lam * v) < 1e-9
+
+
+def test_qrpos_lqpos(oracle):
+    rng = np.random.default_rng(6)
+    M = rng.standard_normal((12, 5))
+    Q, R = oracle.qrpos(M)
+    assert np.allclose(Q @ R, M) and np.allclose(Q.T @ Q, np.eye(5))
+    assert np.all(np.diag(R) > 0) and np.allclose(np.tril(R, -1), 0)
+    L, Qr = oracle.lqpos(M.T)
+    assert np.allclose(L @ Qr, M.T) and np.allclose(Qr @ Qr.T, np.eye(5))
+    assert np.all(np.diag(L) > 0) and np.allclose(np.triu(L, 1), 0)
+    # uniqueness: a sign-flipped input gives the same R and a sign-flipped Q
+    Q2, R2 = oracle.qrpos(-M)
+    assert np.allclose(R2, R) and np.allclose(Q2, -Q)
+
+
+# ---------------------------------------------------------------------------- states / gauge
+
+
+def test_transfer_matrix_and_gauge(oracle):
+    psi = oracle.randomimps(5, [2, 3], seed=0)
+    el, er, em = oracle.mixedcanonical_errors(psi)
+    assert max(el, er, em) < 1e-12
+    # test/algorithms/envs.jl:15-22: TransferMatrix == sequential pushes
+    L = np.eye(5)
+    w = oracle.transfer_left(psi.AL, psi.AL, L.reshape(-1, order="F")).reshape(5, 5, order="F")
+    Lseq = oracle.push_env_left(oracle.push_env_left(L, psi.AL[0]), psi.AL[1])
+    assert np.linalg.norm(w - Lseq) / np.linalg.norm(Lseq) < 1e-12
+    # envs.jl:25-28: AL transfer fixed point eigenvalue 1
+    lam = oracle.dot_psi(psi, psi)
+    assert abs(lam - 1) < 1e-10
+    # api.jl:166-169: regauge!(AC, C) gives AL with AL C = AC
+    ALr = oracle.regauge(psi.AC[0], psi.C[0])
+    assert np.allclose(oracle.mul_AL_C(ALr, psi.C[0]), psi.AC[0], atol=1e-9)
+    assert abs(np.linalg.norm(psi.AC[0]) - 1) < 1e-12
+
+
+# ---------------------------------------------------------------------------- envs / energy
+
+
+def test_energy_product_state_and_rdm_crosscheck(oracle):
+    Ws = [oracle.tfim_mpo()]
+    # test/operators/mpo.jl:83-85: all-up product state, E = -h <sz> = -1
+    psi = oracle.prodimps([2], [0])
+    envs = oracle.dmrg_cache(psi, Ws)
+    assert abs(oracle.expectation_energy(psi, Ws, envs) + 1.0) < 1e-12
+    # api.jl:510-527 (generic state): E_cell = sum_l (-<X_l X_l+1> - <Z_l>)
+    psi = oracle.randomimps(6, [2, 2], seed=3)
+    Ws2 = Ws * 2
+    envs = oracle.dmrg_cache(psi, Ws2, tol=1e-13)
+    E = oracle.expectation_energy(psi, Ws2, envs)
+    X, Z = oracle.sigma_x(), oracle.sigma_z()
+    psi_shift = oracle.CanonicalIMPS(psi.AL[1:] + psi.AL[:1], psi.AR[1:] + psi.AR[:1],
+                                     psi.C[1:] + psi.C[:1], psi.AC[1:] + psi.AC[:1])
+    Erdm = sum(-oracle.local_expectation2(p, 0, np.kron(X, X)) - oracle.local_expectation1(p, 0, Z)
+               for p in (psi, psi_shift))
+    assert abs(E - Erdm) < 1e-9
+    # api.jl:207-210: recalculate! leaves the energy unchanged
+    oracle.recalculate(envs, psi, Ws2, 1e-12)
+    assert abs(oracle.expectation_energy(psi, Ws2, envs) - E) < 1e-9
+    # regularised identity levels are traceless against rho
+    w = Ws2[0].shape[0]
+    for l in range(2):
+        rho = psi.C[l - 1] @ psi.C[l - 1].T
+        assert abs(np.sum(rho * envs.lefts[l][:, w - 1, :].T)) < 1e-10
+        assert np.array_equal(envs.lefts[l][:, 0, :], np.eye(6))
+        assert np.array_equal(envs.rights[l][:, w - 1, :], np.eye(6))
+
+
+def test_galerkin_zero_on_exact_eigenstate(oracle):
+    # diagonal on-site model: the lowest product state is an exact eigen-MPS
+    lam = np.array([-2.0, 1.0])
+    W = oracle.mpohamiltonian(np.diag(lam), [])
+    psi = oracle.prodimps([2], [0])
+    envs = oracle.dmrg_cache(psi, [W])
+    assert oracle.calc_galerkin(psi, [W], envs) < 1e-14
+    assert oracle.expectation_energy(psi, [W], envs) == pytest.approx(-2.0, abs=1e-13)
+
+
+def test_vumps_tfim_critical_energy(oracle):
+    # test/algorithms/vumps.jl:14-18: TFIM J = h = 1, |e + 4/pi| < 1e-6 at D = 12, 2-site cell
+    Ws = [oracle.tfim_mpo()] * 2
+    psi0 = oracle.randomimps(12, [2, 2], seed=0)
+    psi, envs, info = oracle.find_groundstate(psi0, Ws, oracle.VUMPS(tol=1e-10))
+    assert info.converged and info.itererr < 1e-9
+    assert abs(info.energies[-1] / 2 + 4 / math.pi) < 1e-6
+    # api.jl:488: calc_galerkin(psi, H, envs) == info.itererr
+    assert oracle.calc_galerkin(psi, Ws, envs) == info.itererr
+    # Hellmann-Feynman at h = 1: de/dh = -<sz> = -2/pi (critical: D-limited accuracy)
+    assert abs(oracle.local_expectation1(psi, 0, oracle.sigma_z()) - 2 / math.pi) < 1e-3
+
+
+def test_vumps_xxx_energy(oracle):
+    # test/algorithms/vumps.jl:4-12: XXX, |e - (1/4 - ln 2)| < 2e-4, Galerkin < 1e-9
+    Ws = [oracle.xxz_mpo()] * 2
+    psi0 = oracle.randomimps(12, [2, 2], seed=1)
+    psi, envs, info = oracle.find_groundstate(psi0, Ws, oracle.VUMPS(tol=1e-10))
+    assert info.itererr < 1e-9
+    assert abs(info.energies[-1] / 2 - (0.25 - math.log(2))) < 2e-4
