@@ -1,0 +1,245 @@
+/*
+ * imps.h — C ABI of the MI355X (gfx950) VUMPS library (libimps.so).
+ *
+ * The entry points replace the Julia callables of InfiniteMPSAlgorithms.jl's VUMPS
+ * path; each declaration cites the reference interface it stands in for (paths are
+ * relative to the reference repository root).  A host language binds them with its
+ * FFI (Julia `ccall`, Python `ctypes`).
+ *
+ * Conventions
+ *  - Every call returns an int status: IMPS_OK (0) or a negative IMPS_E_* code;
+ *    imps_last_error() describes the last failure on a context.  Nothing throws or
+ *    longjmps across the ABI.
+ *  - Tensors cross the ABI as column-major float64 arrays (Julia layout):
+ *      MPS tensor   A[a,s,b]      at a + D*(s + d*b)            (D, d, D)
+ *      bond matrix  C[a,b]        at a + D*b                    (D, D)
+ *      MPO tensor   W[wl,u,wr,dd] at wl + w*(u + d*(wr + w*dd)) (w, d, w, d)
+ *      environment  G[a,k,b]      at a + D*(k + w*b)            (D, w, D)
+ *    GL = leftenv is (bra, w, ket), GR = rightenv is (ket, w, bra).
+ *  - Sites are 0-based here (the reference is 1-based); site indices wrap
+ *    periodically like PeriodicVector (src/utility.jl:16-33).
+ *  - Host pointers are owned by the caller and only read/written during the call;
+ *    the context owns all device memory.  Entry points with the suffix `_dev` take
+ *    device pointers and run on the context's stream.
+ *  - A context is not thread-safe: use one context per host thread.
+ *  - Scope: real float64 data, a uniform bond dimension D, square Schur MPO tensors
+ *    with the same level count w on every site.
+ */
+#ifndef IMPS_H
+#define IMPS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct imps_ctx imps_ctx;
+
+enum {
+  IMPS_OK = 0,
+  IMPS_E_ARG = -1,        /* invalid argument / dimension mismatch (DimensionMismatch, ArgumentError) */
+  IMPS_E_STATE = -2,      /* required model/state/environments not set */
+  IMPS_E_HIP = -3,        /* HIP runtime failure */
+  IMPS_E_SOLVER = -4,     /* rocSOLVER/rocBLAS failure */
+  IMPS_E_UNSUPPORTED = -5 /* configuration outside the supported scope */
+};
+
+/* Arithmetic of the bulk contractions (H_AC, H_C, transfer pushes). */
+enum {
+  IMPS_PREC_FP64 = 0,  /* FP64 MFMA */
+  IMPS_PREC_FP32 = 1,  /* FP32 MFMA on FP64 data, FP32 accumulation */
+  IMPS_PREC_BF16X3 = 2 /* split-BF16 three-term products, FP32 accumulation */
+};
+
+enum { IMPS_SIDE_LEFT = 0, IMPS_SIDE_RIGHT = 1 };
+
+/* ---- lifecycle ---------------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` with its own stream. */
+int imps_create(int device, imps_ctx** out);
+int imps_destroy(imps_ctx* ctx);
+/* Message of the last failed call on `ctx` (or of the last failed imps_create when
+ * ctx is NULL).  Valid until the next call on the same context. */
+const char* imps_last_error(const imps_ctx* ctx);
+
+/* ---- model and state ---------------------------------------------------------- */
+
+/* SparseIMPO (src/operators/mpohamiltonian.jl:33-56) as the dense Schur tensors of
+ * tompotensors(H) (mpohamiltonian.jl:150): W[site] has shape (w, d, w, d).
+ * Levels 0 and w-1 must be the unit levels (W[0,:,0,:] = W[w-1,:,w-1,:] = I). */
+int imps_set_mpo(imps_ctx* ctx, int nsites, int w, int d, const double* const* W);
+
+/* CanonicalIMPS (src/states/canonicalmps.jl:17-40): AL, AR, AC of shape (D, d, D)
+ * and C of shape (D, D) per site; C[l] is the bond right of site l. */
+int imps_set_state(imps_ctx* ctx, int nsites, int D, const double* const* AL,
+                   const double* const* AR, const double* const* C, const double* const* AC);
+int imps_get_state(imps_ctx* ctx, double* const* AL, double* const* AR, double* const* C,
+                   double* const* AC);
+
+/* CanonicalIMPS(As) (src/states/canonicalmps.jl:151-170): uploads raw tensors and
+ * brings them to the mixed-canonical form on the device with gaugefix!(order = :LR)
+ * (src/states/ortho.jl:64-99), tolerance `tol`, at most `maxiter` sweeps. */
+int imps_set_state_raw(imps_ctx* ctx, int nsites, int D, const double* const* A, double tol,
+                       int maxiter);
+
+/* ---- effective Hamiltonians (src/algorithms/groundstates/effective.jl) -------- */
+
+/* Replaces AC_hamiltonian(site, ψ, H, ψ, envs)(x) (effective.jl:66-68, 90-102):
+ * y[b,u,c] = Σ GL[b,i,a] x[a,s,bb] W[i,u,j,s] GR[bb,j,c] with GL = leftenv(site),
+ * GR = rightenv(site).  x, y: (D, d, D). */
+int imps_hac_apply(imps_ctx* ctx, int site, const double* x, double* y, int prec);
+int imps_hac_apply_dev(imps_ctx* ctx, int site, const double* d_x, double* d_y, int prec);
+
+/* Replaces C_hamiltonian(site, ...)(x) (effective.jl:52-54, 72-76):
+ * y[α,β] = Σ GL[α,w,α'] x[α',β'] GR[β',w,β] with GL = leftenv(site+1),
+ * GR = rightenv(site).  x, y: (D, D). */
+int imps_hc_apply(imps_ctx* ctx, int site, const double* x, double* y, int prec);
+int imps_hc_apply_dev(imps_ctx* ctx, int site, const double* d_x, double* d_y, int prec);
+
+/* ---- transfer matrices and environments --------------------------------------- */
+
+/* TransferMatrix(ψ.AL, ψ.AL; side=:left) * v or TransferMatrix(ψ.AR, ψ.AR;
+ * side=:right) * v over the unit cell (src/transfermatrix.jl:139-166), v: (D, D). */
+int imps_transfer_apply(imps_ctx* ctx, int side, const double* v, double* out);
+
+/* regularize!(v, lvec, rvec) (src/algorithms/groundstates/envs.jl:28-32):
+ * v -= (Σ lvec[a,b] v[b,a]) * rvec, all (n, n), in place on the host array v. */
+int imps_regularize(imps_ctx* ctx, int n, double* v, const double* lvec, const double* rvec);
+
+/* DMRGCache(ψ, H; tol, maxiter, krylovdim) (envs.jl:321-518) when no environments
+ * exist yet, otherwise recalculate!(envs, ψ, H; tol) (envs.jl:530-546): the previous
+ * levels are the GMRES starting vectors. */
+int imps_envs_recalculate(imps_ctx* ctx, double tol, int maxiter, int krylovdim);
+/* Drops the environments so the next recalculation starts from zero vectors. */
+int imps_envs_reset(imps_ctx* ctx);
+
+/* leftenv(envs, site) / rightenv(envs, site) (src/environments.jl:230-233):
+ * (D, w, D) arrays.  imps_set_env overrides one environment (for tests/drivers). */
+int imps_get_env(imps_ctx* ctx, int site, int side, double* out);
+int imps_set_env(imps_ctx* ctx, int site, int side, const double* in);
+
+/* ---- Krylov solvers on device-resident vectors --------------------------------- */
+
+typedef struct imps_krylov_params {
+  double tol;    /* absolute residual tolerance */
+  int krylovdim; /* Defaults.krylovdim = 30 */
+  int maxiter;   /* Defaults.maxiter = 300 (restarts) */
+  int eager;     /* 1: test convergence after every expansion */
+  int prec;      /* IMPS_PREC_* of the operator applications */
+} imps_krylov_params;
+
+typedef struct imps_krylov_info {
+  int converged;
+  int numiter; /* restarts + 1 */
+  int numops;  /* operator applications */
+  double normres;
+} imps_krylov_info;
+
+/* fixedpoint(AC_hamiltonian(site,...), ψ.AC[site], :SR, Lanczos) (src/utility.jl:
+ * 180-186): the lowest eigenpair; the eigenvector (unit norm, sign fixed so that
+ * <x0, v> >= 0) is written to `vec` (host, (D, d, D)) when vec != NULL and kept on
+ * the device as the pending AC of `site`. */
+int imps_eigsolve_ac(imps_ctx* ctx, int site, const imps_krylov_params* p, double* lambda,
+                     double* vec, imps_krylov_info* info);
+/* Same for C_hamiltonian(site, ...) and ψ.C[site] (vumps.jl:14-15). */
+int imps_eigsolve_c(imps_ctx* ctx, int site, const imps_krylov_params* p, double* lambda,
+                    double* vec, imps_krylov_info* info);
+
+/* regauge!(AC, C; alg = QRpos()) (src/states/ortho.jl:214-219) of the pending AC and
+ * C of `site` (from imps_eigsolve_ac/c): AL = Q(QRpos(AC)) * Q(QRpos(C))^T, kept as
+ * the pending AL of `site` and written to `al` (host) when al != NULL. */
+int imps_regauge(imps_ctx* ctx, int site, double* al);
+
+/* gauge_step! (src/algorithms/groundstates/vumps.jl:23-32): AL <- pending ALs,
+ * gaugefix!(ψ, AL, C[end]; order = :R, tol, maxiter) (ortho.jl:95-99, 166-202),
+ * then AC = AL*C.  `sweeps` receives the number of LQ sweeps (may be NULL). */
+int imps_gaugefix_right(imps_ctx* ctx, double tol, int maxiter, int* sweeps);
+
+/* calc_galerkin(ψ, H, envs) (vumps.jl:111-124). */
+int imps_galerkin(imps_ctx* ctx, double* eps);
+/* expectationvalue(ψ, H::SparseIMPO, envs) (src/observables/expval.jl:93-112):
+ * unit-cell total energy divided by dot(ψ, ψ) (canonicalmps.jl:262-269). */
+int imps_energy(imps_ctx* ctx, double* e);
+/* expectationvalue(ψ, (site,) => O) (expval.jl:33-37), O: (d, d). */
+int imps_local_expectation(imps_ctx* ctx, int site, const double* O, double* val);
+
+/* ---- the VUMPS driver ------------------------------------------------------------ */
+
+/* VUMPS(; tol, maxiter, verbosity, alg_gauge, alg_eigsolve, alg_environments)
+ * (src/algorithms/algdefs.jl:26-35) with the DynamicTol defaults of
+ * src/utility.jl:308-364. */
+typedef struct imps_vumps_params {
+  double tol;             /* Defaults.tol = 1e-10 (Galerkin target) */
+  int maxiter;            /* Defaults.maxiter = 300 */
+  int verbosity;          /* 0: silent, >0: one line per iteration on stdout */
+  double tol_min;         /* 1e-14 */
+  double tol_max;         /* 1e-4 */
+  double eigs_tolfactor;  /* 1e-3 */
+  double gauge_tolfactor; /* 1e-6 */
+  double envs_tolfactor;  /* 1e-4 */
+  int krylovdim;          /* 30 */
+  int eig_maxiter;        /* 300 */
+  int gauge_maxiter;      /* 300 */
+  int envs_maxiter;       /* 300 */
+  int prec;               /* IMPS_PREC_* of the bulk contractions */
+  double refine_below;    /* mixed precision: switch to FP64 once eps < refine_below */
+} imps_vumps_params;
+
+/* IterativeConvergenceInfo (algdefs.jl:180-194) plus timings. */
+typedef struct imps_vumps_info {
+  int niter;
+  int converged;
+  double itererr;     /* last Galerkin error */
+  double energy;      /* last expectationvalue(ψ, H, envs) */
+  double seconds;     /* wall time of the loop */
+  double t_galerkin_reached; /* seconds until eps <= tol first held (-1 if never) */
+  int hac_applies;
+  int hc_applies;
+  int prec_switch_iter; /* iteration at which FP64 refinement started (-1 if none) */
+} imps_vumps_info;
+
+/* Per-iteration callback: return nonzero to stop the loop. */
+typedef int (*imps_iter_cb)(int iter, double eps, double energy, void* user);
+
+/* Fills *p with the reference defaults. */
+int imps_vumps_default_params(imps_vumps_params* p);
+
+/* find_groundstate(ψ₀, H::SparseIMPO, alg::VUMPS, envs; which = :SR)
+ * (src/algorithms/groundstates/vumps.jl:51-86) on the context's state and MPO.
+ * Builds DMRGCache(ψ₀, H) first when no environments exist.  `losses` (may be NULL)
+ * receives up to `nloss` Galerkin errors: the initial one, then one per iteration. */
+int imps_vumps_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* info,
+                   double* losses, int nloss, imps_iter_cb cb, void* user);
+
+/* ---- measurement helpers ----------------------------------------------------------- */
+
+/* Runs `reps` H_AC applications on device-resident random data of the current
+ * site-0 shapes and returns the average duration of one application and of each of
+ * its kernels (HIP events on the context's stream). */
+typedef struct imps_hac_timing {
+  double ms_apply;  /* whole application */
+  double ms_gemm1;  /* GL x  */
+  double ms_wmix;   /* W mix */
+  double ms_gemm2;  /* (.) GR */
+} imps_hac_timing;
+int imps_bench_hac(imps_ctx* ctx, int site, int reps, int prec, imps_hac_timing* t);
+
+/* General strided GEMM of the library on device pointers (column-major):
+ * C = alpha*op(A)*op(B) + beta*C, op = 'N' or 'T'.  Exposed for tests. */
+int imps_gemm_dev(imps_ctx* ctx, char transa, char transb, int m, int n, int k, double alpha,
+                  const double* d_a, int lda, const double* d_b, int ldb, double beta, double* d_c,
+                  int ldc, int prec);
+/* Device memory helpers for callers without their own allocator. */
+int imps_dev_alloc(imps_ctx* ctx, size_t bytes, void** d_ptr);
+int imps_dev_free(imps_ctx* ctx, void* d_ptr);
+int imps_memcpy_h2d(imps_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int imps_memcpy_d2h(imps_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+int imps_synchronize(imps_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMPS_H */

@@ -1,0 +1,21 @@
+"""MI355X (gfx950) VUMPS inner loop of InfiniteMPSAlgorithms.jl behind a C ABI.
+
+The numerics live in ``libimps.so`` (HIP kernels + C++ runtime); this package is the
+host-side mirror of the reference's API (``find_groundstate``, ``VUMPS``,
+``AC_hamiltonian``, ``C_hamiltonian``, ``DMRGCache``, ``leftenv``/``rightenv``,
+``TransferMatrix``, ...).
+"""
+from . import _lib
+from .api import (VUMPS, AC_hamiltonian, C_hamiltonian, CanonicalIMPS, Context, DMRGCache,
+                  GPUCache, IterativeConvergenceInfo, Lanczos, SparseIMPO, TransferMatrix,
+                  calc_galerkin, canonicalimps, expectationvalue, find_groundstate, fixedpoint,
+                  leftenv, random_tensors, randomimps, recalculate_, regauge_, regularize_,
+                  rightenv)
+from . import models
+
+__all__ = [
+    "VUMPS", "AC_hamiltonian", "C_hamiltonian", "CanonicalIMPS", "Context", "DMRGCache", "GPUCache",
+    "IterativeConvergenceInfo", "Lanczos", "SparseIMPO", "TransferMatrix", "calc_galerkin",
+    "canonicalimps", "expectationvalue", "find_groundstate", "fixedpoint", "leftenv", "random_tensors",
+    "randomimps", "recalculate_", "regauge_", "regularize_", "rightenv", "models",
+]

@@ -1,0 +1,448 @@
+"""Host-side mirror of InfiniteMPSAlgorithms.jl's VUMPS API on top of libimps.
+
+Names, argument meaning and error behaviour follow the reference (Julia ``!``
+functions become a trailing underscore).  All numerics run in the HIP library; the
+Python objects only hold host copies of tensors and the device context.
+
+Index conventions are the reference's column-major ones: MPS ``A[a, s, b]``, bond
+matrices ``C[a, b]`` (bond right of a site), MPO ``W[wl, u, wr, d]``, environments
+``GL[bra, w, ket]`` and ``GR[ket, w, bra]``.  Sites are 0-based.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, fortran, ptr_array
+
+
+class Context:
+    """One libimps context (a HIP stream plus device-resident state) on `device`."""
+
+    def __init__(self, device=0):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        st = L.imps_create(device, ctypes.byref(h))
+        if st != 0:
+            raise _lib.ImpsError(f"imps_create failed ({st}): {L.imps_last_error(None).decode()}")
+        self.h = h
+        self.L = L
+        self.N = self.w = self.d = self.D = 0
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.L.imps_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        check(self.h, getattr(self.L, name)(self.h, *args), name)
+
+    # ---- model / state
+    def set_mpo(self, H: "SparseIMPO"):
+        Ws = [fortran(W) for W in H.Ws]
+        pp, keep = ptr_array(Ws)
+        self.call("imps_set_mpo", len(Ws), H.w, H.d, pp)
+        self.N, self.w, self.d = len(Ws), H.w, H.d
+
+    def set_state(self, psi: "CanonicalIMPS"):
+        AL, AR, C, AC = ([fortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
+        keep = [ptr_array(x) for x in (AL, AR, C, AC)]
+        self.call("imps_set_state", len(AL), psi.D, *(k[0] for k in keep))
+        self.D = psi.D
+
+    def set_state_raw(self, As, tol, maxiter):
+        As = [fortran(a) for a in As]
+        pp, keep = ptr_array(As)
+        self.call("imps_set_state_raw", len(As), As[0].shape[0], pp, float(tol), int(maxiter))
+        self.D = As[0].shape[0]
+
+    def get_state(self):
+        N, D, d = self.N, self.D, self.d
+        AL = [np.zeros((D, d, D), order="F") for _ in range(N)]
+        AR = [np.zeros((D, d, D), order="F") for _ in range(N)]
+        C = [np.zeros((D, D), order="F") for _ in range(N)]
+        AC = [np.zeros((D, d, D), order="F") for _ in range(N)]
+        keep = [ptr_array(x) for x in (AL, AR, C, AC)]
+        self.call("imps_get_state", *(k[0] for k in keep))
+        return CanonicalIMPS(AL, AR, C, AC)
+
+    def get_env(self, site, side):
+        out = np.zeros((self.D, self.w, self.D), order="F")
+        self.call("imps_get_env", int(site), int(side), dptr(out))
+        return out
+
+    def set_env(self, site, side, G):
+        G = fortran(G)
+        self.call("imps_set_env", int(site), int(side), dptr(G))
+
+
+# ---------------------------------------------------------------------------- data types
+
+
+class SparseIMPO:
+    """SparseIMPO (src/operators/mpohamiltonian.jl:33-56): per-site Schur tensors, stored
+    densely as ``tompotensor`` arrays (w, d, w, d).  Square tensors with the same w and d
+    on every site are supported."""
+
+    def __init__(self, Ws):
+        Ws = [fortran(W) for W in Ws]
+        if not Ws:
+            raise ValueError("Ws must not be empty")
+        w, d = Ws[0].shape[0], Ws[0].shape[1]
+        for W in Ws:
+            if W.shape != (w, d, w, d):
+                raise ValueError(f"all Schur tensors must have shape {(w, d, w, d)} (level chain mismatch)")
+            if not (np.array_equal(W[0, :, 0, :], np.eye(d)) and np.array_equal(W[w - 1, :, w - 1, :], np.eye(d))):
+                raise ValueError("levels 1 and w must be unit levels")
+        self.Ws = Ws
+        self.w, self.d = w, d
+
+    def __len__(self):
+        return len(self.Ws)
+
+    def __getitem__(self, i):
+        return self.Ws[i % len(self.Ws)]
+
+    def phydims(self):
+        return [self.d] * len(self.Ws)
+
+    def tompotensors(self):
+        """tompotensors(H) (mpohamiltonian.jl:150)."""
+        return [W.copy(order="F") for W in self.Ws]
+
+    def bonddim(self, l=None):
+        """Number of Schur levels (mpohamiltonian.jl:72-84)."""
+        return self.w
+
+
+@dataclass
+class CanonicalIMPS:
+    """CanonicalIMPS{Float64} (src/states/canonicalmps.jl:17-40) as host arrays:
+    AL[l] C[l] = AC[l] = C[l-1] AR[l]."""
+
+    AL: list
+    AR: list
+    C: list
+    AC: list
+
+    def __len__(self):
+        return len(self.AL)
+
+    @property
+    def D(self):
+        return self.AL[0].shape[0]
+
+    @property
+    def d(self):
+        return self.AL[0].shape[1]
+
+    def copy(self):
+        return CanonicalIMPS(*[[x.copy(order="F") for x in xs] for xs in (self.AL, self.AR, self.C, self.AC)])
+
+    def norm(self):
+        """norm(ψ) = ‖AC[1]‖ (canonicalmps.jl:238)."""
+        return float(np.linalg.norm(self.AC[0]))
+
+
+def random_tensors(D, phydims, seed):
+    """Raw ψ₀ tensors: N(0,1) entries from numpy PCG64(seed), site by site, column-major
+    (the role of randn in randomimps, src/states/constructors.jl:13-22)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return [np.reshape(rng.standard_normal(D * d * D), (D, d, D), order="F") for d in phydims]
+
+
+def canonicalimps(As, H: SparseIMPO, ctx: Context | None = None, tol=1e-13, maxiter=300):
+    """CanonicalIMPS(As) (canonicalmps.jl:151-170): gaugefix!(order = :LR) from C₀ = I,
+    run on the device.  Returns (ψ, ctx)."""
+    ctx = ctx or Context()
+    ctx.set_mpo(H)
+    ctx.set_state_raw(As, tol, maxiter)
+    return ctx.get_state(), ctx
+
+
+def randomimps(D, H: SparseIMPO, seed=0, ctx: Context | None = None, **kw):
+    """randomimps(Float64, phydims(H); D) (constructors.jl:13-22) with the seeded generator."""
+    return canonicalimps(random_tensors(D, H.phydims(), seed), H, ctx, **kw)
+
+
+# ---------------------------------------------------------------------------- environments
+
+
+class GPUCache:
+    """Environments of the Hamiltonian channel (DMRGCache, envs.jl:248-253) resident on
+    the device.  leftenv/rightenv copy one environment to the host."""
+
+    def __init__(self, psi: CanonicalIMPS, H: SparseIMPO, ctx: Context | None = None):
+        self.ctx = ctx or Context()
+        self.operator = H
+        self.ket = psi
+        if self.ctx.N != len(H) or self.ctx.w != H.w:
+            self.ctx.set_mpo(H)
+        self.ctx.set_state(psi)
+
+    def __len__(self):
+        return len(self.ket)
+
+
+def DMRGCache(psi: CanonicalIMPS, H: SparseIMPO, tol=1e-10, maxiter=300, krylovdim=30, ctx=None):
+    """DMRGCache(ψ, H::SparseIMPO; tol, maxiter, krylovdim) (envs.jl:321-518)."""
+    envs = GPUCache(psi, H, ctx)
+    envs.ctx.call("imps_envs_reset")
+    envs.ctx.call("imps_envs_recalculate", float(tol), int(maxiter), int(krylovdim))
+    return envs
+
+
+def recalculate_(envs: GPUCache, psi: CanonicalIMPS, H: SparseIMPO | None = None, tol=1e-10,
+                 maxiter=300, krylovdim=30):
+    """recalculate!(envs, ψ, H; tol) (envs.jl:530-546): warm-started rebuild."""
+    if psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ket = psi
+    envs.ctx.call("imps_envs_recalculate", float(tol), int(maxiter), int(krylovdim))
+    return envs
+
+
+def leftenv(envs: GPUCache, site):
+    """leftenv(envs, ℓ) (src/environments.jl:230): (D, w, D), periodic site index."""
+    return envs.ctx.get_env(site, _lib.IMPS_SIDE_LEFT)
+
+
+def rightenv(envs: GPUCache, site):
+    """rightenv(envs, ℓ) (src/environments.jl:233)."""
+    return envs.ctx.get_env(site, _lib.IMPS_SIDE_RIGHT)
+
+
+# ---------------------------------------------------------------------------- effective operators
+
+
+class MPO_AC_Hamiltonian:
+    """AC_hamiltonian(site, ψ, H, ψ, envs) (effective.jl:66-68); calling it applies
+    effective.jl:90-102 on the device."""
+
+    def __init__(self, site, envs: GPUCache, prec=_lib.IMPS_PREC_FP64):
+        self.site, self.envs, self.prec = site, envs, prec
+
+    def __call__(self, x):
+        x = fortran(x)
+        y = np.zeros_like(x, order="F")
+        self.envs.ctx.call("imps_hac_apply", int(self.site), dptr(x), dptr(y), int(self.prec))
+        return y
+
+
+class MPO_C_Hamiltonian:
+    """C_hamiltonian(site, ...) (effective.jl:52-54) applying effective.jl:72-76."""
+
+    def __init__(self, site, envs: GPUCache, prec=_lib.IMPS_PREC_FP64):
+        self.site, self.envs, self.prec = site, envs, prec
+
+    def __call__(self, x):
+        x = fortran(x)
+        y = np.zeros_like(x, order="F")
+        self.envs.ctx.call("imps_hc_apply", int(self.site), dptr(x), dptr(y), int(self.prec))
+        return y
+
+
+def AC_hamiltonian(site, below, operator, above, envs: GPUCache, prec=_lib.IMPS_PREC_FP64):
+    return MPO_AC_Hamiltonian(site, envs, prec)
+
+
+def C_hamiltonian(site, below, operator, above, envs: GPUCache, prec=_lib.IMPS_PREC_FP64):
+    return MPO_C_Hamiltonian(site, envs, prec)
+
+
+@dataclass
+class Lanczos:
+    """KrylovKit.Lanczos parameters as used by fixedpoint (utility.jl:180-186)."""
+
+    tol: float = 1e-10
+    krylovdim: int = 30
+    maxiter: int = 300
+    eager: bool = True
+
+
+def fixedpoint(op, x0, which="SR", alg: Lanczos | None = None):
+    """fixedpoint(op, x₀, :SR, Lanczos) (utility.jl:180-186) for the effective operators:
+    the Krylov vectors stay on the device.  Returns (λ, v, info)."""
+    if which != "SR":
+        raise ValueError("only which = :SR is supported for the effective Hamiltonians")
+    alg = alg or Lanczos()
+    p = _lib.KrylovParams(alg.tol, alg.krylovdim, alg.maxiter, 1 if alg.eager else 0, int(op.prec))
+    info = _lib.KrylovInfo()
+    lam = ctypes.c_double()
+    ctx = op.envs.ctx
+    if isinstance(op, MPO_AC_Hamiltonian):
+        v = np.zeros((ctx.D, ctx.d, ctx.D), order="F")
+        ctx.call("imps_eigsolve_ac", int(op.site), ctypes.byref(p), ctypes.byref(lam), dptr(v), ctypes.byref(info))
+    elif isinstance(op, MPO_C_Hamiltonian):
+        v = np.zeros((ctx.D, ctx.D), order="F")
+        ctx.call("imps_eigsolve_c", int(op.site), ctypes.byref(p), ctypes.byref(lam), dptr(v), ctypes.byref(info))
+    else:
+        raise TypeError("fixedpoint: operator must come from AC_hamiltonian or C_hamiltonian")
+    return lam.value, v, info
+
+
+class TransferMatrix:
+    """TransferMatrix(ψ.AL, ψ.AL; side = :left) or (ψ.AR, ψ.AR; side = :right)
+    (src/transfermatrix.jl:122-191), applied on the device to vec'd (D, D) matrices."""
+
+    def __init__(self, envs_or_ctx, side="left"):
+        self.ctx = envs_or_ctx.ctx if isinstance(envs_or_ctx, GPUCache) else envs_or_ctx
+        self.side = _lib.IMPS_SIDE_LEFT if side == "left" else _lib.IMPS_SIDE_RIGHT
+        D = self.ctx.D
+        self.shape = (D * D, D * D)
+
+    def __matmul__(self, v):
+        return self(v)
+
+    def __call__(self, v):
+        D = self.ctx.D
+        X = fortran(np.reshape(v, (D, D), order="F"))
+        out = np.zeros((D, D), order="F")
+        self.ctx.call("imps_transfer_apply", self.side, dptr(X), dptr(out))
+        return out.reshape(-1, order="F") if np.ndim(v) == 1 else out
+
+
+def regularize_(v, lvec, rvec, ctx: Context | None = None):
+    """regularize!(v, lvec, rvec) (envs.jl:28-32): v -= (Σ lvec[a,b] v[b,a]) rvec, in place."""
+    if v.dtype != np.float64 or not v.flags.f_contiguous:
+        raise TypeError("regularize_: v must be a Fortran-ordered float64 array")
+    n = v.shape[0]
+    if v.shape != (n, n) or np.shape(lvec) != (n, n) or np.shape(rvec) != (n, n):
+        raise ValueError("regularize_: DimensionMismatch")
+    own = ctx is None
+    ctx = ctx or Context()
+    l, r = fortran(lvec), fortran(rvec)
+    ctx.call("imps_regularize", n, dptr(v), dptr(l), dptr(r))
+    if own:
+        ctx.close()
+    return v
+
+
+def regauge_(envs: GPUCache, site):
+    """regauge!(AC, C; alg = QRpos()) (ortho.jl:214-219) of the pending (AC, C) produced by
+    fixedpoint on `site`; returns the candidate AL."""
+    ctx = envs.ctx
+    al = np.zeros((ctx.D, ctx.d, ctx.D), order="F")
+    ctx.call("imps_regauge", int(site), dptr(al))
+    return al
+
+
+def calc_galerkin(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
+    """calc_galerkin(ψ, H, envs) (vumps.jl:111-124)."""
+    if psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ket = psi
+    eps = ctypes.c_double()
+    envs.ctx.call("imps_galerkin", ctypes.byref(eps))
+    return eps.value
+
+
+def expectationvalue(psi: CanonicalIMPS, op, envs: GPUCache):
+    """expectationvalue(ψ, H::SparseIMPO, envs) (expval.jl:93-112) or, for
+    op = (site, O), the one-site value (expval.jl:33-37)."""
+    if psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ket = psi
+    val = ctypes.c_double()
+    if isinstance(op, SparseIMPO):
+        envs.ctx.call("imps_energy", ctypes.byref(val))
+    else:
+        site, O = op
+        O = fortran(O)
+        envs.ctx.call("imps_local_expectation", int(site), dptr(O), ctypes.byref(val))
+    return val.value
+
+
+# ---------------------------------------------------------------------------- VUMPS
+
+
+@dataclass
+class VUMPS:
+    """VUMPS(; D, tol, maxiter, verbosity, ...) (algdefs.jl:26-35) with the DynamicTol
+    defaults of utility.jl:308-364.  `prec` selects the arithmetic of the bulk
+    contractions; with a mixed precision the loop refines in FP64 once the Galerkin
+    error drops below `refine_below`."""
+
+    D: int = 64
+    tol: float = 1e-10
+    maxiter: int = 300
+    verbosity: int = 0
+    tol_min: float = 1e-14
+    tol_max: float = 1e-4
+    eigs_tolfactor: float = 1e-3
+    gauge_tolfactor: float = 1e-6
+    envs_tolfactor: float = 1e-4
+    krylovdim: int = 30
+    eig_maxiter: int = 300
+    gauge_maxiter: int = 300
+    envs_maxiter: int = 300
+    prec: int = _lib.IMPS_PREC_FP64
+    refine_below: float = 0.0
+
+    def params(self):
+        return _lib.VumpsParams(self.tol, self.maxiter, self.verbosity, self.tol_min, self.tol_max,
+                                self.eigs_tolfactor, self.gauge_tolfactor, self.envs_tolfactor,
+                                self.krylovdim, self.eig_maxiter, self.gauge_maxiter, self.envs_maxiter,
+                                self.prec, self.refine_below)
+
+
+@dataclass
+class IterativeConvergenceInfo:
+    """IterativeConvergenceInfo (algdefs.jl:180-194) plus the library's timings."""
+
+    niter: int
+    converged: bool
+    losses: list
+    energy: float
+    seconds: float
+    t_galerkin_reached: float
+    hac_applies: int
+    hc_applies: int
+    prec_switch_iter: int
+    energies: list = field(default_factory=list)
+
+    @property
+    def itererr(self):
+        return self.losses[-1] if self.losses else float("nan")
+
+
+def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | None = None,
+                     envs: GPUCache | None = None, seed=0, ctx: Context | None = None):
+    """find_groundstate(ψ₀, H::SparseIMPO, alg::VUMPS, envs) (vumps.jl:51-86).  With
+    psi0 = None a random state of bond dimension alg.D is generated (vumps.jl:96-102).
+    Returns (ψ, envs, info)."""
+    alg = alg or VUMPS()
+    if envs is None:
+        if psi0 is None:
+            psi0, ctx = randomimps(alg.D, H, seed=seed, ctx=ctx)
+        envs = GPUCache(psi0, H, ctx)
+    elif psi0 is not None and psi0 is not envs.ket:
+        envs.ctx.set_state(psi0)
+    p = alg.params()
+    info = _lib.VumpsInfo()
+    losses = np.zeros(alg.maxiter + 1)
+    energies = []
+
+    def _cb(it, eps, f, user):
+        energies.append(f)
+        return 0
+
+    cb = _lib.ITER_CB(_cb)
+    envs.ctx.call("imps_vumps_run", ctypes.byref(p), ctypes.byref(info),
+                  losses.ctypes.data_as(_lib.P_double), len(losses), cb, None)
+    psi = envs.ctx.get_state()
+    envs.ket = psi
+    nl = info.niter + 1
+    res = IterativeConvergenceInfo(info.niter, bool(info.converged), list(losses[:nl]), info.energy,
+                                   info.seconds, info.t_galerkin_reached, info.hac_applies,
+                                   info.hc_applies, info.prec_switch_iter, energies)
+    return psi, envs, res

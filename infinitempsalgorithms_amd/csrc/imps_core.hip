@@ -1,0 +1,1678 @@
+// libimps runtime: device-resident VUMPS state, the effective-Hamiltonian and
+// transfer operators built on the MFMA GEMM, Krylov solvers whose vectors stay in HBM
+// (FP64 Gram-Schmidt, host-side projected problems), QRpos/LQpos gauge fixing, the
+// Schur-MPO environment solves and the VUMPS driver, behind the C ABI of imps.h.
+//
+// Reference files cited below are relative to the InfiniteMPSAlgorithms.jl root.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "imps.h"
+#include "imps_internal.h"
+
+using namespace imps;
+
+namespace {
+
+struct ImpsError : std::runtime_error {
+  int code;
+  ImpsError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      throw ImpsError(IMPS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+#define RBCHK(expr)                                                                          \
+  do {                                                                                       \
+    rocblas_status s_ = (expr);                                                              \
+    if (s_ != rocblas_status_success)                                                        \
+      throw ImpsError(IMPS_E_SOLVER, std::string(#expr) + ": rocblas status " +              \
+                                         std::to_string((int)s_));                           \
+  } while (0)
+
+static void require(bool ok, int code, const std::string& msg) {
+  if (!ok) throw ImpsError(code, msg);
+}
+
+struct DevBuf {
+  double* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct imps_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle rb = nullptr;
+  std::string err;
+  // model: square Schur tensors (w, d, w, d) per site
+  int N = 0, w = 0, d = 0;
+  std::vector<std::vector<double>> Wh;
+  std::vector<double*> Wm;  // per-site H_AC mixing matrix (w d) x (w d)
+  // state
+  int D = 0;
+  std::vector<double*> AL, AR, AC, C, pAC, pC, pAL;
+  // environments
+  bool envs_valid = false;
+  std::vector<double*> GL, GR;
+  // workspaces and pinned host scratch for reductions
+  std::map<std::string, DevBuf> ws;
+  double* hpart = nullptr;
+  double* dpart = nullptr;
+  // counters
+  long hac_applies = 0, hc_applies = 0;
+};
+
+static thread_local std::string g_create_error;
+
+namespace {
+
+// ------------------------------------------------------------------ memory helpers
+
+double* dalloc(size_t n) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)));
+  return (double*)p;
+}
+
+void dfree(double*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+double* ws(imps_ctx* c, const std::string& name, size_t n) {
+  DevBuf& b = c->ws[name];
+  if (b.n < n) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(b.p);
+    b.p = dalloc(n);
+    b.n = n;
+  }
+  return b.p;
+}
+
+void h2d(imps_ctx* c, double* dst, const double* src, size_t n) {
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+}
+
+void d2h(imps_ctx* c, double* dst, const double* src, size_t n) {
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void d2d(imps_ctx* c, double* dst, const double* src, size_t n) {
+  if (dst == src || n == 0) return;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+}
+
+// ------------------------------------------------------------------ BLAS-1 (FP64)
+
+constexpr int kMaxDotBlocks = 1024;
+constexpr int kMaxVec = 64;
+
+void multidot(imps_ctx* c, size_t n, int nv, const double* X, size_t ldx, const double* y,
+              double* out) {
+  require(nv <= kMaxVec, IMPS_E_ARG, "multidot: too many vectors");
+  const int nb = dot_blocks(n);
+  HIPCHK(launch_multidot(n, nv, X, ldx, y, c->dpart, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hpart, c->dpart, sizeof(double) * nb * nv, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int v = 0; v < nv; ++v) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += c->hpart[(size_t)v * nb + b];
+    out[v] = s;
+  }
+}
+
+double dot(imps_ctx* c, size_t n, const double* x, const double* y) {
+  double r;
+  multidot(c, n, 1, x, n, y, &r);
+  return r;
+}
+
+double nrm2(imps_ctx* c, size_t n, const double* x) { return std::sqrt(std::max(0.0, dot(c, n, x, x))); }
+
+void multiaxpy(imps_ctx* c, size_t n, int nv, const double* X, size_t ldx, const double* coef,
+               double alpha, double* y) {
+  HIPCHK(launch_multiaxpy(n, nv, X, ldx, coef, alpha, y, c->stream));
+}
+
+void scal(imps_ctx* c, size_t n, double a, double* x) { HIPCHK(launch_scale(n, a, x, c->stream)); }
+
+void axpby(imps_ctx* c, size_t n, double a, const double* x, double b, double* y) {
+  HIPCHK(launch_axpby(n, a, x, b, y, c->stream));
+}
+
+void fill(imps_ctx* c, size_t n, double v, double* x) { HIPCHK(launch_fill(n, v, x, c->stream)); }
+
+// classical Gram-Schmidt applied twice; returns the coefficients in h[0..nv)
+void cgs2(imps_ctx* c, size_t n, int nv, const double* V, double* w, double* h) {
+  std::vector<double> h1(nv), h2(nv), m(nv);
+  multidot(c, n, nv, V, n, w, h1.data());
+  for (int i = 0; i < nv; ++i) m[i] = -h1[i];
+  multiaxpy(c, n, nv, V, n, m.data(), 1.0, w);
+  multidot(c, n, nv, V, n, w, h2.data());
+  for (int i = 0; i < nv; ++i) m[i] = -h2[i];
+  multiaxpy(c, n, nv, V, n, m.data(), 1.0, w);
+  for (int i = 0; i < nv; ++i) h[i] = h1[i] + h2[i];
+}
+
+// tr(l v) = sum l[a + ldl*b] v[b + ldv*a] for n x n (contiguous l and v)
+double trace_prod(imps_ctx* c, int n, const double* l, const double* v) {
+  int np = 0;
+  double* part = ws(c, "tp_part", (size_t)((n + 31) / 32) * ((n + 31) / 32));
+  HIPCHK(launch_trace_prod_parts(n, l, v, part, &np, c->stream));
+  std::vector<double> h(np);
+  d2h(c, h.data(), part, np);
+  double s = 0.0;
+  for (double x : h) s += x;
+  return s;
+}
+
+// ------------------------------------------------------------------ GEMM wrappers
+
+void gemm(imps_ctx* c, const GemmDesc& g, int prec) {
+  HIPCHK(launch_gemm(g, prec, c->stream));
+}
+
+// C(MxN, ldc) = alpha op(A) op(B) + beta C, column-major
+void gemm2(imps_ctx* c, char ta, char tb, int M, int N, int K, double alpha, const double* A,
+           long lda, const double* B, long ldb, double beta, double* Cm, long ldc, int prec = PREC_FP64) {
+  GemmDesc g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.Kc = K;
+  g.A = A;
+  g.B = B;
+  g.C = Cm;
+  g.ldc = ldc;
+  g.alpha = alpha;
+  g.beta = beta;
+  if (ta == 'N') {
+    g.sAm = 1;
+    g.sAk = lda;
+  } else {
+    g.sAm = lda;
+    g.sAk = 1;
+  }
+  if (tb == 'N') {
+    g.sBk = 1;
+    g.sBn = ldb;
+  } else {
+    g.sBk = ldb;
+    g.sBn = 1;
+  }
+  gemm(c, g, prec);
+}
+
+// ------------------------------------------------------------------ context checks
+
+void need_mpo(imps_ctx* c) { require(c->N > 0 && c->w > 0, IMPS_E_STATE, "MPO not set"); }
+void need_state(imps_ctx* c) { require(c->D > 0 && !c->AL.empty(), IMPS_E_STATE, "state not set"); }
+void need_envs(imps_ctx* c) { require(c->envs_valid, IMPS_E_STATE, "environments not computed"); }
+void need_match(imps_ctx* c) {
+  need_mpo(c);
+  need_state(c);
+  require((int)c->AL.size() == c->N, IMPS_E_ARG, "unit cell of the state and of the MPO differ");
+}
+int wrap(imps_ctx* c, int site) {
+  const int N = (int)c->AL.size() > 0 ? (int)c->AL.size() : c->N;
+  return ((site % N) + N) % N;
+}
+
+// ------------------------------------------------------------------ operators
+
+// H_AC (effective.jl:90-102) as GEMM (GL x) -> W mix -> GEMM (. GR)
+void hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
+  const int D = c->D, w = c->w, d = c->d;
+  site = wrap(c, site);
+  const double* GL = c->GL[site];
+  const double* GR = c->GR[site];
+  double* T1 = ws(c, "hac_T1", (size_t)D * w * d * D);
+  double* T2 = ws(c, "hac_T2", (size_t)D * w * d * D);
+  GemmDesc g1;
+  g1.M = D * w;
+  g1.N = d * D;
+  g1.K = D;
+  g1.Kc = D;
+  g1.A = GL;
+  g1.sAm = 1;
+  g1.sAk = (long)D * w;
+  g1.B = x;
+  g1.sBk = 1;
+  g1.sBn = D;
+  g1.C = T1;
+  g1.ldc = (long)D * w;
+  gemm(c, g1, prec);
+  HIPCHK(launch_wmix(D, w, d, c->Wm[site], T1, T2, c->stream));
+  GemmDesc g2;
+  g2.M = D * d;
+  g2.N = D;
+  g2.K = D * w;
+  g2.Kc = D * w;
+  g2.A = T2;
+  g2.sAm = 1;
+  g2.sAk = (long)D * d;
+  g2.B = GR;
+  g2.sBk = 1;
+  g2.sBn = (long)D * w;
+  g2.C = y;
+  g2.ldc = (long)D * d;
+  gemm(c, g2, prec);
+  c->hac_applies++;
+}
+
+// H_C (effective.jl:72-76): y = sum_k GL_k x GR_k with GL = leftenv(site+1), GR = rightenv(site)
+void hc_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
+  const int D = c->D, w = c->w;
+  site = wrap(c, site);
+  const double* GL = c->GL[wrap(c, site + 1)];
+  const double* GR = c->GR[site];
+  double* T1 = ws(c, "hc_T1", (size_t)D * w * D);
+  gemm2(c, 'N', 'N', D * w, D, D, 1.0, GL, (long)D * w, x, D, 0.0, T1, (long)D * w, prec);
+  if (D % 16 == 0) {
+    GemmDesc g;
+    g.M = D;
+    g.N = D;
+    g.K = D * w;
+    g.Kc = D;
+    g.A = T1;
+    g.sAm = 1;
+    g.sAk = (long)D * w;
+    g.sAc = D;
+    g.B = GR;
+    g.sBk = 1;
+    g.sBc = D;
+    g.sBn = (long)D * w;
+    g.C = y;
+    g.ldc = D;
+    gemm(c, g, prec);
+  } else {
+    double* T1p = ws(c, "hc_T1p", (size_t)D * w * D);
+    for (int k = 0; k < w; ++k)
+      HIPCHK(launch_copy2d(D, D, T1 + (size_t)D * k, (long)D * w, T1p + (size_t)D * D * k, D, c->stream));
+    gemm2(c, 'N', 'N', D, D, D * w, 1.0, T1p, D, GR, (long)D * w, 0.0, y, D, prec);
+  }
+  c->hc_applies++;
+}
+
+// generalized identity-channel pushes; X (D x D, ld ldx) -> out (D x D, ld ldo)
+// left:  out[a',b'] = sum below[a,s,a'] X[a,b] above[b,s,b']
+void push_left2(imps_ctx* c, const double* X, long ldx, const double* above, const double* below,
+                double* out, long ldo, int prec = PREC_FP64) {
+  const int D = c->D, d = c->d;
+  double* Y = ws(c, "pl_Y", (size_t)D * d * D);
+  gemm2(c, 'N', 'N', D, d * D, D, 1.0, X, ldx, above, D, 0.0, Y, D, prec);
+  gemm2(c, 'T', 'N', D, D, d * D, 1.0, below, (long)d * D, Y, (long)d * D, 0.0, out, ldo, prec);
+}
+
+// right: out[a',b'] = sum above[a',s,a] X[a,b] below[b',s,b]
+void push_right2(imps_ctx* c, const double* X, long ldx, const double* above, const double* below,
+                 double* out, long ldo, int prec = PREC_FP64) {
+  const int D = c->D, d = c->d;
+  double* Y = ws(c, "pr_Y", (size_t)D * d * D);
+  gemm2(c, 'N', 'N', d * D, D, D, 1.0, above, (long)d * D, X, ldx, 0.0, Y, (long)d * D, prec);
+  gemm2(c, 'N', 'T', D, D, d * D, 1.0, Y, D, below, D, 0.0, out, ldo, prec);
+}
+
+// MPO block W[l,:,i,:] of site as a (d x d) column-major matrix M[u + d*s]
+std::vector<double> wblock(imps_ctx* c, int site, int l, int i) {
+  const int w = c->w, d = c->d;
+  const std::vector<double>& W = c->Wh[site];
+  std::vector<double> M((size_t)d * d);
+  for (int s = 0; s < d; ++s)
+    for (int u = 0; u < d; ++u) M[u + (size_t)d * s] = W[l + (size_t)w * (u + (size_t)d * (i + (size_t)w * s))];
+  return M;
+}
+
+bool is_zero(const std::vector<double>& M) {
+  for (double v : M)
+    if (v != 0.0) return false;
+  return true;
+}
+
+bool is_eye(const std::vector<double>& M, int d, double atol = 1e-14) {
+  for (int s = 0; s < d; ++s)
+    for (int u = 0; u < d; ++u)
+      if (std::fabs(M[u + (size_t)d * s] - (u == s ? 1.0 : 0.0)) > atol) return false;
+  return true;
+}
+
+// Left slice sum (envs.jl:104-107, 121-136): out = sum_terms push_slice_left(L_t, W_t, A)
+struct SliceTerm {
+  const double* L;  // nullptr: identity
+  long ld;
+  std::vector<double> Wb;
+};
+
+void slice_sum_left(imps_ctx* c, const std::vector<SliceTerm>& terms, const double* A, double* out,
+                    long ldo) {
+  const int D = c->D, d = c->d;
+  if (terms.empty()) {
+    HIPCHK(launch_fill2d(D, D, 0.0, out, ldo, c->stream));
+    return;
+  }
+  double* Z = ws(c, "sl_Z", (size_t)D * d * D);
+  double* Y = ws(c, "sl_Y", (size_t)D * d * D);
+  bool first = true;
+  for (const SliceTerm& t : terms) {
+    const double* src = A;
+    if (t.L) {
+      gemm2(c, 'N', 'N', D, d * D, D, 1.0, t.L, t.ld, A, D, 0.0, Y, D);
+      src = Y;
+    }
+    HIPCHK(launch_mix_mid(D, d, D, t.Wb.data(), src, Z, first ? 0.0 : 1.0, c->stream));
+    first = false;
+  }
+  gemm2(c, 'T', 'N', D, D, d * D, 1.0, A, (long)d * D, Z, (long)d * D, 0.0, out, ldo);
+}
+
+// Right slice sum (envs.jl:110-114, 162-177)
+void slice_sum_right(imps_ctx* c, const std::vector<SliceTerm>& terms, const double* A, double* out,
+                     long ldo) {
+  const int D = c->D, d = c->d;
+  if (terms.empty()) {
+    HIPCHK(launch_fill2d(D, D, 0.0, out, ldo, c->stream));
+    return;
+  }
+  double* Z = ws(c, "sr_Z", (size_t)D * d * D);
+  double* Y = ws(c, "sr_Y", (size_t)D * d * D);
+  bool first = true;
+  for (const SliceTerm& t : terms) {
+    const double* src = A;
+    if (t.L) {
+      gemm2(c, 'N', 'N', d * D, D, D, 1.0, A, (long)d * D, t.L, t.ld, 0.0, Y, (long)d * D);
+      src = Y;
+    }
+    HIPCHK(launch_mix_mid(D, d, D, t.Wb.data(), src, Z, first ? 0.0 : 1.0, c->stream));
+    first = false;
+  }
+  gemm2(c, 'N', 'T', D, D, d * D, 1.0, Z, D, A, D, 0.0, out, ldo);
+}
+
+// ------------------------------------------------------------------ Krylov solvers
+
+typedef std::function<void(const double*, double*)> Op;
+
+struct KInfo {
+  int converged = 0, numiter = 0, numops = 0;
+  double normres = 0.0;
+};
+
+// Lanczos eigsolve, lowest eigenpair (KrylovKit.eigsolve(..., :SR, Lanczos) via
+// fixedpoint, utility.jl:180-186): thick restart keep = div(3 kdim + 2 conv, 5),
+// eager convergence test |beta u_K| <= tol, CGS2 full reorthogonalisation.
+double lanczos(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xout, double tol,
+               int kdim, int maxiter, bool eager, KInfo* info) {
+  const int kmax = (int)std::min<size_t>(kdim, n);
+  require(kmax >= 1 && kmax < kMaxVec, IMPS_E_ARG, "lanczos: krylovdim out of range");
+  double* V = ws(c, "lz_V", (size_t)(kmax + 1) * n);
+  const int ldt = kmax + 1;
+  std::vector<double> T((size_t)ldt * ldt, 0.0);
+  auto Vc = [&](int k) { return V + (size_t)k * n; };
+  const double nx = nrm2(c, n, x0);
+  require(nx > 0 && std::isfinite(nx), IMPS_E_ARG, "lanczos: zero or non-finite start vector");
+  axpby(c, n, 1.0 / nx, x0, 0.0, Vc(0));
+  int numops = 0, numiter = 1, K = 0;
+  double beta = 0.0;
+  std::vector<double> h(kmax + 1);
+  auto expand = [&](int k) {
+    op(Vc(k), Vc(k + 1));
+    numops++;
+    cgs2(c, n, k + 1, V, Vc(k + 1), h.data());
+    beta = nrm2(c, n, Vc(k + 1));
+    for (int i = 0; i <= k; ++i) {
+      T[i + (size_t)ldt * k] = h[i];
+      T[k + (size_t)ldt * i] = h[i];
+    }
+    if (beta > 0) scal(c, n, 1.0 / beta, Vc(k + 1));
+  };
+  expand(0);
+  K = 1;
+  std::vector<double> theta, U;
+  int converged = 0;
+  while (true) {
+    converged = 0;
+    if (K == kmax || beta <= tol || eager) {
+      std::vector<double> A((size_t)K * K);
+      for (int j = 0; j < K; ++j)
+        for (int i = 0; i < K; ++i)
+          A[i + (size_t)K * j] = 0.5 * (T[i + (size_t)ldt * j] + T[j + (size_t)ldt * i]);
+      theta.assign(K, 0.0);
+      U.assign((size_t)K * K, 0.0);
+      sym_eig(K, A.data(), theta.data(), U.data());
+      if (std::fabs(beta * U[(K - 1) + (size_t)K * 0]) <= tol) {
+        converged = 1;
+        break;
+      }
+    }
+    if (K < kmax) {
+      expand(K);
+      K += 1;
+    } else {
+      if (numiter == maxiter) break;
+      const int keep = (3 * kmax + 2 * converged) / 5;
+      double* tmp = ws(c, "lz_tmp", (size_t)keep * n);
+      double* dU = ws(c, "lz_U", (size_t)K * keep);
+      h2d(c, dU, U.data(), (size_t)K * keep);
+      gemm2(c, 'N', 'N', (int)n, keep, K, 1.0, V, (long)n, dU, K, 0.0, tmp, (long)n);
+      d2d(c, V, tmp, (size_t)keep * n);
+      d2d(c, Vc(keep), Vc(K), n);
+      std::vector<double> f(keep);
+      for (int j = 0; j < keep; ++j) f[j] = beta * U[(K - 1) + (size_t)K * j];
+      std::fill(T.begin(), T.end(), 0.0);
+      for (int j = 0; j < keep; ++j) {
+        T[j + (size_t)ldt * j] = theta[j];
+        T[keep + (size_t)ldt * j] = f[j];
+        T[j + (size_t)ldt * keep] = f[j];
+      }
+      K = keep;
+      expand(K);
+      K += 1;
+      numiter++;
+    }
+  }
+  std::vector<double> u0(K);
+  for (int i = 0; i < K; ++i) u0[i] = U[i];
+  multiaxpy(c, n, K, V, n, u0.data(), 0.0, xout);
+  if (dot(c, n, xout, x0) < 0) scal(c, n, -1.0, xout);
+  if (info) {
+    info->converged = converged;
+    info->numiter = numiter;
+    info->numops = numops;
+    info->normres = std::fabs(beta * U[K - 1]);
+  }
+  return theta[0];
+}
+
+// Dominant eigenpair (which = :LM) of a real operator by Arnoldi (KrylovKit.schursolve
+// via gauge_fixedpoint, utility.jl:204-211; eigsolve(..., :LM) via dot, canonicalmps.jl:
+// 262-269); restarts from the dominant Ritz vector.
+double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xout,
+                        double tol, int kdim, int maxiter, bool eager, KInfo* info) {
+  const int kmax = (int)std::min<size_t>(kdim, n);
+  require(kmax >= 1 && kmax < kMaxVec, IMPS_E_ARG, "arnoldi: krylovdim out of range");
+  double* V = ws(c, "ar_V", (size_t)(kmax + 1) * n);
+  auto Vc = [&](int k) { return V + (size_t)k * n; };
+  const int ldh = kmax + 1;
+  std::vector<double> H((size_t)ldh * kmax), h(kmax + 1), u(kmax);
+  int numiter = 1, numops = 0, K = 0;
+  bool conv = false;
+  double lam = 0.0, res = 0.0;
+  const double nx = nrm2(c, n, x0);
+  require(nx > 0 && std::isfinite(nx), IMPS_E_ARG, "arnoldi: zero or non-finite start vector");
+  axpby(c, n, 1.0 / nx, x0, 0.0, Vc(0));
+  while (true) {
+    std::fill(H.begin(), H.end(), 0.0);
+    K = 0;
+    while (true) {
+      op(Vc(K), Vc(K + 1));
+      numops++;
+      cgs2(c, n, K + 1, V, Vc(K + 1), h.data());
+      const double beta = nrm2(c, n, Vc(K + 1));
+      for (int i = 0; i <= K; ++i) H[i + (size_t)ldh * K] = h[i];
+      H[(K + 1) + (size_t)ldh * K] = beta;
+      K++;
+      if (beta > 0) scal(c, n, 1.0 / beta, Vc(K));
+      if (K == kmax || beta <= tol || eager) {
+        hess_dominant(K, H.data(), ldh, &lam, u.data());
+        res = std::fabs(beta * u[K - 1]);
+        if (res <= tol) {
+          conv = true;
+          break;
+        }
+      }
+      if (K == kmax) break;
+    }
+    multiaxpy(c, n, K, V, n, u.data(), 0.0, xout);
+    if (conv || numiter == maxiter) break;
+    const double nr = nrm2(c, n, xout);
+    axpby(c, n, 1.0 / nr, xout, 0.0, Vc(0));
+    numiter++;
+  }
+  if (info) {
+    info->converged = conv ? 1 : 0;
+    info->numiter = numiter;
+    info->numops = numops;
+    info->normres = res;
+  }
+  return lam;
+}
+
+// Restarted GMRES for (a0 + a1 A) x = b (KrylovKit.linsolve via envs.jl:14-19); x holds
+// the starting vector on entry and the solution on exit.  Absolute tolerance.
+void gmres(imps_ctx* c, const Op& op, size_t n, const double* b, double* x, double tol, int kdim,
+           int maxiter, double a0, double a1, KInfo* info) {
+  const int kmax = (int)std::min<size_t>(kdim, n);
+  require(kmax >= 1 && kmax < kMaxVec, IMPS_E_ARG, "gmres: krylovdim out of range");
+  double* V = ws(c, "gm_V", (size_t)(kmax + 1) * n);
+  double* r = ws(c, "gm_r", n);
+  double* t = ws(c, "gm_t", n);
+  auto Vc = [&](int k) { return V + (size_t)k * n; };
+  auto applyA = [&](const double* v, double* out) {
+    op(v, out);
+    axpby(c, n, a0, v, a1, out);
+  };
+  int numops = 1, numiter = 0;
+  applyA(x, t);
+  d2d(c, r, b, n);
+  axpby(c, n, -1.0, t, 1.0, r);
+  double beta = nrm2(c, n, r);
+  if (beta < tol) {
+    if (info) *info = KInfo{1, 0, numops, beta};
+    return;
+  }
+  const int ldh = kmax + 1;
+  std::vector<double> H((size_t)ldh * kmax), cs(kmax), sn(kmax), g(kmax + 1), h(kmax + 1), y(kmax);
+  while (numiter < maxiter) {
+    numiter++;
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    axpby(c, n, 1.0 / beta, r, 0.0, Vc(0));
+    int k = 0;
+    while (true) {
+      applyA(Vc(k), Vc(k + 1));
+      numops++;
+      cgs2(c, n, k + 1, V, Vc(k + 1), h.data());
+      const double hn = nrm2(c, n, Vc(k + 1));
+      for (int i = 0; i <= k; ++i) H[i + (size_t)ldh * k] = h[i];
+      H[(k + 1) + (size_t)ldh * k] = hn;
+      if (hn > 0) scal(c, n, 1.0 / hn, Vc(k + 1));
+      for (int i = 0; i < k; ++i) {
+        const double a = H[i + (size_t)ldh * k], bb = H[(i + 1) + (size_t)ldh * k];
+        H[i + (size_t)ldh * k] = cs[i] * a + sn[i] * bb;
+        H[(i + 1) + (size_t)ldh * k] = -sn[i] * a + cs[i] * bb;
+      }
+      const double a = H[k + (size_t)ldh * k], bb = H[(k + 1) + (size_t)ldh * k];
+      const double rho = std::hypot(a, bb);
+      cs[k] = a / rho;
+      sn[k] = bb / rho;
+      H[k + (size_t)ldh * k] = rho;
+      H[(k + 1) + (size_t)ldh * k] = 0.0;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = cs[k] * g[k];
+      k++;
+      if (std::fabs(g[k]) <= tol || k == kmax) break;
+    }
+    for (int i = k - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int j = i + 1; j < k; ++j) s -= H[i + (size_t)ldh * j] * y[j];
+      y[i] = s / H[i + (size_t)ldh * i];
+    }
+    multiaxpy(c, n, k, V, n, y.data(), 1.0, x);
+    applyA(x, t);
+    numops++;
+    d2d(c, r, b, n);
+    axpby(c, n, -1.0, t, 1.0, r);
+    beta = nrm2(c, n, r);
+    if (beta < tol) {
+      if (info) *info = KInfo{1, numiter, numops, beta};
+      return;
+    }
+  }
+  if (info) *info = KInfo{0, numiter, numops, beta};
+}
+
+// ------------------------------------------------------------------ QRpos / LQpos
+
+// A (m x n, lda = m, m >= n) is overwritten by Q; R (n x n) receives the factor with a
+// non-negative diagonal (FiniteMPSAlgorithms' QRpos).
+void qrpos(imps_ctx* c, int m, int n, double* A, double* R) {
+  double* tau = ws(c, "qr_tau", n);
+  RBCHK(rocsolver_dgeqrf(c->rb, m, n, A, m, tau));
+  HIPCHK(launch_copy_upper(n, A, m, R, c->stream));
+  RBCHK(rocsolver_dorgqr(c->rb, m, n, n, A, m, tau));
+  HIPCHK(launch_qr_signfix(m, n, A, m, R, n, c->stream));
+}
+
+// M (m x n, ld m, m <= n) = L Q with L (m x m) lower, non-negative diagonal and Q (m x n)
+// row-orthonormal (LQpos).  M may alias Q.
+void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q) {
+  double* Mt = ws(c, "lq_Mt", (size_t)m * n);
+  double* R = ws(c, "lq_R", (size_t)m * m);
+  HIPCHK(launch_transpose(m, n, M, m, Mt, n, c->stream));
+  qrpos(c, n, m, Mt, R);
+  HIPCHK(launch_transpose(m, m, R, m, L, m, c->stream));
+  HIPCHK(launch_transpose(n, m, Mt, n, Q, m, c->stream));
+}
+
+// ------------------------------------------------------------------ gauge fixing
+
+constexpr double kTolMin = 1.0e-14, kTolMax = 1.0e-4;
+
+double updatetol(double factor, int it, double eps, double tmin = kTolMin, double tmax = kTolMax) {
+  it = std::max(it, 1);
+  return std::min(std::max(eps * factor / std::sqrt((double)it), tmin), tmax);
+}
+
+// uniform_rightorth! (ortho.jl:166-202): AR, C from A and C0 (C0 is read before C is written)
+int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* C0, double tol,
+                      int maxiter, int eig_miniter = 10) {
+  const int N = (int)A.size(), D = c->D, d = c->d;
+  const size_t DD = (size_t)D * D;
+  double* Cn = c->C[N - 1];
+  double* Cpre = ws(c, "ro_Cpre", DD);
+  double* M = ws(c, "ro_M", (size_t)D * d * D);
+  double* ev = ws(c, "ro_ev", DD);
+  d2d(c, Cpre, C0, DD);
+  d2d(c, Cn, Cpre, DD);
+  scal(c, DD, 1.0 / nrm2(c, DD, Cn), Cn);
+  int it = 0;
+  double eps = std::numeric_limits<double>::infinity();
+  while (true) {
+    if (it >= eig_miniter) {
+      const double tol_e = updatetol(1.0, 1, eps * eps);
+      Op tm = [&](const double* v, double* out) {
+        double* t0 = ws(c, "ro_t0", DD);
+        double* t1 = ws(c, "ro_t1", DD);
+        const double* cur = v;
+        for (int l = N - 1; l >= 0; --l) {
+          double* dst = (l == 0) ? out : ((N - 1 - l) % 2 == 0 ? t0 : t1);
+          push_right2(c, cur, D, A[l], c->AR[l], dst, D);
+          cur = dst;
+        }
+      };
+      arnoldi_dominant(c, tm, DD, Cn, ev, tol_e, 30, 300, true, nullptr);
+      double* Ldum = ws(c, "ro_Ldum", DD);
+      lqpos(c, D, D, ev, Cn, Ldum);
+    }
+    d2d(c, Cpre, Cn, DD);
+    for (int i = N - 1; i >= 0; --i) {
+      gemm2(c, 'N', 'N', D * d, D, D, 1.0, A[i], (long)D * d, c->C[i], D, 0.0, M, (long)D * d);
+      lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i]);
+    }
+    scal(c, DD, 1.0 / nrm2(c, DD, Cn), Cn);
+    axpby(c, DD, 1.0, Cn, -1.0, Cpre);
+    eps = nrm2(c, DD, Cpre);
+    it++;
+    if (eps < tol) return it;
+    if (it > maxiter) {
+      fprintf(stderr, "imps: uniform_rightorth not converged (eps = %.3e)\n", eps);
+      return it;
+    }
+  }
+}
+
+// uniform_leftorth! (ortho.jl:125-164)
+int uniform_leftorth(imps_ctx* c, const std::vector<double*>& A, const double* C0, double tol,
+                     int maxiter, int eig_miniter = 10) {
+  const int N = (int)A.size(), D = c->D, d = c->d;
+  const size_t DD = (size_t)D * D;
+  double* Cn = c->C[N - 1];
+  double* Cpre = ws(c, "lo_Cpre", DD);
+  double* ev = ws(c, "lo_ev", DD);
+  d2d(c, Cpre, C0, DD);
+  d2d(c, Cn, Cpre, DD);
+  scal(c, DD, 1.0 / nrm2(c, DD, Cn), Cn);
+  int it = 0;
+  double eps = std::numeric_limits<double>::infinity();
+  while (true) {
+    if (it >= eig_miniter) {
+      const double tol_e = updatetol(1.0, 1, eps * eps);
+      Op tm = [&](const double* v, double* out) {
+        double* t0 = ws(c, "lo_t0", DD);
+        double* t1 = ws(c, "lo_t1", DD);
+        const double* cur = v;
+        for (int l = 0; l < N; ++l) {
+          double* dst = (l == N - 1) ? out : (l % 2 == 0 ? t0 : t1);
+          push_left2(c, cur, D, A[l], c->AL[l], dst, D);
+          cur = dst;
+        }
+      };
+      arnoldi_dominant(c, tm, DD, Cn, ev, tol_e, 30, 300, true, nullptr);
+      double* Rf = ws(c, "lo_R", DD);
+      qrpos(c, D, D, ev, Rf);
+      d2d(c, Cn, Rf, DD);
+    }
+    d2d(c, Cpre, Cn, DD);
+    for (int i = 0; i < N; ++i) {
+      // M = C[i-1] A[i] written straight into AL[i], then QR in place
+      gemm2(c, 'N', 'N', D, d * D, D, 1.0, c->C[(i - 1 + N) % N], D, A[i], D, 0.0, c->AL[i], D);
+      qrpos(c, D * d, D, c->AL[i], c->C[i]);
+    }
+    scal(c, DD, 1.0 / nrm2(c, DD, Cn), Cn);
+    axpby(c, DD, 1.0, Cn, -1.0, Cpre);
+    eps = nrm2(c, DD, Cpre);
+    it++;
+    if (eps < tol) return it;
+    if (it > maxiter) {
+      fprintf(stderr, "imps: uniform_leftorth not converged (eps = %.3e)\n", eps);
+      return it;
+    }
+  }
+}
+
+void mul_AL_C(imps_ctx* c, const double* AL, const double* Cm, double* AC) {
+  const int D = c->D, d = c->d;
+  gemm2(c, 'N', 'N', D * d, D, D, 1.0, AL, (long)D * d, Cm, D, 0.0, AC, (long)D * d);
+}
+
+// ------------------------------------------------------------------ environments
+
+bool level_identity(imps_ctx* c, int i) {
+  if (i == 0) return true;
+  for (int s = 0; s < c->N; ++s) {
+    if (i == c->w - 1) continue;
+    if (!is_eye(wblock(c, s, i, i), c->d)) return false;
+  }
+  return true;
+}
+
+bool level_empty(imps_ctx* c, int i) {
+  if (i == 0 || i == c->w - 1) return false;
+  for (int s = 0; s < c->N; ++s) {
+    if (!is_zero(wblock(c, s, i, i)) || !is_zero(wblock(c, s, 0, i)) || !is_zero(wblock(c, s, i, c->w - 1)))
+      return false;
+  }
+  return true;
+}
+
+struct EnvStats {
+  int solves = 0, ops = 0;
+};
+
+// DMRGCache(psi, H::SparseIMPO; tol, maxiter, krylovdim, init_lefts, init_rights)
+// (envs.jl:321-518) for square Schur tensors; warm start from the current levels.
+void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* st) {
+  need_match(c);
+  const int N = c->N, D = c->D, w = c->w;
+  const size_t DD = (size_t)D * D;
+  const long ldg = (long)D * w;
+  const bool warm = c->envs_valid;
+  // old levels (GMRES starting vectors) are kept in separate buffers
+  std::vector<double*> oldL(N), oldR(N);
+  for (int l = 0; l < N; ++l) {
+    oldL[l] = ws(c, "env_oldL" + std::to_string(l), (size_t)D * w * D);
+    oldR[l] = ws(c, "env_oldR" + std::to_string(l), (size_t)D * w * D);
+    if (warm) {
+      d2d(c, oldL[l], c->GL[l], (size_t)D * w * D);
+      d2d(c, oldR[l], c->GR[l], (size_t)D * w * D);
+    }
+  }
+  for (int l = 0; l < N; ++l) {
+    fill(c, (size_t)D * w * D, 0.0, c->GL[l]);
+    fill(c, (size_t)D * w * D, 0.0, c->GR[l]);
+    HIPCHK(launch_set_identity(D, c->GL[l], ldg, c->stream));
+    HIPCHK(launch_set_identity(D, c->GR[l] + (size_t)D * (w - 1), ldg, c->stream));
+  }
+  const int kdim = (int)std::min<long>((long)D * D, krylovdim);
+  double* rhs = ws(c, "env_rhs", DD);
+  double* x = ws(c, "env_x", DD);
+  double* rho = ws(c, "env_rho", DD);
+  auto slice = [&](double* G, int k) { return G + (size_t)D * k; };
+
+  auto left_terms = [&](int site, int i, int lmax) {
+    std::vector<SliceTerm> terms;
+    for (int l = 0; l <= lmax; ++l) {
+      std::vector<double> Wb = wblock(c, site, l, i);
+      if (is_zero(Wb)) continue;
+      terms.push_back(SliceTerm{l == 0 ? nullptr : slice(c->GL[site], l), ldg, Wb});
+    }
+    return terms;
+  };
+  auto right_terms = [&](int site, int i) {
+    std::vector<SliceTerm> terms;
+    for (int l = i; l < w; ++l) {
+      std::vector<double> Wb = wblock(c, site, i, l);
+      if (is_zero(Wb)) continue;
+      terms.push_back(SliceTerm{l == w - 1 ? nullptr : slice(c->GR[site], l), ldg, Wb});
+    }
+    return terms;
+  };
+  auto left_cycle = [&](int i, bool closing) {
+    for (int site = 0; site < N; ++site) {
+      const int sn = (site + 1) % N;
+      slice_sum_left(c, left_terms(site, i, closing ? w - 1 : i), c->AL[site], slice(c->GL[sn], i), ldg);
+    }
+  };
+  auto right_cycle = [&](int i) {
+    for (int site = N - 1; site >= 0; --site) {
+      const int sp = (site - 1 + N) % N;
+      slice_sum_right(c, right_terms(site, i), c->AR[site], slice(c->GR[sp], i), ldg);
+    }
+  };
+  auto solve = [&](const Op& op, int which_side, int i) {
+    // x0: previous level (warm start) or the zero level
+    double* G = which_side == 0 ? c->GL[0] : c->GR[N - 1];
+    double* old = which_side == 0 ? oldL[0] : oldR[N - 1];
+    HIPCHK(launch_copy2d(D, D, slice(G, i), ldg, rhs, D, c->stream));
+    if (warm)
+      HIPCHK(launch_copy2d(D, D, slice(old, i), ldg, x, D, c->stream));
+    else
+      fill(c, DD, 0.0, x);
+    KInfo info;
+    gmres(c, op, DD, rhs, x, tol, kdim, maxiter, 1.0, -1.0, &info);
+    if (st) {
+      st->solves++;
+      st->ops += info.numops;
+    }
+    HIPCHK(launch_copy2d(D, D, x, D, slice(G, i), ldg, c->stream));
+  };
+  auto regularize_slice = [&](double* v, long ld, const double* l) {
+    double* tmp = ws(c, "env_reg", DD);
+    HIPCHK(launch_copy2d(D, D, v, ld, tmp, D, c->stream));
+    const double cc = trace_prod(c, D, l, tmp);
+    HIPCHK(launch_add_diag(D, -cc, v, ld, c->stream));
+  };
+  auto ring_left = [&](const double* v, double* out, double* rh) {
+    double* t0 = ws(c, "env_t0", DD);
+    double* t1 = ws(c, "env_t1", DD);
+    const double* cur = v;
+    for (int l = 0; l < N; ++l) {
+      double* dst = (l == N - 1) ? out : (l % 2 == 0 ? t0 : t1);
+      push_left2(c, cur, D, c->AL[l], c->AL[l], dst, D);
+      cur = dst;
+    }
+    if (rh) regularize_slice(out, D, rh);
+  };
+  auto ring_right = [&](const double* v, double* out, double* rh) {
+    double* t0 = ws(c, "env_t0", DD);
+    double* t1 = ws(c, "env_t1", DD);
+    const double* cur = v;
+    for (int l = N - 1; l >= 0; --l) {
+      double* dst = (l == 0) ? out : ((N - 1 - l) % 2 == 0 ? t0 : t1);
+      push_right2(c, cur, D, c->AR[l], c->AR[l], dst, D);
+      cur = dst;
+    }
+    if (rh) regularize_slice(out, D, rh);
+  };
+
+  // ---- left interior levels (envs.jl:364-421)
+  for (int i = 1; i <= w - 2; ++i) {
+    left_cycle(i, false);
+    if (level_identity(c, i)) {
+      gemm2(c, 'N', 'T', D, D, D, 1.0, c->C[N - 1], D, c->C[N - 1], D, 0.0, rho, D);
+      solve([&](const double* v, double* o) { ring_left(v, o, rho); }, 0, i);
+      if (N > 1) left_cycle(i, false);
+      for (int l = 0; l < N; ++l) {
+        double* rl = ws(c, "env_rhol", DD);
+        gemm2(c, 'N', 'T', D, D, D, 1.0, c->C[(l - 1 + N) % N], D, c->C[(l - 1 + N) % N], D, 0.0, rl, D);
+        regularize_slice(slice(c->GL[l], i), ldg, rl);
+      }
+    } else {
+      if (!level_empty(c, i)) {
+        bool zero_diag = true;
+        for (int s = 0; s < N; ++s) zero_diag = zero_diag && is_zero(wblock(c, s, i, i));
+        solve([&](const double* v, double* o) {
+          if (zero_diag) {
+            fill(c, DD, 0.0, o);
+            return;
+          }
+          double* t0 = ws(c, "env_t0", DD);
+          double* t1 = ws(c, "env_t1", DD);
+          const double* cur = v;
+          for (int l = 0; l < N; ++l) {
+            double* dst = (l == N - 1) ? o : (l % 2 == 0 ? t0 : t1);
+            std::vector<SliceTerm> tt{SliceTerm{cur, D, wblock(c, l, i, i)}};
+            slice_sum_left(c, tt, c->AL[l], dst, D);
+            cur = dst;
+          }
+        }, 0, i);
+      }
+      if (N > 1) left_cycle(i, false);
+    }
+  }
+  // ---- left closing level (envs.jl:424-455)
+  {
+    const int i = w - 1;
+    left_cycle(i, true);
+    gemm2(c, 'N', 'T', D, D, D, 1.0, c->C[N - 1], D, c->C[N - 1], D, 0.0, rho, D);
+    solve([&](const double* v, double* o) { ring_left(v, o, rho); }, 0, i);
+    if (N > 1) left_cycle(i, true);
+    for (int l = 0; l < N; ++l) {
+      double* rl = ws(c, "env_rhol", DD);
+      gemm2(c, 'N', 'T', D, D, D, 1.0, c->C[(l - 1 + N) % N], D, c->C[(l - 1 + N) % N], D, 0.0, rl, D);
+      regularize_slice(slice(c->GL[l], i), ldg, rl);
+    }
+  }
+  // ---- right levels (envs.jl:461-515)
+  for (int i = w - 2; i >= 0; --i) {
+    right_cycle(i);
+    if (level_identity(c, i)) {
+      gemm2(c, 'T', 'N', D, D, D, 1.0, c->C[N - 1], D, c->C[N - 1], D, 0.0, rho, D);
+      solve([&](const double* v, double* o) { ring_right(v, o, rho); }, 1, i);
+      if (N > 1) right_cycle(i);
+      for (int l = 0; l < N; ++l) {
+        double* rl = ws(c, "env_rhol", DD);
+        gemm2(c, 'T', 'N', D, D, D, 1.0, c->C[l], D, c->C[l], D, 0.0, rl, D);
+        regularize_slice(slice(c->GR[l], i), ldg, rl);
+      }
+    } else {
+      if (!level_empty(c, i)) {
+        bool zero_diag = true;
+        for (int s = 0; s < N; ++s) zero_diag = zero_diag && is_zero(wblock(c, s, i, i));
+        solve([&](const double* v, double* o) {
+          if (zero_diag) {
+            fill(c, DD, 0.0, o);
+            return;
+          }
+          double* t0 = ws(c, "env_t0", DD);
+          double* t1 = ws(c, "env_t1", DD);
+          const double* cur = v;
+          for (int l = N - 1; l >= 0; --l) {
+            double* dst = (l == 0) ? o : ((N - 1 - l) % 2 == 0 ? t0 : t1);
+            std::vector<SliceTerm> tt{SliceTerm{cur, D, wblock(c, l, i, i)}};
+            slice_sum_right(c, tt, c->AR[l], dst, D);
+            cur = dst;
+          }
+        }, 1, i);
+      }
+      if (N > 1) right_cycle(i);
+    }
+  }
+  c->envs_valid = true;
+}
+
+// ------------------------------------------------------------------ observables
+
+double dot_psi(imps_ctx* c) {
+  const int N = c->N, D = c->D;
+  const size_t DD = (size_t)D * D;
+  double* v0 = ws(c, "dp_v0", DD);
+  double* ev = ws(c, "dp_ev", DD);
+  HIPCHK(launch_set_identity(D, v0, D, c->stream));
+  Op tm = [&](const double* v, double* out) {
+    double* t0 = ws(c, "dp_t0", DD);
+    double* t1 = ws(c, "dp_t1", DD);
+    const double* cur = v;
+    for (int l = 0; l < N; ++l) {
+      double* dst = (l == N - 1) ? out : (l % 2 == 0 ? t0 : t1);
+      push_left2(c, cur, D, c->AL[l], c->AL[l], dst, D);
+      cur = dst;
+    }
+  };
+  return arnoldi_dominant(c, tm, DD, v0, ev, 1.0e-10, 30, 300, true, nullptr);
+}
+
+double energy(imps_ctx* c) {
+  need_envs(c);
+  const int N = c->N, D = c->D, w = c->w, d = c->d;
+  double E = 0.0;
+  for (int site = 0; site < N; ++site) {
+    double* T1 = ws(c, "hac_T1", (size_t)D * w * d * D);
+    double* T2 = ws(c, "hac_T2", (size_t)D * w * d * D);
+    gemm2(c, 'N', 'N', D * w, d * D, D, 1.0, c->GL[site], (long)D * w, c->AC[site], D, 0.0, T1, (long)D * w);
+    HIPCHK(launch_wmix(D, w, d, c->Wm[site], T1, T2, c->stream));
+    E += dot(c, (size_t)D * d * D, c->AC[site], T2 + (size_t)D * d * D * (w - 1));
+  }
+  return E / dot_psi(c);
+}
+
+double galerkin(imps_ctx* c, int prec) {
+  need_envs(c);
+  const int N = c->N, D = c->D, d = c->d;
+  const size_t n = (size_t)D * d * D;
+  double eps = 0.0;
+  for (int site = 0; site < N; ++site) {
+    double* x = ws(c, "gk_x", n);
+    double* q = ws(c, "gk_q", (size_t)D * D);
+    hac_apply(c, site, c->AC[site], x, prec);
+    scal(c, n, 1.0 / nrm2(c, n, x), x);
+    gemm2(c, 'T', 'N', D, D, d * D, 1.0, c->AL[site], (long)d * D, x, (long)d * D, 0.0, q, D);
+    gemm2(c, 'N', 'N', d * D, D, D, -1.0, c->AL[site], (long)d * D, q, D, 1.0, x, (long)d * D);
+    eps = std::max(eps, nrm2(c, n, x));
+  }
+  return eps;
+}
+
+// ------------------------------------------------------------------ state management
+
+void free_state(imps_ctx* c) {
+  for (auto* v : {&c->AL, &c->AR, &c->AC, &c->C, &c->pAC, &c->pC, &c->pAL, &c->GL, &c->GR})
+    for (double*& p : *v) dfree(p);
+  for (auto* v : {&c->AL, &c->AR, &c->AC, &c->C, &c->pAC, &c->pC, &c->pAL, &c->GL, &c->GR}) v->clear();
+  c->envs_valid = false;
+  c->D = 0;
+}
+
+void alloc_state(imps_ctx* c, int N, int D, int d) {
+  free_state(c);
+  c->D = D;
+  const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
+  for (int l = 0; l < N; ++l) {
+    c->AL.push_back(dalloc(nA));
+    c->AR.push_back(dalloc(nA));
+    c->AC.push_back(dalloc(nA));
+    c->C.push_back(dalloc(nC));
+    c->pAC.push_back(dalloc(nA));
+    c->pC.push_back(dalloc(nC));
+    c->pAL.push_back(dalloc(nA));
+  }
+}
+
+void alloc_envs(imps_ctx* c) {
+  if (!c->GL.empty()) return;
+  const size_t n = (size_t)c->D * c->w * c->D;
+  for (int l = 0; l < c->N; ++l) {
+    c->GL.push_back(dalloc(n));
+    c->GR.push_back(dalloc(n));
+  }
+}
+
+void envs_recalculate(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* st) {
+  need_match(c);
+  alloc_envs(c);
+  envs_build(c, tol, maxiter, krylovdim, st);
+}
+
+void gauge_step(imps_ctx* c, double tol, int maxiter, int* sweeps) {
+  const int N = c->N, D = c->D, d = c->d;
+  double* C0 = ws(c, "gs_C0", (size_t)D * D);
+  d2d(c, C0, c->C[N - 1], (size_t)D * D);
+  const int sw = uniform_rightorth(c, c->pAL, C0, tol, maxiter);
+  for (int l = 0; l < N; ++l) {
+    d2d(c, c->AL[l], c->pAL[l], (size_t)D * d * D);
+    mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+  }
+  if (sweeps) *sweeps = sw;
+}
+
+void regauge_site(imps_ctx* c, int site) {
+  const int D = c->D, d = c->d;
+  double* Qac = ws(c, "rg_Qac", (size_t)D * d * D);
+  double* Qc = ws(c, "rg_Qc", (size_t)D * D);
+  double* R = ws(c, "rg_R", (size_t)D * D);
+  d2d(c, Qac, c->pAC[site], (size_t)D * d * D);
+  d2d(c, Qc, c->pC[site], (size_t)D * D);
+  qrpos(c, D * d, D, Qac, R);
+  qrpos(c, D, D, Qc, R);
+  gemm2(c, 'N', 'T', D * d, D, D, 1.0, Qac, (long)D * d, Qc, D, 0.0, c->pAL[site], (long)D * d);
+}
+
+struct EigOut {
+  double lambda;
+  KInfo info;
+};
+
+EigOut eigsolve_ac(imps_ctx* c, int site, double tol, int kdim, int maxiter, bool eager, int prec) {
+  const size_t n = (size_t)c->D * c->d * c->D;
+  EigOut o;
+  Op op = [&](const double* x, double* y) { hac_apply(c, site, x, y, prec); };
+  o.lambda = lanczos(c, op, n, c->AC[site], c->pAC[site], tol, kdim, maxiter, eager, &o.info);
+  return o;
+}
+
+EigOut eigsolve_c(imps_ctx* c, int site, double tol, int kdim, int maxiter, bool eager, int prec) {
+  const size_t n = (size_t)c->D * c->D;
+  EigOut o;
+  Op op = [&](const double* x, double* y) { hc_apply(c, site, x, y, prec); };
+  o.lambda = lanczos(c, op, n, c->C[site], c->pC[site], tol, kdim, maxiter, eager, &o.info);
+  return o;
+}
+
+int run_guarded(imps_ctx* c, const std::function<void()>& f) {
+  try {
+    if (c) HIPCHK(hipSetDevice(c->device));
+    f();
+    return IMPS_OK;
+  } catch (const ImpsError& e) {
+    if (c) c->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    if (c) c->err = e.what();
+    return IMPS_E_ARG;
+  }
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int imps_create(int device, imps_ctx** out) {
+  if (!out) return IMPS_E_ARG;
+  *out = nullptr;
+  imps_ctx* c = new imps_ctx();
+  c->device = device;
+  try {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    RBCHK(rocblas_create_handle(&c->rb));
+    RBCHK(rocblas_set_stream(c->rb, c->stream));
+    HIPCHK(hipHostMalloc((void**)&c->hpart, sizeof(double) * kMaxDotBlocks * kMaxVec, hipHostMallocDefault));
+    c->dpart = dalloc((size_t)kMaxDotBlocks * kMaxVec);
+  } catch (const ImpsError& e) {
+    g_create_error = e.what();
+    if (c->rb) rocblas_destroy_handle(c->rb);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return e.code;
+  }
+  *out = c;
+  return IMPS_OK;
+}
+
+int imps_destroy(imps_ctx* c) {
+  if (!c) return IMPS_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_state(c);
+  for (double*& p : c->Wm) dfree(p);
+  for (auto& kv : c->ws) dfree(kv.second.p);
+  dfree(c->dpart);
+  if (c->hpart) (void)hipHostFree(c->hpart);
+  if (c->rb) rocblas_destroy_handle(c->rb);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return IMPS_OK;
+}
+
+const char* imps_last_error(const imps_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int imps_set_mpo(imps_ctx* c, int nsites, int w, int d, const double* const* W) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    require(nsites > 0 && w >= 2 && d >= 1 && W, IMPS_E_ARG, "imps_set_mpo: bad dimensions");
+    require(w * d <= 64, IMPS_E_UNSUPPORTED, "imps_set_mpo: w*d > 64 is not supported");
+    require(d <= 8, IMPS_E_UNSUPPORTED, "imps_set_mpo: d > 8 is not supported");
+    if (!c->AL.empty()) require((int)c->AL.size() == nsites, IMPS_E_ARG, "imps_set_mpo: unit cell differs from the state");
+    for (double*& p : c->Wm) dfree(p);
+    c->Wm.clear();
+    c->Wh.clear();
+    c->N = nsites;
+    c->w = w;
+    c->d = d;
+    const int wd = w * d;
+    for (int l = 0; l < nsites; ++l) {
+      require(W[l] != nullptr, IMPS_E_ARG, "imps_set_mpo: null tensor");
+      std::vector<double> Wl(W[l], W[l] + (size_t)w * d * w * d);
+      c->Wh.push_back(Wl);
+      // unit corners
+      for (int u = 0; u < d; ++u)
+        for (int s = 0; s < d; ++s) {
+          const double e = (u == s) ? 1.0 : 0.0;
+          require(Wl[0 + (size_t)w * (u + (size_t)d * (0 + (size_t)w * s))] == e &&
+                      Wl[(w - 1) + (size_t)w * (u + (size_t)d * ((w - 1) + (size_t)w * s))] == e,
+                  IMPS_E_ARG, "imps_set_mpo: levels 0 and w-1 must be unit levels");
+        }
+      std::vector<double> Wm((size_t)wd * wd);
+      for (int i = 0; i < w; ++i)
+        for (int u = 0; u < d; ++u)
+          for (int j = 0; j < w; ++j)
+            for (int s = 0; s < d; ++s)
+              Wm[(u + (size_t)d * j) + (size_t)wd * (i + (size_t)w * s)] =
+                  Wl[i + (size_t)w * (u + (size_t)d * (j + (size_t)w * s))];
+      double* dp = dalloc(Wm.size());
+      h2d(c, dp, Wm.data(), Wm.size());
+      c->Wm.push_back(dp);
+    }
+    if (c->envs_valid) {
+      c->envs_valid = false;
+    }
+    for (double*& p : c->GL) dfree(p);
+    for (double*& p : c->GR) dfree(p);
+    c->GL.clear();
+    c->GR.clear();
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_set_state(imps_ctx* c, int nsites, int D, const double* const* AL, const double* const* AR,
+                   const double* const* Cm, const double* const* AC) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_mpo(c);
+    require(nsites == c->N && D >= 1, IMPS_E_ARG, "imps_set_state: unit cell or D mismatch");
+    require(D <= c->d * D, IMPS_E_ARG, "imps_set_state: D exceeds d*D");
+    alloc_state(c, nsites, D, c->d);
+    const size_t nA = (size_t)D * c->d * D, nC = (size_t)D * D;
+    for (int l = 0; l < nsites; ++l) {
+      h2d(c, c->AL[l], AL[l], nA);
+      h2d(c, c->AR[l], AR[l], nA);
+      h2d(c, c->C[l], Cm[l], nC);
+      if (AC)
+        h2d(c, c->AC[l], AC[l], nA);
+      else
+        mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_get_state(imps_ctx* c, double* const* AL, double* const* AR, double* const* Cm,
+                   double* const* AC) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    const int N = (int)c->AL.size();
+    const size_t nA = (size_t)c->D * c->d * c->D, nC = (size_t)c->D * c->D;
+    for (int l = 0; l < N; ++l) {
+      if (AL && AL[l]) d2h(c, AL[l], c->AL[l], nA);
+      if (AR && AR[l]) d2h(c, AR[l], c->AR[l], nA);
+      if (Cm && Cm[l]) d2h(c, Cm[l], c->C[l], nC);
+      if (AC && AC[l]) d2h(c, AC[l], c->AC[l], nA);
+    }
+  });
+}
+
+int imps_set_state_raw(imps_ctx* c, int nsites, int D, const double* const* A, double tol, int maxiter) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_mpo(c);
+    require(nsites == c->N && D >= 1 && A, IMPS_E_ARG, "imps_set_state_raw: unit cell mismatch");
+    alloc_state(c, nsites, D, c->d);
+    const size_t nA = (size_t)D * c->d * D, nC = (size_t)D * D;
+    // raw tensors are staged in the pending AC buffers (the gauge sweeps read copies)
+    std::vector<double*> raw(nsites);
+    for (int l = 0; l < nsites; ++l) {
+      raw[l] = c->pAC[l];
+      h2d(c, raw[l], A[l], nA);
+    }
+    double* I = ws(c, "sr_I", nC);
+    HIPCHK(launch_set_identity(D, I, D, c->stream));
+    // gaugefix!(:LR): left sweeps from C0 = I, then right sweeps from C[end]
+    uniform_leftorth(c, raw, I, tol, maxiter);
+    double* C0 = ws(c, "sr_C0", nC);
+    d2d(c, C0, c->C[nsites - 1], nC);
+    uniform_rightorth(c, c->AL, C0, tol, maxiter);
+    for (int l = 0; l < nsites; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+    c->envs_valid = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
+  if (!c || !x || !y) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    const size_t n = (size_t)c->D * c->d * c->D;
+    double* dx = ws(c, "abi_x", n);
+    double* dy = ws(c, "abi_y", n);
+    h2d(c, dx, x, n);
+    hac_apply(c, site, dx, dy, prec);
+    d2h(c, y, dy, n);
+  });
+}
+
+int imps_hac_apply_dev(imps_ctx* c, int site, const double* d_x, double* d_y, int prec) {
+  if (!c || !d_x || !d_y) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    hac_apply(c, site, d_x, d_y, prec);
+  });
+}
+
+int imps_hc_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
+  if (!c || !x || !y) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    const size_t n = (size_t)c->D * c->D;
+    double* dx = ws(c, "abi_x", n);
+    double* dy = ws(c, "abi_y", n);
+    h2d(c, dx, x, n);
+    hc_apply(c, site, dx, dy, prec);
+    d2h(c, y, dy, n);
+  });
+}
+
+int imps_hc_apply_dev(imps_ctx* c, int site, const double* d_x, double* d_y, int prec) {
+  if (!c || !d_x || !d_y) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    hc_apply(c, site, d_x, d_y, prec);
+  });
+}
+
+int imps_transfer_apply(imps_ctx* c, int side, const double* v, double* out) {
+  if (!c || !v || !out) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    require(side == IMPS_SIDE_LEFT || side == IMPS_SIDE_RIGHT, IMPS_E_ARG, "imps_transfer_apply: bad side");
+    const int N = (int)c->AL.size(), D = c->D;
+    const size_t DD = (size_t)D * D;
+    double* a = ws(c, "tr_a", DD);
+    double* b = ws(c, "tr_b", DD);
+    h2d(c, a, v, DD);
+    if (side == IMPS_SIDE_LEFT) {
+      for (int l = 0; l < N; ++l) {
+        push_left2(c, a, D, c->AL[l], c->AL[l], b, D);
+        std::swap(a, b);
+      }
+    } else {
+      for (int l = N - 1; l >= 0; --l) {
+        push_right2(c, a, D, c->AR[l], c->AR[l], b, D);
+        std::swap(a, b);
+      }
+    }
+    d2h(c, out, a, DD);
+  });
+}
+
+int imps_regularize(imps_ctx* c, int n, double* v, const double* lvec, const double* rvec) {
+  if (!c || !v || !lvec || !rvec || n <= 0) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    const size_t nn = (size_t)n * n;
+    double* dv = ws(c, "rg_v", nn);
+    double* dl = ws(c, "rg_l", nn);
+    double* dr = ws(c, "rg_r", nn);
+    h2d(c, dv, v, nn);
+    h2d(c, dl, lvec, nn);
+    h2d(c, dr, rvec, nn);
+    const double cc = trace_prod(c, n, dl, dv);
+    axpby(c, nn, -cc, dr, 1.0, dv);
+    d2h(c, v, dv, nn);
+  });
+}
+
+int imps_envs_recalculate(imps_ctx* c, double tol, int maxiter, int krylovdim) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    require(tol > 0 && maxiter > 0 && krylovdim > 0, IMPS_E_ARG, "imps_envs_recalculate: bad parameters");
+    envs_recalculate(c, tol, maxiter, krylovdim, nullptr);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_envs_reset(imps_ctx* c) {
+  if (!c) return IMPS_E_ARG;
+  c->envs_valid = false;
+  return IMPS_OK;
+}
+
+int imps_get_env(imps_ctx* c, int site, int side, double* out) {
+  if (!c || !out) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_envs(c);
+    site = wrap(c, site);
+    const size_t n = (size_t)c->D * c->w * c->D;
+    d2h(c, out, side == IMPS_SIDE_LEFT ? c->GL[site] : c->GR[site], n);
+  });
+}
+
+int imps_set_env(imps_ctx* c, int site, int side, const double* in) {
+  if (!c || !in) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    alloc_envs(c);
+    site = wrap(c, site);
+    const size_t n = (size_t)c->D * c->w * c->D;
+    h2d(c, side == IMPS_SIDE_LEFT ? c->GL[site] : c->GR[site], in, n);
+    c->envs_valid = true;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+static void fill_info(imps_krylov_info* info, const KInfo& k) {
+  if (!info) return;
+  info->converged = k.converged;
+  info->numiter = k.numiter;
+  info->numops = k.numops;
+  info->normres = k.normres;
+}
+
+int imps_eigsolve_ac(imps_ctx* c, int site, const imps_krylov_params* p, double* lambda, double* vec,
+                     imps_krylov_info* info) {
+  if (!c || !p) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    site = wrap(c, site);
+    EigOut o = eigsolve_ac(c, site, p->tol, p->krylovdim, p->maxiter, p->eager != 0, p->prec);
+    if (lambda) *lambda = o.lambda;
+    fill_info(info, o.info);
+    if (vec) d2h(c, vec, c->pAC[site], (size_t)c->D * c->d * c->D);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_eigsolve_c(imps_ctx* c, int site, const imps_krylov_params* p, double* lambda, double* vec,
+                    imps_krylov_info* info) {
+  if (!c || !p) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    site = wrap(c, site);
+    EigOut o = eigsolve_c(c, site, p->tol, p->krylovdim, p->maxiter, p->eager != 0, p->prec);
+    if (lambda) *lambda = o.lambda;
+    fill_info(info, o.info);
+    if (vec) d2h(c, vec, c->pC[site], (size_t)c->D * c->D);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_regauge(imps_ctx* c, int site, double* al) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    site = wrap(c, site);
+    regauge_site(c, site);
+    if (al) d2h(c, al, c->pAL[site], (size_t)c->D * c->d * c->D);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_gaugefix_right(imps_ctx* c, double tol, int maxiter, int* sweeps) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    gauge_step(c, tol, maxiter, sweeps);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_galerkin(imps_ctx* c, double* eps) {
+  if (!c || !eps) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    *eps = galerkin(c, PREC_FP64);
+  });
+}
+
+int imps_energy(imps_ctx* c, double* e) {
+  if (!c || !e) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    *e = energy(c);
+  });
+}
+
+int imps_local_expectation(imps_ctx* c, int site, const double* O, double* val) {
+  if (!c || !O || !val) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    site = wrap(c, site);
+    const size_t n = (size_t)c->D * c->d * c->D;
+    double* t = ws(c, "lx_t", n);
+    HIPCHK(launch_mix_mid(c->D, c->d, c->D, O, c->AC[site], t, 0.0, c->stream));
+    *val = dot(c, n, c->AC[site], t) / dot_psi(c);
+  });
+}
+
+int imps_vumps_default_params(imps_vumps_params* p) {
+  if (!p) return IMPS_E_ARG;
+  p->tol = 1.0e-10;
+  p->maxiter = 300;
+  p->verbosity = 0;
+  p->tol_min = 1.0e-14;
+  p->tol_max = 1.0e-4;
+  p->eigs_tolfactor = 1.0e-3;
+  p->gauge_tolfactor = 1.0e-6;
+  p->envs_tolfactor = 1.0e-4;
+  p->krylovdim = 30;
+  p->eig_maxiter = 300;
+  p->gauge_maxiter = 300;
+  p->envs_maxiter = 300;
+  p->prec = IMPS_PREC_FP64;
+  p->refine_below = 0.0;
+  return IMPS_OK;
+}
+
+int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* info, double* losses,
+                   int nloss, imps_iter_cb cb, void* user) {
+  if (!c || !p) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    const int N = c->N;
+    const long hac0 = c->hac_applies, hc0 = c->hc_applies;
+    auto t0 = std::chrono::steady_clock::now();
+    auto secs = [&] {
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    if (!c->envs_valid) envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, nullptr);
+    int prec = p->prec;
+    int switch_iter = -1;
+    double eps = galerkin(c, PREC_FP64);
+    auto tolu = [&](double f, int it, double e) { return updatetol(f, it, e, p->tol_min, p->tol_max); };
+    envs_recalculate(c, tolu(p->envs_tolfactor, 0, eps), p->envs_maxiter, p->krylovdim, nullptr);
+    int nl = 0;
+    if (losses && nl < nloss) losses[nl++] = eps;
+    bool converged = false;
+    double f = 0.0, t_reach = -1.0;
+    int it = 0;
+    for (it = 1; it <= p->maxiter; ++it) {
+      const int siter = it - 1;
+      if (prec != PREC_FP64 && eps < p->refine_below) {
+        prec = PREC_FP64;
+        switch_iter = it;
+      }
+      const double tol_eig = tolu(p->eigs_tolfactor, siter, eps);
+      for (int site = 0; site < N; ++site) {
+        eigsolve_ac(c, site, tol_eig, p->krylovdim, p->eig_maxiter, true, prec);
+        eigsolve_c(c, site, tol_eig, p->krylovdim, p->eig_maxiter, true, prec);
+        regauge_site(c, site);
+      }
+      gauge_step(c, tolu(p->gauge_tolfactor, siter, eps), p->gauge_maxiter, nullptr);
+      envs_recalculate(c, tolu(p->envs_tolfactor, siter, eps), p->envs_maxiter, p->krylovdim, nullptr);
+      eps = galerkin(c, PREC_FP64);
+      if (losses && nl < nloss) losses[nl++] = eps;
+      f = energy(c);
+      if (p->verbosity > 0) printf("VUMPS iter %4d : eps = %.3e f = %.8g\n", it, eps, f);
+      if (eps <= p->tol) {
+        converged = true;
+        t_reach = secs();
+      }
+      if (cb && cb(it, eps, f, user) != 0) break;
+      if (converged) break;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (info) {
+      info->niter = std::min(it, p->maxiter);
+      info->converged = converged ? 1 : 0;
+      info->itererr = eps;
+      info->energy = f;
+      info->seconds = secs();
+      info->t_galerkin_reached = t_reach;
+      info->hac_applies = (int)(c->hac_applies - hac0);
+      info->hc_applies = (int)(c->hc_applies - hc0);
+      info->prec_switch_iter = switch_iter;
+    }
+  });
+}
+
+int imps_bench_hac(imps_ctx* c, int site, int reps, int prec, imps_hac_timing* t) {
+  if (!c || !t || reps <= 0) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    site = wrap(c, site);
+    const int D = c->D, w = c->w, d = c->d;
+    const size_t n = (size_t)D * d * D;
+    double* x = ws(c, "bh_x", n);
+    double* y = ws(c, "bh_y", n);
+    HIPCHK(launch_fill_random(n, 12345, x, c->stream));
+    hac_apply(c, site, x, y, prec);  // warm-up
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    double s_all = 0, s1 = 0, s2 = 0, s3 = 0;
+    double* T1 = ws(c, "hac_T1", (size_t)D * w * d * D);
+    double* T2 = ws(c, "hac_T2", (size_t)D * w * d * D);
+    for (int r = 0; r < reps; ++r) {
+      HIPCHK(hipEventRecord(ev[0], c->stream));
+      gemm2(c, 'N', 'N', D * w, d * D, D, 1.0, c->GL[site], (long)D * w, x, D, 0.0, T1, (long)D * w, prec);
+      HIPCHK(hipEventRecord(ev[1], c->stream));
+      HIPCHK(launch_wmix(D, w, d, c->Wm[site], T1, T2, c->stream));
+      HIPCHK(hipEventRecord(ev[2], c->stream));
+      gemm2(c, 'N', 'N', D * d, D, D * w, 1.0, T2, (long)D * d, c->GR[site], (long)D * w, 0.0, y, (long)D * d, prec);
+      HIPCHK(hipEventRecord(ev[3], c->stream));
+      HIPCHK(hipEventSynchronize(ev[3]));
+      float a, b, cc, tot;
+      HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+      HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+      HIPCHK(hipEventElapsedTime(&cc, ev[2], ev[3]));
+      HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[3]));
+      s1 += a;
+      s2 += b;
+      s3 += cc;
+      s_all += tot;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    t->ms_apply = s_all / reps;
+    t->ms_gemm1 = s1 / reps;
+    t->ms_wmix = s2 / reps;
+    t->ms_gemm2 = s3 / reps;
+  });
+}
+
+int imps_gemm_dev(imps_ctx* c, char transa, char transb, int m, int n, int k, double alpha, const double* d_a,
+                  int lda, const double* d_b, int ldb, double beta, double* d_c, int ldc, int prec) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    require((transa == 'N' || transa == 'T') && (transb == 'N' || transb == 'T'), IMPS_E_ARG, "imps_gemm_dev: trans");
+    require(m >= 0 && n >= 0 && k >= 0, IMPS_E_ARG, "imps_gemm_dev: negative size");
+    require(ldc >= std::max(1, m), IMPS_E_ARG, "imps_gemm_dev: ldc");
+    require(lda >= std::max(1, transa == 'N' ? m : k), IMPS_E_ARG, "imps_gemm_dev: lda");
+    require(ldb >= std::max(1, transb == 'N' ? k : n), IMPS_E_ARG, "imps_gemm_dev: ldb");
+    gemm2(c, transa, transb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, prec);
+  });
+}
+
+int imps_dev_alloc(imps_ctx* c, size_t bytes, void** d_ptr) {
+  if (!c || !d_ptr) return IMPS_E_ARG;
+  return run_guarded(c, [&] { HIPCHK(hipMalloc(d_ptr, std::max<size_t>(bytes, 1))); });
+}
+
+int imps_dev_free(imps_ctx* c, void* d_ptr) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (d_ptr) HIPCHK(hipFree(d_ptr));
+  });
+}
+
+int imps_memcpy_h2d(imps_ctx* c, void* d_dst, const void* h_src, size_t bytes) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_memcpy_d2h(imps_ctx* c, void* h_dst, const void* d_src, size_t bytes) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    HIPCHK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_synchronize(imps_ctx* c) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Internal declarations shared by the libimps translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace imps {
+
+// Strided GEMM on column-major data:
+//   C[m + n*ldc] = alpha * sum_k opA(m,k) * opB(k,n) + beta * C[m + n*ldc]
+// with k = kc*Kc + kk (0 <= kk < Kc) and
+//   opA(m,k) = A[m*sAm + kk*sAk + kc*sAc],  opB(k,n) = B[kk*sBk + kc*sBc + n*sBn].
+// Either sAm or sAk must be 1, and either sBk or sBn must be 1.  When Kc < K, Kc must be
+// a multiple of 16.
+struct GemmDesc {
+  int M = 0, N = 0, K = 0, Kc = 0;
+  const double* A = nullptr;
+  long sAm = 1, sAk = 0, sAc = 0;
+  const double* B = nullptr;
+  long sBk = 1, sBn = 0, sBc = 0;
+  double* C = nullptr;
+  long ldc = 0;
+  double alpha = 1.0, beta = 0.0;
+};
+
+enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
+
+hipError_t launch_gemm(const GemmDesc& g, int prec, hipStream_t s);
+
+// out[a,u,b] = sum_s M[u,s] in[a,s,b] + beta*out[a,u,b]; in/out (D1, d, D2), M (d, d) on host.
+hipError_t launch_mix_mid(int D1, int d, int D2, const double* M, const double* in, double* out,
+                          double beta, hipStream_t s);
+
+// H_AC W-mix: T2[b,u,bb,j] = sum_{i,s} W[i,u,j,s] T1[b,i,s,bb]
+// T1 at b + D*(i + w*(s + d*bb)), T2 at b + D*(u + d*(bb + D*j)); Wm (w*d)x(w*d) on device
+// as Wm[(u + d*j) + wd*(i + w*s)].
+hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2,
+                       hipStream_t s);
+
+// Partial sums for dots: part[v*nb + blk] for v < nv over vectors X + v*ldx against y.
+int dot_blocks(size_t n);
+hipError_t launch_multidot(size_t n, int nv, const double* X, size_t ldx, const double* y,
+                           double* part, hipStream_t s);
+// y = alpha*y + sum_v coef[v] * X[v]  (nv <= 64 coefficients passed by value)
+hipError_t launch_multiaxpy(size_t n, int nv, const double* X, size_t ldx, const double* coef,
+                            double alpha, double* y, hipStream_t s);
+hipError_t launch_scale(size_t n, double a, double* x, hipStream_t s);
+// y = a*x + b*y
+hipError_t launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t s);
+hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s);
+// diagonal of an n x n matrix (leading dimension ld) += v
+hipError_t launch_add_diag(int n, double v, double* x, long ld, hipStream_t s);
+// x[i + ld*j] = v for i < rows, j < cols
+hipError_t launch_fill2d(int rows, int cols, double v, double* x, long ld, hipStream_t s);
+hipError_t launch_set_identity(int n, double* x, long ld, hipStream_t s);
+// out (cols x rows) = in^T, in (rows x cols) column-major with leading dims
+hipError_t launch_transpose(int rows, int cols, const double* in, long ldin, double* out,
+                            long ldout, hipStream_t s);
+// partial sums of sum_{a,b} l[a + n*b] * v[b + n*a]
+hipError_t launch_trace_prod_parts(int n, const double* l, const double* v, double* part,
+                                   int* nparts, hipStream_t s);
+// column/row sign fix for QRpos: Q[:,i] *= s_i, R[i,:] *= s_i with s_i = sign(R[i,i])
+hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr,
+                             hipStream_t s);
+// copy upper triangle of the n x n leading block of A (lda) into R (n x n), zero below
+hipError_t launch_copy_upper(int n, const double* A, long lda, double* R, hipStream_t s);
+// strided 2-D copy: dst[i + j*ldd] = src[i + j*lds]
+hipError_t launch_copy2d(int rows, int cols, const double* src, long lds, double* dst, long ldd,
+                         hipStream_t s);
+// deterministic pseudo-random fill in [-1, 1) (for measurement inputs)
+hipError_t launch_fill_random(size_t n, uint64_t seed, double* x, hipStream_t s);
+
+// ---- host small dense linear algebra (imps_small.cpp) ----
+// Symmetric eigen-decomposition of the n x n column-major A (destroyed): eigenvalues
+// ascending in w, orthonormal eigenvectors in the columns of V.
+void sym_eig(int n, double* A, double* w, double* V);
+// Eigenvalues (wr + i wi) of an upper-Hessenberg n x n column-major matrix H (copied).
+// Returns 0 on success.
+int hess_eigvals(int n, const double* H, int ldh, double* wr, double* wi);
+// Dominant (largest modulus) eigenpair of the n x n upper-Hessenberg H.  For a complex
+// dominant pair the real part of the phase-fixed eigenvector is returned.  u is unit norm.
+// Returns 0 on success, 1 if the dominant value is complex.
+int hess_dominant(int n, const double* H, int ldh, double* lam, double* u);
+
+}  // namespace imps

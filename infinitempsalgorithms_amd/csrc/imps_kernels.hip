@@ -1,0 +1,545 @@
+// gfx950 kernels of libimps: MFMA GEMM for the bulk contractions, the MPO mixing
+// kernels, FP64 reductions and the small bandwidth-bound helpers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+
+#include "imps_internal.h"
+
+namespace imps {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------------
+// FP64 GEMM on MFMA (v_mfma_f64_16x16x4_f64).
+//
+// Block = 4 wavefronts (2 x 2), macro tile BM x BN, K step BK.  Operands are staged
+// global -> registers -> LDS (k-major rows, one padding element per row); the next
+// K tile is fetched into registers while the MFMAs consume the current one.
+// The MFMA is issued with the B fragment as its row operand so that the 16 lanes of
+// an accumulator row hold 16 consecutive m of the column-major C (coalesced stores).
+// ---------------------------------------------------------------------------------
+
+template <int BM, int BN, int BK, bool AMC, bool BKC>
+__global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int AE = BM * BK / 256, BE = BK * BN / 256;
+  static_assert(AE * 256 == BM * BK && BE * 256 == BK * BN, "tile/thread mismatch");
+
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN + 1];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int m0 = blockIdx.x * BM;
+  const int n0 = blockIdx.y * BN;
+  const int wm = (wave & 1) * WM;
+  const int wn = (wave >> 1) * WN;
+  const int M = g.M, N = g.N, K = g.K, Kc = g.Kc;
+
+  v4d acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  double ra[AE], rb[BE];
+
+  auto load_tiles = [&](int k0) {
+    const int kc = k0 / Kc;
+    const int kk0 = k0 - kc * Kc;
+    const double* Ab = g.A + (long)kc * g.sAc + (long)kk0 * g.sAk;
+    const double* Bb = g.B + (long)kc * g.sBc + (long)kk0 * g.sBk;
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      const int idx = tid + 256 * e;
+      int m, k;
+      if (AMC) {
+        m = idx % BM;
+        k = idx / BM;
+      } else {
+        k = idx % BK;
+        m = idx / BK;
+      }
+      const int gm = m0 + m;
+      ra[e] = (gm < M && k0 + k < K) ? Ab[(long)gm * g.sAm + (long)k * g.sAk] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      const int idx = tid + 256 * e;
+      int n, k;
+      if (BKC) {
+        k = idx % BK;
+        n = idx / BK;
+      } else {
+        n = idx % BN;
+        k = idx / BN;
+      }
+      const int gn = n0 + n;
+      rb[e] = (gn < N && k0 + k < K) ? Bb[(long)k * g.sBk + (long)gn * g.sBn] : 0.0;
+    }
+  };
+
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      const int idx = tid + 256 * e;
+      int m, k;
+      if (AMC) {
+        m = idx % BM;
+        k = idx / BM;
+      } else {
+        k = idx % BK;
+        m = idx / BK;
+      }
+      As[k][m] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      const int idx = tid + 256 * e;
+      int n, k;
+      if (BKC) {
+        k = idx % BK;
+        n = idx / BK;
+      } else {
+        n = idx % BN;
+        k = idx / BN;
+      }
+      Bs[k][n] = rb[e];
+    }
+  };
+
+  const int ntiles = (K + BK - 1) / BK;
+  if (ntiles > 0) load_tiles(0);
+  const int lr = lane & 15;
+  const int lk = lane >> 4;
+  for (int kt = 0; kt < ntiles; ++kt) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (kt + 1 < ntiles) load_tiles((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double af[TM], bf[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[i] = As[kk + lk][wm + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[kk + lk][wn + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+  }
+
+  // accumulator (i, j), register r: row n = 4*(lane/16) + r, column m = lane % 16
+  const double alpha = g.alpha, beta = g.beta;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int m = m0 + wm + i * 16 + lr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn + j * 16 + 4 * lk + r;
+        if (n < N) {
+          double* p = g.C + m + (long)n * g.ldc;
+          const double v = alpha * acc[i][j][r];
+          *p = (beta == 0.0) ? v : v + beta * *p;
+        }
+      }
+    }
+  }
+}
+
+template <int BM, int BN, int BK>
+static hipError_t gemm_dispatch_tile(const GemmDesc& g, bool amc, bool bkc, hipStream_t s) {
+  dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN);
+  dim3 block(256);
+  if (amc && bkc)
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, true, true>), grid, block, 0, s, g);
+  else if (amc && !bkc)
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, true, false>), grid, block, 0, s, g);
+  else if (!amc && bkc)
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, false, true>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, false, false>), grid, block, 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm(const GemmDesc& g0, int prec, hipStream_t s) {
+  (void)prec;
+  GemmDesc g = g0;
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (g.Kc <= 0 || g.Kc >= g.K) g.Kc = std::max(g.K, 1);
+  if (g.Kc < g.K && (g.Kc % 16) != 0) return hipErrorInvalidValue;
+  const bool amc = (g.sAm == 1);
+  const bool bkc = (g.sBk == 1);
+  if (!amc && g.sAk != 1) return hipErrorInvalidValue;
+  if (!bkc && g.sBn != 1) return hipErrorInvalidValue;
+  const long big = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
+  if (big >= 192) return gemm_dispatch_tile<128, 128, 16>(g, amc, bkc, s);
+  return gemm_dispatch_tile<64, 64, 16>(g, amc, bkc, s);
+}
+
+// ---------------------------------------------------------------------------------
+// MPO mixing kernels (bandwidth bound)
+// ---------------------------------------------------------------------------------
+
+struct Mat8 {
+  double v[64];
+};
+
+template <int MAXD>
+__global__ void mix_mid_kernel(int D1, int d, int D2, Mat8 M, const double* __restrict__ in,
+                               double* __restrict__ out, double beta) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (a >= D1) return;
+  const size_t base = (size_t)a + (size_t)D1 * d * b;
+  double x[MAXD];
+#pragma unroll
+  for (int s = 0; s < MAXD; ++s)
+    if (s < d) x[s] = in[base + (size_t)D1 * s];
+  for (int u = 0; u < d; ++u) {
+    double acc = 0.0;
+#pragma unroll
+    for (int s = 0; s < MAXD; ++s)
+      if (s < d) acc += M.v[u + d * s] * x[s];
+    double* p = out + base + (size_t)D1 * u;
+    *p = (beta == 0.0) ? acc : acc + beta * *p;
+  }
+}
+
+hipError_t launch_mix_mid(int D1, int d, int D2, const double* M, const double* in, double* out,
+                          double beta, hipStream_t s) {
+  if (d > 8) return hipErrorInvalidValue;
+  Mat8 m{};
+  for (int i = 0; i < d * d; ++i) m.v[i] = M[i];
+  dim3 block(256);
+  dim3 grid((D1 + 255) / 256, D2);
+  hipLaunchKernelGGL((mix_mid_kernel<8>), grid, block, 0, s, D1, d, D2, m, in, out, beta);
+  return hipGetLastError();
+}
+
+template <int MAXIN>
+__global__ void __launch_bounds__(256) wmix_kernel(int D, int w, int d, const double* __restrict__ Wm,
+                                                   const double* __restrict__ T1, double* __restrict__ T2) {
+  __shared__ double Ws[MAXIN * MAXIN];
+  const int wd = w * d;
+  for (int t = threadIdx.x; t < wd * wd; t += blockDim.x) Ws[t] = Wm[t];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int bb = blockIdx.y;
+  if (b >= D) return;
+  double x[MAXIN];
+  const double* src = T1 + b + (size_t)D * wd * bb;
+#pragma unroll
+  for (int q = 0; q < MAXIN; ++q)
+    if (q < wd) x[q] = src[(size_t)D * q];
+  for (int j = 0; j < w; ++j) {
+    for (int u = 0; u < d; ++u) {
+      const int o = u + d * j;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < MAXIN; ++q)
+        if (q < wd) acc += Ws[o + wd * q] * x[q];
+      T2[b + (size_t)D * (u + (size_t)d * (bb + (size_t)D * j))] = acc;
+    }
+  }
+}
+
+hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2,
+                       hipStream_t s) {
+  const int wd = w * d;
+  dim3 block(256);
+  dim3 grid((D + 255) / 256, D);
+  if (wd <= 16)
+    hipLaunchKernelGGL((wmix_kernel<16>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2);
+  else if (wd <= 32)
+    hipLaunchKernelGGL((wmix_kernel<32>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2);
+  else if (wd <= 64)
+    hipLaunchKernelGGL((wmix_kernel<64>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 reductions: per-block partial sums, summed on the host in block order.
+// ---------------------------------------------------------------------------------
+
+static constexpr int RED_THREADS = 256;
+static constexpr int RED_MAXBLK = 1024;
+
+int dot_blocks(size_t n) {
+  size_t nb = (n + (size_t)RED_THREADS * 16 - 1) / ((size_t)RED_THREADS * 16);
+  if (nb < 1) nb = 1;
+  if (nb > (size_t)RED_MAXBLK) nb = RED_MAXBLK;
+  return (int)nb;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <int NV>
+__device__ inline void block_sum_store(double (&acc)[NV], int nvalid, double* part, int stride,
+                                       int blk) {
+  __shared__ double red[NV][RED_THREADS / 64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    const double v = wave_sum(acc[t]);
+    if (lane == 0) red[t][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV && (int)threadIdx.x < nvalid) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < RED_THREADS / 64; ++w) s += red[threadIdx.x][w];
+    part[(size_t)threadIdx.x * stride + blk] = s;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(RED_THREADS) multidot_kernel(size_t n, int nv, const double* __restrict__ X,
+                                                               size_t ldx, const double* __restrict__ y,
+                                                               double* __restrict__ part) {
+  const int nb = gridDim.x;
+  for (int vc = 0; vc < nv; vc += 8) {
+    double acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = 0.0;
+    for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)nb * RED_THREADS) {
+      const double yi = y[i];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (vc + t < nv) acc[t] += X[(size_t)(vc + t) * ldx + i] * yi;
+    }
+    block_sum_store<8>(acc, nv - vc, part + (size_t)vc * nb, nb, blockIdx.x);
+  }
+}
+
+hipError_t launch_multidot(size_t n, int nv, const double* X, size_t ldx, const double* y,
+                           double* part, hipStream_t s) {
+  const int nb = dot_blocks(n);
+  hipLaunchKernelGGL(multidot_kernel, dim3(nb), dim3(RED_THREADS), 0, s, n, nv, X, ldx, y, part);
+  return hipGetLastError();
+}
+
+struct Coef64 {
+  double c[64];
+};
+
+__global__ void multiaxpy_kernel(size_t n, int nv, const double* __restrict__ X, size_t ldx, Coef64 c,
+                                 double alpha, double* __restrict__ y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double acc = (alpha == 0.0) ? 0.0 : alpha * y[i];
+    for (int v = 0; v < nv; ++v) acc += c.c[v] * X[(size_t)v * ldx + i];
+    y[i] = acc;
+  }
+}
+
+static dim3 ew_grid(size_t n) {
+  size_t nb = (n + 255) / 256;
+  if (nb > 8192) nb = 8192;
+  if (nb < 1) nb = 1;
+  return dim3((unsigned)nb);
+}
+
+hipError_t launch_multiaxpy(size_t n, int nv, const double* X, size_t ldx, const double* coef,
+                            double alpha, double* y, hipStream_t s) {
+  if (nv > 64) return hipErrorInvalidValue;
+  Coef64 c{};
+  for (int v = 0; v < nv; ++v) c.c[v] = coef[v];
+  hipLaunchKernelGGL(multiaxpy_kernel, ew_grid(n), dim3(256), 0, s, n, nv, X, ldx, c, alpha, y);
+  return hipGetLastError();
+}
+
+__global__ void scale_kernel(size_t n, double a, double* x) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
+hipError_t launch_scale(size_t n, double a, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(scale_kernel, ew_grid(n), dim3(256), 0, s, n, a, x);
+  return hipGetLastError();
+}
+
+__global__ void axpby_kernel(size_t n, double a, const double* x, double b, double* y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (b == 0.0) ? a * x[i] : a * x[i] + b * y[i];
+}
+
+hipError_t launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(axpby_kernel, ew_grid(n), dim3(256), 0, s, n, a, x, b, y);
+  return hipGetLastError();
+}
+
+__global__ void fill_kernel(size_t n, double v, double* x) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(fill_kernel, ew_grid(n), dim3(256), 0, s, n, v, x);
+  return hipGetLastError();
+}
+
+__global__ void add_diag_kernel(int n, double v, double* x, long ld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[(size_t)i * ld + i] += v;
+}
+
+hipError_t launch_add_diag(int n, double v, double* x, long ld, hipStream_t s) {
+  hipLaunchKernelGGL(add_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, v, x, ld);
+  return hipGetLastError();
+}
+
+__global__ void fill2d_kernel(int rows, int cols, double v, double* x, long ld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < rows && j < cols) x[i + ld * j] = v;
+}
+
+hipError_t launch_fill2d(int rows, int cols, double v, double* x, long ld, hipStream_t s) {
+  hipLaunchKernelGGL(fill2d_kernel, dim3((rows + 255) / 256, cols), dim3(256), 0, s, rows, cols, v, x, ld);
+  return hipGetLastError();
+}
+
+__global__ void set_identity_kernel(int n, double* x, long ld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < n) x[i + ld * j] = (i == j) ? 1.0 : 0.0;
+}
+
+hipError_t launch_set_identity(int n, double* x, long ld, hipStream_t s) {
+  hipLaunchKernelGGL(set_identity_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, x, ld);
+  return hipGetLastError();
+}
+
+__global__ void transpose_kernel(int rows, int cols, const double* __restrict__ in, long ldin,
+                                 double* __restrict__ out, long ldout) {
+  __shared__ double tile[32][33];
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + tx, c = c0 + k;
+    if (r < rows && c < cols) tile[k][tx] = in[r + ldin * c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + tx, r = r0 + k;  // out[c + ldout * r] = in[r, c]
+    if (r < rows && c < cols) out[c + ldout * r] = tile[tx][k];
+  }
+}
+
+hipError_t launch_transpose(int rows, int cols, const double* in, long ldin, double* out,
+                            long ldout, hipStream_t s) {
+  dim3 grid((rows + 31) / 32, (cols + 31) / 32);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, s, rows, cols, in, ldin, out, ldout);
+  return hipGetLastError();
+}
+
+// sum_{a,b} l[a + n*b] * v[b + n*a], one partial per 32 x 32 tile
+__global__ void trace_prod_kernel(int n, const double* __restrict__ l, const double* __restrict__ v,
+                                  double* __restrict__ part) {
+  __shared__ double vt[32][33];
+  __shared__ double red[4];
+  const int a0 = blockIdx.x * 32, b0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  // v[b + n*a] for b in tile rows (contiguous), a in tile columns
+  for (int k = ty; k < 32; k += 8) {
+    const int b = b0 + tx, a = a0 + k;
+    vt[k][tx] = (a < n && b < n) ? v[b + (long)n * a] : 0.0;  // vt[a-a0][b-b0]
+  }
+  __syncthreads();
+  double acc = 0.0;
+  for (int k = ty; k < 32; k += 8) {
+    const int a = a0 + tx, b = b0 + k;
+    if (a < n && b < n) acc += l[a + (long)n * b] * vt[tx][k];
+  }
+  const int t = ty * 32 + tx;
+  acc = wave_sum(acc);
+  if ((t & 63) == 0) red[t >> 6] = acc;
+  __syncthreads();
+  if (t == 0) part[blockIdx.x + gridDim.x * blockIdx.y] = red[0] + red[1] + red[2] + red[3];
+}
+
+hipError_t launch_trace_prod_parts(int n, const double* l, const double* v, double* part,
+                                   int* nparts, hipStream_t s) {
+  dim3 grid((n + 31) / 32, (n + 31) / 32);
+  *nparts = (int)(grid.x * grid.y);
+  hipLaunchKernelGGL(trace_prod_kernel, grid, dim3(32, 8), 0, s, n, l, v, part);
+  return hipGetLastError();
+}
+
+__device__ inline double sgn_nonneg(double x) { return x < 0.0 ? -1.0 : 1.0; }
+
+__global__ void qr_signfix_q_kernel(int m, int n, double* Q, long ldq, const double* R, long ldr) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (r < m && i < n) Q[r + ldq * i] *= sgn_nonneg(R[i + ldr * i]);
+}
+
+__global__ void qr_signfix_r_kernel(int n, double* R, long ldr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = sgn_nonneg(R[i + ldr * i]);
+  for (int j = i; j < n; ++j) R[i + ldr * j] *= s;
+}
+
+hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr, hipStream_t s) {
+  if (Q) hipLaunchKernelGGL(qr_signfix_q_kernel, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, Q, ldq, R, ldr);
+  hipLaunchKernelGGL(qr_signfix_r_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, R, ldr);
+  return hipGetLastError();
+}
+
+__global__ void copy_upper_kernel(int n, const double* A, long lda, double* R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < n) R[i + (long)n * j] = (i <= j) ? A[i + lda * j] : 0.0;
+}
+
+hipError_t launch_copy_upper(int n, const double* A, long lda, double* R, hipStream_t s) {
+  hipLaunchKernelGGL(copy_upper_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, A, lda, R);
+  return hipGetLastError();
+}
+
+__global__ void copy2d_kernel(int rows, int cols, const double* src, long lds, double* dst, long ldd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < rows && j < cols) dst[i + ldd * j] = src[i + lds * j];
+}
+
+hipError_t launch_copy2d(int rows, int cols, const double* src, long lds, double* dst, long ldd,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(copy2d_kernel, dim3((rows + 255) / 256, cols), dim3(256), 0, s, rows, cols, src, lds, dst, ldd);
+  return hipGetLastError();
+}
+
+__global__ void fill_random_kernel(size_t n, uint64_t seed, double* x) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+hipError_t launch_fill_random(size_t n, uint64_t seed, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(fill_random_kernel, ew_grid(n), dim3(256), 0, s, n, seed, x);
+  return hipGetLastError();
+}
+
+}  // namespace imps

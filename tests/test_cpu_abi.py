@@ -1,0 +1,113 @@
+"""CPU tests of the boundary: libimps.so loads, exports every symbol include/imps.h
+declares, its host-side Krylov numerics match numpy, and the host models/validation
+match the oracle.  No GPU calls."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    txt = open(os.path.join(ROOT, "include", "imps.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(imps_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from infinitempsalgorithms_amd import _lib
+    return _lib.load()
+
+
+def test_library_exports_every_declared_symbol(lib):
+    syms = _header_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(lib, s), s
+    from infinitempsalgorithms_amd import _lib
+    bound = {name for name, _, _ in _lib.SIGNATURES}
+    assert set(syms) == bound, set(syms) ^ bound
+
+
+def test_default_params_match_reference_defaults(lib):
+    from infinitempsalgorithms_amd import _lib
+    p = _lib.VumpsParams()
+    assert lib.imps_vumps_default_params(ctypes.byref(p)) == 0
+    # src/utility.jl:308-320
+    assert (p.tol, p.maxiter, p.krylovdim) == (1e-10, 300, 30)
+    assert (p.tol_min, p.tol_max) == (1e-14, 1e-4)
+    assert (p.eigs_tolfactor, p.gauge_tolfactor, p.envs_tolfactor) == (1e-3, 1e-6, 1e-4)
+
+
+def test_host_symmetric_eig(lib):
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 7, 31):
+        A = rng.standard_normal((n, n))
+        A = np.asfortranarray(A + A.T)
+        ref = np.linalg.eigvalsh(A)
+        w = np.zeros(n)
+        V = np.zeros((n, n), order="F")
+        Ac = A.copy(order="F")
+        P = ctypes.POINTER(ctypes.c_double)
+        lib.imps_host_sym_eig(n, Ac.ctypes.data_as(P), w.ctypes.data_as(P), V.ctypes.data_as(P))
+        assert np.allclose(w, ref, atol=1e-12)
+        assert np.allclose(A @ V, V * w, atol=1e-11)
+        assert np.allclose(V.T @ V, np.eye(n), atol=1e-12)
+
+
+def test_host_hessenberg_dominant(lib):
+    from scipy.linalg import hessenberg
+    rng = np.random.default_rng(1)
+    P = ctypes.POINTER(ctypes.c_double)
+    for n in (1, 3, 12, 30):
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        lam = np.concatenate([[2.0], rng.uniform(-1.5, 1.5, n - 1)])
+        A = Q @ np.diag(lam) @ Q.T + 0.1 * rng.standard_normal((n, n)) / np.sqrt(n)
+        H = np.asfortranarray(hessenberg(A))
+        ev = np.linalg.eigvals(H)
+        wr, wi = np.zeros(n), np.zeros(n)
+        lib.imps_host_hess_eigvals(n, H.ctypes.data_as(P), n, wr.ctypes.data_as(P), wi.ctypes.data_as(P))
+        got = np.sort_complex(wr + 1j * wi)
+        assert np.allclose(got, np.sort_complex(ev), atol=1e-9)
+        l = ctypes.c_double()
+        u = np.zeros(n)
+        st = lib.imps_host_hess_dominant(n, H.ctypes.data_as(P), n, ctypes.byref(l), u.ctypes.data_as(P))
+        k = np.argmax(np.abs(ev))
+        if abs(ev[k].imag) < 1e-12:
+            assert st == 0
+            assert l.value == pytest.approx(ev[k].real, abs=1e-10)
+            assert np.linalg.norm(H @ u - l.value * u) < 1e-9
+
+
+def test_models_match_oracle(oracle):
+    from infinitempsalgorithms_amd import models
+    assert np.array_equal(models.tfim_hamiltonian(0.7, 1.3).Ws[0], oracle.tfim_mpo(0.7, 1.3))
+    assert np.array_equal(models.heisenberg_hamiltonian(1.0, 0.8, 0.2).Ws[0], oracle.xxz_mpo(1.0, 0.8, 0.2))
+    assert np.array_equal(models.fermi_hubbard(1.0, 4.0, 2.0).Ws[0], oracle.hubbard_mpo(1.0, 4.0, 2.0))
+    H = models.fermi_hubbard(nsites=2)
+    assert len(H) == 2 and H.w == 6 and H.d == 4
+
+
+def test_sparseimpo_validation():
+    from infinitempsalgorithms_amd import SparseIMPO, models
+    W = models.tfim_hamiltonian().Ws[0]
+    with pytest.raises(ValueError):
+        SparseIMPO([])
+    with pytest.raises(ValueError):
+        SparseIMPO([W, models.heisenberg_hamiltonian().Ws[0]])  # level chain mismatch
+    bad = W.copy()
+    bad[0, :, 0, :] = 0.0
+    with pytest.raises(ValueError):
+        SparseIMPO([bad])
+
+
+def test_random_tensors_match_oracle(oracle):
+    from infinitempsalgorithms_amd import random_tensors
+    a = random_tensors(6, [2, 2], seed=3)
+    b = oracle.random_tensors(6, [2, 2], seed=3)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

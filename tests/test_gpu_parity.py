@@ -1,0 +1,224 @@
+"""GPU parity of libimps (called through the C ABI) against the FP64 oracle.
+
+Tolerances: the bulk contractions are FP64 on MFMA, so single operator applications
+agree with the oracle to relative 1e-12; Krylov/GMRES results agree to their solver
+tolerance; gauge-invariant VUMPS results (energy, Galerkin error) to the stated bounds.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def imps():
+    import infinitempsalgorithms_amd as m
+    return m
+
+
+@pytest.fixture(scope="module")
+def ctx(imps):
+    c = imps.Context(0)
+    yield c
+    c.close()
+
+
+class DevArray:
+    def __init__(self, ctx, a):
+        self.ctx, self.a = ctx, np.asfortranarray(a, dtype=np.float64)
+        self.p = ctypes.c_void_p()
+        ctx.call("imps_dev_alloc", self.a.nbytes, ctypes.byref(self.p))
+        ctx.call("imps_memcpy_h2d", self.p, self.a.ctypes.data_as(ctypes.c_void_p), self.a.nbytes)
+
+    def get(self):
+        out = np.zeros_like(self.a, order="F")
+        self.ctx.call("imps_memcpy_d2h", out.ctypes.data_as(ctypes.c_void_p), self.p, out.nbytes)
+        return out
+
+    def free(self):
+        self.ctx.call("imps_dev_free", self.p)
+
+
+@pytest.mark.parametrize("ta,tb,m,n,k", [
+    ("N", "N", 64, 64, 64), ("N", "N", 130, 77, 45), ("T", "N", 33, 129, 70), ("N", "T", 257, 100, 31),
+    ("T", "T", 5, 3, 2), ("N", "N", 1, 1, 1), ("N", "N", 640, 512, 96), ("T", "N", 1024, 1024, 4096),
+])
+def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    C0 = rng.standard_normal((m, n))
+    ref = 0.7 * (A if ta == "N" else A.T) @ (B if tb == "N" else B.T) - 0.3 * C0
+    dA, dB, dC = DevArray(ctx, A), DevArray(ctx, B), DevArray(ctx, C0)
+    ctx.call("imps_gemm_dev", ta.encode(), tb.encode(), m, n, k, 0.7, dA.p, A.shape[0], dB.p, B.shape[0],
+             -0.3, dC.p, m, 0)
+    got = dC.get()
+    for d in (dA, dB, dC):
+        d.free()
+    assert rel(got, ref) < 1e-13
+
+
+def _setup_envs(imps, ctx, oracle, Wfn, D, N, seed, sym=False):
+    rng = np.random.default_rng(seed)
+    W = Wfn()
+    H = imps.SparseIMPO([W] * N)
+    ctx.set_mpo(H)
+    psi = oracle.randomimps(D, [H.d] * N, seed=seed)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    ctx.set_state(st)
+    w = H.w
+    GLs, GRs = [], []
+    for l in range(N):
+        GL = rng.standard_normal((D, w, D))
+        GR = rng.standard_normal((D, w, D))
+        ctx.set_env(l, 0, GL)
+        ctx.set_env(l, 1, GR)
+        GLs.append(GL)
+        GRs.append(GR)
+    return H, psi, GLs, GRs
+
+
+@pytest.mark.parametrize("model,D", [("tfim", 16), ("xxz", 24), ("hubbard", 20), ("hubbard", 64), ("tfim", 7)])
+def test_hac_hc_match_oracle(imps, ctx, oracle, model, D):
+    Wfn = {"tfim": oracle.tfim_mpo, "xxz": oracle.xxz_mpo, "hubbard": oracle.hubbard_mpo}[model]
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, Wfn, D, 2, seed=D)
+    envs = imps.GPUCache.__new__(imps.GPUCache)
+    envs.ctx, envs.ket, envs.operator = ctx, None, H
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((D, H.d, D))
+    for site in range(2):
+        y = imps.AC_hamiltonian(site, None, H, None, envs)(x)
+        assert rel(y, oracle.hac_apply(GLs[site], H.Ws[site], GRs[site], x)) < 1e-12
+        c = rng.standard_normal((D, D))
+        yc = imps.C_hamiltonian(site, None, H, None, envs)(c)
+        assert rel(yc, oracle.hc_apply(GLs[(site + 1) % 2], GRs[site], c)) < 1e-12
+
+
+def test_hac_full_size_hubbard(imps, ctx, oracle):
+    # the D = 1024 Hubbard shape of the benchmark (w = 6, d = 4), against the dense einsum
+    D = 1024
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 1, seed=2)
+    envs = imps.GPUCache.__new__(imps.GPUCache)
+    envs.ctx, envs.ket, envs.operator = ctx, None, H
+    x = np.random.default_rng(3).standard_normal((D, 4, D))
+    y = imps.AC_hamiltonian(0, None, H, None, envs)(x)
+    assert rel(y, oracle.hac_apply_dense(GLs[0], H.Ws[0], GRs[0], x)) < 1e-12
+    # linearity: H(2x + y) = 2 H(x) + H(y)
+    z = np.random.default_rng(4).standard_normal((D, 4, D))
+    lhs = imps.AC_hamiltonian(0, None, H, None, envs)(2 * x + z)
+    assert rel(lhs, 2 * y + imps.AC_hamiltonian(0, None, H, None, envs)(z)) < 1e-12
+
+
+def test_transfer_and_regularize(imps, ctx, oracle):
+    D = 12
+    H, psi, _, _ = _setup_envs(imps, ctx, oracle, oracle.tfim_mpo, D, 2, seed=5)
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal((D, D))
+    tl = imps.TransferMatrix(ctx, "left")(v)
+    assert rel(tl, oracle.transfer_left(psi.AL, psi.AL, v.reshape(-1, order="F")).reshape(D, D, order="F")) < 1e-12
+    tr = imps.TransferMatrix(ctx, "right")(v)
+    assert rel(tr, oracle.transfer_right(psi.AR, psi.AR, v.reshape(-1, order="F")).reshape(D, D, order="F")) < 1e-12
+    # identity is the fixed point of the AL transfer (test/algorithms/envs.jl:25-28)
+    assert rel(imps.TransferMatrix(ctx, "left")(np.eye(D)), np.eye(D)) < 1e-12
+    M = np.asfortranarray(rng.standard_normal((D, D)))
+    l = rng.standard_normal((D, D))
+    ref = oracle.regularize(M.copy(), l, np.eye(D))
+    imps.regularize_(M, l, np.eye(D), ctx=ctx)
+    assert rel(M, ref) < 1e-13
+
+
+@pytest.mark.parametrize("model,D,N", [("tfim", 8, 1), ("tfim", 12, 2), ("xxz", 10, 2), ("hubbard", 12, 2)])
+def test_envs_energy_galerkin_match_oracle(imps, ctx, oracle, model, D, N):
+    Wfn = {"tfim": oracle.tfim_mpo, "xxz": oracle.xxz_mpo, "hubbard": oracle.hubbard_mpo}[model]
+    W = Wfn()
+    H = imps.SparseIMPO([W] * N)
+    psi = oracle.randomimps(D, [H.d] * N, seed=11)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, H, tol=1e-12, ctx=ctx)
+    ref = oracle.dmrg_cache(psi, [W] * N, tol=1e-12)
+    for l in range(N):
+        assert rel(imps.leftenv(envs, l), ref.lefts[l]) < 1e-8
+        assert rel(imps.rightenv(envs, l), ref.rights[l]) < 1e-8
+    e = imps.expectationvalue(st, H, envs)
+    assert abs(e - oracle.expectation_energy(psi, [W] * N, ref)) < 1e-10
+    g = imps.calc_galerkin(st, H, envs)
+    assert abs(g - oracle.calc_galerkin(psi, [W] * N, ref)) < 1e-9 * max(1.0, g)
+    # warm-started recalculation keeps the energy (test/algorithms/api.jl:207-210)
+    imps.recalculate_(envs, st, H, tol=1e-12)
+    assert abs(imps.expectationvalue(st, H, envs) - e) < 1e-10
+
+
+def test_local_update_and_gauge_match_oracle(imps, ctx, oracle):
+    D, N = 10, 2
+    W = oracle.tfim_mpo()
+    H = imps.SparseIMPO([W] * N)
+    psi = oracle.randomimps(D, [2] * N, seed=21)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, H, tol=1e-12, ctx=ctx)
+    ref = oracle.dmrg_cache(psi, [W] * N, tol=1e-12)
+    tol = 1e-10
+    ALs_ref = []
+    for site in range(N):
+        lam, AC, info = imps.fixedpoint(imps.AC_hamiltonian(site, st, H, st, envs), psi.AC[site], "SR",
+                                        imps.Lanczos(tol=tol))
+        lr, ACr, _ = oracle.lanczos_eigsolve(lambda x: oracle.hac_apply(ref.leftenv(site), W, ref.rightenv(site), x),
+                                             psi.AC[site], tol)
+        assert info.converged and abs(lam - lr) < 1e-9
+        assert rel(AC, ACr) < 1e-6
+        lam, C, info = imps.fixedpoint(imps.C_hamiltonian(site, st, H, st, envs), psi.C[site], "SR",
+                                       imps.Lanczos(tol=tol))
+        lcr, Cr, _ = oracle.lanczos_eigsolve(lambda x: oracle.hc_apply(ref.leftenv(site + 1), ref.rightenv(site), x),
+                                             psi.C[site], tol)
+        assert abs(lam - lcr) < 1e-9 and rel(C, Cr) < 1e-6
+        AL = imps.regauge_(envs, site)
+        ALr = oracle.regauge(ACr, Cr)
+        ALs_ref.append(ALr)
+        assert rel(AL, ALr) < 1e-5
+    sweeps = ctypes.c_int()
+    ctx.call("imps_gaugefix_right", 1e-12, 300, ctypes.byref(sweeps))
+    psi_g = ctx.get_state()
+    ref_psi, ref_sweeps = oracle.gauge_step(psi, ALs_ref, 1e-12, 300)
+    el, er, em = oracle.mixedcanonical_errors(oracle.CanonicalIMPS(psi_g.AL, psi_g.AR, psi_g.C, psi_g.AC))
+    assert max(el, er) < 1e-10
+    for l in range(N):
+        # singular values of C are gauge invariant
+        assert rel(np.linalg.svd(psi_g.C[l], compute_uv=False), np.linalg.svd(ref_psi.C[l], compute_uv=False)) < 1e-6
+
+
+def test_canonicalize_raw_tensors(imps, ctx, oracle):
+    H = imps.models.fermi_hubbard(nsites=2)
+    As = imps.random_tensors(8, H.phydims(), seed=2)
+    psi, _ = imps.canonicalimps(As, H, ctx=ctx)
+    ref = oracle.canonical_from_tensors(As)
+    el, er, em = oracle.mixedcanonical_errors(oracle.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC))
+    assert max(el, er) < 1e-12 and em < 1e-10
+    for l in range(2):
+        assert rel(psi.C[l], ref.C[l]) < 1e-8
+        assert rel(psi.AL[l], ref.AL[l]) < 1e-8
+
+
+@pytest.mark.parametrize("N,D,seed", [(1, 16, 0), (2, 12, 0)])
+def test_vumps_tfim_matches_oracle(imps, ctx, oracle, N, D, seed):
+    # configuration "TFIM J=h=1, N=1, D=16" and the reference's D=12 2-site test
+    W = oracle.tfim_mpo()
+    H = imps.SparseIMPO([W] * N)
+    psi0 = oracle.randomimps(D, [2] * N, seed=seed)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+    envs = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    psi, envs, info = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-10), envs=envs)
+    _, _, rinfo = oracle.find_groundstate(psi0, [W] * N, oracle.VUMPS(tol=1e-10))
+    assert info.converged and info.itererr <= 1e-10
+    e = info.energy / N
+    assert abs(e + 4 / math.pi) < 1e-6
+    assert abs(info.energy - rinfo.energies[-1]) < 1e-9
+    assert abs(info.niter - rinfo.niter) <= max(3, rinfo.niter // 10)
+    for a, b in zip(info.losses[:5], rinfo.losses[:5]):
+        assert abs(a - b) <= 1e-6 * max(b, 1e-12) + 1e-9
