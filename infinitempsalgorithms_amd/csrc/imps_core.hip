@@ -1186,7 +1186,8 @@ int imps_set_mpo(imps_ctx* c, int nsites, int w, int d, const double* const* W) 
     require(nsites > 0 && w >= 2 && d >= 1 && W, IMPS_E_ARG, "imps_set_mpo: bad dimensions");
     require(w * d <= 64, IMPS_E_UNSUPPORTED, "imps_set_mpo: w*d > 64 is not supported");
     require(d <= 8, IMPS_E_UNSUPPORTED, "imps_set_mpo: d > 8 is not supported");
-    if (!c->AL.empty()) require((int)c->AL.size() == nsites, IMPS_E_ARG, "imps_set_mpo: unit cell differs from the state");
+    // a model with another unit cell or physical dimension invalidates the state
+    if (!c->AL.empty() && ((int)c->AL.size() != nsites || c->d != d)) free_state(c);
     for (double*& p : c->Wm) dfree(p);
     c->Wm.clear();
     c->Wh.clear();

@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
     }
   }
 
-  // accumulator (i, j), register r: row n = 4*(lane/16) + r, column m = lane % 16
+  // v_mfma_f64_16x16x4_f64 accumulator layout: register r of lane l holds row
+  // (l / 16) + 4 r, column l % 16; rows are n, columns m here
   const double alpha = g.alpha, beta = g.beta;
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
@@ -146,7 +147,7 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
     for (int j = 0; j < TN; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = n0 + wn + j * 16 + 4 * lk + r;
+        const int n = n0 + wn + j * 16 + lk + 4 * r;
         if (n < N) {
           double* p = g.C + m + (long)n * g.ldc;
           const double v = alpha * acc[i][j][r];
