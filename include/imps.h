@@ -198,6 +198,15 @@ typedef struct imps_vumps_info {
   int hac_applies;
   int hc_applies;
   int prec_switch_iter; /* iteration at which FP64 refinement started (-1 if none) */
+  /* wall seconds spent in each phase of the loop */
+  double t_local;    /* localupdate_step!: AC/C eigensolves and regauge! */
+  double t_gauge;    /* gauge_step! */
+  double t_envs;     /* recalculate! */
+  double t_galerkin; /* calc_galerkin */
+  double t_energy;   /* expectationvalue */
+  int gauge_sweeps;  /* LQ sweeps summed over iterations */
+  int gmres_ops;     /* environment-solver operator applications */
+  int qr_fallbacks;  /* Cholesky-QR breakdowns answered by Householder QR */
 } imps_vumps_info;
 
 /* Per-iteration callback: return nonzero to stop the loop. */

@@ -43,7 +43,9 @@ class VumpsParams(ctypes.Structure):
 class VumpsInfo(ctypes.Structure):
     _fields_ = [("niter", c_int), ("converged", c_int), ("itererr", c_double), ("energy", c_double),
                 ("seconds", c_double), ("t_galerkin_reached", c_double), ("hac_applies", c_int),
-                ("hc_applies", c_int), ("prec_switch_iter", c_int)]
+                ("hc_applies", c_int), ("prec_switch_iter", c_int), ("t_local", c_double),
+                ("t_gauge", c_double), ("t_envs", c_double), ("t_galerkin", c_double), ("t_energy", c_double),
+                ("gauge_sweeps", c_int), ("gmres_ops", c_int), ("qr_fallbacks", c_int)]
 
 
 class HacTiming(ctypes.Structure):

@@ -409,6 +409,7 @@ class IterativeConvergenceInfo:
     hc_applies: int
     prec_switch_iter: int
     energies: list = field(default_factory=list)
+    timings: dict = field(default_factory=dict)
 
     @property
     def itererr(self):
@@ -416,10 +417,11 @@ class IterativeConvergenceInfo:
 
 
 def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | None = None,
-                     envs: GPUCache | None = None, seed=0, ctx: Context | None = None):
+                     envs: GPUCache | None = None, seed=0, ctx: Context | None = None, callback=None):
     """find_groundstate(ψ₀, H::SparseIMPO, alg::VUMPS, envs) (vumps.jl:51-86).  With
     psi0 = None a random state of bond dimension alg.D is generated (vumps.jl:96-102).
-    Returns (ψ, envs, info)."""
+    `callback(iter, eps, energy)` is called after every iteration (alg.finalize's slot);
+    a truthy return value stops the loop.  Returns (ψ, envs, info)."""
     alg = alg or VUMPS()
     if envs is None:
         if psi0 is None:
@@ -434,6 +436,8 @@ def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | Non
 
     def _cb(it, eps, f, user):
         energies.append(f)
+        if callback is not None and callback(it, eps, f):
+            return 1
         return 0
 
     cb = _lib.ITER_CB(_cb)
@@ -444,5 +448,8 @@ def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | Non
     nl = info.niter + 1
     res = IterativeConvergenceInfo(info.niter, bool(info.converged), list(losses[:nl]), info.energy,
                                    info.seconds, info.t_galerkin_reached, info.hac_applies,
-                                   info.hc_applies, info.prec_switch_iter, energies)
+                                   info.hc_applies, info.prec_switch_iter, energies,
+                                   {k: getattr(info, k) for k in ("t_local", "t_gauge", "t_envs", "t_galerkin",
+                                                                  "t_energy", "gauge_sweeps", "gmres_ops",
+                                                                  "qr_fallbacks")})
     return psi, envs, res

@@ -635,23 +635,128 @@ void gmres(imps_ctx* c, const Op& op, size_t n, const double* b, double* x, doub
 
 // ------------------------------------------------------------------ QRpos / LQpos
 
-// A (m x n, lda = m, m >= n) is overwritten by Q; R (n x n) receives the factor with a
-// non-negative diagonal (FiniteMPSAlgorithms' QRpos).
-void qrpos(imps_ctx* c, int m, int n, double* A, double* R) {
+// Householder QR (rocSOLVER geqrf/orgqr) with the sign fix of QRpos.
+void qrpos_householder(imps_ctx* c, int m, int n, double* A, double* R) {
   double* tau = ws(c, "qr_tau", n);
   RBCHK(rocsolver_dgeqrf(c->rb, m, n, A, m, tau));
   HIPCHK(launch_copy_upper(n, A, m, R, c->stream));
   RBCHK(rocsolver_dorgqr(c->rb, m, n, n, A, m, tau));
-  HIPCHK(launch_qr_signfix(m, n, A, m, R, n, c->stream));
+  HIPCHK(launch_qr_signfix(m, n, A, m, R, n, ws(c, "qr_sign", n), c->stream));
+}
+
+// Cholesky factor of the n x n G in place (upper: G = R^T R, lower: G = L L^T); false on
+// breakdown.
+bool chol(imps_ctx* c, int n, double* G, bool upper) {
+  int* info = (int*)ws(c, "chol_info", 1);
+  RBCHK(rocsolver_dpotrf(c->rb, upper ? rocblas_fill_upper : rocblas_fill_lower, n, G, n, info));
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return h == 0;
+}
+
+double diag_sum(imps_ctx* c, int n, const double* G) {
+  double* dg = ws(c, "chol_diag", n);
+  HIPCHK(launch_copy2d(1, n, G, (long)n + 1, dg, 1, c->stream));
+  std::vector<double> h(n);
+  d2h(c, h.data(), dg, n);
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s;
+}
+
+// Shifted Cholesky QR, three passes (one shifted pass, two plain passes): A (m x n, m >= n)
+// is overwritten by Q, R (n x n) gets the upper factor with a positive diagonal, which is
+// the QRpos factorisation.  Returns false (A restored) if a Cholesky factorisation breaks
+// down.
+bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R) {
+  const size_t nn = (size_t)n * n, mn = (size_t)m * n;
+  double* G = ws(c, "cq_G", nn);
+  double* R2 = ws(c, "cq_R2", nn);
+  double* bak = ws(c, "cq_bak", mn);
+  d2d(c, bak, A, mn);
+  const double one = 1.0;
+  for (int pass = 0; pass < 3; ++pass) {
+    gemm2(c, 'T', 'N', n, n, m, 1.0, A, m, A, m, 0.0, G, n);
+    if (pass == 0) {
+      const double shift = 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * diag_sum(c, n, G);
+      HIPCHK(launch_add_diag(n, shift, G, n, c->stream));
+    }
+    if (!chol(c, n, G, true)) {
+      d2d(c, A, bak, mn);
+      return false;
+    }
+    RBCHK(rocblas_dtrsm(c->rb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                        rocblas_diagonal_non_unit, m, n, &one, G, n, A, m));
+    if (pass == 0) {
+      // potrf leaves the strictly lower triangle of G untouched: clear it before R is
+      // used as a general matrix by trmm
+      d2d(c, R, G, nn);
+      HIPCHK(launch_tri_keep(n, R, n, true, c->stream));
+    } else {
+      RBCHK(rocblas_dtrmm(c->rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, n, n, &one, G, n, R, n, R2, n));
+      d2d(c, R, R2, nn);
+    }
+  }
+  HIPCHK(launch_tri_keep(n, R, n, true, c->stream));
+  return true;
+}
+
+// LQ counterpart: M (m x n, ld m, m <= n) = L Q computed as Q (in place of M) and L (m x m,
+// lower, positive diagonal).
+bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L) {
+  const size_t mm = (size_t)m * m, mn = (size_t)m * n;
+  double* G = ws(c, "cq_G", mm);
+  double* L2 = ws(c, "cq_R2", mm);
+  double* bak = ws(c, "cq_bak", mn);
+  d2d(c, bak, M, mn);
+  const double one = 1.0;
+  for (int pass = 0; pass < 3; ++pass) {
+    gemm2(c, 'N', 'T', m, m, n, 1.0, M, m, M, m, 0.0, G, m);
+    if (pass == 0) {
+      const double shift = 11.0 * ((double)m * n + (double)m * (m + 1)) * 1.1102230246251565e-16 * diag_sum(c, m, G);
+      HIPCHK(launch_add_diag(m, shift, G, m, c->stream));
+    }
+    if (!chol(c, m, G, false)) {
+      d2d(c, M, bak, mn);
+      return false;
+    }
+    RBCHK(rocblas_dtrsm(c->rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                        rocblas_diagonal_non_unit, m, n, &one, G, m, M, m));
+    if (pass == 0) {
+      d2d(c, L, G, mm);
+      HIPCHK(launch_tri_keep(m, L, m, false, c->stream));
+    } else {
+      RBCHK(rocblas_dtrmm(c->rb, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, m, m, &one, G, m, L, m, L2, m));
+      d2d(c, L, L2, mm);
+    }
+  }
+  HIPCHK(launch_tri_keep(m, L, m, false, c->stream));
+  return true;
+}
+
+long g_qr_fallbacks = 0;
+
+// A (m x n, lda = m, m >= n) is overwritten by Q; R (n x n) receives the factor with a
+// non-negative diagonal (FiniteMPSAlgorithms' QRpos).
+void qrpos(imps_ctx* c, int m, int n, double* A, double* R) {
+  if (cholqr3(c, m, n, A, R)) return;
+  g_qr_fallbacks++;
+  qrpos_householder(c, m, n, A, R);
 }
 
 // M (m x n, ld m, m <= n) = L Q with L (m x m) lower, non-negative diagonal and Q (m x n)
-// row-orthonormal (LQpos).  M may alias Q.
+// row-orthonormal (LQpos).  Q may alias M.
 void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q) {
+  d2d(c, Q, M, (size_t)m * n);
+  if (cholqr3_lq(c, m, n, Q, L)) return;
+  g_qr_fallbacks++;
   double* Mt = ws(c, "lq_Mt", (size_t)m * n);
   double* R = ws(c, "lq_R", (size_t)m * m);
-  HIPCHK(launch_transpose(m, n, M, m, Mt, n, c->stream));
-  qrpos(c, n, m, Mt, R);
+  HIPCHK(launch_transpose(m, n, Q, m, Mt, n, c->stream));
+  qrpos_householder(c, n, m, Mt, R);
   HIPCHK(launch_transpose(m, m, R, m, L, m, c->stream));
   HIPCHK(launch_transpose(n, m, Mt, n, Q, m, c->stream));
 }
@@ -1534,12 +1639,23 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     auto secs = [&] {
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
-    if (!c->envs_valid) envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, nullptr);
+    const long qf0 = g_qr_fallbacks;
+    EnvStats est;
+    double tl = 0, tg = 0, te = 0, tk = 0, tf = 0;
+    int sweeps_total = 0;
+    auto lap = [&](double& acc, auto&& fn) {
+      auto a = std::chrono::steady_clock::now();
+      fn();
+      HIPCHK(hipStreamSynchronize(c->stream));
+      acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    };
+    if (!c->envs_valid) lap(te, [&] { envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, &est); });
     int prec = p->prec;
     int switch_iter = -1;
-    double eps = galerkin(c, PREC_FP64);
+    double eps = 0.0;
+    lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
     auto tolu = [&](double f, int it, double e) { return updatetol(f, it, e, p->tol_min, p->tol_max); };
-    envs_recalculate(c, tolu(p->envs_tolfactor, 0, eps), p->envs_maxiter, p->krylovdim, nullptr);
+    lap(te, [&] { envs_recalculate(c, tolu(p->envs_tolfactor, 0, eps), p->envs_maxiter, p->krylovdim, &est); });
     int nl = 0;
     if (losses && nl < nloss) losses[nl++] = eps;
     bool converged = false;
@@ -1552,17 +1668,25 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
         switch_iter = it;
       }
       const double tol_eig = tolu(p->eigs_tolfactor, siter, eps);
-      for (int site = 0; site < N; ++site) {
-        eigsolve_ac(c, site, tol_eig, p->krylovdim, p->eig_maxiter, true, prec);
-        eigsolve_c(c, site, tol_eig, p->krylovdim, p->eig_maxiter, true, prec);
-        regauge_site(c, site);
-      }
-      gauge_step(c, tolu(p->gauge_tolfactor, siter, eps), p->gauge_maxiter, nullptr);
-      envs_recalculate(c, tolu(p->envs_tolfactor, siter, eps), p->envs_maxiter, p->krylovdim, nullptr);
-      eps = galerkin(c, PREC_FP64);
+      lap(tl, [&] {
+        for (int site = 0; site < N; ++site) {
+          eigsolve_ac(c, site, tol_eig, p->krylovdim, p->eig_maxiter, true, prec);
+          eigsolve_c(c, site, tol_eig, p->krylovdim, p->eig_maxiter, true, prec);
+          regauge_site(c, site);
+        }
+      });
+      int sw = 0;
+      lap(tg, [&] { gauge_step(c, tolu(p->gauge_tolfactor, siter, eps), p->gauge_maxiter, &sw); });
+      sweeps_total += sw;
+      lap(te, [&] { envs_recalculate(c, tolu(p->envs_tolfactor, siter, eps), p->envs_maxiter, p->krylovdim, &est); });
+      lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
       if (losses && nl < nloss) losses[nl++] = eps;
-      f = energy(c);
-      if (p->verbosity > 0) printf("VUMPS iter %4d : eps = %.3e f = %.8g\n", it, eps, f);
+      lap(tf, [&] { f = energy(c); });
+      if (p->verbosity > 0) {
+        printf("VUMPS iter %4d : eps = %.3e f = %.8g  [%.2fs: local %.2f gauge %.2f (%d sweeps) envs %.2f]\n", it, eps, f,
+               secs(), tl, tg, sw, te);
+        fflush(stdout);
+      }
       if (eps <= p->tol) {
         converged = true;
         t_reach = secs();
@@ -1581,6 +1705,14 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       info->hac_applies = (int)(c->hac_applies - hac0);
       info->hc_applies = (int)(c->hc_applies - hc0);
       info->prec_switch_iter = switch_iter;
+      info->t_local = tl;
+      info->t_gauge = tg;
+      info->t_envs = te;
+      info->t_galerkin = tk;
+      info->t_energy = tf;
+      info->gauge_sweeps = sweeps_total;
+      info->gmres_ops = est.ops;
+      info->qr_fallbacks = (int)(g_qr_fallbacks - qf0);
     }
   });
 }

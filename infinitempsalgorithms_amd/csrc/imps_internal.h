@@ -63,9 +63,12 @@ hipError_t launch_transpose(int rows, int cols, const double* in, long ldin, dou
 // partial sums of sum_{a,b} l[a + n*b] * v[b + n*a]
 hipError_t launch_trace_prod_parts(int n, const double* l, const double* v, double* part,
                                    int* nparts, hipStream_t s);
-// column/row sign fix for QRpos: Q[:,i] *= s_i, R[i,:] *= s_i with s_i = sign(R[i,i])
-hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr,
+// column/row sign fix for QRpos: Q[:,i] *= s_i, R[i,:] *= s_i with s_i = sign(R[i,i]);
+// sbuf: n doubles of device scratch
+hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr, double* sbuf,
                              hipStream_t s);
+// zero the strictly lower (keep_upper) or strictly upper triangle of an n x n matrix
+hipError_t launch_tri_keep(int n, double* A, long ld, bool keep_upper, hipStream_t s);
 // copy upper triangle of the n x n leading block of A (lda) into R (n x n), zero below
 hipError_t launch_copy_upper(int n, const double* A, long lda, double* R, hipStream_t s);
 // strided 2-D copy: dst[i + j*ldd] = src[i + j*lds]

@@ -486,22 +486,39 @@ hipError_t launch_trace_prod_parts(int n, const double* l, const double* v, doub
 
 __device__ inline double sgn_nonneg(double x) { return x < 0.0 ? -1.0 : 1.0; }
 
-__global__ void qr_signfix_q_kernel(int m, int n, double* Q, long ldq, const double* R, long ldr) {
+__global__ void qr_signs_kernel(int n, const double* R, long ldr, double* sg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sg[i] = sgn_nonneg(R[i + ldr * i]);
+}
+
+__global__ void qr_signfix_q_kernel(int m, int n, double* Q, long ldq, const double* sg) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
-  if (r < m && i < n) Q[r + ldq * i] *= sgn_nonneg(R[i + ldr * i]);
+  if (r < m && i < n) Q[r + ldq * i] *= sg[i];
 }
 
-__global__ void qr_signfix_r_kernel(int n, double* R, long ldr) {
+__global__ void qr_signfix_r_kernel(int n, double* R, long ldr, const double* sg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // row
+  const int j = blockIdx.y;                             // column
+  if (i < n && i <= j) R[i + ldr * j] *= sg[i];
+}
+
+hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr, double* sbuf,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(qr_signs_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, R, ldr, sbuf);
+  if (Q) hipLaunchKernelGGL(qr_signfix_q_kernel, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, Q, ldq, sbuf);
+  hipLaunchKernelGGL(qr_signfix_r_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, R, ldr, sbuf);
+  return hipGetLastError();
+}
+
+__global__ void tri_keep_kernel(int n, double* A, long ld, bool keep_upper) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double s = sgn_nonneg(R[i + ldr * i]);
-  for (int j = i; j < n; ++j) R[i + ldr * j] *= s;
+  const int j = blockIdx.y;
+  if (i < n && ((keep_upper && i > j) || (!keep_upper && i < j))) A[i + ld * j] = 0.0;
 }
 
-hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr, hipStream_t s) {
-  if (Q) hipLaunchKernelGGL(qr_signfix_q_kernel, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, Q, ldq, R, ldr);
-  hipLaunchKernelGGL(qr_signfix_r_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, R, ldr);
+hipError_t launch_tri_keep(int n, double* A, long ld, bool keep_upper, hipStream_t s) {
+  hipLaunchKernelGGL(tri_keep_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, A, ld, keep_upper);
   return hipGetLastError();
 }
 
