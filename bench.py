@@ -1,14 +1,23 @@
-"""Benchmark: H_AC TFLOP/s at D = 1024 (JW Fermi-Hubbard, N = 2, d = 4, w = 6) on one MI355X,
-plus (with --ttg) VUMPS time-to-Galerkin < 1e-8 on the same configuration.
+"""Benchmark on one MI355X: H_AC TFLOP/s at D = 1024 for the JW Fermi-Hubbard chain
+(t = 1, U = 4, mu = 2, N = 2, d = 4, w = 6), plus VUMPS outer-loop timings on the same
+configuration.
 
 A "step" is one application of the effective Hamiltonian H_AC (effective.jl:90-102) to a
 device-resident (D, d, D) tensor with device-resident environments.  Algorithmic work per
-step: F_HAC = 4 w d D^3 flop (dense-level two-GEMM form).  Prints one JSON line.
+step: F_HAC = 4 w d D^3 flop (dense-level two-GEMM form).  `value` is F_HAC / (wall time
+per step), inputs resident in HBM, timed with a device synchronisation on both sides.
 
-The cpu_baseline leg times the FP64 CPU restatement of the same contraction (numpy /
-OpenBLAS on the host cores) on a bounded sample.
+Also reported:
+  roofline      the dominant GEMM of H_AC against the FP64 MFMA peak (HIP events on the
+                library's stream);
+  vumps         seconds per VUMPS iteration (and its phases) for a few iterations at
+                D = 1024 from a random canonical state (disable with --no-vumps);
+  time_to_galerkin  with --ttg: the full run to Galerkin < 1e-8 (long);
+  cpu_baseline  the FP64 CPU restatement of H_AC (numpy / OpenBLAS) on a bounded sample.
+Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -20,6 +29,10 @@ sys.path.insert(0, ROOT)
 METRIC = "VUMPS time-to-Galerkin<1e-8 and H_AC TFLOP/s at D=1024, mixed vs FP64 GPU"
 PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+# HBM bytes per launch of the dominant H_AC GEMM measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (FETCH_SIZE scaled by 1.995 from a copy kernel of known size); keyed by
+# (D, precision, kernel).  profiles/hac_d1024_hubbard_traffic.md holds the runs.
+TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 766.8 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 606.8 * 2 ** 20}
 
 
 def log(msg):
@@ -45,7 +58,7 @@ def cpu_baseline(D, d, w, seconds=15.0):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": 4 * w * d * D ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
             "sample": f"{n} H_AC applications of the numpy FP64 restatement (oracle/imps_oracle.py, "
-                      f"einsum -> OpenBLAS) at D={D}, d={d}, w={w}; {dt * 1e3:.1f} ms each"}
+                      f"einsum -> OpenBLAS, {cores} threads) at D={D}, d={d}, w={w}; {dt * 1e3:.1f} ms each"}
 
 
 def main():
@@ -55,16 +68,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--prec", default="fp64", choices=sorted(PREC))
     ap.add_argument("--D", type=int, default=1024)
-    ap.add_argument("--ttg", action="store_true", help="also run VUMPS to Galerkin < 1e-8")
-    ap.add_argument("--ttg-maxiter", type=int, default=400)
+    ap.add_argument("--no-vumps", action="store_true", help="skip the VUMPS iteration timing")
+    ap.add_argument("--vumps-iters", type=int, default=3)
+    ap.add_argument("--ttg", action="store_true", help="run VUMPS to Galerkin < 1e-8 (long)")
+    ap.add_argument("--ttg-maxiter", type=int, default=2000)
+    ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is "
-              "described in DESIGN.md and is not part of this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the path would shard "
+              "over several GPUs is described in DESIGN.md and is not part of this build", file=sys.stderr)
         sys.exit(2)
-
-    import ctypes
 
     import numpy as np
 
@@ -73,35 +87,46 @@ def main():
 
     D, d, w, N = args.D, 4, 6, 2
     H = imps.models.fermi_hubbard(t=1.0, U=4.0, mu=2.0, nsites=N)
+    prec = PREC[args.prec]
+    F = 4.0 * w * d * D ** 3
+    out = {"metric": METRIC}
+
+    # ---- H_AC throughput on device-resident data
     ctx = imps.Context(0)
-    log(f"bench: Hubbard N={N} d={d} w={w} D={D}: canonicalising a random state on the device")
-    t0 = time.perf_counter()
-    psi, _ = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=50)
-    log(f"bench: state ready in {time.perf_counter() - t0:.1f}s; building random device environments")
+    ctx.set_mpo(H)
     rng = np.random.default_rng(7)
+    rnd = [np.asfortranarray(rng.standard_normal((D, d, D))) for _ in range(N)]
+    rc = [np.asfortranarray(rng.standard_normal((D, D))) for _ in range(N)]
+    ctx.set_state(imps.CanonicalIMPS(rnd, rnd, rc, rnd))
     for l in range(N):
         ctx.set_env(l, 0, rng.standard_normal((D, w, D)))
         ctx.set_env(l, 1, rng.standard_normal((D, w, D)))
-    prec = PREC[args.prec]
     n = D * d * D
-    x = ctypes.c_void_p()
-    y = ctypes.c_void_p()
+    x, y = ctypes.c_void_p(), ctypes.c_void_p()
     ctx.call("imps_dev_alloc", n * 8, ctypes.byref(x))
     ctx.call("imps_dev_alloc", n * 8, ctypes.byref(y))
     xh = np.asfortranarray(rng.standard_normal(n))
     ctx.call("imps_memcpy_h2d", x, xh.ctypes.data_as(ctypes.c_void_p), n * 8)
+    if args.calib:
+        nb = 256 << 20
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        ctx.call("imps_dev_alloc", nb, ctypes.byref(a))
+        ctx.call("imps_dev_alloc", nb, ctypes.byref(b))
+        for _ in range(3):
+            ctx.call("imps_dev_copy", b, a, nb)
+        ctx.call("imps_synchronize")
+        ctx.call("imps_dev_free", a)
+        ctx.call("imps_dev_free", b)
     for _ in range(args.warmup):
         ctx.call("imps_hac_apply_dev", 0, x, y, prec)
     ctx.call("imps_synchronize")
-    log(f"bench: timing {args.steps} H_AC applications ({args.prec})")
+    log(f"bench: timing {args.steps} H_AC applications at D={D} ({args.prec})")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.call("imps_hac_apply_dev", 0, x, y, prec)
     ctx.call("imps_synchronize")
     dt = (time.perf_counter() - t0) / args.steps
-    F = 4.0 * w * d * D ** 3
     value = F / dt / 1e12
-    # per-kernel durations on the library's stream (HIP events)
     t = _lib.HacTiming()
     ctx.call("imps_bench_hac", 0, max(10, args.steps // 2), prec, ctypes.byref(t))
     g1 = 2.0 * w * d * D ** 3
@@ -113,40 +138,53 @@ def main():
     dom = max(("gemm1_GLx", "gemm2_GR"), key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["tflops"]
     peak = PEAKS[args.prec]
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+    ctx.call("imps_dev_free", x)
+    ctx.call("imps_dev_free", y)
+    ctx.close()
+    out.update({
+        "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32-on-f64", "bf16x3": "bf16x3-on-f64"}[args.prec],
-        "data": "synthetic (random canonical state, random environments)",
-        "config": {"workload": "H_AC apply, JW Fermi-Hubbard t=1 U=4 mu=2, N=2, d=4, w=6",
-                   "D": D, "flop_per_step": F, "parallelism": "single GPU"},
+        "data": "synthetic (random tensors and environments of the benchmark shape)",
+        "config": {"workload": "H_AC apply, JW Fermi-Hubbard t=1 U=4 mu=2, N=2, d=4, w=6, D=1024"
+                   if D == 1024 else f"H_AC apply, JW Fermi-Hubbard, N=2, d=4, w=6, D={D}",
+                   "D": D, "d": d, "w": w, "flop_per_step": F, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None},
+                     "frac": achieved / peak, "traffic": TRAFFIC.get((D, args.prec, dom))},
         "kernels": kernels,
-    }
-    if args.ttg:
-        log("bench: VUMPS to Galerkin < 1e-8 (FP64) from the canonicalised random state")
-        ctx.call("imps_envs_reset")
-        ctx.set_state(psi)
+    })
+
+    # ---- VUMPS outer iterations on the same configuration
+    if not args.no_vumps or args.ttg:
+        log(f"bench: canonicalising a random D={D} state (PCG64 seed 2) on the device")
+        ctx = imps.Context(0)
+        t0 = time.perf_counter()
+        psi, ctx = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=100)
+        t_setup = time.perf_counter() - t0
         envs = imps.GPUCache(psi, H, ctx)
-        _, _, info = imps.find_groundstate(psi, H, imps.VUMPS(tol=1e-8, maxiter=args.ttg_maxiter), envs=envs)
-        out["time_to_galerkin"] = {"fp64": {"seconds": info.t_galerkin_reached if info.converged else None,
-                                            "loop_seconds": info.seconds, "niter": info.niter,
-                                            "itererr": info.itererr, "energy_per_site": info.energy / N,
-                                            "hac_applies": info.hac_applies}}
+        maxit = args.ttg_maxiter if args.ttg else args.vumps_iters
+        tstart = time.perf_counter()
+
+        def cb(it, eps, f):
+            log(f"bench: VUMPS iter {it} eps {eps:.3e} e/site {f / N:.12f} t {time.perf_counter() - tstart:.1f}s")
+            return False
+
+        alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec)
+        _, _, info = imps.find_groundstate(psi, H, alg, envs=envs, callback=cb)
+        rec = {"setup_seconds": t_setup, "iterations": info.niter, "loop_seconds": info.seconds,
+               "seconds_per_iteration": info.seconds / max(info.niter, 1), "itererr": info.itererr,
+               "energy_per_site": info.energy / N, "hac_applies": info.hac_applies,
+               "hc_applies": info.hc_applies, "phases_seconds": info.timings}
+        if args.ttg:
+            out["time_to_galerkin"] = {args.prec: {**rec, "seconds": info.t_galerkin_reached if info.converged else None,
+                                                   "converged": info.converged}}
+        else:
+            out["vumps"] = rec
+        ctx.close()
+
     if not args.no_cpu:
         log("bench: CPU baseline (numpy restatement)")
         out["cpu_baseline"] = cpu_baseline(D, d, w)
-    ctx.call("imps_dev_free", x)
-    ctx.call("imps_dev_free", y)
     print(json.dumps(out), flush=True)
 
 

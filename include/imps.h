@@ -240,6 +240,9 @@ int imps_bench_hac(imps_ctx* ctx, int site, int reps, int prec, imps_hac_timing*
 int imps_gemm_dev(imps_ctx* ctx, char transa, char transb, int m, int n, int k, double alpha,
                   const double* d_a, int lda, const double* d_b, int ldb, double beta, double* d_c,
                   int ldc, int prec);
+/* Device-to-device copy of `bytes` bytes with the library's copy kernel on the
+ * context's stream (also the known-size reference for HBM traffic counters). */
+int imps_dev_copy(imps_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 /* Device memory helpers for callers without their own allocator. */
 int imps_dev_alloc(imps_ctx* ctx, size_t bytes, void** d_ptr);
 int imps_dev_free(imps_ctx* ctx, void* d_ptr);

@@ -89,6 +89,7 @@ SIGNATURES = [
     ("imps_bench_hac", c_int, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(HacTiming)]),
     ("imps_gemm_dev", c_int, [c_void_p, c_char, c_char, c_int, c_int, c_int, c_double, c_void_p, c_int,
                               c_void_p, c_int, c_double, c_void_p, c_int, c_int]),
+    ("imps_dev_copy", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     ("imps_dev_alloc", c_int, [c_void_p, c_size_t, ctypes.POINTER(c_void_p)]),
     ("imps_dev_free", c_int, [c_void_p, c_void_p]),
     ("imps_memcpy_h2d", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),

@@ -1774,6 +1774,20 @@ int imps_gemm_dev(imps_ctx* c, char transa, char transb, int m, int n, int k, do
   });
 }
 
+int imps_dev_copy(imps_ctx* c, void* d_dst, const void* d_src, size_t bytes) {
+  if (!c || (bytes % sizeof(double)) != 0) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    const size_t n = bytes / sizeof(double);
+    const int rows = 1 << 20;
+    const size_t cols = (n + rows - 1) / rows;
+    for (size_t j = 0; j < cols; ++j) {
+      const size_t off = j * (size_t)rows;
+      const int r = (int)std::min<size_t>(rows, n - off);
+      HIPCHK(launch_copy2d(r, 1, (const double*)d_src + off, r, (double*)d_dst + off, r, c->stream));
+    }
+  });
+}
+
 int imps_dev_alloc(imps_ctx* c, size_t bytes, void** d_ptr) {
   if (!c || !d_ptr) return IMPS_E_ARG;
   return run_guarded(c, [&] { HIPCHK(hipMalloc(d_ptr, std::max<size_t>(bytes, 1))); });
