@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--vumps-iters", type=int, default=3)
     ap.add_argument("--ttg", action="store_true", help="run VUMPS to Galerkin < 1e-8 (long)")
     ap.add_argument("--ttg-maxiter", type=int, default=2000)
+    ap.add_argument("--ttg-seconds", type=float, default=1e9, help="stop the --ttg run after this wall time")
+    ap.add_argument("--refine-below", type=float, default=1e-6, help="mixed precision: FP64 below this Galerkin error")
     ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -165,19 +167,30 @@ def main():
         maxit = args.ttg_maxiter if args.ttg else args.vumps_iters
         tstart = time.perf_counter()
 
-        def cb(it, eps, f):
-            log(f"bench: VUMPS iter {it} eps {eps:.3e} e/site {f / N:.12f} t {time.perf_counter() - tstart:.1f}s")
-            return False
+        trace = []
 
-        alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec)
+        def cb(it, eps, f):
+            el = time.perf_counter() - tstart
+            trace.append((it, eps, el))
+            log(f"bench: VUMPS iter {it} eps {eps:.3e} e/site {f / N:.12f} t {el:.1f}s")
+            return args.ttg and el > args.ttg_seconds
+
+        alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec, refine_below=args.refine_below)
         _, _, info = imps.find_groundstate(psi, H, alg, envs=envs, callback=cb)
         rec = {"setup_seconds": t_setup, "iterations": info.niter, "loop_seconds": info.seconds,
                "seconds_per_iteration": info.seconds / max(info.niter, 1), "itererr": info.itererr,
                "energy_per_site": info.energy / N, "hac_applies": info.hac_applies,
                "hc_applies": info.hc_applies, "phases_seconds": info.timings}
         if args.ttg:
+            marks = {}
+            for it, eps, el in trace:
+                for thr in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8):
+                    if eps <= thr and thr not in marks:
+                        marks[thr] = {"iter": it, "seconds": el}
             out["time_to_galerkin"] = {args.prec: {**rec, "seconds": info.t_galerkin_reached if info.converged else None,
-                                                   "converged": info.converged}}
+                                                   "converged": info.converged,
+                                                   "prec_switch_iter": info.prec_switch_iter,
+                                                   "first_reached": {f"{k:.0e}": v for k, v in marks.items()}}}
         else:
             out["vumps"] = rec
         ctx.close()

@@ -189,7 +189,10 @@ double trace_prod(imps_ctx* c, int n, const double* l, const double* v) {
 
 // ------------------------------------------------------------------ GEMM wrappers
 
-void gemm(imps_ctx* c, const GemmDesc& g, int prec) {
+void gemm(imps_ctx* c, const GemmDesc& g0, int prec) {
+  GemmDesc g = g0;
+  const size_t nw = gemm_workspace(g.M, g.N, g.K);
+  if (nw > 0 && g.W == nullptr) g.W = ws(c, "gemm_splitk", nw);
   HIPCHK(launch_gemm(g, prec, c->stream));
 }
 
@@ -1652,6 +1655,9 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     if (!c->envs_valid) lap(te, [&] { envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, &est); });
     int prec = p->prec;
     int switch_iter = -1;
+    std::vector<double> lam_ac(N, 1.0), lam_c(N, 1.0);
+    double best_eps = std::numeric_limits<double>::infinity();
+    int since_best = 0;
     double eps = 0.0;
     lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
     auto tolu = [&](double f, int it, double e) { return updatetol(f, it, e, p->tol_min, p->tol_max); };
@@ -1663,15 +1669,22 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     int it = 0;
     for (it = 1; it <= p->maxiter; ++it) {
       const int siter = it - 1;
-      if (prec != PREC_FP64 && eps < p->refine_below) {
+      // FP64 refinement: below refine_below, or once the reduced-precision phase has
+      // stopped improving (no new best Galerkin error for 10 iterations)
+      if (prec != PREC_FP64 && (eps < p->refine_below || since_best >= 10)) {
         prec = PREC_FP64;
         switch_iter = it;
       }
-      const double tol_eig = tolu(p->eigs_tolfactor, siter, eps);
+      double tol_eig = tolu(p->eigs_tolfactor, siter, eps);
+      // a reduced-precision operator cannot resolve residuals below its own rounding
+      // level: the eigensolver tolerance is floored there until the FP64 refinement
+      const double floor_rel = prec == PREC_FP32 ? 1e-6 : (prec == PREC_BF16X3 ? 1e-4 : 0.0);
       lap(tl, [&] {
         for (int site = 0; site < N; ++site) {
-          eigsolve_ac(c, site, tol_eig, p->krylovdim, p->eig_maxiter, true, prec);
-          eigsolve_c(c, site, tol_eig, p->krylovdim, p->eig_maxiter, true, prec);
+          const double ta = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_ac[site])));
+          lam_ac[site] = eigsolve_ac(c, site, ta, p->krylovdim, p->eig_maxiter, true, prec).lambda;
+          const double tc = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_c[site])));
+          lam_c[site] = eigsolve_c(c, site, tc, p->krylovdim, p->eig_maxiter, true, prec).lambda;
           regauge_site(c, site);
         }
       });
@@ -1680,6 +1693,12 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       sweeps_total += sw;
       lap(te, [&] { envs_recalculate(c, tolu(p->envs_tolfactor, siter, eps), p->envs_maxiter, p->krylovdim, &est); });
       lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
+      if (eps < best_eps) {
+        best_eps = eps;
+        since_best = 0;
+      } else {
+        since_best++;
+      }
       if (losses && nl < nloss) losses[nl++] = eps;
       lap(tf, [&] { f = energy(c); });
       if (p->verbosity > 0) {

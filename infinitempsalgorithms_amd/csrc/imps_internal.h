@@ -25,7 +25,11 @@ struct GemmDesc {
   double* C = nullptr;
   long ldc = 0;
   double alpha = 1.0, beta = 0.0;
+  // optional split-K workspace of gemm_workspace(M, N, K) doubles; nullptr disables split-K
+  double* W = nullptr;
 };
+
+size_t gemm_workspace(int M, int N, int K);
 
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
 

@@ -12,24 +12,51 @@ namespace imps {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------------
-// FP64 GEMM on MFMA (v_mfma_f64_16x16x4_f64).
+// GEMM on MFMA, FP64 data in HBM.
+//
+// TC = double: v_mfma_f64_16x16x4_f64 (FP64 products and accumulation).
+// TC = float : operands rounded to FP32 when staged into LDS, v_mfma_f32_16x16x4_f32
+//              (FP32 products and accumulation), result widened to FP64 in the epilogue.
 //
 // Block = 4 wavefronts (2 x 2), macro tile BM x BN, K step BK.  Operands are staged
 // global -> registers -> LDS (k-major rows, one padding element per row); the next
 // K tile is fetched into registers while the MFMAs consume the current one.
 // The MFMA is issued with the B fragment as its row operand so that the 16 lanes of
 // an accumulator row hold 16 consecutive m of the column-major C (coalesced stores).
+// Accumulator layouts (measured): f64 16x16x4 register r of lane l holds row l/16 + 4r;
+// f32 16x16x4 holds row 4(l/16) + r; both hold column l % 16.
 // ---------------------------------------------------------------------------------
 
-template <int BM, int BN, int BK, bool AMC, bool BKC>
-__global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
+template <typename TC>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lk, int r) { return lk + 4 * r; }
+};
+template <>
+struct MfmaTraits<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lk, int r) { return 4 * lk + r; }
+};
+
+template <typename TC, int BM, int BN, int BK, bool AMC, bool BKC>
+__global__ void __launch_bounds__(256) gemm_kernel(GemmDesc g) {
+  typedef MfmaTraits<TC> MT;
+  typedef typename MT::v4 v4;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   constexpr int AE = BM * BK / 256, BE = BK * BN / 256;
   static_assert(AE * 256 == BM * BK && BE * 256 == BK * BN, "tile/thread mismatch");
 
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
+  __shared__ TC As[BK][BM + 1];
+  __shared__ TC Bs[BK][BN + 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -39,12 +66,17 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
   const int wm = (wave & 1) * WM;
   const int wn = (wave >> 1) * WN;
   const int M = g.M, N = g.N, K = g.K, Kc = g.Kc;
+  // split-K: blockIdx.z owns K tiles [kt0, kt1)
+  const int ntiles_all = (K + BK - 1) / BK;
+  const int per = (ntiles_all + gridDim.z - 1) / gridDim.z;
+  const int kt0 = blockIdx.z * per;
+  const int kt1 = min(ntiles_all, kt0 + per);
 
-  v4d acc[TM][TN];
+  v4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < TN; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
   double ra[AE], rb[BE];
 
@@ -95,7 +127,7 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
         k = idx % BK;
         m = idx / BK;
       }
-      As[k][m] = ra[e];
+      As[k][m] = (TC)ra[e];
     }
 #pragma unroll
     for (int e = 0; e < BE; ++e) {
@@ -108,22 +140,21 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
         n = idx % BN;
         k = idx / BN;
       }
-      Bs[k][n] = rb[e];
+      Bs[k][n] = (TC)rb[e];
     }
   };
 
-  const int ntiles = (K + BK - 1) / BK;
-  if (ntiles > 0) load_tiles(0);
+  if (kt0 < kt1) load_tiles(kt0 * BK);
   const int lr = lane & 15;
   const int lk = lane >> 4;
-  for (int kt = 0; kt < ntiles; ++kt) {
+  for (int kt = kt0; kt < kt1; ++kt) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (kt + 1 < ntiles) load_tiles((kt + 1) * BK);
+    if (kt + 1 < kt1) load_tiles((kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double af[TM], bf[TN];
+      TC af[TM], bf[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) af[i] = As[kk + lk][wm + i * 16 + lr];
 #pragma unroll
@@ -131,14 +162,21 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TN; ++j) acc[i][j] = MT::mma(bf[j], af[i], acc[i][j]);
     }
   }
 
-  // v_mfma_f64_16x16x4_f64 accumulator layout: register r of lane l holds row
-  // (l / 16) + 4 r, column l % 16; rows are n, columns m here
-  const double alpha = g.alpha, beta = g.beta;
+  // split-K partial results go to the workspace slice of this z; the reduction kernel
+  // applies alpha and beta
+  double* Cout = g.C;
+  long ldc = g.ldc;
+  double alpha = g.alpha, beta = g.beta;
+  if (gridDim.z > 1) {
+    Cout = g.W + (size_t)blockIdx.z * M * N;
+    ldc = M;
+    alpha = 1.0;
+    beta = 0.0;
+  }
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
     const int m = m0 + wm + i * 16 + lr;
@@ -147,10 +185,10 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
     for (int j = 0; j < TN; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = n0 + wn + j * 16 + lk + 4 * r;
+        const int n = n0 + wn + j * 16 + MT::row(lk, r);
         if (n < N) {
-          double* p = g.C + m + (long)n * g.ldc;
-          const double v = alpha * acc[i][j][r];
+          double* p = Cout + m + (long)n * ldc;
+          const double v = alpha * (double)acc[i][j][r];
           *p = (beta == 0.0) ? v : v + beta * *p;
         }
       }
@@ -158,23 +196,62 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmDesc g) {
   }
 }
 
-template <int BM, int BN, int BK>
-static hipError_t gemm_dispatch_tile(const GemmDesc& g, bool amc, bool bkc, hipStream_t s) {
-  dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN);
+// C = alpha * sum_z W[z] + beta * C
+__global__ void splitk_reduce_kernel(int M, int N, int nz, const double* __restrict__ W, double alpha,
+                                     double beta, double* C, long ldc) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
+  if (m >= M) return;
+  double s = 0.0;
+  for (int z = 0; z < nz; ++z) s += W[(size_t)z * M * N + m + (size_t)M * n];
+  double* p = C + m + ldc * n;
+  *p = (beta == 0.0) ? alpha * s : alpha * s + beta * *p;
+}
+
+template <typename TC, int BM, int BN, int BK>
+static hipError_t gemm_dispatch_tile(const GemmDesc& g, bool amc, bool bkc, int splitk, hipStream_t s) {
+  dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, splitk);
   dim3 block(256);
   if (amc && bkc)
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, true, true>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, true>), grid, block, 0, s, g);
   else if (amc && !bkc)
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, true, false>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, false>), grid, block, 0, s, g);
   else if (!amc && bkc)
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, false, true>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, true>), grid, block, 0, s, g);
   else
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, false, false>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, false>), grid, block, 0, s, g);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || splitk == 1) return e;
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3((g.M + 255) / 256, g.N), dim3(256), 0, s, g.M, g.N, splitk,
+                     g.W, g.alpha, g.beta, g.C, g.ldc);
   return hipGetLastError();
 }
 
+size_t gemm_workspace(int M, int N, int K) {
+  // doubles needed by the split-K choice of gemm_select (0: no split-K for this shape)
+  (void)K;
+  const long t128 = (long)((M + 127) / 128) * ((N + 127) / 128);
+  const long t64 = (long)((M + 63) / 64) * ((N + 63) / 64);
+  if (t128 >= 192 || t64 >= 512) return 0;
+  return (size_t)8 * M * N;
+}
+
+template <typename TC>
+static hipError_t gemm_select(const GemmDesc& g, bool amc, bool bkc, hipStream_t s) {
+  const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
+  if (t128 >= 192) return gemm_dispatch_tile<TC, 128, 128, 16>(g, amc, bkc, 1, s);
+  const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
+  int splitk = 1;
+  if (g.W != nullptr && t64 < 512) {
+    // long-K shapes: split K so that about 1024 workgroups run; each split keeps >= 256 K
+    const int ktiles = (g.K + 15) / 16;
+    splitk = (int)std::min<long>(8, std::max<long>(1, 1024 / std::max<long>(t64, 1)));
+    splitk = std::min(splitk, std::max(1, ktiles / 16));
+  }
+  return gemm_dispatch_tile<TC, 64, 64, 16>(g, amc, bkc, splitk, s);
+}
+
 hipError_t launch_gemm(const GemmDesc& g0, int prec, hipStream_t s) {
-  (void)prec;
   GemmDesc g = g0;
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (g.Kc <= 0 || g.Kc >= g.K) g.Kc = std::max(g.K, 1);
@@ -183,9 +260,9 @@ hipError_t launch_gemm(const GemmDesc& g0, int prec, hipStream_t s) {
   const bool bkc = (g.sBk == 1);
   if (!amc && g.sAk != 1) return hipErrorInvalidValue;
   if (!bkc && g.sBn != 1) return hipErrorInvalidValue;
-  const long big = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
-  if (big >= 192) return gemm_dispatch_tile<128, 128, 16>(g, amc, bkc, s);
-  return gemm_dispatch_tile<64, 64, 16>(g, amc, bkc, s);
+  if (prec == PREC_FP32) return gemm_select<float>(g, amc, bkc, s);
+  if (prec == PREC_FP64) return gemm_select<double>(g, amc, bkc, s);
+  return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------------

@@ -65,6 +65,23 @@ def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
     assert rel(got, ref) < 1e-13
 
 
+@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 130, 77, 45), ("T", "N", 1024, 1024, 4096), ("N", "T", 640, 512, 96)])
+def test_gemm_fp32_mfma(ctx, ta, tb, m, n, k):
+    # FP32 products/accumulation on FP64 data: error ~ 2^-24 sqrt(k) relative
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    ref = (A if ta == "N" else A.T) @ (B if tb == "N" else B.T)
+    dA, dB, dC = DevArray(ctx, A), DevArray(ctx, B), DevArray(ctx, np.zeros((m, n)))
+    ctx.call("imps_gemm_dev", ta.encode(), tb.encode(), m, n, k, 1.0, dA.p, A.shape[0], dB.p, B.shape[0],
+             0.0, dC.p, m, 1)
+    got = dC.get()
+    for d in (dA, dB, dC):
+        d.free()
+    err = rel(got, ref)
+    assert 1e-9 < err < 2e-6  # genuinely FP32, and as accurate as FP32 allows
+
+
 def _setup_envs(imps, ctx, oracle, Wfn, D, N, seed, sym=False):
     rng = np.random.default_rng(seed)
     W = Wfn()
@@ -222,3 +239,32 @@ def test_vumps_tfim_matches_oracle(imps, ctx, oracle, N, D, seed):
     assert abs(info.niter - rinfo.niter) <= max(3, rinfo.niter // 10)
     for a, b in zip(info.losses[:5], rinfo.losses[:5]):
         assert abs(a - b) <= 1e-6 * max(b, 1e-12) + 1e-9
+
+
+def test_hac_fp32_against_fp64(imps, ctx, oracle):
+    D = 128
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 2, seed=9)
+    envs = imps.GPUCache.__new__(imps.GPUCache)
+    envs.ctx, envs.ket, envs.operator = ctx, None, H
+    x = np.random.default_rng(5).standard_normal((D, 4, D))
+    y64 = oracle.hac_apply(GLs[0], H.Ws[0], GRs[0], x)
+    y32 = imps.AC_hamiltonian(0, None, H, None, envs, prec=1)(x)
+    assert 1e-9 < rel(y32, y64) < 5e-6
+
+
+def test_vumps_mixed_precision_refines_to_fp64(imps, ctx, oracle):
+    # FP32 bulk contractions, then FP64 refinement: same converged energy as the FP64 run
+    W = oracle.tfim_mpo()
+    N, D = 2, 12
+    H = imps.SparseIMPO([W] * N)
+    psi0 = oracle.randomimps(D, [2] * N, seed=0)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+    envs = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, i64 = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-10), envs=envs)
+    envs = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, imx = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-10, prec=1, refine_below=1e-6), envs=envs)
+    assert imx.converged and imx.itererr <= 1e-10
+    assert imx.prec_switch_iter > 1
+    assert abs(imx.energy - i64.energy) < 1e-9
