@@ -222,6 +222,29 @@ int imps_vumps_default_params(imps_vumps_params* p);
 int imps_vumps_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* info,
                    double* losses, int nloss, imps_iter_cb cb, void* user);
 
+/* ---- batched sweeps ------------------------------------------------------------------ */
+
+/* One instance of a parameter sweep: its MPO, raw ψ₀ tensors (canonicalised on the device
+ * like CanonicalIMPS(As)) and the results. */
+typedef struct imps_batch_instance {
+  int nsites, w, d, D;
+  const double* const* W; /* nsites tensors (w, d, w, d) */
+  const double* const* A; /* nsites raw tensors (D, d, D) */
+  /* outputs */
+  int status; /* IMPS_OK or the failing call's code */
+  int niter;
+  int converged;
+  double itererr;
+  double energy; /* unit-cell energy */
+  double seconds;
+} imps_batch_instance;
+
+/* Runs find_groundstate (vumps.jl:51-86) on every instance with the parameters `p`,
+ * `nworkers` host threads each owning a context (its own HIP stream) on `device`, so the
+ * instances' kernels overlap on the GPU.  Instances are taken largest D^3 first. */
+int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imps_vumps_params* p,
+                     int nworkers, double gauge_tol);
+
 /* ---- measurement helpers ----------------------------------------------------------- */
 
 /* Runs `reps` H_AC applications on device-resident random data of the current

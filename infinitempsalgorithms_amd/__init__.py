@@ -8,7 +8,8 @@ host-side mirror of the reference's API (``find_groundstate``, ``VUMPS``,
 from . import _lib
 from .api import (VUMPS, AC_hamiltonian, C_hamiltonian, CanonicalIMPS, Context, DMRGCache,
                   GPUCache, IterativeConvergenceInfo, Lanczos, SparseIMPO, TransferMatrix,
-                  calc_galerkin, canonicalimps, expectationvalue, find_groundstate, fixedpoint,
+                  calc_galerkin, canonicalimps, expectationvalue, find_groundstate,
+                  find_groundstates_batched, fixedpoint,
                   leftenv, random_tensors, randomimps, recalculate_, regauge_, regularize_,
                   rightenv)
 from . import models
@@ -16,6 +17,6 @@ from . import models
 __all__ = [
     "VUMPS", "AC_hamiltonian", "C_hamiltonian", "CanonicalIMPS", "Context", "DMRGCache", "GPUCache",
     "IterativeConvergenceInfo", "Lanczos", "SparseIMPO", "TransferMatrix", "calc_galerkin",
-    "canonicalimps", "expectationvalue", "find_groundstate", "fixedpoint", "leftenv", "random_tensors",
+    "canonicalimps", "expectationvalue", "find_groundstate", "find_groundstates_batched", "fixedpoint", "leftenv", "random_tensors",
     "randomimps", "recalculate_", "regauge_", "regularize_", "rightenv", "models",
 ]

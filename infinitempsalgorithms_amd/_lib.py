@@ -53,6 +53,12 @@ class HacTiming(ctypes.Structure):
                 ("ms_gemm2", c_double)]
 
 
+class BatchInstance(ctypes.Structure):
+    _fields_ = [("nsites", c_int), ("w", c_int), ("d", c_int), ("D", c_int), ("W", PP_double), ("A", PP_double),
+                ("status", c_int), ("niter", c_int), ("converged", c_int), ("itererr", c_double),
+                ("energy", c_double), ("seconds", c_double)]
+
+
 ITER_CB = ctypes.CFUNCTYPE(c_int, c_int, c_double, c_double, c_void_p)
 
 # (name, restype, argtypes) of every symbol declared in include/imps.h
@@ -86,6 +92,8 @@ SIGNATURES = [
     ("imps_vumps_default_params", c_int, [ctypes.POINTER(VumpsParams)]),
     ("imps_vumps_run", c_int, [c_void_p, ctypes.POINTER(VumpsParams), ctypes.POINTER(VumpsInfo), P_double,
                                c_int, ITER_CB, c_void_p]),
+    ("imps_vumps_batch", c_int, [c_int, c_int, ctypes.POINTER(BatchInstance), ctypes.POINTER(VumpsParams), c_int,
+                                 c_double]),
     ("imps_bench_hac", c_int, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(HacTiming)]),
     ("imps_gemm_dev", c_int, [c_void_p, c_char, c_char, c_int, c_int, c_int, c_double, c_void_p, c_int,
                               c_void_p, c_int, c_double, c_void_p, c_int, c_int]),

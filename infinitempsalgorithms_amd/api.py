@@ -185,8 +185,7 @@ class GPUCache:
         self.ctx = ctx or Context()
         self.operator = H
         self.ket = psi
-        if self.ctx.N != len(H) or self.ctx.w != H.w:
-            self.ctx.set_mpo(H)
+        self.ctx.set_mpo(H)
         self.ctx.set_state(psi)
 
     def __len__(self):
@@ -453,3 +452,29 @@ def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | Non
                                                                   "t_energy", "gauge_sweeps", "gmres_ops",
                                                                   "qr_fallbacks")})
     return psi, envs, res
+
+
+def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, device=0, gauge_tol=1e-12):
+    """Batched parameter sweep: find_groundstate (vumps.jl:51-86) for every
+    (H::SparseIMPO, raw tensors As) in `problems`, the instances running concurrently on
+    the GPU (one context and stream per worker thread).  Returns one dict per problem
+    with niter, converged, itererr, energy (unit cell) and seconds."""
+    alg = alg or VUMPS()
+    L = _lib.load()
+    keep = []
+    arr = (_lib.BatchInstance * len(problems))()
+    for k, (H, As) in enumerate(problems):
+        Ws = [fortran(W) for W in H.Ws]
+        As = [fortran(a) for a in As]
+        pw, kw = ptr_array(Ws)
+        pa, ka = ptr_array(As)
+        keep.append((Ws, As, kw, ka))
+        arr[k] = _lib.BatchInstance(len(Ws), H.w, H.d, As[0].shape[0], pw, pa, 0, 0, 0, 0.0, 0.0, 0.0)
+    p = alg.params()
+    st = L.imps_vumps_batch(int(device), len(problems), arr, ctypes.byref(p), int(nworkers), float(gauge_tol))
+    out = [{"status": r.status, "niter": r.niter, "converged": bool(r.converged), "itererr": r.itererr,
+            "energy": r.energy, "seconds": r.seconds} for r in arr]
+    if st != 0:
+        bad = [k for k, r in enumerate(out) if r["status"] != 0]
+        raise _lib.ImpsError(f"imps_vumps_batch failed with status {st} on instances {bad}")
+    return out

@@ -9,6 +9,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "imps.h"
@@ -163,16 +165,27 @@ void axpby(imps_ctx* c, size_t n, double a, const double* x, double b, double* y
 
 void fill(imps_ctx* c, size_t n, double v, double* x) { HIPCHK(launch_fill(n, v, x, c->stream)); }
 
-// classical Gram-Schmidt applied twice; returns the coefficients in h[0..nv)
-void cgs2(imps_ctx* c, size_t n, int nv, const double* V, double* w, double* h) {
-  std::vector<double> h1(nv), h2(nv), m(nv);
-  multidot(c, n, nv, V, n, w, h1.data());
-  for (int i = 0; i < nv; ++i) m[i] = -h1[i];
-  multiaxpy(c, n, nv, V, n, m.data(), 1.0, w);
-  multidot(c, n, nv, V, n, w, h2.data());
-  for (int i = 0; i < nv; ++i) m[i] = -h2[i];
-  multiaxpy(c, n, nv, V, n, m.data(), 1.0, w);
-  for (int i = 0; i < nv; ++i) h[i] = h1[i] + h2[i];
+// Classical Gram-Schmidt (two passes) of w against V[0..nv) followed by the
+// normalisation of w, with the coefficients reduced and applied on the device: one host
+// synchronisation.  h[0..nv) receives the summed coefficients and *beta the norm of w
+// before normalisation (w is left as is when that norm is zero).
+void cgs2_normalize(imps_ctx* c, size_t n, int nv, const double* V, double* w, double* h, double* beta) {
+  require(nv <= kMaxVec, IMPS_E_ARG, "cgs2: too many vectors");
+  double* coef = ws(c, "ks_coef", 2 * kMaxVec + 2);
+  const int nb = dot_blocks(n);
+  HIPCHK(launch_multidot(n, nv, V, n, w, c->dpart, c->stream));
+  HIPCHK(launch_partials_reduce(nv, nb, c->dpart, coef, c->stream));
+  HIPCHK(launch_multiaxpy_dev(n, nv, V, n, coef, -1.0, 1.0, w, c->stream));
+  HIPCHK(launch_multidot(n, nv, V, n, w, c->dpart, c->stream));
+  HIPCHK(launch_partials_reduce(nv, nb, c->dpart, coef + kMaxVec, c->stream));
+  HIPCHK(launch_multiaxpy_dev(n, nv, V, n, coef + kMaxVec, -1.0, 1.0, w, c->stream));
+  HIPCHK(launch_multidot(n, 1, w, n, w, c->dpart, c->stream));
+  HIPCHK(launch_partials_reduce(1, nb, c->dpart, coef + 2 * kMaxVec, c->stream));
+  HIPCHK(launch_scale_inv_sqrt(n, w, coef + 2 * kMaxVec, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hpart, coef, sizeof(double) * (2 * kMaxVec + 1), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nv; ++i) h[i] = c->hpart[i] + c->hpart[kMaxVec + i];
+  *beta = std::sqrt(std::max(0.0, c->hpart[2 * kMaxVec]));
 }
 
 // tr(l v) = sum l[a + ldl*b] v[b + ldv*a] for n x n (contiguous l and v)
@@ -292,7 +305,7 @@ void hc_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
   const double* GR = c->GR[site];
   double* T1 = ws(c, "hc_T1", (size_t)D * w * D);
   gemm2(c, 'N', 'N', D * w, D, D, 1.0, GL, (long)D * w, x, D, 0.0, T1, (long)D * w, prec);
-  if (D % 16 == 0) {
+  if (D % 32 == 0) {
     GemmDesc g;
     g.M = D;
     g.N = D;
@@ -441,13 +454,11 @@ double lanczos(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xo
   auto expand = [&](int k) {
     op(Vc(k), Vc(k + 1));
     numops++;
-    cgs2(c, n, k + 1, V, Vc(k + 1), h.data());
-    beta = nrm2(c, n, Vc(k + 1));
+    cgs2_normalize(c, n, k + 1, V, Vc(k + 1), h.data(), &beta);
     for (int i = 0; i <= k; ++i) {
       T[i + (size_t)ldt * k] = h[i];
       T[k + (size_t)ldt * i] = h[i];
     }
-    if (beta > 0) scal(c, n, 1.0 / beta, Vc(k + 1));
   };
   expand(0);
   K = 1;
@@ -530,12 +541,11 @@ double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, d
     while (true) {
       op(Vc(K), Vc(K + 1));
       numops++;
-      cgs2(c, n, K + 1, V, Vc(K + 1), h.data());
-      const double beta = nrm2(c, n, Vc(K + 1));
+      double beta = 0.0;
+      cgs2_normalize(c, n, K + 1, V, Vc(K + 1), h.data(), &beta);
       for (int i = 0; i <= K; ++i) H[i + (size_t)ldh * K] = h[i];
       H[(K + 1) + (size_t)ldh * K] = beta;
       K++;
-      if (beta > 0) scal(c, n, 1.0 / beta, Vc(K));
       if (K == kmax || beta <= tol || eager) {
         hess_dominant(K, H.data(), ldh, &lam, u.data());
         res = std::fabs(beta * u[K - 1]);
@@ -596,11 +606,10 @@ void gmres(imps_ctx* c, const Op& op, size_t n, const double* b, double* x, doub
     while (true) {
       applyA(Vc(k), Vc(k + 1));
       numops++;
-      cgs2(c, n, k + 1, V, Vc(k + 1), h.data());
-      const double hn = nrm2(c, n, Vc(k + 1));
+      double hn = 0.0;
+      cgs2_normalize(c, n, k + 1, V, Vc(k + 1), h.data(), &hn);
       for (int i = 0; i <= k; ++i) H[i + (size_t)ldh * k] = h[i];
       H[(k + 1) + (size_t)ldh * k] = hn;
-      if (hn > 0) scal(c, n, 1.0 / hn, Vc(k + 1));
       for (int i = 0; i < k; ++i) {
         const double a = H[i + (size_t)ldh * k], bb = H[(i + 1) + (size_t)ldh * k];
         H[i + (size_t)ldh * k] = cs[i] * a + sn[i] * bb;
@@ -648,12 +657,32 @@ void qrpos_householder(imps_ctx* c, int m, int n, double* A, double* R) {
 }
 
 // Cholesky factor of the n x n G in place (upper: G = R^T R, lower: G = L L^T); false on
-// breakdown.
+// breakdown.  Blocked right-looking: 64 x 64 diagonal blocks factored in LDS, panel solves,
+// trailing updates on the MFMA GEMM.  The lower factor is always computed; for `upper`
+// it is mirrored into the upper triangle (R = L^T).  The other triangle is left holding
+// scratch values, as with LAPACK's potrf.
 bool chol(imps_ctx* c, int n, double* G, bool upper) {
-  int* info = (int*)ws(c, "chol_info", 1);
-  RBCHK(rocsolver_dpotrf(c->rb, upper ? rocblas_fill_upper : rocblas_fill_lower, n, G, n, info));
+  int* fail = (int*)ws(c, "chol_info", 1);
+  double* Linv = ws(c, "chol_linv", 64 * 64);
+  double* X = ws(c, "chol_panel", (size_t)n * 64);
+  HIPCHK(launch_fill_int(1, 0, fail, c->stream));
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int nb = std::min(64, n - j0);
+    double* Djj = G + j0 + (size_t)n * j0;
+    HIPCHK(launch_chol_diag(nb, Djj, n, Linv, fail, c->stream));
+    const int rows = n - j0 - nb;
+    if (rows > 0) {
+      // panel: X = P L^{-T} (MFMA GEMM with the inverted diagonal block), written back to P,
+      // then the trailing update G22 -= X X^T
+      double* P = G + (j0 + nb) + (size_t)n * j0;
+      gemm2(c, 'N', 'T', rows, nb, nb, 1.0, P, n, Linv, 64, 0.0, X, rows);
+      HIPCHK(launch_copy2d(rows, nb, X, rows, P, n, c->stream));
+      gemm2(c, 'N', 'T', rows, rows, nb, -1.0, X, rows, X, rows, 1.0, G + (j0 + nb) + (size_t)n * (j0 + nb), n);
+    }
+  }
+  if (upper) HIPCHK(launch_mirror_lower(n, G, n, c->stream));
   int h = 0;
-  HIPCHK(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&h, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return h == 0;
 }
@@ -740,7 +769,7 @@ bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L) {
   return true;
 }
 
-long g_qr_fallbacks = 0;
+std::atomic<long> g_qr_fallbacks(0);
 
 // A (m x n, lda = m, m >= n) is overwritten by Q; R (n x n) receives the factor with a
 // non-negative diagonal (FiniteMPSAlgorithms' QRpos).
@@ -1734,6 +1763,49 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       info->qr_fallbacks = (int)(g_qr_fallbacks - qf0);
     }
   });
+}
+
+int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imps_vumps_params* p, int nworkers,
+                     double gauge_tol) {
+  if (ninst < 0 || (ninst > 0 && !inst) || !p || nworkers < 1) return IMPS_E_ARG;
+  std::vector<int> order(ninst);
+  for (int i = 0; i < ninst; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (double)inst[a].D * inst[a].D * inst[a].D > (double)inst[b].D * inst[b].D * inst[b].D;
+  });
+  std::atomic<int> next(0);
+  std::atomic<int> worst(IMPS_OK);
+  auto worker = [&]() {
+    imps_ctx* c = nullptr;
+    const int st0 = imps_create(device, &c);
+    if (st0 != IMPS_OK) {
+      worst = st0;
+      return;
+    }
+    while (true) {
+      const int k = next.fetch_add(1);
+      if (k >= ninst) break;
+      imps_batch_instance& in = inst[order[k]];
+      auto t0 = std::chrono::steady_clock::now();
+      imps_vumps_info info{};
+      int st = imps_set_mpo(c, in.nsites, in.w, in.d, in.W);
+      if (st == IMPS_OK) st = imps_set_state_raw(c, in.nsites, in.D, in.A, gauge_tol, p->gauge_maxiter);
+      if (st == IMPS_OK) st = imps_vumps_run(c, p, &info, nullptr, 0, nullptr, nullptr);
+      in.status = st;
+      in.niter = info.niter;
+      in.converged = info.converged;
+      in.itererr = info.itererr;
+      in.energy = info.energy;
+      in.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (st != IMPS_OK) worst = st;
+    }
+    imps_destroy(c);
+  };
+  std::vector<std::thread> th;
+  const int nw = std::min(nworkers, std::max(ninst, 1));
+  for (int i = 0; i < nw; ++i) th.emplace_back(worker);
+  for (auto& t : th) t.join();
+  return worst.load();
 }
 
 int imps_bench_hac(imps_ctx* c, int site, int reps, int prec, imps_hac_timing* t) {

@@ -53,6 +53,13 @@ hipError_t launch_multidot(size_t n, int nv, const double* X, size_t ldx, const 
 hipError_t launch_multiaxpy(size_t n, int nv, const double* X, size_t ldx, const double* coef,
                             double alpha, double* y, hipStream_t s);
 hipError_t launch_scale(size_t n, double a, double* x, hipStream_t s);
+// out[v] = sum_b part[v*nb + b] (one thread per v, fixed order)
+hipError_t launch_partials_reduce(int nv, int nb, const double* part, double* out, hipStream_t s);
+// y = alpha*y + scale * sum_v coef[v] * X[v] with the coefficients in device memory
+hipError_t launch_multiaxpy_dev(size_t n, int nv, const double* X, size_t ldx, const double* coef,
+                                double scale, double alpha, double* y, hipStream_t s);
+// x *= 1/sqrt(*sq) when *sq > 0 (sq in device memory)
+hipError_t launch_scale_inv_sqrt(size_t n, double* x, const double* sq, hipStream_t s);
 // y = a*x + b*y
 hipError_t launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t s);
 hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s);
@@ -71,6 +78,14 @@ hipError_t launch_trace_prod_parts(int n, const double* l, const double* v, doub
 // sbuf: n doubles of device scratch
 hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr, double* sbuf,
                              hipStream_t s);
+// Blocked Cholesky helper (lower factor G = L L^T, 64 x 64 diagonal blocks): factor the
+// nb x nb diagonal block at A (leading dimension lda) in place, write its inverse to Linv
+// (64 x 64, ld 64, zero outside the lower nb x nb triangle); *fail = 1 on a non-positive
+// pivot
+hipError_t launch_chol_diag(int nb, double* A, long lda, double* Linv, int* fail, hipStream_t s);
+// A[j + ld i] = A[i + ld j] for i > j (mirror the lower triangle into the upper one)
+hipError_t launch_mirror_lower(int n, double* A, long ld, hipStream_t s);
+hipError_t launch_fill_int(int n, int v, int* x, hipStream_t s);
 // zero the strictly lower (keep_upper) or strictly upper triangle of an n x n matrix
 hipError_t launch_tri_keep(int n, double* A, long ld, bool keep_upper, hipStream_t s);
 // copy upper triangle of the n x n leading block of A (lda) into R (n x n), zero below

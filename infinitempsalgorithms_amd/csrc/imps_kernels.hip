@@ -47,7 +47,7 @@ struct MfmaTraits<float> {
 };
 
 template <typename TC, int BM, int BN, int BK, bool AMC, bool BKC>
-__global__ void __launch_bounds__(256) gemm_kernel(GemmDesc g) {
+__global__ void __launch_bounds__(256, 2) gemm_kernel(GemmDesc g) {
   typedef MfmaTraits<TC> MT;
   typedef typename MT::v4 v4;
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -196,6 +196,168 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmDesc g) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Split-BF16 three-term GEMM: every FP64 operand x is staged as x_hi = bf16(x) and
+// x_lo = bf16(x - x_hi) (|x - x_hi - x_lo| <~ 2^-17 |x|); a*b is accumulated in FP32 as
+// a_hi b_hi + a_hi b_lo + a_lo b_hi with v_mfma_f32_16x16x32_bf16.  LDS rows are k-contiguous
+// (BK = 32 bf16 plus 8 of padding) so each lane fetches its 8 k-values with one 16-byte read.
+// ---------------------------------------------------------------------------------
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline void split_bf16(double x, __bf16& hi, __bf16& lo) {
+  const float xf = (float)x;
+  hi = (__bf16)xf;
+  lo = (__bf16)(xf - (float)hi);
+}
+
+template <int BM, int BN, bool AMC, bool BKC>
+__global__ void __launch_bounds__(256) gemm_bf16x3_kernel(GemmDesc g) {
+  constexpr int BK = 32, LDK = BK + 8;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int AE = BM * BK / 256, BE = BK * BN / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Al[BM][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[BN][LDK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int m0 = blockIdx.x * BM;
+  const int n0 = blockIdx.y * BN;
+  const int wm = (wave & 1) * WM;
+  const int wn = (wave >> 1) * WN;
+  const int M = g.M, N = g.N, K = g.K, Kc = g.Kc;
+  const int ntiles_all = (K + BK - 1) / BK;
+  const int per = (ntiles_all + gridDim.z - 1) / gridDim.z;
+  const int kt0 = blockIdx.z * per;
+  const int kt1 = min(ntiles_all, kt0 + per);
+  v4f acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  double ra[AE], rb[BE];
+  auto load_tiles = [&](int k0) {
+    const int kc = k0 / Kc;
+    const int kk0 = k0 - kc * Kc;
+    const double* Ab = g.A + (long)kc * g.sAc + (long)kk0 * g.sAk;
+    const double* Bb = g.B + (long)kc * g.sBc + (long)kk0 * g.sBk;
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      const int idx = tid + 256 * e;
+      int m, k;
+      if (AMC) {
+        m = idx % BM;
+        k = idx / BM;
+      } else {
+        k = idx % BK;
+        m = idx / BK;
+      }
+      const int gm = m0 + m;
+      ra[e] = (gm < M && k0 + k < K) ? Ab[(long)gm * g.sAm + (long)k * g.sAk] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      const int idx = tid + 256 * e;
+      int n, k;
+      if (BKC) {
+        k = idx % BK;
+        n = idx / BK;
+      } else {
+        n = idx % BN;
+        k = idx / BN;
+      }
+      const int gn = n0 + n;
+      rb[e] = (gn < N && k0 + k < K) ? Bb[(long)k * g.sBk + (long)gn * g.sBn] : 0.0;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      const int idx = tid + 256 * e;
+      int m, k;
+      if (AMC) {
+        m = idx % BM;
+        k = idx / BM;
+      } else {
+        k = idx % BK;
+        m = idx / BK;
+      }
+      split_bf16(ra[e], Ah[m][k], Al[m][k]);
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      const int idx = tid + 256 * e;
+      int n, k;
+      if (BKC) {
+        k = idx % BK;
+        n = idx / BK;
+      } else {
+        n = idx % BN;
+        k = idx / BN;
+      }
+      split_bf16(rb[e], Bh[n][k], Bl[n][k]);
+    }
+  };
+  if (kt0 < kt1) load_tiles(kt0 * BK);
+  const int lr = lane & 15;
+  const int lk = lane >> 4;
+  for (int kt = kt0; kt < kt1; ++kt) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (kt + 1 < kt1) load_tiles((kt + 1) * BK);
+    v8bf ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      ah[i] = *(const v8bf*)&Ah[wm + i * 16 + lr][8 * lk];
+      al[i] = *(const v8bf*)&Al[wm + i * 16 + lr][8 * lk];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      bh[j] = *(const v8bf*)&Bh[wn + j * 16 + lr][8 * lk];
+      bl[j] = *(const v8bf*)&Bl[wn + j * 16 + lr][8 * lk];
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+      }
+  }
+  double* Cout = g.C;
+  long ldc = g.ldc;
+  double alpha = g.alpha, beta = g.beta;
+  if (gridDim.z > 1) {
+    Cout = g.W + (size_t)blockIdx.z * M * N;
+    ldc = M;
+    alpha = 1.0;
+    beta = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int m = m0 + wm + i * 16 + lr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn + j * 16 + 4 * lk + r;
+        if (n < N) {
+          double* p = Cout + m + (long)n * ldc;
+          const double v = alpha * (double)acc[i][j][r];
+          *p = (beta == 0.0) ? v : v + beta * *p;
+        }
+      }
+    }
+  }
+}
+
 // C = alpha * sum_z W[z] + beta * C
 __global__ void splitk_reduce_kernel(int M, int N, int nz, const double* __restrict__ W, double alpha,
                                      double beta, double* C, long ldc) {
@@ -220,6 +382,25 @@ static hipError_t gemm_dispatch_tile(const GemmDesc& g, bool amc, bool bkc, int 
     hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, true>), grid, block, 0, s, g);
   else
     hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, false>), grid, block, 0, s, g);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || splitk == 1) return e;
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3((g.M + 255) / 256, g.N), dim3(256), 0, s, g.M, g.N, splitk,
+                     g.W, g.alpha, g.beta, g.C, g.ldc);
+  return hipGetLastError();
+}
+
+template <int BM, int BN>
+static hipError_t gemm_bf16x3_tile(const GemmDesc& g, bool amc, bool bkc, int splitk, hipStream_t s) {
+  dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, splitk);
+  dim3 block(256);
+  if (amc && bkc)
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, true, true>), grid, block, 0, s, g);
+  else if (amc && !bkc)
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, true, false>), grid, block, 0, s, g);
+  else if (!amc && bkc)
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, false, true>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, false, false>), grid, block, 0, s, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || splitk == 1) return e;
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3((g.M + 255) / 256, g.N), dim3(256), 0, s, g.M, g.N, splitk,
@@ -260,6 +441,19 @@ hipError_t launch_gemm(const GemmDesc& g0, int prec, hipStream_t s) {
   const bool bkc = (g.sBk == 1);
   if (!amc && g.sAk != 1) return hipErrorInvalidValue;
   if (!bkc && g.sBn != 1) return hipErrorInvalidValue;
+  if (prec == PREC_BF16X3) {
+    if (g.Kc < g.K && (g.Kc % 32) != 0) return hipErrorInvalidValue;
+    const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
+    if (t128 >= 192) return gemm_bf16x3_tile<128, 128>(g, amc, bkc, 1, s);
+    const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
+    int splitk = 1;
+    if (g.W != nullptr && t64 < 512) {
+      const int ktiles = (g.K + 31) / 32;
+      splitk = (int)std::min<long>(8, std::max<long>(1, 1024 / std::max<long>(t64, 1)));
+      splitk = std::min(splitk, std::max(1, ktiles / 8));
+    }
+    return gemm_bf16x3_tile<64, 64>(g, amc, bkc, splitk, s);
+  }
   if (prec == PREC_FP32) return gemm_select<float>(g, amc, bkc, s);
   if (prec == PREC_FP64) return gemm_select<double>(g, amc, bkc, s);
   return hipErrorInvalidValue;
@@ -443,9 +637,55 @@ hipError_t launch_multiaxpy(size_t n, int nv, const double* X, size_t ldx, const
   return hipGetLastError();
 }
 
+__global__ void partials_reduce_kernel(int nv, int nb, const double* __restrict__ part, double* __restrict__ out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[(size_t)v * nb + b];
+  out[v] = s;
+}
+
+hipError_t launch_partials_reduce(int nv, int nb, const double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(partials_reduce_kernel, dim3((nv + 63) / 64), dim3(64), 0, s, nv, nb, part, out);
+  return hipGetLastError();
+}
+
+__global__ void multiaxpy_dev_kernel(size_t n, int nv, const double* __restrict__ X, size_t ldx,
+                                     const double* __restrict__ coef, double scale, double alpha,
+                                     double* __restrict__ y) {
+  __shared__ double c[64];
+  if (threadIdx.x < nv) c[threadIdx.x] = scale * coef[threadIdx.x];
+  __syncthreads();
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double acc = (alpha == 0.0) ? 0.0 : alpha * y[i];
+    for (int v = 0; v < nv; ++v) acc += c[v] * X[(size_t)v * ldx + i];
+    y[i] = acc;
+  }
+}
+
+__global__ void scale_inv_sqrt_kernel(size_t n, double* x, const double* sq) {
+  const double q = *sq;
+  if (!(q > 0.0)) return;
+  const double a = 1.0 / sqrt(q);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
 __global__ void scale_kernel(size_t n, double a, double* x) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     x[i] *= a;
+}
+
+hipError_t launch_multiaxpy_dev(size_t n, int nv, const double* X, size_t ldx, const double* coef,
+                                double scale, double alpha, double* y, hipStream_t s) {
+  if (nv > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(multiaxpy_dev_kernel, ew_grid(n), dim3(256), 0, s, n, nv, X, ldx, coef, scale, alpha, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_inv_sqrt(size_t n, double* x, const double* sq, hipStream_t s) {
+  hipLaunchKernelGGL(scale_inv_sqrt_kernel, ew_grid(n), dim3(256), 0, s, n, x, sq);
+  return hipGetLastError();
 }
 
 hipError_t launch_scale(size_t n, double a, double* x, hipStream_t s) {
@@ -585,6 +825,88 @@ hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long 
   hipLaunchKernelGGL(qr_signs_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, R, ldr, sbuf);
   if (Q) hipLaunchKernelGGL(qr_signfix_q_kernel, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, Q, ldq, sbuf);
   hipLaunchKernelGGL(qr_signfix_r_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, R, ldr, sbuf);
+  return hipGetLastError();
+}
+
+// Right-looking unblocked Cholesky of one diagonal block held in LDS (nb <= 64), then the
+// in-place inverse of the factor (column sweep from the last column, each column a
+// triangular matrix-vector product with the already inverted trailing block).  L goes
+// back to A; L^{-1} (64 x 64, zero above the diagonal and beyond nb) goes to Linv.
+__global__ void __launch_bounds__(256) chol_diag_kernel(int nb, double* A, long lda, double* Linv, int* fail) {
+  __shared__ double S[64][65];
+  __shared__ double y[64];
+  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
+    const int i = t % nb, j = t / nb;
+    S[i][j] = (i >= j) ? A[i + lda * j] : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < nb; ++k) {
+    if (threadIdx.x == 0) {
+      double p = S[k][k];
+      if (!(p > 0.0)) {
+        *fail = 1;
+        p = 1.0;
+      }
+      S[k][k] = sqrt(p);
+    }
+    __syncthreads();
+    const double piv = S[k][k];
+    for (int i = k + 1 + threadIdx.x; i < nb; i += blockDim.x) S[i][k] /= piv;
+    __syncthreads();
+    const int m = nb - k - 1;
+    for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+      const int i = k + 1 + t % m, j = k + 1 + t / m;
+      if (i >= j) S[i][j] -= S[i][k] * S[j][k];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
+    const int i = t % nb, j = t / nb;
+    if (i >= j) A[i + lda * j] = S[i][j];
+  }
+  __syncthreads();
+  for (int j = nb - 1; j >= 0; --j) {
+    if (threadIdx.x == 0) S[j][j] = 1.0 / S[j][j];
+    __syncthreads();
+    for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) {
+      double acc = 0.0;
+      for (int k = j + 1; k <= i; ++k) acc += S[i][k] * S[k][j];
+      y[i] = acc;
+    }
+    __syncthreads();
+    for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) S[i][j] = -S[j][j] * y[i];
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
+    const int i = t % 64, j = t / 64;
+    Linv[i + 64 * j] = (i < nb && j < nb && i >= j) ? S[i][j] : 0.0;
+  }
+}
+
+hipError_t launch_chol_diag(int nb, double* A, long lda, double* Linv, int* fail, hipStream_t s) {
+  if (nb > 64 || nb < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, nb, A, lda, Linv, fail);
+  return hipGetLastError();
+}
+
+__global__ void mirror_lower_kernel(int n, double* A, long ld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < n && i > j) A[j + ld * i] = A[i + ld * j];
+}
+
+hipError_t launch_mirror_lower(int n, double* A, long ld, hipStream_t s) {
+  hipLaunchKernelGGL(mirror_lower_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, A, ld);
+  return hipGetLastError();
+}
+
+__global__ void fill_int_kernel(int n, int v, int* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+hipError_t launch_fill_int(int n, int v, int* x, hipStream_t s) {
+  hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, v, x);
   return hipGetLastError();
 }
 

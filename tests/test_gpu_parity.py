@@ -82,6 +82,23 @@ def test_gemm_fp32_mfma(ctx, ta, tb, m, n, k):
     assert 1e-9 < err < 2e-6  # genuinely FP32, and as accurate as FP32 allows
 
 
+@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 130, 77, 45), ("T", "N", 1024, 1024, 4096), ("N", "T", 640, 512, 96)])
+def test_gemm_bf16x3_mfma(ctx, ta, tb, m, n, k):
+    # a*b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi (bf16 parts, FP32 accumulation): ~2^-16 relative
+    rng = np.random.default_rng(m + n + k + 1)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    ref = (A if ta == "N" else A.T) @ (B if tb == "N" else B.T)
+    dA, dB, dC = DevArray(ctx, A), DevArray(ctx, B), DevArray(ctx, np.zeros((m, n)))
+    ctx.call("imps_gemm_dev", ta.encode(), tb.encode(), m, n, k, 1.0, dA.p, A.shape[0], dB.p, B.shape[0],
+             0.0, dC.p, m, 2)
+    got = dC.get()
+    for d in (dA, dB, dC):
+        d.free()
+    err = rel(got, ref)
+    assert 1e-8 < err < 5e-5
+
+
 def _setup_envs(imps, ctx, oracle, Wfn, D, N, seed, sym=False):
     rng = np.random.default_rng(seed)
     W = Wfn()
@@ -268,3 +285,85 @@ def test_vumps_mixed_precision_refines_to_fp64(imps, ctx, oracle):
     assert imx.converged and imx.itererr <= 1e-10
     assert imx.prec_switch_iter > 1
     assert abs(imx.energy - i64.energy) < 1e-9
+
+
+def test_local_expectation_matches_oracle(imps, ctx, oracle):
+    # expectationvalue(psi, (i,) => O) (expval.jl:33-37) on a random canonical state
+    W = oracle.tfim_mpo()
+    H = imps.SparseIMPO([W] * 2)
+    psi = oracle.randomimps(10, [2, 2], seed=31)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, H, ctx=ctx)
+    for site, O in ((0, oracle.sigma_z()), (1, oracle.sigma_x())):
+        got = imps.expectationvalue(st, (site, O), envs)
+        assert abs(got - oracle.local_expectation1(psi, site, O)) < 1e-12
+
+
+def test_hellmann_feynman_tfim(imps, ctx, oracle):
+    # H = -J sum sx sx - h sum sz: de/dh = -<sz> at the variational optimum; central FD of
+    # converged energies with step 1e-4 (D = 16, one-site cell)
+    def run(h):
+        H = imps.models.tfim_hamiltonian(J=1.0, h=h, nsites=1)
+        psi0 = oracle.randomimps(16, [2], seed=0)
+        st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+        envs = imps.GPUCache(st, H, ctx)
+        ctx.call("imps_envs_reset")
+        psi, envs, info = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-11, maxiter=400), envs=envs)
+        return psi, envs, info
+    h0, dh = 1.3, 1e-4
+    psi, envs, info = run(h0)
+    sz = imps.expectationvalue(psi, (0, oracle.sigma_z()), envs)
+    ep = run(h0 + dh)[2].energy
+    em = run(h0 - dh)[2].energy
+    fd = (ep - em) / (2 * dh)
+    assert abs(fd + sz) < 1e-6
+    # gapped point: D = 16 is converged to the closed form e(h) = -(1/pi) int sqrt(1 + h^2 + 2h cos k)
+    ks = np.linspace(0, np.pi, 200001)
+    e_exact = -np.trapezoid(np.sqrt(1 + h0 ** 2 + 2 * h0 * np.cos(ks)), ks) / np.pi
+    assert abs(info.energy - e_exact) < 1e-9
+
+
+def test_batched_sweep_matches_single_runs(imps, ctx, oracle):
+    # several TFIM instances with uneven D run concurrently (one stream per worker); each
+    # equals its own single run and the closed form e(h) at these gapped fields
+    cases = [(0.6, 8), (1.4, 12), (0.5, 16), (1.5, 10), (0.8, 12)]
+    alg = imps.VUMPS(tol=1e-10, maxiter=300)
+    problems = []
+    for k, (h, D) in enumerate(cases):
+        H = imps.models.tfim_hamiltonian(J=1.0, h=h, nsites=1)
+        problems.append((H, imps.random_tensors(D, [2], seed=100 + k)))
+    res = imps.find_groundstates_batched(problems, alg, nworkers=3)
+    ks = np.linspace(0, np.pi, 200001)
+    for (h, D), (H, As), r in zip(cases, problems, res):
+        assert r["converged"] and r["itererr"] <= 1e-10
+        psi0, c2 = imps.canonicalimps(As, H, ctx=ctx, tol=1e-12)
+        envs = imps.GPUCache(psi0, H, c2)
+        ctx.call("imps_envs_reset")
+        _, _, info = imps.find_groundstate(psi0, H, alg, envs=envs)
+        assert abs(r["energy"] - info.energy) < 1e-10
+        e_exact = -np.trapezoid(np.sqrt(1 + h ** 2 + 2 * h * np.cos(ks)), ks) / np.pi
+        assert abs(r["energy"] - e_exact) < 1e-7
+
+
+def test_hac_bf16x3_and_mixed_vumps(imps, ctx, oracle):
+    D = 64
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 2, seed=19)
+    envs = imps.GPUCache.__new__(imps.GPUCache)
+    envs.ctx, envs.ket, envs.operator = ctx, None, H
+    x = np.random.default_rng(8).standard_normal((D, 4, D))
+    y64 = oracle.hac_apply(GLs[1], H.Ws[1], GRs[1], x)
+    yb = imps.AC_hamiltonian(1, None, H, None, envs, prec=2)(x)
+    assert 1e-8 < rel(yb, y64) < 1e-4
+    # split-BF16 bulk phase, then FP64 refinement: converged energy equals the FP64 run's
+    W = oracle.tfim_mpo()
+    Ht = imps.SparseIMPO([W] * 2)
+    psi0 = oracle.randomimps(12, [2, 2], seed=0)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+    e = imps.GPUCache(st, Ht, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, i64 = imps.find_groundstate(st, Ht, imps.VUMPS(tol=1e-10), envs=e)
+    e = imps.GPUCache(st, Ht, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, ib = imps.find_groundstate(st, Ht, imps.VUMPS(tol=1e-10, prec=2, refine_below=1e-4), envs=e)
+    assert ib.converged and ib.prec_switch_iter > 1
+    assert abs(ib.energy - i64.energy) < 1e-9
