@@ -162,6 +162,16 @@ int imps_galerkin(imps_ctx* ctx, double* eps);
 /* expectationvalue(ψ, H::SparseIMPO, envs) (src/observables/expval.jl:93-112):
  * unit-cell total energy divided by dot(ψ, ψ) (canonicalmps.jl:262-269). */
 int imps_energy(imps_ctx* ctx, double* e);
+/* Gradient of the unit-cell energy of expectationvalue(ψ, H, envs) (expval.jl:93-112) with
+ * respect to every MPO tensor at fixed ψ and environments: the reverse of the H_AC
+ * contraction (effective.jl:90-102),
+ *   dW[site][i,u,j,s] = Σ GL[b,i,a] AC[a,s,bb] GR[bb,j,c] AC[b,u,c] / dot(ψ, ψ),
+ * i.e. ⟨AC| GL_i ⊗ δW ⊗ GR_j |AC⟩.  At a converged VUMPS state ψ is stationary, so for
+ * every non-structural Schur block this is the derivative of the ground-state energy
+ * (Hellmann–Feynman); the unit corners (0,0) and (w-1,w-1) are structural.  The reference
+ * has no automatic differentiation.  dW: nsites host arrays (w, d, w, d). */
+int imps_energy_gradient(imps_ctx* ctx, double* const* dW);
+
 /* expectationvalue(ψ, (site,) => O) (expval.jl:33-37), O: (d, d). */
 int imps_local_expectation(imps_ctx* ctx, int site, const double* O, double* val);
 

@@ -8,7 +8,7 @@ host-side mirror of the reference's API (``find_groundstate``, ``VUMPS``,
 from . import _lib
 from .api import (VUMPS, AC_hamiltonian, C_hamiltonian, CanonicalIMPS, Context, DMRGCache,
                   GPUCache, IterativeConvergenceInfo, Lanczos, SparseIMPO, TransferMatrix,
-                  calc_galerkin, canonicalimps, expectationvalue, find_groundstate,
+                  calc_galerkin, canonicalimps, energy_gradient, expectationvalue, find_groundstate,
                   find_groundstates_batched, fixedpoint,
                   leftenv, random_tensors, randomimps, recalculate_, regauge_, regularize_,
                   rightenv)
@@ -17,6 +17,6 @@ from . import models
 __all__ = [
     "VUMPS", "AC_hamiltonian", "C_hamiltonian", "CanonicalIMPS", "Context", "DMRGCache", "GPUCache",
     "IterativeConvergenceInfo", "Lanczos", "SparseIMPO", "TransferMatrix", "calc_galerkin",
-    "canonicalimps", "expectationvalue", "find_groundstate", "find_groundstates_batched", "fixedpoint", "leftenv", "random_tensors",
+    "canonicalimps", "energy_gradient", "expectationvalue", "find_groundstate", "find_groundstates_batched", "fixedpoint", "leftenv", "random_tensors",
     "randomimps", "recalculate_", "regauge_", "regularize_", "rightenv", "models",
 ]

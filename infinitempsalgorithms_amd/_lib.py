@@ -88,6 +88,7 @@ SIGNATURES = [
     ("imps_gaugefix_right", c_int, [c_void_p, c_double, c_int, ctypes.POINTER(c_int)]),
     ("imps_galerkin", c_int, [c_void_p, P_double]),
     ("imps_energy", c_int, [c_void_p, P_double]),
+    ("imps_energy_gradient", c_int, [c_void_p, PP_double]),
     ("imps_local_expectation", c_int, [c_void_p, c_int, P_double, P_double]),
     ("imps_vumps_default_params", c_int, [ctypes.POINTER(VumpsParams)]),
     ("imps_vumps_run", c_int, [c_void_p, ctypes.POINTER(VumpsParams), ctypes.POINTER(VumpsInfo), P_double,

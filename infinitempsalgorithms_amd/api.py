@@ -361,6 +361,20 @@ def expectationvalue(psi: CanonicalIMPS, op, envs: GPUCache):
     return val.value
 
 
+def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
+    """d e_cell / d W[site] at fixed ψ and environments (the reverse of the H_AC
+    contraction, effective.jl:90-102, applied to the energy of expval.jl:93-112).  At a
+    converged VUMPS state this is the derivative of the ground-state energy with respect
+    to each non-structural MPO block.  Returns one (w, d, w, d) array per site."""
+    if psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ket = psi
+    out = [np.zeros((H.w, H.d, H.w, H.d), order="F") for _ in range(len(H))]
+    pp, keep = ptr_array(out)
+    envs.ctx.call("imps_energy_gradient", pp)
+    return out
+
+
 # ---------------------------------------------------------------------------- VUMPS
 
 

@@ -1155,6 +1155,30 @@ double energy(imps_ctx* c) {
   return E / dot_psi(c);
 }
 
+// d e_cell / d W[site][i,u,j,s] at fixed psi: sum GL[b,i,a] AC[a,s,bb] GR[bb,j,c] AC[b,u,c],
+// divided by dot(psi, psi).  At a variational fixed point this is the derivative of the
+// converged energy for every non-structural Schur block.
+void energy_gradient(imps_ctx* c, int site, double* out) {
+  const int D = c->D, w = c->w, d = c->d, wd = w * d;
+  double* T1 = ws(c, "hac_T1", (size_t)D * w * d * D);
+  double* T3 = ws(c, "hac_T2", (size_t)D * w * d * D);
+  double* Gp = ws(c, "eg_part", (size_t)wd * wd * D);
+  double* G = ws(c, "eg_sum", (size_t)wd * wd);
+  gemm2(c, 'N', 'N', D * w, d * D, D, 1.0, c->GL[site], (long)D * w, c->AC[site], D, 0.0, T1, (long)D * w);
+  gemm2(c, 'N', 'T', D * d, D * w, D, 1.0, c->AC[site], (long)D * d, c->GR[site], (long)D * w, 0.0, T3, (long)D * d);
+  HIPCHK(launch_gsand(D, w, d, T1, T3, Gp, c->stream));
+  HIPCHK(launch_partials_reduce(wd * wd, D, Gp, G, c->stream));
+  std::vector<double> h((size_t)wd * wd);
+  d2h(c, h.data(), G, h.size());
+  const double nrm = dot_psi(c);
+  for (int i = 0; i < w; ++i)
+    for (int u = 0; u < d; ++u)
+      for (int j = 0; j < w; ++j)
+        for (int s2 = 0; s2 < d; ++s2)
+          out[i + (size_t)w * (u + (size_t)d * (j + (size_t)w * s2))] =
+              h[(i + (size_t)w * s2) + (size_t)wd * (u + (size_t)d * j)] / nrm;
+}
+
 double galerkin(imps_ctx* c, int prec) {
   need_envs(c);
   const int N = c->N, D = c->D, d = c->d;
@@ -1626,6 +1650,16 @@ int imps_energy(imps_ctx* c, double* e) {
   return run_guarded(c, [&] {
     need_match(c);
     *e = energy(c);
+  });
+}
+
+int imps_energy_gradient(imps_ctx* c, double* const* dW) {
+  if (!c || !dW) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    require(c->w * c->d <= 32, IMPS_E_UNSUPPORTED, "imps_energy_gradient: w*d > 32 is not supported");
+    for (int site = 0; site < c->N; ++site) energy_gradient(c, site, dW[site]);
   });
 }
 

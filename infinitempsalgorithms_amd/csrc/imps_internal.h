@@ -45,6 +45,10 @@ hipError_t launch_mix_mid(int D1, int d, int D2, const double* M, const double* 
 hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2,
                        hipStream_t s);
 
+// Energy-gradient sandwich partials: Gp[o*D + bb] = sum_b T1[b,i,s,bb] T3[b,u,bb,j] with
+// o = (i + w s) + w d (u + d j); T1 as in launch_wmix, T3 laid out like its T2; w d <= 32.
+hipError_t launch_gsand(int D, int w, int d, const double* T1, const double* T3, double* Gp, hipStream_t s);
+
 // Partial sums for dots: part[v*nb + blk] for v < nv over vectors X + v*ldx against y.
 int dot_blocks(size_t n);
 hipError_t launch_multidot(size_t n, int nv, const double* X, size_t ldx, const double* y,

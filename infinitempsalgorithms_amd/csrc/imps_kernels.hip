@@ -542,6 +542,50 @@ hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* 
   return hipGetLastError();
 }
 
+// Energy gradient sandwich: for one bb per workgroup,
+//   Gp[o * D + bb] = sum_b T1[b, i, s, bb] * T3[b, u, bb, j],  o = q + wd r, q = i + w s, r = u + d j
+// with T1 at b + D(i + w(s + d bb)) and T3 at b + D(u + d(bb + D j)); wd <= 32.
+__global__ void __launch_bounds__(256) gsand_kernel(int D, int w, int d, const double* __restrict__ T1,
+                                                    const double* __restrict__ T3, double* __restrict__ Gp) {
+  __shared__ double S1[64][33];
+  __shared__ double S3[64][33];
+  const int wd = w * d;
+  const int bb = blockIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b0 = 0; b0 < D; b0 += 64) {
+    const int nbk = min(64, D - b0);
+    for (int t = threadIdx.x; t < nbk * wd; t += blockDim.x) {
+      const int bl = t % nbk, q = t / nbk;
+      S1[bl][q] = T1[(size_t)(b0 + bl) + (size_t)D * q + (size_t)D * wd * bb];
+      const int u = q % d, j = q / d;  // q doubles as r = u + d j here
+      S3[bl][q] = T3[(size_t)(b0 + bl) + (size_t)D * u + (size_t)D * d * bb + (size_t)D * D * d * j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int o = threadIdx.x + 256 * e;
+      if (o < wd * wd) {
+        const int q = o % wd, r = o / wd;
+        double sacc = 0.0;
+        for (int bl = 0; bl < nbk; ++bl) sacc += S1[bl][q] * S3[bl][r];
+        acc[e] += sacc;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int o = threadIdx.x + 256 * e;
+    if (o < wd * wd) Gp[(size_t)o * D + bb] = acc[e];
+  }
+}
+
+hipError_t launch_gsand(int D, int w, int d, const double* T1, const double* T3, double* Gp, hipStream_t s) {
+  if (w * d > 32) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(gsand_kernel, dim3(D), dim3(256), 0, s, D, w, d, T1, T3, Gp);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------------
 // FP64 reductions: per-block partial sums, summed on the host in block order.
 // ---------------------------------------------------------------------------------

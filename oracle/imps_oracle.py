@@ -940,6 +940,21 @@ def expectation_energy(psi, Ws, envs):
     return E / dot_psi(psi, psi)
 
 
+def energy_gradient(psi, Ws, envs):
+    """d e_cell / d W[site] at fixed psi and environments: the sandwich
+    <AC| GL_i (x) dW (x) GR_j |AC> / dot(psi, psi) (reverse of effective.jl:90-102 applied
+    to the energy of expval.jl:93-112).  The reference has no automatic differentiation;
+    at a variational fixed point this is the Hellmann-Feynman derivative."""
+    nrm = dot_psi(psi, psi)
+    out = []
+    for l in range(len(psi)):
+        AC = psi.AC[l]
+        t1 = np.einsum("bia,asc->bisc", envs.leftenv(l), AC, optimize=True)
+        t3 = np.einsum("buc,xjc->buxj", AC, envs.rightenv(l), optimize=True)
+        out.append(np.einsum("bisx,buxj->iujs", t1, t3, optimize=True) / nrm)
+    return out
+
+
 def local_expectation1(psi, site, O):
     """expval.jl:33-37."""
     AC = psi.AC[site]

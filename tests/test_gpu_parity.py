@@ -367,3 +367,33 @@ def test_hac_bf16x3_and_mixed_vumps(imps, ctx, oracle):
     _, _, ib = imps.find_groundstate(st, Ht, imps.VUMPS(tol=1e-10, prec=2, refine_below=1e-4), envs=e)
     assert ib.converged and ib.prec_switch_iter > 1
     assert abs(ib.energy - i64.energy) < 1e-9
+
+
+def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
+    # reverse H_AC sandwich against the oracle on a random state
+    Wx = oracle.xxz_mpo(1.0, 0.8)
+    H = imps.SparseIMPO([Wx] * 2)
+    psi = oracle.randomimps(16, [2, 2], seed=12)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, H, tol=1e-12, ctx=ctx)
+    G = imps.energy_gradient(st, H, envs)
+    Gr = oracle.energy_gradient(psi, [Wx] * 2, oracle.dmrg_cache(psi, [Wx] * 2, tol=1e-12))
+    for a, b in zip(G, Gr):
+        assert rel(a, b) < 1e-9
+    # dE/dDelta of the converged XXZ cell energy vs central differences (step 1e-5):
+    # Delta enters only the SzSz channel block W[0, :, 3, :] = J Delta Sz
+    def run(delta):
+        Hd = imps.models.heisenberg_hamiltonian(J=1.0, Delta=delta, nsites=2)
+        p0 = oracle.randomimps(16, [2, 2], seed=1)
+        s0 = imps.CanonicalIMPS(p0.AL, p0.AR, p0.C, p0.AC)
+        e = imps.GPUCache(s0, Hd, ctx)
+        ctx.call("imps_envs_reset")
+        return imps.find_groundstate(s0, Hd, imps.VUMPS(tol=1e-11, maxiter=500), envs=e), Hd
+    d0, dd = 1.3, 1e-5
+    (psi_c, envs_c, info), Hd = run(d0)
+    Gc = imps.energy_gradient(psi_c, Hd, envs_c)
+    sz = 0.5 * oracle.sigma_z()
+    grad = sum(np.sum(g[0, :, 3, :] * sz) for g in Gc)
+    fd = (run(d0 + dd)[0][2].energy - run(d0 - dd)[0][2].energy) / (2 * dd)
+    assert info.itererr < 1e-9
+    assert abs(grad - fd) < 1e-6

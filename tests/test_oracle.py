@@ -333,3 +333,27 @@ def test_vumps_xxx_energy(oracle):
     psi, envs, info = oracle.find_groundstate(psi0, Ws, oracle.VUMPS(tol=1e-10))
     assert info.itererr < 1e-9
     assert abs(info.energies[-1] / 2 - (0.25 - math.log(2))) < 2e-4
+
+
+def test_energy_gradient_onsite_block_and_fd(oracle):
+    # on-site block of the sandwich contracted with O is the one-site expectation value
+    W = oracle.tfim_mpo(1.0, 1.3)
+    psi = oracle.randomimps(6, [2], seed=4)
+    envs = oracle.dmrg_cache(psi, [W], tol=1e-12)
+    G = oracle.energy_gradient(psi, [W], envs)[0]
+    w = W.shape[0]
+    for O in (oracle.sigma_z(), oracle.sigma_x()):
+        assert abs(np.sum(G[0, :, w - 1, :] * O) - oracle.local_expectation1(psi, 0, O)) < 1e-12
+    # at the variational optimum the sandwich is the derivative of the converged energy:
+    # de/dh with H = -J sum sx sx - h sum sz (central differences, step 1e-5)
+    def e_of(h):
+        Wh = oracle.tfim_mpo(1.0, h)
+        p0 = oracle.randomimps(8, [2], seed=1)
+        p, en, info = oracle.find_groundstate(p0, [Wh], oracle.VUMPS(tol=1e-11))
+        return p, en, info
+    h0, dh = 1.3, 1e-5
+    p, en, info = e_of(h0)
+    G = oracle.energy_gradient(p, [oracle.tfim_mpo(1.0, h0)], en)[0]
+    grad = np.sum(G[0, :, w - 1, :] * (-oracle.sigma_z()))
+    fd = (e_of(h0 + dh)[2].energies[-1] - e_of(h0 - dh)[2].energies[-1]) / (2 * dh)
+    assert abs(grad - fd) < 1e-6
