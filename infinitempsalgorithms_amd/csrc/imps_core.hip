@@ -661,40 +661,35 @@ void qrpos_householder(imps_ctx* c, int m, int n, double* A, double* R) {
 // trailing updates on the MFMA GEMM.  The lower factor is always computed; for `upper`
 // it is mirrored into the upper triangle (R = L^T).  The other triangle is left holding
 // scratch values, as with LAPACK's potrf.
-bool chol(imps_ctx* c, int n, double* G, bool upper) {
-  int* fail = (int*)ws(c, "chol_info", 1);
-  double* Linv = ws(c, "chol_linv", 64 * 64);
+// Blocked Cholesky G = L L^T (lower factor in place; mirrored to the upper triangle when
+// `upper`, so that G then also holds R = L^T).  Enqueued only: a non-positive pivot sets
+// *fail (substituting 1 so the factorisation continues with finite values); the caller
+// clears *fail beforehand and reads it afterwards.
+void chol_async(imps_ctx* c, int n, double* G, bool upper, int* fail) {
+  double* Ut = ws(c, "chol_ut", 64 * 64);
   double* X = ws(c, "chol_panel", (size_t)n * 64);
-  HIPCHK(launch_fill_int(1, 0, fail, c->stream));
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int nb = std::min(64, n - j0);
     double* Djj = G + j0 + (size_t)n * j0;
-    HIPCHK(launch_chol_diag(nb, Djj, n, Linv, fail, c->stream));
+    HIPCHK(launch_chol_diag(nb, Djj, n, Ut, fail, c->stream));
     const int rows = n - j0 - nb;
     if (rows > 0) {
-      // panel: X = P L^{-T} (MFMA GEMM with the inverted diagonal block), written back to P,
-      // then the trailing update G22 -= X X^T
+      // panel: X = P L^{-T} = P U (MFMA GEMM with the inverted diagonal block), written back
+      // to P, then the trailing update G22 -= X X^T
       double* P = G + (j0 + nb) + (size_t)n * j0;
-      gemm2(c, 'N', 'T', rows, nb, nb, 1.0, P, n, Linv, 64, 0.0, X, rows);
+      gemm2(c, 'N', 'N', rows, nb, nb, 1.0, P, n, Ut, 64, 0.0, X, rows);
       HIPCHK(launch_copy2d(rows, nb, X, rows, P, n, c->stream));
       gemm2(c, 'N', 'T', rows, rows, nb, -1.0, X, rows, X, rows, 1.0, G + (j0 + nb) + (size_t)n * (j0 + nb), n);
     }
   }
   if (upper) HIPCHK(launch_mirror_lower(n, G, n, c->stream));
-  int h = 0;
-  HIPCHK(hipMemcpyAsync(&h, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return h == 0;
 }
 
-double diag_sum(imps_ctx* c, int n, const double* G) {
-  double* dg = ws(c, "chol_diag", n);
-  HIPCHK(launch_copy2d(1, n, G, (long)n + 1, dg, 1, c->stream));
-  std::vector<double> h(n);
-  d2h(c, h.data(), dg, n);
-  double s = 0.0;
-  for (double v : h) s += v;
-  return s;
+int read_flag(imps_ctx* c, const int* flag) {
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return h;
 }
 
 // Shifted Cholesky QR, three passes (one shifted pass, two plain passes): A (m x n, m >= n)
@@ -706,18 +701,16 @@ bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R) {
   double* G = ws(c, "cq_G", nn);
   double* R2 = ws(c, "cq_R2", nn);
   double* bak = ws(c, "cq_bak", mn);
+  int* fail = (int*)ws(c, "chol_info", 1);
   d2d(c, bak, A, mn);
+  HIPCHK(launch_fill_int(1, 0, fail, c->stream));
   const double one = 1.0;
   for (int pass = 0; pass < 3; ++pass) {
     gemm2(c, 'T', 'N', n, n, m, 1.0, A, m, A, m, 0.0, G, n);
-    if (pass == 0) {
-      const double shift = 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * diag_sum(c, n, G);
-      HIPCHK(launch_add_diag(n, shift, G, n, c->stream));
-    }
-    if (!chol(c, n, G, true)) {
-      d2d(c, A, bak, mn);
-      return false;
-    }
+    if (pass == 0)
+      HIPCHK(launch_add_diag_trace(n, 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16, G, n,
+                                   c->stream));
+    chol_async(c, n, G, true, fail);
     RBCHK(rocblas_dtrsm(c->rb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                         rocblas_diagonal_non_unit, m, n, &one, G, n, A, m));
     if (pass == 0) {
@@ -731,6 +724,11 @@ bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R) {
       d2d(c, R, R2, nn);
     }
   }
+  // a breakdown in any pass: restore A (the passes after it ran on substituted pivots)
+  if (read_flag(c, fail) != 0) {
+    d2d(c, A, bak, mn);
+    return false;
+  }
   HIPCHK(launch_tri_keep(n, R, n, true, c->stream));
   return true;
 }
@@ -742,18 +740,16 @@ bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L) {
   double* G = ws(c, "cq_G", mm);
   double* L2 = ws(c, "cq_R2", mm);
   double* bak = ws(c, "cq_bak", mn);
+  int* fail = (int*)ws(c, "chol_info", 1);
   d2d(c, bak, M, mn);
+  HIPCHK(launch_fill_int(1, 0, fail, c->stream));
   const double one = 1.0;
   for (int pass = 0; pass < 3; ++pass) {
     gemm2(c, 'N', 'T', m, m, n, 1.0, M, m, M, m, 0.0, G, m);
-    if (pass == 0) {
-      const double shift = 11.0 * ((double)m * n + (double)m * (m + 1)) * 1.1102230246251565e-16 * diag_sum(c, m, G);
-      HIPCHK(launch_add_diag(m, shift, G, m, c->stream));
-    }
-    if (!chol(c, m, G, false)) {
-      d2d(c, M, bak, mn);
-      return false;
-    }
+    if (pass == 0)
+      HIPCHK(launch_add_diag_trace(m, 11.0 * ((double)m * n + (double)m * (m + 1)) * 1.1102230246251565e-16, G, m,
+                                   c->stream));
+    chol_async(c, m, G, false, fail);
     RBCHK(rocblas_dtrsm(c->rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                         rocblas_diagonal_non_unit, m, n, &one, G, m, M, m));
     if (pass == 0) {
@@ -764,6 +760,10 @@ bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L) {
                           rocblas_diagonal_non_unit, m, m, &one, G, m, L, m, L2, m));
       d2d(c, L, L2, mm);
     }
+  }
+  if (read_flag(c, fail) != 0) {
+    d2d(c, M, bak, mn);
+    return false;
   }
   HIPCHK(launch_tri_keep(m, L, m, false, c->stream));
   return true;

@@ -83,10 +83,12 @@ hipError_t launch_trace_prod_parts(int n, const double* l, const double* v, doub
 hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr, double* sbuf,
                              hipStream_t s);
 // Blocked Cholesky helper (lower factor G = L L^T, 64 x 64 diagonal blocks): factor the
-// nb x nb diagonal block at A (leading dimension lda) in place, write its inverse to Linv
-// (64 x 64, ld 64, zero outside the lower nb x nb triangle); *fail = 1 on a non-positive
+// nb x nb diagonal block at A (leading dimension lda) in place, write U = L^{-T} to Ut
+// (64 x 64, ld 64, zero outside the upper nb x nb triangle); *fail = 1 on a non-positive
 // pivot
-hipError_t launch_chol_diag(int nb, double* A, long lda, double* Linv, int* fail, hipStream_t s);
+hipError_t launch_chol_diag(int nb, double* A, long lda, double* Ut, int* fail, hipStream_t s);
+// G[i + ld i] += factor * trace(G) (n x n), the shift computed on the device
+hipError_t launch_add_diag_trace(int n, double factor, double* G, long ld, hipStream_t s);
 // A[j + ld i] = A[i + ld j] for i > j (mirror the lower triangle into the upper one)
 hipError_t launch_mirror_lower(int n, double* A, long ld, hipStream_t s);
 hipError_t launch_fill_int(int n, int v, int* x, hipStream_t s);

@@ -46,8 +46,11 @@ struct MfmaTraits<float> {
   static __device__ inline int row(int lk, int r) { return 4 * lk + r; }
 };
 
-template <typename TC, int BM, int BN, int BK, bool AMC, bool BKC>
-__global__ void __launch_bounds__(256, 2) gemm_kernel(GemmDesc g) {
+// VEC: every tile is full (M % BM, N % BN, K % BK, Kc % BK all zero) and each thread moves
+// 16-byte pairs along the contiguous direction of A and B (even strides, 16-byte aligned bases),
+// so the tile loads need no bounds checks.
+template <typename TC, int BM, int BN, int BK, bool AMC, bool BKC, int OCC, bool VEC>
+__global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
   typedef MfmaTraits<TC> MT;
   typedef typename MT::v4 v4;
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -79,37 +82,66 @@ __global__ void __launch_bounds__(256, 2) gemm_kernel(GemmDesc g) {
     for (int j = 0; j < TN; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
   double ra[AE], rb[BE];
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  static_assert(!VEC || (AE % 2 == 0 && BE % 2 == 0), "paired loads need an even count");
+
+  // tile element e of this thread: (m, k) of A and (k, n) of B
+  auto a_index = [&](int e, int& m, int& k) {
+    const int idx = VEC ? tid + 256 * (e / 2) : tid + 256 * e;
+    if (AMC) {
+      m = VEC ? (idx % (BM / 2)) * 2 + (e & 1) : idx % BM;
+      k = VEC ? idx / (BM / 2) : idx / BM;
+    } else {
+      k = VEC ? (idx % (BK / 2)) * 2 + (e & 1) : idx % BK;
+      m = VEC ? idx / (BK / 2) : idx / BK;
+    }
+  };
+  auto b_index = [&](int e, int& k, int& n) {
+    const int idx = VEC ? tid + 256 * (e / 2) : tid + 256 * e;
+    if (BKC) {
+      k = VEC ? (idx % (BK / 2)) * 2 + (e & 1) : idx % BK;
+      n = VEC ? idx / (BK / 2) : idx / BK;
+    } else {
+      n = VEC ? (idx % (BN / 2)) * 2 + (e & 1) : idx % BN;
+      k = VEC ? idx / (BN / 2) : idx / BN;
+    }
+  };
 
   auto load_tiles = [&](int k0) {
     const int kc = k0 / Kc;
     const int kk0 = k0 - kc * Kc;
     const double* Ab = g.A + (long)kc * g.sAc + (long)kk0 * g.sAk;
     const double* Bb = g.B + (long)kc * g.sBc + (long)kk0 * g.sBk;
+    if (VEC) {
+#pragma unroll
+      for (int e = 0; e < AE; e += 2) {
+        int m, k;
+        a_index(e, m, k);
+        const v2d v = *(const v2d*)(Ab + (long)(m0 + m) * g.sAm + (long)k * g.sAk);
+        ra[e] = v[0];
+        ra[e + 1] = v[1];
+      }
+#pragma unroll
+      for (int e = 0; e < BE; e += 2) {
+        int k, n;
+        b_index(e, k, n);
+        const v2d v = *(const v2d*)(Bb + (long)k * g.sBk + (long)(n0 + n) * g.sBn);
+        rb[e] = v[0];
+        rb[e + 1] = v[1];
+      }
+      return;
+    }
 #pragma unroll
     for (int e = 0; e < AE; ++e) {
-      const int idx = tid + 256 * e;
       int m, k;
-      if (AMC) {
-        m = idx % BM;
-        k = idx / BM;
-      } else {
-        k = idx % BK;
-        m = idx / BK;
-      }
+      a_index(e, m, k);
       const int gm = m0 + m;
       ra[e] = (gm < M && k0 + k < K) ? Ab[(long)gm * g.sAm + (long)k * g.sAk] : 0.0;
     }
 #pragma unroll
     for (int e = 0; e < BE; ++e) {
-      const int idx = tid + 256 * e;
-      int n, k;
-      if (BKC) {
-        k = idx % BK;
-        n = idx / BK;
-      } else {
-        n = idx % BN;
-        k = idx / BN;
-      }
+      int k, n;
+      b_index(e, k, n);
       const int gn = n0 + n;
       rb[e] = (gn < N && k0 + k < K) ? Bb[(long)k * g.sBk + (long)gn * g.sBn] : 0.0;
     }
@@ -118,28 +150,14 @@ __global__ void __launch_bounds__(256, 2) gemm_kernel(GemmDesc g) {
   auto store_tiles = [&]() {
 #pragma unroll
     for (int e = 0; e < AE; ++e) {
-      const int idx = tid + 256 * e;
       int m, k;
-      if (AMC) {
-        m = idx % BM;
-        k = idx / BM;
-      } else {
-        k = idx % BK;
-        m = idx / BK;
-      }
+      a_index(e, m, k);
       As[k][m] = (TC)ra[e];
     }
 #pragma unroll
     for (int e = 0; e < BE; ++e) {
-      const int idx = tid + 256 * e;
-      int n, k;
-      if (BKC) {
-        k = idx % BK;
-        n = idx / BK;
-      } else {
-        n = idx % BN;
-        k = idx / BN;
-      }
+      int k, n;
+      b_index(e, k, n);
       Bs[k][n] = (TC)rb[e];
     }
   };
@@ -370,18 +388,18 @@ __global__ void splitk_reduce_kernel(int M, int N, int nz, const double* __restr
   *p = (beta == 0.0) ? alpha * s : alpha * s + beta * *p;
 }
 
-template <typename TC, int BM, int BN, int BK>
+template <typename TC, int BM, int BN, int BK, int OCC, bool VEC>
 static hipError_t gemm_dispatch_tile(const GemmDesc& g, bool amc, bool bkc, int splitk, hipStream_t s) {
   dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, splitk);
   dim3 block(256);
   if (amc && bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, true>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, true, OCC, VEC>), grid, block, 0, s, g);
   else if (amc && !bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, false>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, false, OCC, VEC>), grid, block, 0, s, g);
   else if (!amc && bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, true>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, true, OCC, VEC>), grid, block, 0, s, g);
   else
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, false>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, false, OCC, VEC>), grid, block, 0, s, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || splitk == 1) return e;
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3((g.M + 255) / 256, g.N), dim3(256), 0, s, g.M, g.N, splitk,
@@ -417,10 +435,24 @@ size_t gemm_workspace(int M, int N, int K) {
   return (size_t)8 * M * N;
 }
 
+// true when every BM x BN x 16 tile is full and the paired 16-byte loads stay aligned
+static bool gemm_vec_ok(const GemmDesc& g, bool amc, bool bkc, int BM, int BN) {
+  auto even = [](long x) { return (x & 1) == 0; };
+  if (g.M % BM || g.N % BN || g.K % 16 || g.Kc % 16) return false;
+  if (((uintptr_t)g.A & 15) || ((uintptr_t)g.B & 15)) return false;
+  const bool a_ok = amc ? (even(g.sAk) && even(g.sAc)) : (even(g.sAm) && even(g.sAc));
+  const bool b_ok = bkc ? (even(g.sBn) && even(g.sBc)) : (even(g.sBk) && even(g.sBc));
+  return a_ok && b_ok;
+}
+
 template <typename TC>
 static hipError_t gemm_select(const GemmDesc& g, bool amc, bool bkc, hipStream_t s) {
+  const long t64x128 = (long)((g.M + 63) / 64) * ((g.N + 127) / 128);
+  // large shapes: 64 x 128 tiles at four workgroups per CU with paired loads
+  if (t64x128 >= 384 && gemm_vec_ok(g, amc, bkc, 64, 128))
+    return gemm_dispatch_tile<TC, 64, 128, 16, 4, true>(g, amc, bkc, 1, s);
   const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
-  if (t128 >= 192) return gemm_dispatch_tile<TC, 128, 128, 16>(g, amc, bkc, 1, s);
+  if (t128 >= 192) return gemm_dispatch_tile<TC, 128, 128, 16, 2, false>(g, amc, bkc, 1, s);
   const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
   int splitk = 1;
   if (g.W != nullptr && t64 < 512) {
@@ -429,7 +461,9 @@ static hipError_t gemm_select(const GemmDesc& g, bool amc, bool bkc, hipStream_t
     splitk = (int)std::min<long>(8, std::max<long>(1, 1024 / std::max<long>(t64, 1)));
     splitk = std::min(splitk, std::max(1, ktiles / 16));
   }
-  return gemm_dispatch_tile<TC, 64, 64, 16>(g, amc, bkc, splitk, s);
+  if (gemm_vec_ok(g, amc, bkc, 64, 64))
+    return gemm_dispatch_tile<TC, 64, 64, 16, 4, true>(g, amc, bkc, splitk, s);
+  return gemm_dispatch_tile<TC, 64, 64, 16, 2, false>(g, amc, bkc, splitk, s);
 }
 
 hipError_t launch_gemm(const GemmDesc& g0, int prec, hipStream_t s) {
@@ -681,16 +715,26 @@ hipError_t launch_multiaxpy(size_t n, int nv, const double* X, size_t ldx, const
   return hipGetLastError();
 }
 
-__global__ void partials_reduce_kernel(int nv, int nb, const double* __restrict__ part, double* __restrict__ out) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
+// out[v] = sum_b part[v nb + b]: one workgroup per v, strided partial sums, then an LDS tree
+// (fixed order, so repeated calls give identical results)
+__global__ void __launch_bounds__(256) partials_reduce_kernel(int nv, int nb, const double* __restrict__ part,
+                                                             double* __restrict__ out) {
+  __shared__ double red[256];
+  const int v = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(size_t)v * nb + b];
-  out[v] = s;
+  for (int b = threadIdx.x; b < nb; b += 256) s += part[(size_t)v * nb + b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[v] = red[0];
 }
 
 hipError_t launch_partials_reduce(int nv, int nb, const double* part, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(partials_reduce_kernel, dim3((nv + 63) / 64), dim3(64), 0, s, nv, nb, part, out);
+  if (nv <= 0) return hipSuccess;
+  hipLaunchKernelGGL(partials_reduce_kernel, dim3(nv), dim3(256), 0, s, nv, nb, part, out);
   return hipGetLastError();
 }
 
@@ -872,64 +916,111 @@ hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long 
   return hipGetLastError();
 }
 
-// Right-looking unblocked Cholesky of one diagonal block held in LDS (nb <= 64), then the
-// in-place inverse of the factor (column sweep from the last column, each column a
-// triangular matrix-vector product with the already inverted trailing block).  L goes
-// back to A; L^{-1} (64 x 64, zero above the diagonal and beyond nb) goes to Linv.
-__global__ void __launch_bounds__(256) chol_diag_kernel(int nb, double* A, long lda, double* Linv, int* fail) {
-  __shared__ double S[64][65];
-  __shared__ double y[64];
-  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
-    const int i = t % nb, j = t / nb;
-    S[i][j] = (i >= j) ? A[i + lda * j] : 0.0;
+// Cholesky of one diagonal block (nb <= 64, padded to 64 x 64 with an identity block) and the
+// inverse of its factor, by two wavefronts in lockstep over the 64 right-looking steps.
+// Wave 0, lane i = row i of the block in registers: at step k it publishes the (not yet
+// scaled) column c = A^(k)[:, k] to LDS; then every row finalises L[i][k] = c_i / sqrt(c_k)
+// and applies the rank-1 update A^(k+1)[i][j] = A^(k)[i][j] - c_i c_j / c_k (j > k).
+// Wave 1, lane j = column j of X = L^{-1}, runs the column-oriented forward substitution on
+// the same broadcast column: X[k][j] = x_j[k] / L[k][k], x_j[i] -= L[i][k] X[k][j] (i > k).
+// One barrier per step; the column buffer is double-buffered so that a wave one step ahead
+// never overwrites values the other wave still reads.  L goes back to A (lower triangle);
+// U = L^{-T} (64 x 64, zero below the diagonal and beyond nb) goes to Ut, so that the panel
+// solve X = P L^{-T} is the product P U.
+__global__ void __launch_bounds__(128) chol_diag_kernel(int nb, double* A, long lda, double* Ut, int* fail) {
+  __shared__ __attribute__((aligned(16))) double col[2][64];
+  __shared__ double Ls[64][65];  // Ls[k][i] = block entry (i, k): input, then L
+  __shared__ double Us[64][65];  // Us[k][j] = X[k][j] = U[j][k]
+  const int lane = threadIdx.x & 63;
+  for (int t = threadIdx.x; t < 64 * 64; t += 128) {
+    const int i = t & 63, j = t >> 6;
+    double x = (i == j) ? 1.0 : 0.0;
+    if (i < nb && j < nb && i >= j) x = A[i + lda * j];
+    Ls[j][i] = x;
   }
   __syncthreads();
-  for (int k = 0; k < nb; ++k) {
-    if (threadIdx.x == 0) {
-      double p = S[k][k];
+  // wave-uniform role: each wave runs its own straight-line loop; both meet at the same
+  // 64 barriers
+  const bool fact = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
+  double v[64];
+  if (fact) {
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) v[j] = Ls[j][lane];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+      double* c = col[k & 1];
+      c[lane] = v[k];
+      __syncthreads();
+      double p = c[k];
       if (!(p > 0.0)) {
-        *fail = 1;
+        bad = true;
         p = 1.0;
       }
-      S[k][k] = sqrt(p);
+      const double piv = sqrt(p);
+      const double inv = 1.0 / piv;
+      // row lane: L[lane][k] final (rows above k keep zero); one reciprocal per step keeps
+      // the division sequences (and their registers) out of the unrolled loop
+      Ls[k][lane] = (lane > k) ? v[k] * inv : (lane == k ? piv : 0.0);
+      const double t = (lane > k) ? v[k] * (inv * inv) : 0.0;
+#pragma unroll
+      for (int j = k + 1; j < 64; ++j) {
+        v[j] -= t * c[j];
+        // broadcast reads in groups of 8: bounded register pressure
+        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
     }
-    __syncthreads();
-    const double piv = S[k][k];
-    for (int i = k + 1 + threadIdx.x; i < nb; i += blockDim.x) S[i][k] /= piv;
-    __syncthreads();
-    const int m = nb - k - 1;
-    for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
-      const int i = k + 1 + t % m, j = k + 1 + t / m;
-      if (i >= j) S[i][j] -= S[i][k] * S[j][k];
+    if (bad && lane == 0) *fail = 1;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) v[i] = (lane == i) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+      const double* c = col[k & 1];
+      __syncthreads();
+      double p = c[k];
+      if (!(p > 0.0)) p = 1.0;
+      const double inv = 1.0 / sqrt(p);
+      // column lane: X[k][lane] final
+      const double y = v[k] * inv;
+      Us[k][lane] = y;
+      const double sc = y * inv;
+#pragma unroll
+      for (int i = k + 1; i < 64; ++i) v[i] -= c[i] * sc;
     }
-    __syncthreads();
-  }
-  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
-    const int i = t % nb, j = t / nb;
-    if (i >= j) A[i + lda * j] = S[i][j];
   }
   __syncthreads();
-  for (int j = nb - 1; j >= 0; --j) {
-    if (threadIdx.x == 0) S[j][j] = 1.0 / S[j][j];
-    __syncthreads();
-    for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) {
-      double acc = 0.0;
-      for (int k = j + 1; k <= i; ++k) acc += S[i][k] * S[k][j];
-      y[i] = acc;
-    }
-    __syncthreads();
-    for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) S[i][j] = -S[j][j] * y[i];
-    __syncthreads();
-  }
-  for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
-    const int i = t % 64, j = t / 64;
-    Linv[i + 64 * j] = (i < nb && j < nb && i >= j) ? S[i][j] : 0.0;
+  for (int t = threadIdx.x; t < 64 * 64; t += 128) {
+    const int i = t & 63, j = t >> 6;
+    if (i < nb && j < nb && i >= j) A[i + lda * j] = Ls[j][i];
+    Ut[t] = (i < nb && j < nb) ? Us[j][i] : 0.0;
   }
 }
 
-hipError_t launch_chol_diag(int nb, double* A, long lda, double* Linv, int* fail, hipStream_t s) {
+hipError_t launch_chol_diag(int nb, double* A, long lda, double* Ut, int* fail, hipStream_t s) {
   if (nb > 64 || nb < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, nb, A, lda, Linv, fail);
+  hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(128), 0, s, nb, A, lda, Ut, fail);
+  return hipGetLastError();
+}
+
+// G[i][i] += factor * trace(G) for an n x n matrix (one workgroup: the trace, then the shift)
+__global__ void __launch_bounds__(256) add_diag_trace_kernel(int n, double factor, double* G, long ld) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += G[i + ld * i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double shift = factor * red[0];
+  for (int i = threadIdx.x; i < n; i += 256) G[i + ld * i] += shift;
+}
+
+hipError_t launch_add_diag_trace(int n, double factor, double* G, long ld, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(add_diag_trace_kernel, dim3(1), dim3(256), 0, s, n, factor, G, ld);
   return hipGetLastError();
 }
 

@@ -49,6 +49,9 @@ class DevArray:
 @pytest.mark.parametrize("ta,tb,m,n,k", [
     ("N", "N", 64, 64, 64), ("N", "N", 130, 77, 45), ("T", "N", 33, 129, 70), ("N", "T", 257, 100, 31),
     ("T", "T", 5, 3, 2), ("N", "N", 1, 1, 1), ("N", "N", 640, 512, 96), ("T", "N", 1024, 1024, 4096),
+    # 64 x 128 tiles with paired loads, all four operand orientations; K % 16 != 0 falls back
+    ("N", "N", 2048, 1536, 64), ("T", "N", 1920, 1664, 80), ("N", "T", 2048, 1536, 48),
+    ("T", "T", 2048, 1536, 32), ("N", "N", 2048, 1536, 33),
 ])
 def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
     rng = np.random.default_rng(m * 7 + n * 3 + k)
@@ -65,7 +68,8 @@ def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
     assert rel(got, ref) < 1e-13
 
 
-@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 130, 77, 45), ("T", "N", 1024, 1024, 4096), ("N", "T", 640, 512, 96)])
+@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 130, 77, 45), ("T", "N", 1024, 1024, 4096), ("N", "T", 640, 512, 96),
+                                         ("N", "N", 2048, 1536, 256)])
 def test_gemm_fp32_mfma(ctx, ta, tb, m, n, k):
     # FP32 products/accumulation on FP64 data: error ~ 2^-24 sqrt(k) relative
     rng = np.random.default_rng(m + n + k)
@@ -236,6 +240,22 @@ def test_canonicalize_raw_tensors(imps, ctx, oracle):
     for l in range(2):
         assert rel(psi.C[l], ref.C[l]) < 1e-8
         assert rel(psi.AL[l], ref.AL[l]) < 1e-8
+
+
+def test_canonicalize_rank_deficient_tensor(imps, ctx, oracle):
+    # A[:, :, D-1] = 0 gives the left QRs an exactly zero column: the shifted first Cholesky
+    # QR pass keeps it zero, the second pass meets a zero pivot, and the Householder QRpos
+    # fallback has to produce the factorisation
+    H = imps.models.tfim_hamiltonian(nsites=1)
+    As = imps.random_tensors(8, H.phydims(), seed=5)
+    As[0][:, :, 7] = 0.0
+    psi, _ = imps.canonicalimps(As, H, ctx=ctx)
+    ref = oracle.canonical_from_tensors(As)
+    el, er, em = oracle.mixedcanonical_errors(oracle.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC))
+    assert max(el, er) < 1e-12 and em < 1e-10
+    sv = np.linalg.svd(psi.C[0], compute_uv=False)
+    assert sv[-1] < 1e-12 * sv[0]
+    assert np.abs(sv - np.linalg.svd(ref.C[0], compute_uv=False)).max() < 1e-10 * sv[0]
 
 
 @pytest.mark.parametrize("N,D,seed", [(1, 16, 0), (2, 12, 0)])
