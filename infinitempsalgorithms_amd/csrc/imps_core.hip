@@ -79,6 +79,7 @@ struct imps_ctx {
   std::map<std::string, DevBuf> ws;
   double* hpart = nullptr;
   double* dpart = nullptr;
+  hipEvent_t ev_coef = nullptr;  // marks the arrival of Gram-Schmidt coefficients in hpart
   // counters
   long hac_applies = 0, hc_applies = 0;
 };
@@ -166,26 +167,45 @@ void axpby(imps_ctx* c, size_t n, double a, const double* x, double b, double* y
 void fill(imps_ctx* c, size_t n, double v, double* x) { HIPCHK(launch_fill(n, v, x, c->stream)); }
 
 // Classical Gram-Schmidt (two passes) of w against V[0..nv) followed by the
-// normalisation of w, with the coefficients reduced and applied on the device: one host
-// synchronisation.  h[0..nv) receives the summed coefficients and *beta the norm of w
-// before normalisation (w is left as is when that norm is zero).
-void cgs2_normalize(imps_ctx* c, size_t n, int nv, const double* V, double* w, double* h, double* beta) {
+// normalisation of w, with the coefficients reduced and applied on the device.
+// cgs2_enqueue only enqueues (the coefficients travel to pinned host memory behind an
+// event), so that work depending on the normalised w can be enqueued before the host
+// reads them; cgs2_wait blocks until they arrived: h[0..nv) receives the summed
+// coefficients and *beta the norm of w before normalisation (w is left as is when that norm
+// is zero).
+void cgs2_enqueue(imps_ctx* c, size_t n, int nv, const double* V, double* w) {
   require(nv <= kMaxVec, IMPS_E_ARG, "cgs2: too many vectors");
   double* coef = ws(c, "ks_coef", 2 * kMaxVec + 2);
   const int nb = dot_blocks(n);
   HIPCHK(launch_multidot(n, nv, V, n, w, c->dpart, c->stream));
   HIPCHK(launch_partials_reduce(nv, nb, c->dpart, coef, c->stream));
-  HIPCHK(launch_multiaxpy_dev(n, nv, V, n, coef, -1.0, 1.0, w, c->stream));
-  HIPCHK(launch_multidot(n, nv, V, n, w, c->dpart, c->stream));
-  HIPCHK(launch_partials_reduce(nv, nb, c->dpart, coef + kMaxVec, c->stream));
-  HIPCHK(launch_multiaxpy_dev(n, nv, V, n, coef + kMaxVec, -1.0, 1.0, w, c->stream));
-  HIPCHK(launch_multidot(n, 1, w, n, w, c->dpart, c->stream));
+  if (nv <= 32) {
+    // fused sweeps: three reads of the basis instead of four
+    HIPCHK(launch_axpy_reduce(n, nv, V, n, coef, w, c->dpart, true, c->stream));
+    HIPCHK(launch_partials_reduce(nv, nb, c->dpart, coef + kMaxVec, c->stream));
+    HIPCHK(launch_axpy_reduce(n, nv, V, n, coef + kMaxVec, w, c->dpart, false, c->stream));
+  } else {
+    HIPCHK(launch_multiaxpy_dev(n, nv, V, n, coef, -1.0, 1.0, w, c->stream));
+    HIPCHK(launch_multidot(n, nv, V, n, w, c->dpart, c->stream));
+    HIPCHK(launch_partials_reduce(nv, nb, c->dpart, coef + kMaxVec, c->stream));
+    HIPCHK(launch_multiaxpy_dev(n, nv, V, n, coef + kMaxVec, -1.0, 1.0, w, c->stream));
+    HIPCHK(launch_multidot(n, 1, w, n, w, c->dpart, c->stream));
+  }
   HIPCHK(launch_partials_reduce(1, nb, c->dpart, coef + 2 * kMaxVec, c->stream));
   HIPCHK(launch_scale_inv_sqrt(n, w, coef + 2 * kMaxVec, c->stream));
   HIPCHK(hipMemcpyAsync(c->hpart, coef, sizeof(double) * (2 * kMaxVec + 1), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipEventRecord(c->ev_coef, c->stream));
+}
+
+void cgs2_wait(imps_ctx* c, int nv, double* h, double* beta) {
+  HIPCHK(hipEventSynchronize(c->ev_coef));
   for (int i = 0; i < nv; ++i) h[i] = c->hpart[i] + c->hpart[kMaxVec + i];
   *beta = std::sqrt(std::max(0.0, c->hpart[2 * kMaxVec]));
+}
+
+void cgs2_normalize(imps_ctx* c, size_t n, int nv, const double* V, double* w, double* h, double* beta) {
+  cgs2_enqueue(c, n, nv, V, w);
+  cgs2_wait(c, nv, h, beta);
 }
 
 // tr(l v) = sum l[a + ldl*b] v[b + ldv*a] for n x n (contiguous l and v)
@@ -451,10 +471,22 @@ double lanczos(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xo
   int numops = 0, numiter = 1, K = 0;
   double beta = 0.0;
   std::vector<double> h(kmax + 1);
+  // Expansion k: V_{k+1} = normalised (op V_k) orthogonalised against V_0..V_k.  The next
+  // application op(V_{k+1}) is enqueued right behind the Gram-Schmidt kernels, before the host
+  // waits for the coefficients and runs the convergence test, so the device does not idle
+  // on the host; when the test converges (or a restart follows) that application is simply
+  // not used.
+  bool pending = false;  // op(V_K) already enqueued
   auto expand = [&](int k) {
-    op(Vc(k), Vc(k + 1));
+    if (!pending) op(Vc(k), Vc(k + 1));
+    pending = false;
     numops++;
-    cgs2_normalize(c, n, k + 1, V, Vc(k + 1), h.data(), &beta);
+    cgs2_enqueue(c, n, k + 1, V, Vc(k + 1));
+    if (k + 1 < kmax) {
+      op(Vc(k + 1), Vc(k + 2));
+      pending = true;
+    }
+    cgs2_wait(c, k + 1, h.data(), &beta);
     for (int i = 0; i <= k; ++i) {
       T[i + (size_t)ldt * k] = h[i];
       T[k + (size_t)ldt * i] = h[i];
@@ -466,7 +498,8 @@ double lanczos(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xo
   int converged = 0;
   while (true) {
     converged = 0;
-    if (K == kmax || beta <= tol || eager) {
+    const bool check = (K == kmax || beta <= tol || eager);
+    if (check) {
       std::vector<double> A((size_t)K * K);
       for (int j = 0; j < K; ++j)
         for (int i = 0; i < K; ++i)
@@ -538,11 +571,20 @@ double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, d
   while (true) {
     std::fill(H.begin(), H.end(), 0.0);
     K = 0;
+    // the next application is enqueued behind the Gram-Schmidt kernels, before the host waits
+    // for the coefficients and computes the Ritz pair; unused if that test converges
+    bool pending = false;  // op(V_K) already enqueued
     while (true) {
-      op(Vc(K), Vc(K + 1));
+      if (!pending) op(Vc(K), Vc(K + 1));
+      pending = false;
       numops++;
       double beta = 0.0;
-      cgs2_normalize(c, n, K + 1, V, Vc(K + 1), h.data(), &beta);
+      cgs2_enqueue(c, n, K + 1, V, Vc(K + 1));
+      if (K + 1 < kmax) {
+        op(Vc(K + 1), Vc(K + 2));
+        pending = true;
+      }
+      cgs2_wait(c, K + 1, h.data(), &beta);
       for (int i = 0; i <= K; ++i) H[i + (size_t)ldh * K] = h[i];
       H[(K + 1) + (size_t)ldh * K] = beta;
       K++;
@@ -1313,6 +1355,7 @@ int imps_create(int device, imps_ctx** out) {
     RBCHK(rocblas_set_stream(c->rb, c->stream));
     HIPCHK(hipHostMalloc((void**)&c->hpart, sizeof(double) * kMaxDotBlocks * kMaxVec, hipHostMallocDefault));
     c->dpart = dalloc((size_t)kMaxDotBlocks * kMaxVec);
+    HIPCHK(hipEventCreateWithFlags(&c->ev_coef, hipEventDisableTiming));
   } catch (const ImpsError& e) {
     g_create_error = e.what();
     if (c->rb) rocblas_destroy_handle(c->rb);
@@ -1333,6 +1376,7 @@ int imps_destroy(imps_ctx* c) {
   for (auto& kv : c->ws) dfree(kv.second.p);
   dfree(c->dpart);
   if (c->hpart) (void)hipHostFree(c->hpart);
+  if (c->ev_coef) (void)hipEventDestroy(c->ev_coef);
   if (c->rb) rocblas_destroy_handle(c->rb);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -53,6 +53,10 @@ hipError_t launch_gsand(int D, int w, int d, const double* T1, const double* T3,
 int dot_blocks(size_t n);
 hipError_t launch_multidot(size_t n, int nv, const double* X, size_t ldx, const double* y,
                            double* part, hipStream_t s);
+// fused Gram-Schmidt sweep (nv <= 32, coef on the device): w -= X coef, then part receives the
+// per-block shares of X_v . w (dots) or of w . w (!dots), laid out as launch_multidot's
+hipError_t launch_axpy_reduce(size_t n, int nv, const double* X, size_t ldx, const double* coef, double* w,
+                              double* part, bool dots, hipStream_t s);
 // y = alpha*y + sum_v coef[v] * X[v]  (nv <= 64 coefficients passed by value)
 hipError_t launch_multiaxpy(size_t n, int nv, const double* X, size_t ldx, const double* coef,
                             double alpha, double* y, hipStream_t s);

@@ -661,21 +661,24 @@ __device__ inline void block_sum_store(double (&acc)[NV], int nvalid, double* pa
   __syncthreads();
 }
 
+// part[v nb + b] = block b's share of X_v . y: vectors in chunks of 32, so y is read once
+// per 32 vectors
 __global__ void __launch_bounds__(RED_THREADS) multidot_kernel(size_t n, int nv, const double* __restrict__ X,
                                                                size_t ldx, const double* __restrict__ y,
                                                                double* __restrict__ part) {
+  constexpr int CH = 32;
   const int nb = gridDim.x;
-  for (int vc = 0; vc < nv; vc += 8) {
-    double acc[8];
+  for (int vc = 0; vc < nv; vc += CH) {
+    double acc[CH];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) acc[t] = 0.0;
+    for (int t = 0; t < CH; ++t) acc[t] = 0.0;
     for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)nb * RED_THREADS) {
       const double yi = y[i];
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
+      for (int t = 0; t < CH; ++t)
         if (vc + t < nv) acc[t] += X[(size_t)(vc + t) * ldx + i] * yi;
     }
-    block_sum_store<8>(acc, nv - vc, part + (size_t)vc * nb, nb, blockIdx.x);
+    block_sum_store<CH>(acc, nv - vc, part + (size_t)vc * nb, nb, blockIdx.x);
   }
 }
 
@@ -683,6 +686,54 @@ hipError_t launch_multidot(size_t n, int nv, const double* X, size_t ldx, const 
                            double* part, hipStream_t s) {
   const int nb = dot_blocks(n);
   hipLaunchKernelGGL(multidot_kernel, dim3(nb), dim3(RED_THREADS), 0, s, n, nv, X, ldx, y, part);
+  return hipGetLastError();
+}
+
+// Fused Gram-Schmidt sweeps over w (nv <= 32 basis vectors, coefficients on the device):
+//   DOTS: w -= X coef, then part <- block shares of X_v . w (the next pass's coefficients)
+//   NORM: w -= X coef, then part <- block shares of w . w
+// Each basis vector is read once per sweep: the subtraction and the following reduction
+// share the loads.
+template <bool DOTS>
+__global__ void __launch_bounds__(RED_THREADS) axpy_reduce_kernel(size_t n, int nv, const double* __restrict__ X,
+                                                                  size_t ldx, const double* __restrict__ coef,
+                                                                  double* __restrict__ w, double* __restrict__ part) {
+  constexpr int MAXV = 32;
+  constexpr int NA = DOTS ? MAXV : 1;
+  __shared__ double c[MAXV];
+  if (threadIdx.x < MAXV) c[threadIdx.x] = (threadIdx.x < (unsigned)nv) ? coef[threadIdx.x] : 0.0;
+  __syncthreads();
+  const int nb = gridDim.x;
+  double acc[NA];
+#pragma unroll
+  for (int t = 0; t < NA; ++t) acc[t] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)nb * RED_THREADS) {
+    double xv[MAXV];
+    double wi = w[i];
+#pragma unroll
+    for (int v = 0; v < MAXV; ++v) {
+      xv[v] = (v < nv) ? X[(size_t)v * ldx + i] : 0.0;
+      wi -= c[v] * xv[v];
+    }
+    w[i] = wi;
+    if (DOTS) {
+#pragma unroll
+      for (int v = 0; v < NA; ++v) acc[v] += xv[v] * wi;
+    } else {
+      acc[0] += wi * wi;
+    }
+  }
+  block_sum_store<NA>(acc, DOTS ? nv : 1, part, nb, blockIdx.x);
+}
+
+hipError_t launch_axpy_reduce(size_t n, int nv, const double* X, size_t ldx, const double* coef, double* w,
+                              double* part, bool dots, hipStream_t s) {
+  if (nv > 32 || nv < 1) return hipErrorInvalidValue;
+  const int nb = dot_blocks(n);
+  if (dots)
+    hipLaunchKernelGGL(axpy_reduce_kernel<true>, dim3(nb), dim3(RED_THREADS), 0, s, n, nv, X, ldx, coef, w, part);
+  else
+    hipLaunchKernelGGL(axpy_reduce_kernel<false>, dim3(nb), dim3(RED_THREADS), 0, s, n, nv, X, ldx, coef, w, part);
   return hipGetLastError();
 }
 
