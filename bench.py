@@ -32,7 +32,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 # HBM bytes per launch of the dominant H_AC GEMM measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (FETCH_SIZE scaled by 1.995 from a copy kernel of known size); keyed by
 # (D, precision, kernel).  profiles/hac_d1024_hubbard_traffic.md holds the runs.
-TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 662.3 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 606.7 * 2 ** 20}
+TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 1036.9 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 607.9 * 2 ** 20}
 
 
 def log(msg):
@@ -69,6 +69,12 @@ def main():
     ap.add_argument("--prec", default="fp64", choices=sorted(PREC))
     ap.add_argument("--D", type=int, default=1024)
     ap.add_argument("--no-vumps", action="store_true", help="skip the VUMPS iteration timing")
+    ap.add_argument("--vumps-only", action="store_true", help="only the VUMPS iterations (for kernel traces)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="batched parameter sweep: 64 TFIM instances, h in U[0.5,1.5], D in {32,64,128,256}")
+    ap.add_argument("--sweep-workers", type=int, default=8)
+    ap.add_argument("--gradient", action="store_true",
+                    help="energy gradients: TFIM D=128 de/dh and XXZ D=512 de/dDelta against finite differences")
     ap.add_argument("--vumps-iters", type=int, default=3)
     ap.add_argument("--ttg", action="store_true", help="run VUMPS to Galerkin < 1e-8 (long)")
     ap.add_argument("--ttg-maxiter", type=int, default=2000)
@@ -93,7 +99,30 @@ def main():
     F = 4.0 * w * d * D ** 3
     out = {"metric": METRIC}
 
+    if args.sweep or args.gradient:
+        if args.sweep:
+            out["sweep"] = parameter_sweep(args, imps, prec)
+        if args.gradient:
+            out["gradient"] = gradients(imps)
+        print(json.dumps(out), flush=True)
+        return
+
     # ---- H_AC throughput on device-resident data
+    if not args.vumps_only:
+        hac_throughput(args, imps, _lib, H, prec, out, D, d, w, N, F)
+
+    # ---- VUMPS outer iterations on the same configuration
+    if not args.no_vumps or args.ttg or args.vumps_only:
+        vumps_iterations(args, imps, H, prec, out, D, N)
+
+    if not args.no_cpu and not args.vumps_only:
+        log("bench: CPU baseline (numpy restatement)")
+        out["cpu_baseline"] = cpu_baseline(D, d, w)
+    print(json.dumps(out), flush=True)
+
+
+def hac_throughput(args, imps, _lib, H, prec, out, D, d, w, N, F):
+    import numpy as np
     ctx = imps.Context(0)
     ctx.set_mpo(H)
     rng = np.random.default_rng(7)
@@ -156,49 +185,135 @@ def main():
         "kernels": kernels,
     })
 
-    # ---- VUMPS outer iterations on the same configuration
-    if not args.no_vumps or args.ttg:
-        log(f"bench: canonicalising a random D={D} state (PCG64 seed 2) on the device")
+
+def tfim_exact_energy(h, J=1.0, n=200000):
+    """Ground-state energy per site of -J sum sx sx - h sum sz (free fermions):
+    -(1/pi) int_0^pi sqrt(J^2 + h^2 - 2 J h cos k) dk, midpoint rule."""
+    import numpy as np
+    k = (np.arange(n) + 0.5) * np.pi / n
+    return -float(np.mean(np.sqrt(J * J + h * h - 2 * J * h * np.cos(k))))
+
+
+def parameter_sweep(args, imps, prec):
+    """64 TFIM instances (N=1, d=2, w=3): h_k ~ U[0.5, 1.5] and D_k in {32, 64, 128, 256},
+    both drawn from PCG64(3); instance k starts from random_tensors(D_k, [2], seed=k).
+    Work = sum_k (H_AC applications of k) x 4 w d D_k^3."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(3))
+    hs = rng.uniform(0.5, 1.5, 64)
+    Ds = rng.choice([32, 64, 128, 256], 64)
+    problems = [(imps.models.tfim_hamiltonian(J=1.0, h=float(h), nsites=1), imps.random_tensors(int(D), [2], seed=k))
+                for k, (h, D) in enumerate(zip(hs, Ds))]
+    alg = imps.VUMPS(tol=1e-8, maxiter=200, prec=prec, refine_below=args.refine_below)
+    log(f"bench: batched sweep of {len(problems)} TFIM instances, {args.sweep_workers} workers")
+    t0 = time.perf_counter()
+    res = imps.find_groundstates_batched(problems, alg, nworkers=args.sweep_workers)
+    wall = time.perf_counter() - t0
+    flop = sum(r["hac_applies"] * 4.0 * 3 * 2 * float(D) ** 3 for r, D in zip(res, Ds))
+    err = [abs(r["energy"] - tfim_exact_energy(float(h))) for r, h in zip(res, hs)]
+    return {"instances": len(res), "workers": args.sweep_workers, "prec": args.prec, "wall_seconds": wall,
+            "converged": sum(r["converged"] for r in res), "total_iterations": sum(r["niter"] for r in res),
+            "hac_applies": sum(r["hac_applies"] for r in res), "hac_tflops": flop / wall / 1e12,
+            "max_itererr": max(r["itererr"] for r in res),
+            "energy_error_vs_exact": {"max": max(err), "median": float(np.median(err))},
+            "per_D": {int(D): {"n": int(sum(Ds == D)), "mean_seconds": float(np.mean([r["seconds"] for r, x in zip(res, Ds) if x == D])),
+                               "mean_iters": float(np.mean([r["niter"] for r, x in zip(res, Ds) if x == D]))}
+                      for D in (32, 64, 128, 256)}}
+
+
+def gradients(imps):
+    """de/dh of TFIM (D=128, h=1.3) against -<sz> and central differences of the converged
+    energy (step 1e-5), and de/dDelta of XXZ (D=512, Delta drawn from U[0.5,1.5] with PCG64(5))
+    against central differences; the gradient contracts imps_energy_gradient with dW/dparameter
+    (the MPOs are linear in h and Delta)."""
+    import numpy as np
+    out = {}
+
+    def solve(H, D, seed, tol, psi0=None, maxiter=1000):
         ctx = imps.Context(0)
+        if psi0 is None:
+            psi0, ctx = imps.randomimps(D, H, seed=seed, ctx=ctx, tol=1e-12, maxiter=100)
+        envs = imps.GPUCache(psi0, H, ctx)
         t0 = time.perf_counter()
-        psi, ctx = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=100)
-        t_setup = time.perf_counter() - t0
-        envs = imps.GPUCache(psi, H, ctx)
-        maxit = args.ttg_maxiter if args.ttg else args.vumps_iters
-        tstart = time.perf_counter()
+        psi, envs, info = imps.find_groundstate(psi0, H, imps.VUMPS(tol=tol, maxiter=maxiter), envs=envs)
+        log(f"bench: VUMPS D={D}: {info.niter} iterations, eps {info.itererr:.2e}, {time.perf_counter() - t0:.1f}s")
+        return psi, envs, info, ctx
 
-        trace = []
-
-        def cb(it, eps, f):
-            el = time.perf_counter() - tstart
-            trace.append((it, eps, el))
-            log(f"bench: VUMPS iter {it} eps {eps:.3e} e/site {f / N:.12f} t {el:.1f}s")
-            return args.ttg and el > args.ttg_seconds
-
-        alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec, refine_below=args.refine_below)
-        _, _, info = imps.find_groundstate(psi, H, alg, envs=envs, callback=cb)
-        rec = {"setup_seconds": t_setup, "iterations": info.niter, "loop_seconds": info.seconds,
-               "seconds_per_iteration": info.seconds / max(info.niter, 1), "itererr": info.itererr,
-               "energy_per_site": info.energy / N, "hac_applies": info.hac_applies,
-               "hc_applies": info.hc_applies, "phases_seconds": info.timings}
-        if args.ttg:
-            marks = {}
-            for it, eps, el in trace:
-                for thr in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8):
-                    if eps <= thr and thr not in marks:
-                        marks[thr] = {"iter": it, "seconds": el}
-            out["time_to_galerkin"] = {args.prec: {**rec, "seconds": info.t_galerkin_reached if info.converged else None,
-                                                   "converged": info.converged,
-                                                   "prec_switch_iter": info.prec_switch_iter,
-                                                   "first_reached": {f"{k:.0e}": v for k, v in marks.items()}}}
-        else:
-            out["vumps"] = rec
+    def energy_at(H, D, tol, psi0, maxiter=1000):
+        _, _, info, ctx = solve(H, D, 0, tol, psi0=psi0, maxiter=maxiter)
         ctx.close()
+        return info.energy
 
-    if not args.no_cpu:
-        log("bench: CPU baseline (numpy restatement)")
-        out["cpu_baseline"] = cpu_baseline(D, d, w)
-    print(json.dumps(out), flush=True)
+    def grad_along(psi, H, Hp, envs):
+        t0 = time.perf_counter()
+        G = imps.energy_gradient(psi, H, envs)
+        dt = time.perf_counter() - t0
+        return sum(float(np.sum(g * (Wp - W))) for g, W, Wp in zip(G, H.Ws, Hp.Ws)), dt
+
+    # TFIM, N=1, D=128
+    h, dh = 1.3, 1e-5
+    H = imps.models.tfim_hamiltonian(J=1.0, h=h, nsites=1)
+    psi, envs, info, ctx = solve(H, 128, 0, 1e-10)
+    g, dt = grad_along(psi, H, imps.models.tfim_hamiltonian(J=1.0, h=h + 1.0, nsites=1), envs)
+    sz = imps.expectationvalue(psi, (0, imps.models.sigma_z()), envs)
+    ep = energy_at(imps.models.tfim_hamiltonian(J=1.0, h=h + dh, nsites=1), 128, 1e-10, psi)
+    em = energy_at(imps.models.tfim_hamiltonian(J=1.0, h=h - dh, nsites=1), 128, 1e-10, psi)
+    fd = (ep - em) / (2 * dh)
+    out["tfim_D128"] = {"h": h, "itererr": info.itererr, "de_dh": g, "minus_sz": -sz, "fd": fd,
+                        "abs_err_vs_sz": abs(g + sz), "abs_err_vs_fd": abs(g - fd), "gradient_seconds": dt}
+    ctx.close()
+    # XXZ, N=2, D=512
+    delta = float(np.random.Generator(np.random.PCG64(5)).uniform(0.5, 1.5))
+    dd = 1e-5
+    H = imps.models.heisenberg_hamiltonian(J=1.0, Delta=delta, nsites=2)
+    psi, envs, info, ctx = solve(H, 512, 1, 1e-9, maxiter=300)
+    g, dt = grad_along(psi, H, imps.models.heisenberg_hamiltonian(J=1.0, Delta=delta + 1.0, nsites=2), envs)
+    ep = energy_at(imps.models.heisenberg_hamiltonian(J=1.0, Delta=delta + dd, nsites=2), 512, 1e-9, psi, maxiter=300)
+    em = energy_at(imps.models.heisenberg_hamiltonian(J=1.0, Delta=delta - dd, nsites=2), 512, 1e-9, psi, maxiter=300)
+    fd = (ep - em) / (2 * dd)
+    out["xxz_D512"] = {"Delta": delta, "itererr": info.itererr, "de_dDelta_cell": g, "fd": fd,
+                       "abs_err_vs_fd": abs(g - fd), "gradient_seconds": dt}
+    ctx.close()
+    return out
+
+
+def vumps_iterations(args, imps, H, prec, out, D, N):
+    log(f"bench: canonicalising a random D={D} state (PCG64 seed 2) on the device")
+    ctx = imps.Context(0)
+    t0 = time.perf_counter()
+    psi, ctx = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=100)
+    t_setup = time.perf_counter() - t0
+    envs = imps.GPUCache(psi, H, ctx)
+    maxit = args.ttg_maxiter if args.ttg else args.vumps_iters
+    tstart = time.perf_counter()
+
+    trace = []
+
+    def cb(it, eps, f):
+        el = time.perf_counter() - tstart
+        trace.append((it, eps, el))
+        log(f"bench: VUMPS iter {it} eps {eps:.3e} e/site {f / N:.12f} t {el:.1f}s")
+        return args.ttg and el > args.ttg_seconds
+
+    alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec, refine_below=args.refine_below)
+    _, _, info = imps.find_groundstate(psi, H, alg, envs=envs, callback=cb)
+    rec = {"setup_seconds": t_setup, "iterations": info.niter, "loop_seconds": info.seconds,
+           "seconds_per_iteration": info.seconds / max(info.niter, 1), "itererr": info.itererr,
+           "energy_per_site": info.energy / N, "hac_applies": info.hac_applies,
+           "hc_applies": info.hc_applies, "phases_seconds": info.timings}
+    if args.ttg:
+        marks = {}
+        for it, eps, el in trace:
+            for thr in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8):
+                if eps <= thr and thr not in marks:
+                    marks[thr] = {"iter": it, "seconds": el}
+        out["time_to_galerkin"] = {args.prec: {**rec, "seconds": info.t_galerkin_reached if info.converged else None,
+                                               "converged": info.converged,
+                                               "prec_switch_iter": info.prec_switch_iter,
+                                               "first_reached": {f"{k:.0e}": v for k, v in marks.items()}}}
+    else:
+        out["vumps"] = rec
+    ctx.close()
 
 
 if __name__ == "__main__":

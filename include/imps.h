@@ -247,6 +247,7 @@ typedef struct imps_batch_instance {
   double itererr;
   double energy; /* unit-cell energy */
   double seconds;
+  int hac_applies; /* H_AC applications of the instance (canonicalisation excluded) */
 } imps_batch_instance;
 
 /* Runs find_groundstate (vumps.jl:51-86) on every instance with the parameters `p`,

@@ -56,7 +56,7 @@ class HacTiming(ctypes.Structure):
 class BatchInstance(ctypes.Structure):
     _fields_ = [("nsites", c_int), ("w", c_int), ("d", c_int), ("D", c_int), ("W", PP_double), ("A", PP_double),
                 ("status", c_int), ("niter", c_int), ("converged", c_int), ("itererr", c_double),
-                ("energy", c_double), ("seconds", c_double)]
+                ("energy", c_double), ("seconds", c_double), ("hac_applies", c_int)]
 
 
 ITER_CB = ctypes.CFUNCTYPE(c_int, c_int, c_double, c_double, c_void_p)

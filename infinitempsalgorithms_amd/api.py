@@ -472,7 +472,7 @@ def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, de
     """Batched parameter sweep: find_groundstate (vumps.jl:51-86) for every
     (H::SparseIMPO, raw tensors As) in `problems`, the instances running concurrently on
     the GPU (one context and stream per worker thread).  Returns one dict per problem
-    with niter, converged, itererr, energy (unit cell) and seconds."""
+    with niter, converged, itererr, energy (unit cell), seconds and hac_applies."""
     alg = alg or VUMPS()
     L = _lib.load()
     keep = []
@@ -483,11 +483,11 @@ def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, de
         pw, kw = ptr_array(Ws)
         pa, ka = ptr_array(As)
         keep.append((Ws, As, kw, ka))
-        arr[k] = _lib.BatchInstance(len(Ws), H.w, H.d, As[0].shape[0], pw, pa, 0, 0, 0, 0.0, 0.0, 0.0)
+        arr[k] = _lib.BatchInstance(len(Ws), H.w, H.d, As[0].shape[0], pw, pa, 0, 0, 0, 0.0, 0.0, 0.0, 0)
     p = alg.params()
     st = L.imps_vumps_batch(int(device), len(problems), arr, ctypes.byref(p), int(nworkers), float(gauge_tol))
     out = [{"status": r.status, "niter": r.niter, "converged": bool(r.converged), "itererr": r.itererr,
-            "energy": r.energy, "seconds": r.seconds} for r in arr]
+            "energy": r.energy, "seconds": r.seconds, "hac_applies": r.hac_applies} for r in arr]
     if st != 0:
         bad = [k for k, r in enumerate(out) if r["status"] != 0]
         raise _lib.ImpsError(f"imps_vumps_batch failed with status {st} on instances {bad}")

@@ -1874,6 +1874,7 @@ int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imp
       in.converged = info.converged;
       in.itererr = info.itererr;
       in.energy = info.energy;
+      in.hac_applies = info.hac_applies;
       in.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       if (st != IMPS_OK) worst = st;
     }
