@@ -230,6 +230,31 @@ def test_local_update_and_gauge_match_oracle(imps, ctx, oracle):
         assert rel(np.linalg.svd(psi_g.C[l], compute_uv=False), np.linalg.svd(ref_psi.C[l], compute_uv=False)) < 1e-6
 
 
+@pytest.mark.parametrize("kdim", [12, 30, 40])
+def test_lanczos_basis_sizes_match_oracle(imps, ctx, oracle, kdim):
+    # krylovdim <= 31 runs the fused (three-sweep) Gram-Schmidt step, 40 the four-sweep one;
+    # small bases restart several times
+    D, N = 16, 2
+    W = oracle.xxz_mpo(1.0, 0.8)
+    H = imps.SparseIMPO([W] * N)
+    psi = oracle.randomimps(D, [2] * N, seed=31)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, H, tol=1e-12, ctx=ctx)
+    ref = oracle.dmrg_cache(psi, [W] * N, tol=1e-12)
+    n = 2 * D * D
+    Hm = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        Hm[:, j] = oracle.hac_apply(ref.leftenv(0), W, ref.rightenv(0), e.reshape((D, 2, D), order="F")).reshape(-1, order="F")
+    ev = np.linalg.eigvalsh(0.5 * (Hm + Hm.T))
+    lam, AC, info = imps.fixedpoint(imps.AC_hamiltonian(0, st, H, st, envs), psi.AC[0], "SR",
+                                    imps.Lanczos(tol=1e-11, krylovdim=kdim, maxiter=500))
+    assert info.converged and abs(lam - ev[0]) < 1e-9 * max(1.0, abs(ev[0]))
+    r = (Hm @ AC.reshape(-1, order="F")) - lam * AC.reshape(-1, order="F")
+    assert np.linalg.norm(r) < 1e-9 and abs(np.linalg.norm(AC) - 1.0) < 1e-12
+
+
 def test_canonicalize_raw_tensors(imps, ctx, oracle):
     H = imps.models.fermi_hubbard(nsites=2)
     As = imps.random_tensors(8, H.phydims(), seed=2)
