@@ -260,12 +260,16 @@ int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imp
 
 /* Runs `reps` H_AC applications on device-resident random data of the current
  * site-0 shapes and returns the average duration of one application and of each of
- * its kernels (HIP events on the context's stream). */
+ * its kernels (HIP events on the context's stream), and the flops of each GEMM launch as
+ * executed (2 (w-1) d D^3 when the environments' structural levels are identities, else
+ * 2 w d D^3). */
 typedef struct imps_hac_timing {
   double ms_apply;  /* whole application */
   double ms_gemm1;  /* GL x  */
   double ms_wmix;   /* W mix */
   double ms_gemm2;  /* (.) GR */
+  double flop_gemm1;
+  double flop_gemm2;
 } imps_hac_timing;
 int imps_bench_hac(imps_ctx* ctx, int site, int reps, int prec, imps_hac_timing* t);
 

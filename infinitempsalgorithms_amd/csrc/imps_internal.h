@@ -41,9 +41,10 @@ hipError_t launch_mix_mid(int D1, int d, int D2, const double* M, const double* 
 
 // H_AC W-mix: T2[b,u,bb,j] = sum_{i,s} W[i,u,j,s] T1[b,i,s,bb]
 // T1 at b + D*(i + w*(s + d*bb)), T2 at b + D*(u + d*(bb + D*j)); Wm (w*d)x(w*d) on device
-// as Wm[(u + d*j) + wd*(i + w*s)].
-hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2,
-                       hipStream_t s);
+// as Wm[(u + d*j) + wd*(i + w*s)].  X0 (optional): level i = 0 read from x at b + D*(s + d*bb)
+// instead of T1; Y (optional): channel j = w-1 written to y at b + D*(u + d*bb) instead of T2.
+hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2, const double* X0,
+                       double* Y, hipStream_t s);
 
 // Energy-gradient sandwich partials: Gp[o*D + bb] = sum_b T1[b,i,s,bb] T3[b,u,bb,j] with
 // o = (i + w s) + w d (u + d j); T1 as in launch_wmix, T3 laid out like its T2; w d <= 32.

@@ -533,9 +533,12 @@ hipError_t launch_mix_mid(int D1, int d, int D2, const double* M, const double* 
   return hipGetLastError();
 }
 
+// X0 != nullptr: level i = 0 of T1 is read from X0 (x[b, s, bb] at b + D(s + d bb)) instead;
+// Y != nullptr: channel j = w-1 is written to Y (y[b, u, bb] at b + D(u + d bb)) instead of T2.
 template <int MAXIN>
 __global__ void __launch_bounds__(256) wmix_kernel(int D, int w, int d, const double* __restrict__ Wm,
-                                                   const double* __restrict__ T1, double* __restrict__ T2) {
+                                                   const double* __restrict__ T1, double* __restrict__ T2,
+                                                   const double* __restrict__ X0, double* __restrict__ Y) {
   __shared__ double Ws[MAXIN * MAXIN];
   const int wd = w * d;
   for (int t = threadIdx.x; t < wd * wd; t += blockDim.x) Ws[t] = Wm[t];
@@ -545,32 +548,38 @@ __global__ void __launch_bounds__(256) wmix_kernel(int D, int w, int d, const do
   if (b >= D) return;
   double x[MAXIN];
   const double* src = T1 + b + (size_t)D * wd * bb;
+  const double* src0 = X0 ? X0 + b + (size_t)D * d * bb : nullptr;
 #pragma unroll
-  for (int q = 0; q < MAXIN; ++q)
-    if (q < wd) x[q] = src[(size_t)D * q];
+  for (int q = 0; q < MAXIN; ++q) {
+    if (q < wd) {
+      const int i = q % w;
+      x[q] = (src0 && i == 0) ? src0[(size_t)D * (q / w)] : src[(size_t)D * q];
+    }
+  }
   for (int j = 0; j < w; ++j) {
+    double* dst = (Y && j == w - 1) ? Y + b + (size_t)D * d * bb : T2 + b + (size_t)D * d * (bb + (size_t)D * j);
     for (int u = 0; u < d; ++u) {
       const int o = u + d * j;
       double acc = 0.0;
 #pragma unroll
       for (int q = 0; q < MAXIN; ++q)
         if (q < wd) acc += Ws[o + wd * q] * x[q];
-      T2[b + (size_t)D * (u + (size_t)d * (bb + (size_t)D * j))] = acc;
+      dst[(size_t)D * u] = acc;
     }
   }
 }
 
-hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2,
-                       hipStream_t s) {
+hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2, const double* X0,
+                       double* Y, hipStream_t s) {
   const int wd = w * d;
   dim3 block(256);
   dim3 grid((D + 255) / 256, D);
   if (wd <= 16)
-    hipLaunchKernelGGL((wmix_kernel<16>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2);
+    hipLaunchKernelGGL((wmix_kernel<16>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2, X0, Y);
   else if (wd <= 32)
-    hipLaunchKernelGGL((wmix_kernel<32>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2);
+    hipLaunchKernelGGL((wmix_kernel<32>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2, X0, Y);
   else if (wd <= 64)
-    hipLaunchKernelGGL((wmix_kernel<64>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2);
+    hipLaunchKernelGGL((wmix_kernel<64>), grid, block, 0, s, D, w, d, Wm_dev, T1, T2, X0, Y);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -103,7 +103,9 @@ def test_gemm_bf16x3_mfma(ctx, ta, tb, m, n, k):
     assert 1e-8 < err < 5e-5
 
 
-def _setup_envs(imps, ctx, oracle, Wfn, D, N, seed, sym=False):
+def _setup_envs(imps, ctx, oracle, Wfn, D, N, seed, sym=False, eye_levels=False):
+    # eye_levels: GL level 0 and GR level w-1 set to the identity, as DMRGCache pins them (the
+    # effective operators then take their copy path for those levels)
     rng = np.random.default_rng(seed)
     W = Wfn()
     H = imps.SparseIMPO([W] * N)
@@ -116,6 +118,9 @@ def _setup_envs(imps, ctx, oracle, Wfn, D, N, seed, sym=False):
     for l in range(N):
         GL = rng.standard_normal((D, w, D))
         GR = rng.standard_normal((D, w, D))
+        if eye_levels:
+            GL[:, 0, :] = np.eye(D)
+            GR[:, w - 1, :] = np.eye(D)
         ctx.set_env(l, 0, GL)
         ctx.set_env(l, 1, GR)
         GLs.append(GL)
@@ -123,10 +128,11 @@ def _setup_envs(imps, ctx, oracle, Wfn, D, N, seed, sym=False):
     return H, psi, GLs, GRs
 
 
+@pytest.mark.parametrize("eye", [False, True])
 @pytest.mark.parametrize("model,D", [("tfim", 16), ("xxz", 24), ("hubbard", 20), ("hubbard", 64), ("tfim", 7)])
-def test_hac_hc_match_oracle(imps, ctx, oracle, model, D):
+def test_hac_hc_match_oracle(imps, ctx, oracle, model, D, eye):
     Wfn = {"tfim": oracle.tfim_mpo, "xxz": oracle.xxz_mpo, "hubbard": oracle.hubbard_mpo}[model]
-    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, Wfn, D, 2, seed=D)
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, Wfn, D, 2, seed=D, eye_levels=eye)
     envs = imps.GPUCache.__new__(imps.GPUCache)
     envs.ctx, envs.ket, envs.operator = ctx, None, H
     rng = np.random.default_rng(1)
@@ -139,10 +145,11 @@ def test_hac_hc_match_oracle(imps, ctx, oracle, model, D):
         assert rel(yc, oracle.hc_apply(GLs[(site + 1) % 2], GRs[site], c)) < 1e-12
 
 
-def test_hac_full_size_hubbard(imps, ctx, oracle):
+@pytest.mark.parametrize("eye", [False, True])
+def test_hac_full_size_hubbard(imps, ctx, oracle, eye):
     # the D = 1024 Hubbard shape of the benchmark (w = 6, d = 4), against the dense einsum
     D = 1024
-    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 1, seed=2)
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 1, seed=2, eye_levels=eye)
     envs = imps.GPUCache.__new__(imps.GPUCache)
     envs.ctx, envs.ket, envs.operator = ctx, None, H
     x = np.random.default_rng(3).standard_normal((D, 4, D))
