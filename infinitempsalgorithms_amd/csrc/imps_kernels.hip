@@ -1017,10 +1017,11 @@ __global__ void __launch_bounds__(128) chol_diag_kernel(int nb, double* A, long 
         bad = true;
         p = 1.0;
       }
-      const double piv = sqrt(p);
-      const double inv = 1.0 / piv;
-      // row lane: L[lane][k] final (rows above k keep zero); one reciprocal per step keeps
-      // the division sequences (and their registers) out of the unrolled loop
+      // one reciprocal square root per step keeps division sequences (and their registers)
+      // out of the unrolled loop
+      const double inv = rsqrt(p);
+      const double piv = p * inv;
+      // row lane: L[lane][k] final (rows above k keep zero)
       Ls[k][lane] = (lane > k) ? v[k] * inv : (lane == k ? piv : 0.0);
       const double t = (lane > k) ? v[k] * (inv * inv) : 0.0;
 #pragma unroll
@@ -1040,7 +1041,7 @@ __global__ void __launch_bounds__(128) chol_diag_kernel(int nb, double* A, long 
       __syncthreads();
       double p = c[k];
       if (!(p > 0.0)) p = 1.0;
-      const double inv = 1.0 / sqrt(p);
+      const double inv = rsqrt(p);
       // column lane: X[k][lane] final
       const double y = v[k] * inv;
       Us[k][lane] = y;
