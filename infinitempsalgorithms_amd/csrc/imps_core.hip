@@ -85,6 +85,13 @@ struct imps_ctx {
   hipEvent_t ev_coef = nullptr;  // marks the arrival of Gram-Schmidt coefficients in hpart
   // counters
   long hac_applies = 0, hc_applies = 0;
+  // workers for the concurrent per-site eigensolves of the local update: each has its own
+  // stream, workspaces and reduction buffers and borrows this context's model, state and
+  // environments (device pointers); a worker is never a parent
+  std::vector<imps_ctx*> workers;
+  bool is_worker = false;
+  hipEvent_t ev_ready = nullptr;  // parent: state and environments enqueued
+  hipEvent_t ev_done = nullptr;   // worker: its solve enqueued
 };
 
 static thread_local std::string g_create_error;
@@ -1354,6 +1361,103 @@ EigOut eigsolve_c(imps_ctx* c, int site, double tol, int kdim, int maxiter, bool
   return o;
 }
 
+// ------------------------------------------------------------------ concurrent local update
+
+imps_ctx* worker(imps_ctx* c, int k) {
+  while ((int)c->workers.size() <= k) {
+    imps_ctx* w = new imps_ctx();
+    w->is_worker = true;
+    w->device = c->device;
+    c->workers.push_back(w);
+    HIPCHK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void**)&w->hpart, sizeof(double) * kMaxDotBlocks * kMaxVec, hipHostMallocDefault));
+    w->dpart = dalloc((size_t)kMaxDotBlocks * kMaxVec);
+    HIPCHK(hipEventCreateWithFlags(&w->ev_coef, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
+  }
+  imps_ctx* w = c->workers[k];
+  w->N = c->N;
+  w->w = c->w;
+  w->d = c->d;
+  w->D = c->D;
+  w->Wm = c->Wm;
+  w->AL = c->AL;
+  w->AR = c->AR;
+  w->AC = c->AC;
+  w->C = c->C;
+  w->pAC = c->pAC;
+  w->pC = c->pC;
+  w->pAL = c->pAL;
+  w->GL = c->GL;
+  w->GR = c->GR;
+  w->gl_id = c->gl_id;
+  w->gr_id = c->gr_id;
+  w->envs_valid = c->envs_valid;
+  return w;
+}
+
+// frees what a worker owns (never the borrowed model, state or environments)
+void destroy_worker(imps_ctx* w) {
+  if (w->stream) (void)hipStreamSynchronize(w->stream);
+  for (auto& kv : w->ws) dfree(kv.second.p);
+  dfree(w->dpart);
+  if (w->hpart) (void)hipHostFree(w->hpart);
+  if (w->ev_coef) (void)hipEventDestroy(w->ev_coef);
+  if (w->ev_done) (void)hipEventDestroy(w->ev_done);
+  if (w->stream) (void)hipStreamDestroy(w->stream);
+  delete w;
+}
+
+// localupdate_step! (vumps.jl:7-19): the AC and C eigensolves of all sites depend only on the
+// current state and environments, so the 2N solves run concurrently, one host thread and one
+// stream each (their device work overlaps; each solve's arithmetic is the same as when run
+// alone).  Then regauge! per site on this context's stream.
+void local_update(imps_ctx* c, const std::vector<double>& tol_ac, const std::vector<double>& tol_c, int kdim,
+                  int maxiter, int prec, std::vector<double>& lam_ac, std::vector<double>& lam_c) {
+  const int N = c->N;
+  if (!c->ev_ready) HIPCHK(hipEventCreateWithFlags(&c->ev_ready, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(c->ev_ready, c->stream));
+  const int njobs = 2 * N;
+  std::vector<imps_ctx*> ws_(njobs);
+  for (int k = 0; k < njobs; ++k) {
+    ws_[k] = worker(c, k);
+    HIPCHK(hipStreamWaitEvent(ws_[k]->stream, c->ev_ready, 0));
+  }
+  std::vector<int> code(njobs, IMPS_OK);
+  std::vector<std::string> msg(njobs);
+  std::vector<std::thread> th;
+  for (int k = 0; k < njobs; ++k) {
+    th.emplace_back([&, k] {
+      imps_ctx* w = ws_[k];
+      try {
+        HIPCHK(hipSetDevice(w->device));
+        const int site = k / 2;
+        if (k % 2 == 0)
+          lam_ac[site] = eigsolve_ac(w, site, tol_ac[site], kdim, maxiter, true, prec).lambda;
+        else
+          lam_c[site] = eigsolve_c(w, site, tol_c[site], kdim, maxiter, true, prec).lambda;
+        HIPCHK(hipEventRecord(w->ev_done, w->stream));
+      } catch (const ImpsError& e) {
+        code[k] = e.code;
+        msg[k] = e.what();
+      } catch (const std::exception& e) {
+        code[k] = IMPS_E_ARG;
+        msg[k] = e.what();
+      }
+    });
+  }
+  for (auto& t : th) t.join();
+  for (int k = 0; k < njobs; ++k)
+    if (code[k] != IMPS_OK) throw ImpsError(code[k], msg[k]);
+  for (int k = 0; k < njobs; ++k) {
+    HIPCHK(hipStreamWaitEvent(c->stream, ws_[k]->ev_done, 0));
+    c->hac_applies += ws_[k]->hac_applies;
+    c->hc_applies += ws_[k]->hc_applies;
+    ws_[k]->hac_applies = ws_[k]->hc_applies = 0;
+  }
+  for (int site = 0; site < N; ++site) regauge_site(c, site);
+}
+
 int run_guarded(imps_ctx* c, const std::function<void()>& f) {
   try {
     if (c) HIPCHK(hipSetDevice(c->device));
@@ -1402,6 +1506,9 @@ int imps_destroy(imps_ctx* c) {
   if (!c) return IMPS_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  for (imps_ctx* w : c->workers) destroy_worker(w);
+  c->workers.clear();
+  if (c->ev_ready) (void)hipEventDestroy(c->ev_ready);
   free_state(c);
   for (double*& p : c->Wm) dfree(p);
   for (auto& kv : c->ws) dfree(kv.second.p);
@@ -1831,13 +1938,12 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       // level: the eigensolver tolerance is floored there until the FP64 refinement
       const double floor_rel = prec == PREC_FP32 ? 1e-6 : (prec == PREC_BF16X3 ? 1e-4 : 0.0);
       lap(tl, [&] {
+        std::vector<double> ta(N), tc(N);
         for (int site = 0; site < N; ++site) {
-          const double ta = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_ac[site])));
-          lam_ac[site] = eigsolve_ac(c, site, ta, p->krylovdim, p->eig_maxiter, true, prec).lambda;
-          const double tc = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_c[site])));
-          lam_c[site] = eigsolve_c(c, site, tc, p->krylovdim, p->eig_maxiter, true, prec).lambda;
-          regauge_site(c, site);
+          ta[site] = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_ac[site])));
+          tc[site] = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_c[site])));
         }
+        local_update(c, ta, tc, p->krylovdim, p->eig_maxiter, prec, lam_ac, lam_c);
       });
       int sw = 0;
       lap(tg, [&] { gauge_step(c, tolu(p->gauge_tolfactor, siter, eps), p->gauge_maxiter, &sw); });
