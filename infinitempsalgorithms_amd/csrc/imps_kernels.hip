@@ -637,7 +637,9 @@ static constexpr int RED_THREADS = 256;
 static constexpr int RED_MAXBLK = 1024;
 
 int dot_blocks(size_t n) {
-  size_t nb = (n + (size_t)RED_THREADS * 16 - 1) / ((size_t)RED_THREADS * 16);
+  // about four elements per thread, at most RED_MAXBLK workgroups (sixteen elements per
+  // thread for the largest vectors): small vectors still spread over many CUs
+  size_t nb = (n + (size_t)RED_THREADS * 4 - 1) / ((size_t)RED_THREADS * 4);
   if (nb < 1) nb = 1;
   if (nb > (size_t)RED_MAXBLK) nb = RED_MAXBLK;
   return (int)nb;
