@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 
 #include "imps_internal.h"
 
@@ -107,25 +108,53 @@ __global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
     }
   };
 
+  // Stage bases (wave-uniform): K chunk kc and offset kk inside it, advanced by BK per stage
+  // (no division in the loop); VEC loads add per-thread byte offsets computed once (they fit
+  // in 31 bits: the host checks the extent of the operands).
+  int kc = (kt0 * BK) / Kc;
+  int kk = kt0 * BK - kc * Kc;
+  const double* Ab = g.A + (long)kc * g.sAc + (long)kk * g.sAk;
+  const double* Bb = g.B + (long)kc * g.sBc + (long)kk * g.sBk;
+  auto advance = [&]() {
+    kk += BK;
+    if (kk >= Kc) {
+      kk -= Kc;
+      ++kc;
+      Ab = g.A + (long)kc * g.sAc + (long)kk * g.sAk;
+      Bb = g.B + (long)kc * g.sBc + (long)kk * g.sBk;
+    } else {
+      Ab += (long)BK * g.sAk;
+      Bb += (long)BK * g.sBk;
+    }
+  };
+  // byte offsets (unsigned 32-bit): a wave-uniform base plus one VGPR per load
+  unsigned offA[VEC ? AE / 2 : 1], offB[VEC ? BE / 2 : 1];
+  if (VEC) {
+#pragma unroll
+    for (int e = 0; e < AE; e += 2) {
+      int m, k;
+      a_index(e, m, k);
+      offA[e / 2] = (unsigned)(8 * ((long)(m0 + m) * g.sAm + (long)k * g.sAk));
+    }
+#pragma unroll
+    for (int e = 0; e < BE; e += 2) {
+      int k, n;
+      b_index(e, k, n);
+      offB[e / 2] = (unsigned)(8 * ((long)k * g.sBk + (long)(n0 + n) * g.sBn));
+    }
+  }
+
   auto load_tiles = [&](int k0) {
-    const int kc = k0 / Kc;
-    const int kk0 = k0 - kc * Kc;
-    const double* Ab = g.A + (long)kc * g.sAc + (long)kk0 * g.sAk;
-    const double* Bb = g.B + (long)kc * g.sBc + (long)kk0 * g.sBk;
     if (VEC) {
 #pragma unroll
       for (int e = 0; e < AE; e += 2) {
-        int m, k;
-        a_index(e, m, k);
-        const v2d v = *(const v2d*)(Ab + (long)(m0 + m) * g.sAm + (long)k * g.sAk);
+        const v2d v = *(const v2d*)((const char*)Ab + offA[e / 2]);
         ra[e] = v[0];
         ra[e + 1] = v[1];
       }
 #pragma unroll
       for (int e = 0; e < BE; e += 2) {
-        int k, n;
-        b_index(e, k, n);
-        const v2d v = *(const v2d*)(Bb + (long)k * g.sBk + (long)(n0 + n) * g.sBn);
+        const v2d v = *(const v2d*)((const char*)Bb + offB[e / 2]);
         rb[e] = v[0];
         rb[e + 1] = v[1];
       }
@@ -169,14 +198,17 @@ __global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (kt + 1 < kt1) load_tiles((kt + 1) * BK);
+    if (kt + 1 < kt1) {
+      advance();
+      load_tiles((kt + 1) * BK);
+    }
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
+    for (int k4 = 0; k4 < BK; k4 += 4) {
       TC af[TM], bf[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) af[i] = As[kk + lk][wm + i * 16 + lr];
+      for (int i = 0; i < TM; ++i) af[i] = As[k4 + lk][wm + i * 16 + lr];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bf[j] = Bs[kk + lk][wn + j * 16 + lr];
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[k4 + lk][wn + j * 16 + lr];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -435,10 +467,14 @@ size_t gemm_workspace(int M, int N, int K) {
   return (size_t)8 * M * N;
 }
 
-// true when every BM x BN x 16 tile is full and the paired 16-byte loads stay aligned
+// true when every BM x BN x 16 tile is full, the paired 16-byte loads stay aligned and the
+// per-thread byte offsets inside one K stage fit in 31 bits
 static bool gemm_vec_ok(const GemmDesc& g, bool amc, bool bkc, int BM, int BN) {
   auto even = [](long x) { return (x & 1) == 0; };
   if (g.M % BM || g.N % BN || g.K % 16 || g.Kc % 16) return false;
+  const long extA = (long)g.M * std::labs(g.sAm) + 16L * std::labs(g.sAk);
+  const long extB = 16L * std::labs(g.sBk) + (long)g.N * std::labs(g.sBn);
+  if (extA >= (1L << 28) || extB >= (1L << 28)) return false;
   if (((uintptr_t)g.A & 15) || ((uintptr_t)g.B & 15)) return false;
   const bool a_ok = amc ? (even(g.sAk) && even(g.sAc)) : (even(g.sAm) && even(g.sAc));
   const bool b_ok = bkc ? (even(g.sBn) && even(g.sBc)) : (even(g.sBk) && even(g.sBc));

@@ -52,6 +52,8 @@ class DevArray:
     # 64 x 128 tiles with paired loads, all four operand orientations; K % 16 != 0 falls back
     ("N", "N", 2048, 1536, 64), ("T", "N", 1920, 1664, 80), ("N", "T", 2048, 1536, 48),
     ("T", "T", 2048, 1536, 32), ("N", "N", 2048, 1536, 33),
+    # long K on a full-tile grid
+    ("N", "N", 2048, 1536, 2304),
 ])
 def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
     rng = np.random.default_rng(m * 7 + n * 3 + k)
