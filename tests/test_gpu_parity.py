@@ -312,6 +312,23 @@ def test_vumps_tfim_matches_oracle(imps, ctx, oracle, N, D, seed):
         assert abs(a - b) <= 1e-6 * max(b, 1e-12) + 1e-9
 
 
+def test_vumps_xxz_two_site_trajectory_matches_oracle(imps, ctx, oracle):
+    # the C2 comparison (XXZ, N=2, real S+- form) at D=64: per-iteration Galerkin errors
+    # within 1e-6 relative and the energy within 1e-9 after 10 iterations
+    W = oracle.xxz_mpo(1.0, 1.0)
+    H = imps.SparseIMPO([W] * 2)
+    psi0 = oracle.randomimps(64, [2, 2], seed=1)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+    envs = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, info = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-14, maxiter=10), envs=envs)
+    _, _, rinfo = oracle.find_groundstate(psi0, [W] * 2, oracle.VUMPS(tol=1e-14, maxiter=10))
+    assert info.niter == rinfo.niter == 10
+    for a, b in zip(info.losses, rinfo.losses):
+        assert abs(a - b) <= 1e-6 * b
+    assert abs(info.energy - rinfo.energies[-1]) < 1e-9
+
+
 def test_hac_fp32_against_fp64(imps, ctx, oracle):
     D = 128
     H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 2, seed=9)
