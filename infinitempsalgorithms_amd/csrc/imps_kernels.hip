@@ -1015,81 +1015,103 @@ hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long 
 }
 
 // Cholesky of one diagonal block (nb <= 64, padded to 64 x 64 with an identity block) and the
-// inverse of its factor, by two wavefronts in lockstep over the 64 right-looking steps.
-// Wave 0, lane i = row i of the block in registers: at step k it publishes the (not yet
-// scaled) column c = A^(k)[:, k] to LDS; then every row finalises L[i][k] = c_i / sqrt(c_k)
-// and applies the rank-1 update A^(k+1)[i][j] = A^(k)[i][j] - c_i c_j / c_k (j > k).
-// Wave 1, lane j = column j of X = L^{-1}, runs the column-oriented forward substitution on
-// the same broadcast column: X[k][j] = x_j[k] / L[k][k], x_j[i] -= L[i][k] X[k][j] (i > k).
-// One barrier per step; the column buffer is double-buffered so that a wave one step ahead
-// never overwrites values the other wave still reads.  L goes back to A (lower triangle);
+// inverse of its factor, by four wavefronts in lockstep over the 64 right-looking steps (one
+// barrier per step).  Step k: the wave owning column k publishes it, not yet scaled, as
+// c = A^(k)[:, k]; every wave reads the pivot c_k from LDS.
+//  - factor waves (lane i = row i; columns 0-31 and 32-63): L[i][k] = c_i / sqrt(c_k) and the
+//    rank-1 update A[i][j] -= c_i c_j / c_k for the wave's columns j > k;
+//  - inverse waves (lane j = column j of X = L^{-1}; rows 0-31 and 32-63): X[k][j] =
+//    x_j[k] / L[k][k] and x_j[i] -= L[i][k] X[k][j] for the wave's rows i > k.  The rows-32-63
+//    wave applies the steps k < 32 one step late, with X[k][j] / L[k][k] handed over in LDS.
+// The column buffer is triple-buffered (the late wave still reads column k-1 while column k+1
+// is published) and the hand-over buffer double-buffered.  L goes back to A (lower triangle);
 // U = L^{-T} (64 x 64, zero below the diagonal and beyond nb) goes to Ut, so that the panel
 // solve X = P L^{-T} is the product P U.
-__global__ void __launch_bounds__(128) chol_diag_kernel(int nb, double* A, long lda, double* Ut, int* fail) {
-  __shared__ __attribute__((aligned(16))) double col[2][64];
+template <int J0>
+__device__ inline void chol_fact_wave(int lane, double (*c)[64], double (*Ls)[65], int* fail) {
+  double v[32];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) v[j] = Ls[J0 + j][lane];
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 64; ++k) {
+    double* ck = c[k % 3];
+    const bool own = (k >= J0 && k < J0 + 32);
+    if (own) ck[lane] = v[k - J0 < 0 ? 0 : (k - J0 > 31 ? 31 : k - J0)];
+    __syncthreads();
+    if (!own && k >= J0 + 31) continue;  // no column of this wave at or right of k
+    double p = ck[k];
+    if (!(p > 0.0)) {
+      bad = true;
+      p = 1.0;
+    }
+    const double inv = rsqrt(p);
+    const double ci = ck[lane];
+    const double t = (lane > k) ? ci * (inv * inv) : 0.0;
+    if (own) Ls[k][lane] = (lane > k) ? ci * inv : (lane == k ? p * inv : 0.0);
+#pragma unroll
+    for (int j = 0; j < 32; ++j)
+      if (J0 + j > k) v[j] -= t * ck[J0 + j];
+  }
+  if (bad && lane == 0) *fail = 1;
+}
+
+template <int I0>
+__device__ inline void chol_inv_wave(int lane, double (*c)[64], double (*scb)[64], double (*Us)[65]) {
+  double v[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) v[i] = (lane == I0 + i) ? 1.0 : 0.0;
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 64; ++k) {
+    __syncthreads();
+    if (I0 == 32 && k >= 1 && k - 1 < 32) {
+      const double* cp = c[(k - 1) % 3];
+      const double sc = scb[(k - 1) & 1][lane];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) v[i] -= cp[32 + i] * sc;
+    }
+    const bool own = (k >= I0 && k < I0 + 32);
+    if (!own) continue;
+    const double* ck = c[k % 3];
+    double p = ck[k];
+    if (!(p > 0.0)) p = 1.0;
+    const double inv = rsqrt(p);
+    const double y = v[k - I0 < 0 ? 0 : (k - I0 > 31 ? 31 : k - I0)] * inv;
+    Us[k][lane] = y;
+    const double sc = y * inv;
+    if (I0 == 0) scb[k & 1][lane] = sc;
+#pragma unroll
+    for (int i = 0; i < 32; ++i)
+      if (I0 + i > k) v[i] -= ck[I0 + i] * sc;
+  }
+}
+
+__global__ void __launch_bounds__(256) chol_diag_kernel(int nb, double* A, long lda, double* Ut, int* fail) {
+  __shared__ double c[3][64];
+  __shared__ double scb[2][64];
   __shared__ double Ls[64][65];  // Ls[k][i] = block entry (i, k): input, then L
   __shared__ double Us[64][65];  // Us[k][j] = X[k][j] = U[j][k]
   const int lane = threadIdx.x & 63;
-  for (int t = threadIdx.x; t < 64 * 64; t += 128) {
+  for (int t = threadIdx.x; t < 64 * 64; t += 256) {
     const int i = t & 63, j = t >> 6;
     double x = (i == j) ? 1.0 : 0.0;
     if (i < nb && j < nb && i >= j) x = A[i + lda * j];
     Ls[j][i] = x;
   }
   __syncthreads();
-  // wave-uniform role: each wave runs its own straight-line loop; both meet at the same
-  // 64 barriers
-  const bool fact = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
-  double v[64];
-  if (fact) {
-    bool bad = false;
-#pragma unroll
-    for (int j = 0; j < 64; ++j) v[j] = Ls[j][lane];
-#pragma unroll
-    for (int k = 0; k < 64; ++k) {
-      double* c = col[k & 1];
-      c[lane] = v[k];
-      __syncthreads();
-      double p = c[k];
-      if (!(p > 0.0)) {
-        bad = true;
-        p = 1.0;
-      }
-      // one reciprocal square root per step keeps division sequences (and their registers)
-      // out of the unrolled loop
-      const double inv = rsqrt(p);
-      const double piv = p * inv;
-      // row lane: L[lane][k] final (rows above k keep zero)
-      Ls[k][lane] = (lane > k) ? v[k] * inv : (lane == k ? piv : 0.0);
-      const double t = (lane > k) ? v[k] * (inv * inv) : 0.0;
-#pragma unroll
-      for (int j = k + 1; j < 64; ++j) {
-        v[j] -= t * c[j];
-        // broadcast reads in groups of 8: bounded register pressure
-        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-    if (bad && lane == 0) *fail = 1;
-  } else {
-#pragma unroll
-    for (int i = 0; i < 64; ++i) v[i] = (lane == i) ? 1.0 : 0.0;
-#pragma unroll
-    for (int k = 0; k < 64; ++k) {
-      const double* c = col[k & 1];
-      __syncthreads();
-      double p = c[k];
-      if (!(p > 0.0)) p = 1.0;
-      const double inv = rsqrt(p);
-      // column lane: X[k][lane] final
-      const double y = v[k] * inv;
-      Us[k][lane] = y;
-      const double sc = y * inv;
-#pragma unroll
-      for (int i = k + 1; i < 64; ++i) v[i] -= c[i] * sc;
-    }
-  }
+  // wave-uniform role: each wave runs its own straight-line loop; all meet at the same 64
+  // barriers
+  const int role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (role == 0)
+    chol_fact_wave<0>(lane, c, Ls, fail);
+  else if (role == 1)
+    chol_fact_wave<32>(lane, c, Ls, fail);
+  else if (role == 2)
+    chol_inv_wave<0>(lane, c, scb, Us);
+  else
+    chol_inv_wave<32>(lane, c, scb, Us);
   __syncthreads();
-  for (int t = threadIdx.x; t < 64 * 64; t += 128) {
+  for (int t = threadIdx.x; t < 64 * 64; t += 256) {
     const int i = t & 63, j = t >> 6;
     if (i < nb && j < nb && i >= j) A[i + lda * j] = Ls[j][i];
     Ut[t] = (i < nb && j < nb) ? Us[j][i] : 0.0;
@@ -1098,7 +1120,7 @@ __global__ void __launch_bounds__(128) chol_diag_kernel(int nb, double* A, long 
 
 hipError_t launch_chol_diag(int nb, double* A, long lda, double* Ut, int* fail, hipStream_t s) {
   if (nb > 64 || nb < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(128), 0, s, nb, A, lda, Ut, fail);
+  hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, nb, A, lda, Ut, fail);
   return hipGetLastError();
 }
 
