@@ -709,23 +709,28 @@ __device__ inline void block_sum_store(double (&acc)[NV], int nvalid, double* pa
 }
 
 // part[v nb + b] = block b's share of X_v . y: vectors in chunks of 32, so y is read once
-// per 32 vectors
+// per 32 vectors; each element's loads are all issued before its products (as in the fused
+// sweeps below)
 __global__ void __launch_bounds__(RED_THREADS) multidot_kernel(size_t n, int nv, const double* __restrict__ X,
                                                                size_t ldx, const double* __restrict__ y,
                                                                double* __restrict__ part) {
   constexpr int CH = 32;
   const int nb = gridDim.x;
   for (int vc = 0; vc < nv; vc += CH) {
+    const int nc = min(CH, nv - vc);
+    const double* Xc = X + (size_t)vc * ldx;
     double acc[CH];
 #pragma unroll
     for (int t = 0; t < CH; ++t) acc[t] = 0.0;
     for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)nb * RED_THREADS) {
+      double xv[CH];
+#pragma unroll
+      for (int t = 0; t < CH; ++t) xv[t] = (t < nc) ? Xc[(size_t)t * ldx + i] : 0.0;
       const double yi = y[i];
 #pragma unroll
-      for (int t = 0; t < CH; ++t)
-        if (vc + t < nv) acc[t] += X[(size_t)(vc + t) * ldx + i] * yi;
+      for (int t = 0; t < CH; ++t) acc[t] += xv[t] * yi;
     }
-    block_sum_store<CH>(acc, nv - vc, part + (size_t)vc * nb, nb, blockIdx.x);
+    block_sum_store<CH>(acc, nc, part + (size_t)vc * nb, nb, blockIdx.x);
   }
 }
 
