@@ -605,9 +605,77 @@ __global__ void __launch_bounds__(256) wmix_kernel(int D, int w, int d, const do
   }
 }
 
+// The same mix on FP64 MFMA tiles (w d <= 32, D % 32 == 0): per bb, OUT (o x b) = W (o x q) .
+// IN (q x b) as 16 x 16 x 4 products.  A wave owns 32 consecutive b of one bb (lane li: b0 + 2 li
+// and b0 + 2 li + 1, two column tiles), all o (two 16-row tiles) and all q (eight k-steps of 4),
+// so every lane moves 16-byte pairs: the W coefficients sit in registers and nothing goes
+// through LDS.
+__global__ void __launch_bounds__(256) wmix_mfma_kernel(int D, int w, int d, const double* __restrict__ Wm,
+                                                        const double* __restrict__ T1, double* __restrict__ T2,
+                                                        const double* __restrict__ X0, double* __restrict__ Y) {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  const int wd = w * d;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int bb = blockIdx.y;
+  const int b0 = (blockIdx.x * 4 + wave) * 32;
+  if (b0 >= D) return;
+  // A fragments: lane holds W[o = 16 t + li][q = 4 s + lk] (zero padded to 32 x 32)
+  double wa[2][8];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int o = 16 * t + li, q = 4 * s + lk;
+      wa[t][s] = (o < wd && q < wd) ? Wm[o + wd * q] : 0.0;
+    }
+  const int b = b0 + 2 * li;
+  v2 xin[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int q = 4 * s + lk;
+    xin[s] = v2{0.0, 0.0};
+    if (q < wd) {
+      const int i = q % w;
+      xin[s] = (X0 && i == 0) ? *(const v2*)&X0[b + (size_t)D * (q / w + (size_t)d * bb)]
+                              : *(const v2*)&T1[b + (size_t)D * (q + (size_t)wd * bb)];
+    }
+  }
+  v4 acc[2][2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) acc[t][0] = acc[t][1] = v4{0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[t][s], xin[s][0], acc[t][0], 0, 0, 0);
+      acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[t][s], xin[s][1], acc[t][1], 0, 0, 0);
+    }
+  // accumulator register r of lane l: o = 16 t + lk + 4 r, b = b0 + 2 li (+1 for the odd tile)
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int o = 16 * t + lk + 4 * r;
+      if (o < wd) {
+        const int u = o % d, j = o / d;
+        double* dst = (Y && j == w - 1) ? Y + b + (size_t)D * (u + (size_t)d * bb)
+                                        : T2 + b + (size_t)D * (u + (size_t)d * (bb + (size_t)D * j));
+        *(v2*)dst = v2{acc[t][0][r], acc[t][1][r]};
+      }
+    }
+}
+
 hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2, const double* X0,
                        double* Y, hipStream_t s) {
   const int wd = w * d;
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  if (wd <= 32 && D % 32 == 0 && al16(T1) && al16(T2) && (!X0 || al16(X0)) && (!Y || al16(Y))) {
+    hipLaunchKernelGGL(wmix_mfma_kernel, dim3(D / 128 + (D % 128 ? 1 : 0), D), dim3(256), 0, s, D, w, d, Wm_dev,
+                       T1, T2, X0, Y);
+    return hipGetLastError();
+  }
   dim3 block(256);
   dim3 grid((D + 255) / 256, D);
   if (wd <= 16)
