@@ -32,7 +32,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 # HBM bytes per launch of the dominant H_AC GEMM measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (FETCH_SIZE scaled by 1.995 from a copy kernel of known size); keyed by
 # (D, precision, kernel).  profiles/hac_d1024_hubbard_traffic.md holds the runs.
-TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 832.4 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 545.4 * 2 ** 20}
+TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 834.9 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 931.7 * 2 ** 20}
 
 
 def log(msg):
