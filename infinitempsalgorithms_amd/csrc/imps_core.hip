@@ -87,9 +87,8 @@ struct imps_ctx {
   long hac_applies = 0, hc_applies = 0;
   // workers for the concurrent per-site eigensolves of the local update: each has its own
   // stream, workspaces and reduction buffers and borrows this context's model, state and
-  // environments (device pointers); a worker is never a parent
+  // environments (device pointers); workers have no workers of their own
   std::vector<imps_ctx*> workers;
-  bool is_worker = false;
   hipEvent_t ev_ready = nullptr;  // parent: state and environments enqueued
   hipEvent_t ev_done = nullptr;   // worker: its solve enqueued
 };
@@ -1366,7 +1365,6 @@ EigOut eigsolve_c(imps_ctx* c, int site, double tol, int kdim, int maxiter, bool
 imps_ctx* worker(imps_ctx* c, int k) {
   while ((int)c->workers.size() <= k) {
     imps_ctx* w = new imps_ctx();
-    w->is_worker = true;
     w->device = c->device;
     c->workers.push_back(w);
     HIPCHK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
@@ -1418,17 +1416,17 @@ void local_update(imps_ctx* c, const std::vector<double>& tol_ac, const std::vec
   if (!c->ev_ready) HIPCHK(hipEventCreateWithFlags(&c->ev_ready, hipEventDisableTiming));
   HIPCHK(hipEventRecord(c->ev_ready, c->stream));
   const int njobs = 2 * N;
-  std::vector<imps_ctx*> ws_(njobs);
+  std::vector<imps_ctx*> wk(njobs);
   for (int k = 0; k < njobs; ++k) {
-    ws_[k] = worker(c, k);
-    HIPCHK(hipStreamWaitEvent(ws_[k]->stream, c->ev_ready, 0));
+    wk[k] = worker(c, k);
+    HIPCHK(hipStreamWaitEvent(wk[k]->stream, c->ev_ready, 0));
   }
   std::vector<int> code(njobs, IMPS_OK);
   std::vector<std::string> msg(njobs);
   std::vector<std::thread> th;
   for (int k = 0; k < njobs; ++k) {
     th.emplace_back([&, k] {
-      imps_ctx* w = ws_[k];
+      imps_ctx* w = wk[k];
       try {
         HIPCHK(hipSetDevice(w->device));
         const int site = k / 2;
@@ -1448,12 +1446,16 @@ void local_update(imps_ctx* c, const std::vector<double>& tol_ac, const std::vec
   }
   for (auto& t : th) t.join();
   for (int k = 0; k < njobs; ++k)
-    if (code[k] != IMPS_OK) throw ImpsError(code[k], msg[k]);
+    if (code[k] != IMPS_OK) {
+      // let the other solves' enqueued work finish before the state is touched again
+      for (imps_ctx* w : wk) (void)hipStreamSynchronize(w->stream);
+      throw ImpsError(code[k], msg[k]);
+    }
   for (int k = 0; k < njobs; ++k) {
-    HIPCHK(hipStreamWaitEvent(c->stream, ws_[k]->ev_done, 0));
-    c->hac_applies += ws_[k]->hac_applies;
-    c->hc_applies += ws_[k]->hc_applies;
-    ws_[k]->hac_applies = ws_[k]->hc_applies = 0;
+    HIPCHK(hipStreamWaitEvent(c->stream, wk[k]->ev_done, 0));
+    c->hac_applies += wk[k]->hac_applies;
+    c->hc_applies += wk[k]->hc_applies;
+    wk[k]->hac_applies = wk[k]->hc_applies = 0;
   }
   for (int site = 0; site < N; ++site) regauge_site(c, site);
 }
