@@ -173,6 +173,11 @@ def hac_throughput(args, imps, _lib, H, prec, out, D, d, w, N, F):
         "wmix": {"ms": t.ms_wmix, "gbs": 2 * w * d * D * D * 8 / (t.ms_wmix * 1e-3) / 1e9},
         "gemm2_GR": {"ms": t.ms_gemm2, "flop": t.flop_gemm2, "tflops": t.flop_gemm2 / (t.ms_gemm2 * 1e-3) / 1e12},
     }
+    if t.ms_prepare > 0:
+        # split-BF16 planes: the environments are split once per eigensolve; inside a solve an
+        # application costs ms_apply (the value loop above splits them on every call)
+        kernels["apply_in_eigensolve"] = {"ms": t.ms_apply, "tflops": F / (t.ms_apply * 1e-3) / 1e12}
+        kernels["env_split_per_eigensolve"] = {"ms": t.ms_prepare}
     dom = max(("gemm1_GLx", "gemm2_GR"), key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["tflops"]
     peak = PEAKS[args.prec]

@@ -262,7 +262,8 @@ int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imp
  * site-0 shapes and returns the average duration of one application and of each of
  * its kernels (HIP events on the context's stream), and the flops of each GEMM launch as
  * executed (2 (w-1) d D^3 when the environments' structural levels are identities, else
- * 2 w d D^3). */
+ * 2 w d D^3).  In the split-BF16 mode the GL x bracket includes the split of x into planes,
+ * and the split of the environments into planes (once per eigensolve) is timed apart. */
 typedef struct imps_hac_timing {
   double ms_apply;  /* whole application */
   double ms_gemm1;  /* GL x  */
@@ -270,6 +271,7 @@ typedef struct imps_hac_timing {
   double ms_gemm2;  /* (.) GR */
   double flop_gemm1;
   double flop_gemm2;
+  double ms_prepare; /* per-eigensolve operand split of the split-BF16 mode (0 otherwise) */
 } imps_hac_timing;
 int imps_bench_hac(imps_ctx* ctx, int site, int reps, int prec, imps_hac_timing* t);
 

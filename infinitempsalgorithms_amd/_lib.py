@@ -50,7 +50,8 @@ class VumpsInfo(ctypes.Structure):
 
 class HacTiming(ctypes.Structure):
     _fields_ = [("ms_apply", c_double), ("ms_gemm1", c_double), ("ms_wmix", c_double),
-                ("ms_gemm2", c_double), ("flop_gemm1", c_double), ("flop_gemm2", c_double)]
+                ("ms_gemm2", c_double), ("flop_gemm1", c_double), ("flop_gemm2", c_double),
+                ("ms_prepare", c_double)]
 
 
 class BatchInstance(ctypes.Structure):

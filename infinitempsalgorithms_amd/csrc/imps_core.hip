@@ -78,6 +78,9 @@ struct imps_ctx {
   // per site: level 0 of GL / level w-1 of GR is exactly the identity (the Schur structure of
   // DMRGCache pins them); the effective operators then apply those levels as copies
   std::vector<char> gl_id, gr_id;
+  // site whose GL/GR split-BF16 planes are current for the eigensolve in progress (-1: none;
+  // then a split-BF16 H_AC application splits them itself)
+  int planes_site = -1;
   // workspaces and pinned host scratch for reductions
   std::map<std::string, DevBuf> ws;
   double* hpart = nullptr;
@@ -286,6 +289,58 @@ int wrap(imps_ctx* c, int site) {
 
 // ------------------------------------------------------------------ operators
 
+// Split-BF16 planes path of H_AC: environments with identity structural levels, D % 128 == 0,
+// w <= 8, d <= 4.  Other shapes use the split-on-load GEMM of launch_gemm.
+bool hac_planes_ok(imps_ctx* c, int site) {
+  return c->D % 128 == 0 && c->w >= 2 && c->w <= 8 && c->d <= 4 && c->gl_id[site] && c->gr_id[site];
+}
+
+uint16_t* ws_bf16(imps_ctx* c, const std::string& name, size_t n) {
+  return (uint16_t*)ws(c, name, (n + 7) / 4);  // 16-byte multiple
+}
+
+// GL levels as [m = b + D i][k = a'] rows and GR as [n = b'][k = bb + D j] rows, both planes
+void hac_planes_prepare(imps_ctx* c, int site) {
+  const int D = c->D, w = c->w;
+  const size_t ne = (size_t)D * w * D;
+  HIPCHK(launch_split_planes(D * w, D, c->GL[site], (long)D * w, true, ws_bf16(c, "hp_GLh", ne),
+                             ws_bf16(c, "hp_GLl", ne), D, c->stream));
+  HIPCHK(launch_split_planes(D, D * w, c->GR[site], (long)D * w, false, ws_bf16(c, "hp_GRh", ne),
+                             ws_bf16(c, "hp_GRl", ne), (long)D * w, c->stream));
+}
+
+// H_AC with identity structural levels on planes: split x, GEMM 1 (levels 1..w-1 of GL, FP32
+// out), mix (level 0 from x, level w-1 of GR straight into y, other levels as planes), GEMM 2
+// (levels 0..w-2 of GR) accumulating onto y.
+void hac_apply_planes(imps_ctx* c, int site, const double* x, double* y, hipEvent_t* ev) {
+  const int D = c->D, w = c->w, d = c->d;
+  if (c->planes_site != site) hac_planes_prepare(c, site);
+  const size_t ne = (size_t)D * w * D, nx = (size_t)D * d * D;
+  const uint16_t* GLh = ws_bf16(c, "hp_GLh", ne);
+  const uint16_t* GLl = ws_bf16(c, "hp_GLl", ne);
+  const uint16_t* GRh = ws_bf16(c, "hp_GRh", ne);
+  const uint16_t* GRl = ws_bf16(c, "hp_GRl", ne);
+  uint16_t* xh = ws_bf16(c, "hp_xh", nx);
+  uint16_t* xl = ws_bf16(c, "hp_xl", nx);
+  float* T1 = (float*)ws(c, "hp_T1", (nx * w + 1) / 2);
+  const size_t nt = nx * (w - 1);
+  uint16_t* Th = ws_bf16(c, "hp_T2h", nt);
+  uint16_t* Tl = ws_bf16(c, "hp_T2l", nt);
+  if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
+  HIPCHK(launch_split_planes(d * D, D, x, D, false, xh, xl, D, c->stream));
+  const size_t lvl = (size_t)D * D;  // one level of GL planes
+  HIPCHK(launch_gemm_planes(D * (w - 1), d * D, D, GLh + lvl, GLl + lvl, D, xh, xl, D, T1 + D, (long)D * w, false,
+                            1.0, 0.0, c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
+  HIPCHK(launch_wmix_planes(D, w, d, c->Wm[site], T1, x, Th, Tl, y, c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
+  const long KT = (long)D * (w - 1);
+  HIPCHK(launch_gemm_planes(D * d, D, (int)KT, Th, Tl, KT, GRh, GRl, (long)D * w, y, (long)D * d, true, 1.0, 1.0,
+                            c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
+  c->hac_applies++;
+}
+
 // H_AC (effective.jl:90-102) as GEMM (GL x) -> W mix -> GEMM (. GR).  When GL's level 0 and
 // GR's level w-1 are the identity (environments built by DMRGCache), their products are the
 // operands themselves: GEMM 1 skips level 0 (the mix reads x instead), the mix writes the
@@ -300,6 +355,10 @@ void hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec, hipE
   double* T1 = ws(c, "hac_T1", (size_t)D * w * d * D);
   double* T2 = ws(c, "hac_T2", (size_t)D * w * d * D);
   const bool skip = w >= 2 && c->gl_id[site] && c->gr_id[site];
+  if (prec == PREC_BF16X3 && hac_planes_ok(c, site)) {
+    hac_apply_planes(c, site, x, y, ev);
+    return;
+  }
   const int lv = skip ? 1 : 0;  // levels handled as copies at each end
   GemmDesc g1;
   g1.M = D * (w - lv);
@@ -1347,6 +1406,16 @@ struct EigOut {
 EigOut eigsolve_ac(imps_ctx* c, int site, double tol, int kdim, int maxiter, bool eager, int prec) {
   const size_t n = (size_t)c->D * c->d * c->D;
   EigOut o;
+  site = wrap(c, site);
+  // the environments stay fixed during the solve: split their planes once
+  struct PlanesScope {
+    imps_ctx* c;
+    ~PlanesScope() { c->planes_site = -1; }
+  } scope{c};
+  if (prec == PREC_BF16X3 && hac_planes_ok(c, site)) {
+    hac_planes_prepare(c, site);
+    c->planes_site = site;
+  }
   Op op = [&](const double* x, double* y) { hac_apply(c, site, x, y, prec); };
   o.lambda = lanczos(c, op, n, c->AC[site], c->pAC[site], tol, kdim, maxiter, eager, &o.info);
   return o;
@@ -2050,6 +2119,28 @@ int imps_bench_hac(imps_ctx* c, int site, int reps, int prec, imps_hac_timing* t
     double* x = ws(c, "bh_x", n);
     double* y = ws(c, "bh_y", n);
     HIPCHK(launch_fill_random(n, 12345, x, c->stream));
+    struct PlanesScope {
+      imps_ctx* c;
+      ~PlanesScope() { c->planes_site = -1; }
+    } scope{c};
+    t->ms_prepare = 0.0;
+    if (prec == PREC_BF16X3 && hac_planes_ok(c, site)) {
+      // the per-eigensolve operand split, timed on its own
+      hac_planes_prepare(c, site);  // allocates the planes
+      hipEvent_t p0, p1;
+      HIPCHK(hipEventCreate(&p0));
+      HIPCHK(hipEventCreate(&p1));
+      HIPCHK(hipEventRecord(p0, c->stream));
+      hac_planes_prepare(c, site);
+      HIPCHK(hipEventRecord(p1, c->stream));
+      HIPCHK(hipEventSynchronize(p1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, p0, p1));
+      t->ms_prepare = ms;
+      (void)hipEventDestroy(p0);
+      (void)hipEventDestroy(p1);
+      c->planes_site = site;
+    }
     hac_apply(c, site, x, y, prec);  // warm-up
     hipEvent_t ev[4];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));

@@ -46,6 +46,24 @@ hipError_t launch_mix_mid(int D1, int d, int D2, const double* M, const double* 
 hipError_t launch_wmix(int D, int w, int d, const double* Wm_dev, const double* T1, double* T2, const double* X0,
                        double* Y, hipStream_t s);
 
+// Split-BF16 planes (x_hi = bf16(float(x)), x_lo = bf16(float(x) - x_hi), stored as raw bf16):
+// out[r*ldo + c] = split(in[r*ldi + c]) (trans = false; cols % 8 == 0) or split(in[r + ldi*c])
+// (trans = true; rows % 64 == cols % 64 == 0).
+hipError_t launch_split_planes(int rows, int cols, const double* in, long ldi, bool trans, uint16_t* hi,
+                               uint16_t* lo, long ldo, hipStream_t s);
+// C(m, n) = alpha sum_k A(m, k) B(k, n) + beta C(m, n) from planes: A(m, k) at m*lda + k,
+// B(k, n) at n*ldb + k, C(m, n) at m + n*ldc (FP64 when c_double, else FP32 with beta = 0).
+// Three bf16 products per element pair, FP32 accumulation.  M % 128 == 0, N % 64 == 0,
+// K % 64 == 0, lda, ldb multiples of 8 and 16-byte aligned planes.
+hipError_t launch_gemm_planes(int M, int N, int K, const uint16_t* Ah, const uint16_t* Al, long lda,
+                              const uint16_t* Bh, const uint16_t* Bl, long ldb, void* C, long ldc, bool c_double,
+                              double alpha, double beta, hipStream_t s);
+// The H_AC mix of launch_wmix for identity structural levels on planes: T1 in FP32 (levels
+// 1..w-1 at b + D*(i + w*(s + d*bb))), level 0 from X0; channel j = w-1 to Y (FP64), channels
+// j < w-1 as planes Th/Tl[(b + D*u)*D*(w-1) + D*j + bb].  D % 32 == 0, 2 <= w <= 8, d <= 4.
+hipError_t launch_wmix_planes(int D, int w, int d, const double* Wm, const float* T1, const double* X0,
+                              uint16_t* Th, uint16_t* Tl, double* Y, hipStream_t s);
+
 // Energy-gradient sandwich partials: Gp[o*D + bb] = sum_b T1[b,i,s,bb] T3[b,u,bb,j] with
 // o = (i + w s) + w d (u + d j); T1 as in launch_wmix, T3 laid out like its T2; w d <= 32.
 hipError_t launch_gsand(int D, int w, int d, const double* T1, const double* T3, double* Gp, hipStream_t s);

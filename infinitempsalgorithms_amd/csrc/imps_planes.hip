@@ -1,0 +1,394 @@
+// Split-BF16 "planes" path of H_AC (effective.jl:90-102) for the mixed-precision bulk
+// contractions: every FP64 operand x is stored once as two bf16 planes, x_hi = bf16(x) and
+// x_lo = bf16(x - x_hi), laid out K-contiguous (row-major over the contraction index), and the
+// GEMMs stream those planes into v_mfma_f32_16x16x32_bf16 as a_hi b_hi + a_hi b_lo + a_lo b_hi
+// with FP32 accumulation (the same three-term product as the on-the-fly split GEMM).
+//
+//   split   : FP64 -> (hi, lo) planes, optionally transposing (GL levels -> [m][k] rows)
+//   gemm    : C(m, n) = alpha sum_k A(m, k) B(k, n) (+ beta C), A(m, k) at m lda + k,
+//             B(k, n) at n ldb + k, C(m, n) at m + n ldc in FP32 or FP64
+//   wmix    : the MPO mix of H_AC between the two GEMMs, reading GEMM 1's FP32 output and
+//             writing GEMM 2's A operand as planes (transposed through LDS so that its rows are
+//             contiguous along the contraction index)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "imps_internal.h"
+
+namespace imps {
+
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__device__ inline void split_f32(float xf, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)xf;
+  lo = (__bf16)(xf - (float)hi);
+}
+
+// out[r ldo + c] = split(in[r ldi + c]); cols % 8 == 0; one thread per 8 consecutive c
+__global__ void __launch_bounds__(256) split_rows_kernel(int rows, int cols, const double* __restrict__ in,
+                                                         long ldi, __bf16* __restrict__ hi, __bf16* __restrict__ lo,
+                                                         long ldo) {
+  const int cpr = cols / 8;
+  const long total = (long)rows * cpr;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < total; t += (long)gridDim.x * 256) {
+    const int r = (int)(t / cpr);
+    const int c = 8 * (int)(t - (long)r * cpr);
+    const double* p = in + (long)r * ldi + c;
+    v8bf h, l;
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+      const v2d v = *(const v2d*)(p + e);
+      __bf16 a, b;
+      split_f32((float)v[0], a, b);
+      h[e] = a;
+      l[e] = b;
+      split_f32((float)v[1], a, b);
+      h[e + 1] = a;
+      l[e + 1] = b;
+    }
+    *(v8bf*)(hi + (long)r * ldo + c) = h;
+    *(v8bf*)(lo + (long)r * ldo + c) = l;
+  }
+}
+
+// out[r ldo + c] = split(in[r + ldi c]) through a 64 x 64 LDS tile; rows % 64 == cols % 64 == 0
+__global__ void __launch_bounds__(256) split_trans_kernel(const double* __restrict__ in, long ldi,
+                                                          __bf16* __restrict__ hi, __bf16* __restrict__ lo,
+                                                          long ldo) {
+  __shared__ float T[64][65];
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int t = threadIdx.x;
+  // read: lanes along r (contiguous in `in`)
+  for (int e = 0; e < 16; ++e) {
+    const int c = (t >> 6) + 4 * e, r = t & 63;
+    T[c][r] = (float)in[(long)(r0 + r) + ldi * (c0 + c)];
+  }
+  __syncthreads();
+  // write: thread -> row r, 16 consecutive c
+  const int r = t >> 2, cq = 16 * (t & 3);
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    v8bf h, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      __bf16 a, b;
+      split_f32(T[cq + 8 * half + e][r], a, b);
+      h[e] = a;
+      l[e] = b;
+    }
+    const long o = (long)(r0 + r) * ldo + c0 + cq + 8 * half;
+    *(v8bf*)(hi + o) = h;
+    *(v8bf*)(lo + o) = l;
+  }
+}
+
+struct PlaneGemm {
+  int M, N, K;
+  const __bf16 *Ah, *Al;
+  long lda;
+  const __bf16 *Bh, *Bl;
+  long ldb;
+  void* C;
+  long ldc;
+  float alpha;
+  double beta;
+};
+
+// Block = 4 waves (2 x 2), tile BM x BN, K stage BK (multiple of 32).  Each thread moves
+// 16-byte chunks (8 bf16 along k) of both planes global -> registers -> LDS ([row][k], 8 bf16
+// of padding per row: the 16 rows read by one ds_read_b128 fall on disjoint banks); the next
+// stage is fetched into registers while the MFMAs consume the current one.  The B fragment is
+// the MFMA's row operand, so the 16 lanes of an accumulator row hold 16 consecutive m.
+template <int BM, int BN, int BK, typename TO>
+__global__ void __launch_bounds__(256, 2) gemm_planes_kernel(PlaneGemm g) {
+  constexpr int LDK = BK + 8;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int CH = BK / 8;          // 16-byte chunks per row and stage
+  constexpr int RSTEP = 256 / CH;     // rows covered by one pass of the block
+  constexpr int AE = BM / RSTEP, BE = BN / RSTEP;
+  static_assert(AE * RSTEP == BM && BE * RSTEP == BN && BK % 32 == 0, "tile/thread mismatch");
+  __shared__ __attribute__((aligned(16))) __bf16 Ash[BM][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Asl[BM][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bsh[BN][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bsl[BN][LDK];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
+  const int crow = tid / CH, ck = 8 * (tid % CH);
+
+  // wave-uniform stage bases (bytes) plus one 32-bit offset per chunk (checked on the host)
+  const char* pah = (const char*)(g.Ah + (long)m0 * g.lda);
+  const char* pal = (const char*)(g.Al + (long)m0 * g.lda);
+  const char* pbh = (const char*)(g.Bh + (long)n0 * g.ldb);
+  const char* pbl = (const char*)(g.Bl + (long)n0 * g.ldb);
+  unsigned offA[AE], offB[BE];
+#pragma unroll
+  for (int e = 0; e < AE; ++e) offA[e] = (unsigned)(2 * ((long)(crow + e * RSTEP) * g.lda + ck));
+#pragma unroll
+  for (int e = 0; e < BE; ++e) offB[e] = (unsigned)(2 * ((long)(crow + e * RSTEP) * g.ldb + ck));
+
+  v8bf rah[AE], ral[AE], rbh[BE], rbl[BE];
+  auto load = [&]() {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      rah[e] = *(const v8bf*)(pah + offA[e]);
+      ral[e] = *(const v8bf*)(pal + offA[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      rbh[e] = *(const v8bf*)(pbh + offB[e]);
+      rbl[e] = *(const v8bf*)(pbl + offB[e]);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      *(v8bf*)&Ash[crow + e * RSTEP][ck] = rah[e];
+      *(v8bf*)&Asl[crow + e * RSTEP][ck] = ral[e];
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      *(v8bf*)&Bsh[crow + e * RSTEP][ck] = rbh[e];
+      *(v8bf*)&Bsl[crow + e * RSTEP][ck] = rbl[e];
+    }
+  };
+
+  v4f acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nk = g.K / BK;
+  load();
+  for (int kt = 0; kt < nk; ++kt) {
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (kt + 1 < nk) {
+      pah += 2 * BK;
+      pal += 2 * BK;
+      pbh += 2 * BK;
+      pbl += 2 * BK;
+      load();
+    }
+#pragma unroll
+    for (int k32 = 0; k32 < BK; k32 += 32) {
+      v8bf ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        ah[i] = *(const v8bf*)&Ash[wm + 16 * i + lr][k32 + 8 * lk];
+        al[i] = *(const v8bf*)&Asl[wm + 16 * i + lr][k32 + 8 * lk];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        bh[j] = *(const v8bf*)&Bsh[wn + 16 * j + lr][k32 + 8 * lk];
+        bl[j] = *(const v8bf*)&Bsl[wn + 16 * j + lr][k32 + 8 * lk];
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+  // accumulator register r of lane l: m = l % 16, n = 4 (l / 16) + r within the 16 x 16 tile
+  TO* C = (TO*)g.C;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int m = m0 + wm + 16 * i + lr;
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn + 16 * j + 4 * lk + r;
+        TO* p = C + m + (long)n * g.ldc;
+        if (sizeof(TO) == 4) {
+          *p = (TO)(g.alpha * acc[i][j][r]);
+        } else {
+          const double v = (double)g.alpha * (double)acc[i][j][r];
+          *p = (TO)(g.beta == 0.0 ? v : v + g.beta * (double)*p);
+        }
+      }
+  }
+}
+
+// The MPO mix of H_AC on planes.  Tile: 16 consecutive a x 32 consecutive bb per block; thread t
+// owns a = a0 + t % 16 and bb = bb0 + t / 16 + 16 r (r = 0, 1) with all w d input channels in
+// registers:
+//   in(i, s) = X0[a + D (s + d bb)] for level i = 0 (the identity level of GL), else
+//              T1[a + D i + D w (s + d bb)] (FP32 output of GEMM 1)
+//   out(u, j) = sum_{i,s} W[(u + d j) + w d (i + w s)] in(i, s), skipping all-zero d x d blocks
+// Level j = w-1 (the identity level of GR) goes to Y[a + D (u + d bb)] (FP64); levels j < w-1
+// are staged in LDS and written as planes Th/Tl[(a + D u) KT + D j + bb], KT = D (w - 1), with
+// 4 lanes per 32-element row segment (the transpose that makes GEMM 2's A operand K-contiguous).
+template <int MW, int MD>
+__global__ void __launch_bounds__(256) wmix_planes_kernel(int D, int w, int d, const double* __restrict__ Wm,
+                                                          const float* __restrict__ T1,
+                                                          const double* __restrict__ X0, __bf16* __restrict__ Th,
+                                                          __bf16* __restrict__ Tl, double* __restrict__ Y) {
+  __shared__ float Ws[MW * MD * MW * MD];
+  __shared__ int nzb[MW * MW];
+  __shared__ float Lt[MD][32][17];
+  const int wd = w * d;
+  const int t = threadIdx.x;
+  for (int e = t; e < wd * wd; e += 256) Ws[e] = (float)Wm[e];
+  __syncthreads();
+  if (t < w * w) {
+    const int j = t / w, i = t % w;
+    int nz = 0;
+    for (int u = 0; u < d; ++u)
+      for (int s = 0; s < d; ++s) nz |= Ws[(u + d * j) + wd * (i + w * s)] != 0.f;
+    nzb[j * MW + i] = nz;
+  }
+  __syncthreads();
+  const int a0 = blockIdx.x * 16, bb0 = blockIdx.y * 32;
+  const int ta = t & 15, tb = t >> 4;
+  const int a = a0 + ta;
+  float in[2][MW][MD];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int bb = bb0 + tb + 16 * r;
+#pragma unroll
+    for (int i = 0; i < MW; ++i)
+#pragma unroll
+      for (int s = 0; s < MD; ++s) {
+        in[r][i][s] = 0.f;
+        if (i < w && s < d)
+          in[r][i][s] = (i == 0) ? (float)X0[a + (long)D * (s + (long)d * bb)]
+                                 : T1[a + (long)D * i + (long)D * w * (s + (long)d * bb)];
+      }
+  }
+  const long KT = (long)D * (w - 1);
+  // transposed write: row (a0 + ra, u = ru) of the tile, 8 bb from 8 q
+  const int ra = (t >> 2) & 15, ru = t >> 6, q = t & 3;
+  for (int j = 0; j < w; ++j) {
+    float o[2][MD];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int u = 0; u < MD; ++u) o[r][u] = 0.f;
+#pragma unroll
+    for (int i = 0; i < MW; ++i) {
+      if (i >= w || !nzb[j * MW + i]) continue;
+#pragma unroll
+      for (int s = 0; s < MD; ++s) {
+        if (s >= d) continue;
+#pragma unroll
+        for (int u = 0; u < MD; ++u) {
+          if (u >= d) continue;
+          const float wv = Ws[(u + d * j) + wd * (i + w * s)];
+          o[0][u] += wv * in[0][i][s];
+          o[1][u] += wv * in[1][i][s];
+        }
+      }
+    }
+    if (j == w - 1) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int bb = bb0 + tb + 16 * r;
+#pragma unroll
+        for (int u = 0; u < MD; ++u)
+          if (u < d) Y[a + (long)D * (u + (long)d * bb)] = (double)o[r][u];
+      }
+      continue;
+    }
+    __syncthreads();  // the previous level's reads of Lt are done
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int u = 0; u < MD; ++u)
+        if (u < d) Lt[u][tb + 16 * r][ta] = o[r][u];
+    __syncthreads();
+    if (ru < d) {
+      v8bf h, l;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 x, y;
+        split_f32(Lt[ru][8 * q + e][ra], x, y);
+        h[e] = x;
+        l[e] = y;
+      }
+      const long off = (long)(a0 + ra + (long)D * ru) * KT + (long)D * j + bb0 + 8 * q;
+      *(v8bf*)(Th + off) = h;
+      *(v8bf*)(Tl + off) = l;
+    }
+  }
+}
+
+bool al16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+}  // namespace
+
+hipError_t launch_split_planes(int rows, int cols, const double* in, long ldi, bool trans, uint16_t* hi,
+                               uint16_t* lo, long ldo, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  if (!al16(hi) || !al16(lo) || ldo % 8 != 0 || ldo < cols) return hipErrorInvalidValue;
+  if (trans) {
+    if (rows % 64 || cols % 64 || ldi < rows) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(split_trans_kernel, dim3(rows / 64, cols / 64), dim3(256), 0, s, in, ldi, (__bf16*)hi,
+                       (__bf16*)lo, ldo);
+  } else {
+    if (cols % 8 || ldi % 2 || ldi < cols || !al16(in)) return hipErrorInvalidValue;
+    const long chunks = (long)rows * (cols / 8);
+    const int blocks = (int)std::min<long>(4096, (chunks + 255) / 256);
+    hipLaunchKernelGGL(split_rows_kernel, dim3(blocks), dim3(256), 0, s, rows, cols, in, ldi, (__bf16*)hi,
+                       (__bf16*)lo, ldo);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_planes(int M, int N, int K, const uint16_t* Ah, const uint16_t* Al, long lda,
+                              const uint16_t* Bh, const uint16_t* Bl, long ldb, void* C, long ldc, bool c_double,
+                              double alpha, double beta, hipStream_t s) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  PlaneGemm g{M, N, K, (const __bf16*)Ah, (const __bf16*)Al, lda, (const __bf16*)Bh, (const __bf16*)Bl, ldb,
+              C, ldc, (float)alpha, beta};
+  constexpr int BK = 64;
+  // full tiles, 16-byte chunks and 32-bit chunk offsets (rows of one tile times the stride)
+  if (K % BK || lda % 8 || ldb % 8 || lda < K || ldb < K) return hipErrorInvalidValue;
+  if (!al16(Ah) || !al16(Al) || !al16(Bh) || !al16(Bl)) return hipErrorInvalidValue;
+  if (!c_double && beta != 0.0) return hipErrorInvalidValue;
+  const long t128 = (long)(M / 128) * (N / 128);
+  const bool big = M % 128 == 0 && N % 128 == 0 && t128 >= 512;
+  const int BM = 128, BN = big ? 128 : 64;
+  if (M % BM || N % BN || ldc < M) return hipErrorInvalidValue;
+  if ((long)BM * lda * 2 >= (1L << 31) || (long)BN * ldb * 2 >= (1L << 31)) return hipErrorInvalidValue;
+  dim3 grid(M / BM, N / BN);
+  if (big) {
+    if (c_double)
+      hipLaunchKernelGGL((gemm_planes_kernel<128, 128, BK, double>), grid, dim3(256), 0, s, g);
+    else
+      hipLaunchKernelGGL((gemm_planes_kernel<128, 128, BK, float>), grid, dim3(256), 0, s, g);
+  } else {
+    if (c_double)
+      hipLaunchKernelGGL((gemm_planes_kernel<128, 64, BK, double>), grid, dim3(256), 0, s, g);
+    else
+      hipLaunchKernelGGL((gemm_planes_kernel<128, 64, BK, float>), grid, dim3(256), 0, s, g);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_wmix_planes(int D, int w, int d, const double* Wm, const float* T1, const double* X0,
+                              uint16_t* Th, uint16_t* Tl, double* Y, hipStream_t s) {
+  if (D % 32 || w < 2 || w > 8 || d > 4 || d < 1) return hipErrorInvalidValue;
+  if (!al16(Th) || !al16(Tl)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((wmix_planes_kernel<8, 4>), dim3(D / 16, D / 32), dim3(256), 0, s, D, w, d, Wm, T1, X0,
+                     (__bf16*)Th, (__bf16*)Tl, Y);
+  return hipGetLastError();
+}
+
+}  // namespace imps

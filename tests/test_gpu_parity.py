@@ -440,6 +440,56 @@ def test_hac_bf16x3_and_mixed_vumps(imps, ctx, oracle):
     assert abs(ib.energy - i64.energy) < 1e-9
 
 
+@pytest.mark.parametrize("model,D", [("hubbard", 128), ("xxz", 256), ("tfim", 128)])
+def test_hac_bf16x3_planes_match_oracle(imps, ctx, oracle, model, D):
+    # identity structural levels and D % 128 == 0: the split-BF16 planes kernels (split, two
+    # plane GEMMs, transposing mix).  Three bf16 products per pair with FP32 accumulation: the
+    # relative error sits near 2^-17 (about 4.5e-6 in an emulation of the same arithmetic);
+    # a dropped correction term would give ~2^-9, a wrong index O(1).
+    Wfn = {"tfim": oracle.tfim_mpo, "xxz": oracle.xxz_mpo, "hubbard": oracle.hubbard_mpo}[model]
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, Wfn, D, 2, seed=D + 1, eye_levels=True)
+    envs = imps.GPUCache.__new__(imps.GPUCache)
+    envs.ctx, envs.ket, envs.operator = ctx, None, H
+    rng = np.random.default_rng(21)
+    for site in range(2):
+        x = rng.standard_normal((D, H.d, D))
+        ref = oracle.hac_apply(GLs[site], H.Ws[site], GRs[site], x)
+        yb = imps.AC_hamiltonian(site, None, H, None, envs, prec=2)(x)
+        assert 1e-8 < rel(yb, ref) < 2e-5
+        # linearity up to the same rounding
+        z = rng.standard_normal((D, H.d, D))
+        lhs = imps.AC_hamiltonian(site, None, H, None, envs, prec=2)(2 * x + z)
+        rhs = 2 * yb + imps.AC_hamiltonian(site, None, H, None, envs, prec=2)(z)
+        assert rel(lhs, rhs) < 2e-5
+
+
+def test_hac_bf16x3_planes_full_size_hubbard(imps, ctx, oracle):
+    D = 1024
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 1, seed=2, eye_levels=True)
+    envs = imps.GPUCache.__new__(imps.GPUCache)
+    envs.ctx, envs.ket, envs.operator = ctx, None, H
+    x = np.random.default_rng(3).standard_normal((D, 4, D))
+    y = imps.AC_hamiltonian(0, None, H, None, envs, prec=2)(x)
+    assert 1e-8 < rel(y, oracle.hac_apply_dense(GLs[0], H.Ws[0], GRs[0], x)) < 2e-5
+
+
+def test_vumps_bf16x3_planes_refined_matches_fp64(imps, ctx, oracle):
+    # TFIM h = 1.2, D = 128: split-BF16 planes bulk phase, FP64 from Galerkin 1e-4 on; the
+    # converged energy equals the plain FP64 run's
+    W = oracle.tfim_mpo(1.0, 1.2)
+    H = imps.SparseIMPO([W])
+    psi0 = oracle.randomimps(128, [2], seed=0)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+    e = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, i64 = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-9, maxiter=300), envs=e)
+    e = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, ib = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-9, maxiter=300, prec=2, refine_below=1e-4), envs=e)
+    assert i64.converged and ib.converged and ib.prec_switch_iter > 1
+    assert abs(ib.energy - i64.energy) < 1e-9
+
+
 def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     # reverse H_AC sandwich against the oracle on a random state
     Wx = oracle.xxz_mpo(1.0, 0.8)
