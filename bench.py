@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--ttg-maxiter", type=int, default=2000)
     ap.add_argument("--ttg-seconds", type=float, default=1e9, help="stop the --ttg run after this wall time")
     ap.add_argument("--refine-below", type=float, default=1e-6, help="mixed precision: FP64 below this Galerkin error")
+    ap.add_argument("--eig-floor", type=float, default=0.0,
+                    help="mixed precision: eigensolver tolerance floor (0: library default)")
     ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -307,7 +309,7 @@ def vumps_iterations(args, imps, H, prec, out, D, N):
         log(f"bench: VUMPS iter {it} eps {eps:.3e} e/site {f / N:.12f} t {el:.1f}s")
         return args.ttg and el > args.ttg_seconds
 
-    alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec, refine_below=args.refine_below)
+    alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec, refine_below=args.refine_below, eig_floor=args.eig_floor)
     _, _, info = imps.find_groundstate(psi, H, alg, envs=envs, callback=cb)
     rec = {"setup_seconds": t_setup, "iterations": info.niter, "loop_seconds": info.seconds,
            "seconds_per_iteration": info.seconds / max(info.niter, 1), "itererr": info.itererr,

@@ -195,6 +195,8 @@ typedef struct imps_vumps_params {
   int envs_maxiter;       /* 300 */
   int prec;               /* IMPS_PREC_* of the bulk contractions */
   double refine_below;    /* mixed precision: switch to FP64 once eps < refine_below */
+  double eig_floor;       /* mixed precision: eigensolver tolerance floor relative to |lambda| in the
+                             reduced-precision phase (0: 1e-6 for FP32, 1e-4 for split-BF16) */
 } imps_vumps_params;
 
 /* IterativeConvergenceInfo (algdefs.jl:180-194) plus timings. */

@@ -37,7 +37,7 @@ class VumpsParams(ctypes.Structure):
                 ("tol_max", c_double), ("eigs_tolfactor", c_double), ("gauge_tolfactor", c_double),
                 ("envs_tolfactor", c_double), ("krylovdim", c_int), ("eig_maxiter", c_int),
                 ("gauge_maxiter", c_int), ("envs_maxiter", c_int), ("prec", c_int),
-                ("refine_below", c_double)]
+                ("refine_below", c_double), ("eig_floor", c_double)]
 
 
 class VumpsInfo(ctypes.Structure):

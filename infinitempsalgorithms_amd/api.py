@@ -383,7 +383,8 @@ class VUMPS:
     """VUMPS(; D, tol, maxiter, verbosity, ...) (algdefs.jl:26-35) with the DynamicTol
     defaults of utility.jl:308-364.  `prec` selects the arithmetic of the bulk
     contractions; with a mixed precision the loop refines in FP64 once the Galerkin
-    error drops below `refine_below`."""
+    error drops below `refine_below`; `eig_floor` (relative to |λ|) bounds the eigensolver
+    tolerance of the reduced-precision phase (0: the library's default per precision)."""
 
     D: int = 64
     tol: float = 1e-10
@@ -400,12 +401,13 @@ class VUMPS:
     envs_maxiter: int = 300
     prec: int = _lib.IMPS_PREC_FP64
     refine_below: float = 0.0
+    eig_floor: float = 0.0
 
     def params(self):
         return _lib.VumpsParams(self.tol, self.maxiter, self.verbosity, self.tol_min, self.tol_max,
                                 self.eigs_tolfactor, self.gauge_tolfactor, self.envs_tolfactor,
                                 self.krylovdim, self.eig_maxiter, self.gauge_maxiter, self.envs_maxiter,
-                                self.prec, self.refine_below)
+                                self.prec, self.refine_below, self.eig_floor)
 
 
 @dataclass

@@ -1957,6 +1957,7 @@ int imps_vumps_default_params(imps_vumps_params* p) {
   p->envs_maxiter = 300;
   p->prec = IMPS_PREC_FP64;
   p->refine_below = 0.0;
+  p->eig_floor = 0.0;
   return IMPS_OK;
 }
 
@@ -1986,7 +1987,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     int switch_iter = -1;
     std::vector<double> lam_ac(N, 1.0), lam_c(N, 1.0);
     double best_eps = std::numeric_limits<double>::infinity();
-    int since_best = 0;
+    bool stalled = false;
     double eps = 0.0;
     lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
     auto tolu = [&](double f, int it, double e) { return updatetol(f, it, e, p->tol_min, p->tol_max); };
@@ -1998,16 +1999,19 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     int it = 0;
     for (it = 1; it <= p->maxiter; ++it) {
       const int siter = it - 1;
-      // FP64 refinement: below refine_below, or once the reduced-precision phase has
-      // stopped improving (no new best Galerkin error for 10 iterations)
-      if (prec != PREC_FP64 && (eps < p->refine_below || since_best >= 10)) {
+      // FP64 refinement: below refine_below, or as soon as the reduced-precision phase
+      // stops improving (an iteration after the third that gains less than 10 % on the best
+      // Galerkin error so far: the operator's rounding level has been reached)
+      if (prec != PREC_FP64 && (eps < p->refine_below || (it > 4 && stalled))) {
         prec = PREC_FP64;
         switch_iter = it;
       }
       double tol_eig = tolu(p->eigs_tolfactor, siter, eps);
       // a reduced-precision operator cannot resolve residuals below its own rounding
       // level: the eigensolver tolerance is floored there until the FP64 refinement
-      const double floor_rel = prec == PREC_FP32 ? 1e-6 : (prec == PREC_BF16X3 ? 1e-4 : 0.0);
+      const double floor_rel = prec == PREC_FP64 ? 0.0
+                               : p->eig_floor > 0.0 ? p->eig_floor
+                                                    : (prec == PREC_FP32 ? 1e-6 : 1e-4);
       lap(tl, [&] {
         std::vector<double> ta(N), tc(N);
         for (int site = 0; site < N; ++site) {
@@ -2021,12 +2025,8 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       sweeps_total += sw;
       lap(te, [&] { envs_recalculate(c, tolu(p->envs_tolfactor, siter, eps), p->envs_maxiter, p->krylovdim, &est); });
       lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
-      if (eps < best_eps) {
-        best_eps = eps;
-        since_best = 0;
-      } else {
-        since_best++;
-      }
+      stalled = eps > 0.9 * best_eps;
+      best_eps = std::min(best_eps, eps);
       if (losses && nl < nloss) losses[nl++] = eps;
       lap(tf, [&] { f = energy(c); });
       if (p->verbosity > 0) {

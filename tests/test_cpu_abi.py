@@ -41,6 +41,8 @@ def test_default_params_match_reference_defaults(lib):
     assert (p.tol, p.maxiter, p.krylovdim) == (1e-10, 300, 30)
     assert (p.tol_min, p.tol_max) == (1e-14, 1e-4)
     assert (p.eigs_tolfactor, p.gauge_tolfactor, p.envs_tolfactor) == (1e-3, 1e-6, 1e-4)
+    # plain FP64, no mixed-precision switch or floor
+    assert (p.prec, p.refine_below, p.eig_floor) == (0, 0.0, 0.0)
 
 
 def test_host_symmetric_eig(lib):
