@@ -10,6 +10,8 @@ per step), inputs resident in HBM, timed with a device synchronisation on both s
 Also reported:
   roofline      the dominant GEMM of H_AC against the FP64 MFMA peak (HIP events on the
                 library's stream);
+  mixed         (FP64 runs) the same H_AC in the split-BF16 planes mode: wall-clock value,
+                per-application cost inside an eigensolve, kernels and roofline;
   vumps         seconds per VUMPS iteration (and its phases) for a few iterations at
                 D = 1024 from a random canonical state (disable with --no-vumps);
   time_to_galerkin  with --ttg: the full run to Galerkin < 1e-8 (long);
@@ -84,6 +86,7 @@ def main():
                     help="mixed precision: eigensolver tolerance floor (0: library default)")
     ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the split-BF16 H_AC line of an FP64 run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the path would shard "
@@ -111,7 +114,25 @@ def main():
 
     # ---- H_AC throughput on device-resident data
     if not args.vumps_only:
-        hac_throughput(args, imps, _lib, H, prec, out, D, d, w, N, F)
+        r = hac_throughput(args, imps, _lib, H, args.prec, D, d, w, N, F)
+        out.update({
+            "value": r["value"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": {"fp64": "f64", "fp32": "f32-on-f64", "bf16x3": "bf16x3-on-f64"}[args.prec],
+            "data": "synthetic (random tensors and environments of the benchmark shape)",
+            "config": {"workload": "H_AC apply, JW Fermi-Hubbard t=1 U=4 mu=2, N=2, d=4, w=6, D=1024"
+                       if D == 1024 else f"H_AC apply, JW Fermi-Hubbard, N=2, d=4, w=6, D={D}",
+                       "D": D, "d": d, "w": w, "flop_per_step": F, "parallelism": "single GPU"},
+            "roofline": r["roofline"], "kernels": r["kernels"],
+        })
+        if args.prec == "fp64" and not args.no_mixed:
+            # the mixed-precision bulk path of the same H_AC (split-BF16 planes, FP32 accumulation)
+            m = hac_throughput(args, imps, _lib, H, "bf16x3", D, d, w, N, F)
+            k = m["kernels"]
+            out["mixed"] = {"prec": "bf16x3", "value": m["value"], "ms_per_step": m["ms_per_step"],
+                            "value_in_eigensolve": k["apply_in_eigensolve"]["tflops"] if "apply_in_eigensolve" in k else None,
+                            "speedup_vs_fp64": r["ms_per_step"] / m["ms_per_step"],
+                            "roofline": m["roofline"], "kernels": k}
 
     # ---- VUMPS outer iterations on the same configuration
     if not args.no_vumps or args.ttg or args.vumps_only:
@@ -123,8 +144,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def hac_throughput(args, imps, _lib, H, prec, out, D, d, w, N, F):
+def hac_throughput(args, imps, _lib, H, pname, D, d, w, N, F):
+    """Times H_AC at precision `pname` on random device-resident data; returns the wall-clock
+    value, the per-kernel HIP-event timings and the roofline entry of the dominant GEMM."""
     import numpy as np
+    prec = PREC[pname]
     ctx = imps.Context(0)
     ctx.set_mpo(H)
     rng = np.random.default_rng(7)
@@ -159,45 +183,43 @@ def hac_throughput(args, imps, _lib, H, prec, out, D, d, w, N, F):
     for _ in range(args.warmup):
         ctx.call("imps_hac_apply_dev", 0, x, y, prec)
     ctx.call("imps_synchronize")
-    log(f"bench: timing {args.steps} H_AC applications at D={D} ({args.prec})")
+    log(f"bench: timing {args.steps} H_AC applications at D={D} ({pname})")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.call("imps_hac_apply_dev", 0, x, y, prec)
     ctx.call("imps_synchronize")
     dt = (time.perf_counter() - t0) / args.steps
-    value = F / dt / 1e12
     t = _lib.HacTiming()
     ctx.call("imps_bench_hac", 0, max(10, args.steps // 2), prec, ctypes.byref(t))
+    ctx.call("imps_dev_free", x)
+    ctx.call("imps_dev_free", y)
+    ctx.close()
     # per-launch algorithmic flops of the GEMMs as executed: 2 (w-1) d D^3 each, the identity
-    # levels being copies (F above keeps the full 4 w d D^3 of the H_AC definition)
+    # levels being copies (F keeps the full 4 w d D^3 of the H_AC definition)
+    planes = t.ms_prepare > 0
+    # mix bytes: FP64 in and out per channel, or (planes) FP32 in / bf16 hi+lo out for the
+    # levels through the GEMMs plus the FP64 x level in and y level out
+    mix_bytes = (2 * w * d * D * D * 8 if not planes
+                 else 2 * ((w - 1) * d * D * D * 4 + d * D * D * 8))
     kernels = {
         "gemm1_GLx": {"ms": t.ms_gemm1, "flop": t.flop_gemm1, "tflops": t.flop_gemm1 / (t.ms_gemm1 * 1e-3) / 1e12},
-        "wmix": {"ms": t.ms_wmix, "gbs": 2 * w * d * D * D * 8 / (t.ms_wmix * 1e-3) / 1e9},
+        "wmix": {"ms": t.ms_wmix, "bytes": mix_bytes, "gbs": mix_bytes / (t.ms_wmix * 1e-3) / 1e9},
         "gemm2_GR": {"ms": t.ms_gemm2, "flop": t.flop_gemm2, "tflops": t.flop_gemm2 / (t.ms_gemm2 * 1e-3) / 1e12},
     }
-    if t.ms_prepare > 0:
+    if planes:
         # split-BF16 planes: the environments are split once per eigensolve; inside a solve an
-        # application costs ms_apply (the value loop above splits them on every call)
+        # application costs ms_apply (the wall-clock loop above splits them on every call)
         kernels["apply_in_eigensolve"] = {"ms": t.ms_apply, "tflops": F / (t.ms_apply * 1e-3) / 1e12}
         kernels["env_split_per_eigensolve"] = {"ms": t.ms_prepare}
     dom = max(("gemm1_GLx", "gemm2_GR"), key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["tflops"]
-    peak = PEAKS[args.prec]
-    ctx.call("imps_dev_free", x)
-    ctx.call("imps_dev_free", y)
-    ctx.close()
-    out.update({
-        "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32-on-f64", "bf16x3": "bf16x3-on-f64"}[args.prec],
-        "data": "synthetic (random tensors and environments of the benchmark shape)",
-        "config": {"workload": "H_AC apply, JW Fermi-Hubbard t=1 U=4 mu=2, N=2, d=4, w=6, D=1024"
-                   if D == 1024 else f"H_AC apply, JW Fermi-Hubbard, N=2, d=4, w=6, D={D}",
-                   "D": D, "d": d, "w": w, "flop_per_step": F, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": TRAFFIC.get((D, args.prec, dom))},
-        "kernels": kernels,
-    })
+    roof = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAKS[pname], "unit": "TFLOP/s",
+            "frac": achieved / PEAKS[pname], "traffic": TRAFFIC.get((D, pname, dom))}
+    if pname == "bf16x3":
+        # three bf16 MFMA products per FP64-equivalent multiply-add, against the dense bf16 peak
+        roof.update({"achieved": 3 * achieved, "frac": 3 * achieved / PEAKS[pname],
+                     "unit": "TFLOP/s (bf16 MFMA: 3 products per FP64-equivalent flop)"})
+    return {"value": F / dt / 1e12, "ms_per_step": dt * 1e3, "kernels": kernels, "roofline": roof}
 
 
 def tfim_exact_energy(h, J=1.0, n=200000):
