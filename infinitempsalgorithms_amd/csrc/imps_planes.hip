@@ -5,7 +5,8 @@
 // with FP32 accumulation (the same three-term product as the on-the-fly split GEMM).
 //
 //   split   : FP64 -> (hi, lo) planes, optionally transposing (GL levels -> [m][k] rows)
-//   gemm    : C(m, n) = alpha sum_k A(m, k) B(k, n) (+ beta C), A(m, k) at m lda + k,
+//   gemm    : (register-staged tiles, or an LDS-DMA ring for long K on small grids)
+//             C(m, n) = alpha sum_k A(m, k) B(k, n) (+ beta C), A(m, k) at m lda + k,
 //             B(k, n) at n ldb + k, C(m, n) at m + n ldc in FP32 or FP64
 //   wmix    : the MPO mix of H_AC between the two GEMMs, reading GEMM 1's FP32 output and
 //             writing GEMM 2's A operand as planes (transposed through LDS so that its rows are
@@ -205,24 +206,177 @@ __global__ void __launch_bounds__(256, 2) gemm_planes_kernel(PlaneGemm g) {
         }
     }
   }
-  // accumulator register r of lane l: m = l % 16, n = 4 (l / 16) + r within the 16 x 16 tile
+  // accumulator register r of lane l: m = l % 16, n = 4 (l / 16) + r within the 16 x 16 tile.
+  // With beta the old values of a row of tiles are loaded together before any store.
   TO* C = (TO*)g.C;
+  const bool with_beta = sizeof(TO) == 8 && g.beta != 0.0;
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
     const int m = m0 + wm + 16 * i + lr;
+    TO* p[TN][4];
 #pragma unroll
     for (int j = 0; j < TN; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = n0 + wn + 16 * j + 4 * lk + r;
-        TO* p = C + m + (long)n * g.ldc;
-        if (sizeof(TO) == 4) {
-          *p = (TO)(g.alpha * acc[i][j][r]);
-        } else {
-          const double v = (double)g.alpha * (double)acc[i][j][r];
-          *p = (TO)(g.beta == 0.0 ? v : v + g.beta * (double)*p);
-        }
+      for (int r = 0; r < 4; ++r) p[j][r] = C + m + (long)(n0 + wn + 16 * j + 4 * lk + r) * g.ldc;
+    if (with_beta) {
+      double old[TN][4];
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) old[j][r] = (double)*p[j][r];
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) *p[j][r] = (TO)((double)g.alpha * (double)acc[i][j][r] + g.beta * old[j][r]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) *p[j][r] = (TO)(g.alpha * acc[i][j][r]);
+    }
+  }
+}
+
+// Long-K variant for grids of about one workgroup per CU: 128 x 128 tile, four waves of 64 x 64,
+// v_mfma_f32_32x32x16_bf16, and the operand stages fetched straight into LDS
+// (global_load_lds_dwordx4: lane L of a wave writes 16 bytes at the wave-uniform LDS address
+// + 16 L) through a ring of NS stages of BK = 32, so NS - 1 stages are in flight while one is
+// consumed.  One DMA instruction fills 16 rows x 4 chunks; chunk c of row r is stored in slot
+// c ^ swz(r), which keeps the 8- and 16-lane groups of a fragment read on disjoint banks.  The
+// stage barrier is a bare s_barrier: __syncthreads() would also wait for every outstanding DMA
+// (vmcnt(0)) and empty the ring; each wave instead waits for its own DMAs of the stage about to
+// be read, then the barrier makes all of them visible.
+__device__ __forceinline__ int ring_swz(int row) {
+  return (((row >> 2) & 1) << 1) | (((row >> 1) ^ (row >> 3)) & 1);
+}
+
+// s_waitcnt vmcnt(N) leaving the other counters alone (expcnt 7, lgkmcnt 15)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+template <int NS, typename TO>
+__global__ void __launch_bounds__(256, 1) gemm_planes_ring_kernel(PlaneGemm g) {
+  constexpr int BM = 128, BN = 128, BK = 32, PL = 128 * BK;
+  constexpr int PER_STAGE = 8;  // DMA instructions per wave and stage (2 blocks x 4 planes)
+  static_assert(NS >= 2 && NS <= 4, "ring depth");
+  __shared__ __attribute__((aligned(1024))) __bf16 ring[NS][4][PL];
+  typedef float v16f __attribute__((ext_vector_type(16)));
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  // this wave fills blocks q = wave and wave + 4 (rows 16 q .. 16 q + 15) of every plane
+  unsigned offA[2], offB[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int r = 16 * (wave + 4 * e) + (lane >> 2);
+    const int c = (lane & 3) ^ ring_swz(r);
+    offA[e] = (unsigned)(2 * ((long)r * g.lda + 8 * c));
+    offB[e] = (unsigned)(2 * ((long)r * g.ldb + 8 * c));
+  }
+  const char* pah = (const char*)(g.Ah + (long)m0 * g.lda);
+  const char* pal = (const char*)(g.Al + (long)m0 * g.lda);
+  const char* pbh = (const char*)(g.Bh + (long)n0 * g.ldb);
+  const char* pbl = (const char*)(g.Bl + (long)n0 * g.ldb);
+  auto issue = [&](int stage, int buf) {
+    const long kb = 2L * BK * stage;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int q = 16 * BK * (wave + 4 * e);
+      __builtin_amdgcn_global_load_lds((const void*)(pah + kb + offA[e]), (void*)&ring[buf][0][q], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(pal + kb + offA[e]), (void*)&ring[buf][1][q], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(pbh + kb + offB[e]), (void*)&ring[buf][2][q], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(pbl + kb + offB[e]), (void*)&ring[buf][3][q], 16, 0, 0);
+    }
+  };
+  v16f acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  // fragment (row, 8-element chunk) offsets of this lane for the two k16 halves of a stage
+  const int lr = lane & 31, lk = lane >> 5;
+  int fa[2][2], fb[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int ra = wm + 32 * i + lr, rb = wn + 32 * i + lr;
+      const int c = 2 * h + lk;
+      fa[h][i] = ra * BK + 8 * (c ^ ring_swz(ra));
+      fb[h][i] = rb * BK + 8 * (c ^ ring_swz(rb));
+    }
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      v8bf ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ah[i] = *(const v8bf*)&ring[buf][0][fa[h][i]];
+        al[i] = *(const v8bf*)&ring[buf][1][fa[h][i]];
+        bh[i] = *(const v8bf*)&ring[buf][2][fb[h][i]];
+        bl[i] = *(const v8bf*)&ring[buf][3][fb[h][i]];
       }
+      // the small products first, each accumulator four instructions apart
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+    }
+  };
+  const int nk = g.K / BK;
+#pragma unroll
+  for (int st = 0; st < NS - 1; ++st)
+    if (st < nk) issue(st, st);
+  for (int kt = 0; kt < nk; ++kt) {
+    // stages kt+1 .. kt+ahead may stay in flight; stage kt must have landed
+    const int ahead = min(NS - 2, nk - 1 - kt);
+    if (ahead >= 2)
+      wait_vmcnt<2 * PER_STAGE>();
+    else if (ahead == 1)
+      wait_vmcnt<PER_STAGE>();
+    else
+      wait_vmcnt<0>();
+    asm volatile("s_barrier" ::: "memory");
+    // the buffer of stage kt - 1 is free: every wave has passed its reads
+    if (kt + NS - 1 < nk) issue(kt + NS - 1, (kt + NS - 1) % NS);
+    compute(kt % NS);
+  }
+  // 32 x 32 accumulator, register r of lane l: m = l % 32, n = 8 (r / 4) + 4 (l / 32) + r % 4.
+  // With beta the 16 old values of a tile are loaded together before any store.
+  TO* C = (TO*)g.C;
+  const bool with_beta = sizeof(TO) == 8 && g.beta != 0.0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int m = m0 + wm + 32 * i + lr;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      TO* p[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) p[r] = C + m + (long)(n0 + wn + 32 * j + 8 * (r / 4) + 4 * lk + (r % 4)) * g.ldc;
+      if (with_beta) {
+        double old[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) old[r] = (double)*p[r];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) *p[r] = (TO)((double)g.alpha * (double)acc[i][j][r] + g.beta * old[r]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) *p[r] = (TO)(g.alpha * acc[i][j][r]);
+      }
+    }
   }
 }
 
@@ -363,12 +517,20 @@ hipError_t launch_gemm_planes(int M, int N, int K, const uint16_t* Ah, const uin
   if (!al16(Ah) || !al16(Al) || !al16(Bh) || !al16(Bl)) return hipErrorInvalidValue;
   if (!c_double && beta != 0.0) return hipErrorInvalidValue;
   const long t128 = (long)(M / 128) * (N / 128);
-  const bool big = M % 128 == 0 && N % 128 == 0 && t128 >= 512;
-  const int BM = 128, BN = big ? 128 : 64;
+  const bool full128 = M % 128 == 0 && N % 128 == 0;
+  const bool big = full128 && t128 >= 512;
+  // long K on about one workgroup per CU: the LDS-ring kernel keeps three stages in flight
+  const bool ring = full128 && !big && K >= 4 * 32;
+  const int BM = 128, BN = (big || ring) ? 128 : 64;
   if (M % BM || N % BN || ldc < M) return hipErrorInvalidValue;
   if ((long)BM * lda * 2 >= (1L << 31) || (long)BN * ldb * 2 >= (1L << 31)) return hipErrorInvalidValue;
   dim3 grid(M / BM, N / BN);
-  if (big) {
+  if (ring) {
+    if (c_double)
+      hipLaunchKernelGGL((gemm_planes_ring_kernel<4, double>), grid, dim3(256), 0, s, g);
+    else
+      hipLaunchKernelGGL((gemm_planes_ring_kernel<4, float>), grid, dim3(256), 0, s, g);
+  } else if (big) {
     if (c_double)
       hipLaunchKernelGGL((gemm_planes_kernel<128, 128, BK, double>), grid, dim3(256), 0, s, g);
     else
