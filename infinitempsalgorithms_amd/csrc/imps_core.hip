@@ -81,6 +81,7 @@ struct imps_ctx {
   // site whose GL/GR split-BF16 planes are current for the eigensolve in progress (-1: none;
   // then a split-BF16 H_AC application splits them itself)
   int planes_site = -1;
+  int hc_planes_site = -1;  // the same for H_C (leftenv(site + 1), rightenv(site))
   // workspaces and pinned host scratch for reductions
   std::map<std::string, DevBuf> ws;
   double* hpart = nullptr;
@@ -397,6 +398,54 @@ void hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec, hipE
   c->hac_applies++;
 }
 
+// Split-BF16 planes path of H_C (identity structural levels, D % 128 == 0): with T1_k = GL_k x,
+// y = T1_{w-1} + sum_{k < w-1} T1_k GR_k.  GEMM 1 makes T1_k for k >= 1 in FP32; T1_0 = x and the
+// FP32 levels 1..w-2 are split (transposed) into the K-contiguous planes of the second GEMM,
+// which runs split-K into FP32 parts; the reduction adds the parts and T1_{w-1} into y.
+bool hc_planes_ok(imps_ctx* c, int site) {
+  return c->D % 128 == 0 && c->w >= 2 && c->gl_id[wrap(c, site + 1)] && c->gr_id[site];
+}
+
+void hc_planes_prepare(imps_ctx* c, int site) {
+  const int D = c->D, w = c->w;
+  const size_t ne = (size_t)D * w * D;
+  HIPCHK(launch_split_planes(D * w, D, c->GL[wrap(c, site + 1)], (long)D * w, true, ws_bf16(c, "hcp_GLh", ne),
+                             ws_bf16(c, "hcp_GLl", ne), D, c->stream));
+  HIPCHK(launch_split_planes(D, D * w, c->GR[site], (long)D * w, false, ws_bf16(c, "hcp_GRh", ne),
+                             ws_bf16(c, "hcp_GRl", ne), (long)D * w, c->stream));
+}
+
+void hc_apply_planes(imps_ctx* c, int site, const double* x, double* y) {
+  const int D = c->D, w = c->w;
+  if (c->hc_planes_site != site) hc_planes_prepare(c, site);
+  const size_t ne = (size_t)D * w * D, nd = (size_t)D * D;
+  const long KT = (long)D * (w - 1);
+  const uint16_t* GLh = ws_bf16(c, "hcp_GLh", ne);
+  const uint16_t* GLl = ws_bf16(c, "hcp_GLl", ne);
+  const uint16_t* GRh = ws_bf16(c, "hcp_GRh", ne);
+  const uint16_t* GRl = ws_bf16(c, "hcp_GRl", ne);
+  uint16_t* xh = ws_bf16(c, "hcp_xh", nd);
+  uint16_t* xl = ws_bf16(c, "hcp_xl", nd);
+  uint16_t* Ah = ws_bf16(c, "hcp_Ah", nd * (w - 1));
+  uint16_t* Al = ws_bf16(c, "hcp_Al", nd * (w - 1));
+  float* T1 = (float*)ws(c, "hcp_T1", (ne + 1) / 2);
+  // split-K depth of GEMM 2: about 512 workgroups of 128 x 64
+  const long tiles = (long)(D / 128) * (D / 64);
+  int S = 1;
+  while (S < 8 && tiles * S * 2 <= 512 && KT % (64L * S * 2) == 0) S *= 2;
+  float* P = (float*)ws(c, "hcp_P", (nd * S + 1) / 2);
+  // B of GEMM 1: x as [b][a'] rows; level 0 of GEMM 2's A: x transposed into [a][b]
+  HIPCHK(launch_split_planes(D, D, x, D, false, xh, xl, D, c->stream));
+  HIPCHK(launch_split_planes(D, D, x, D, true, Ah, Al, KT, c->stream));
+  HIPCHK(launch_gemm_planes(D * (w - 1), D, D, GLh + nd, GLl + nd, D, xh, xl, D, T1 + D, (long)D * w, false, 1.0, 0.0,
+                            c->stream));
+  if (w > 2)
+    HIPCHK(launch_split_planes_levels(D, D, w - 2, T1 + D, (long)D * w, D, Ah + D, Al + D, KT, D, c->stream));
+  HIPCHK(launch_gemm_planes_splitk(D, D, (int)KT, S, Ah, Al, KT, GRh, GRl, (long)D * w, P, c->stream));
+  HIPCHK(launch_planes_reduce(D, S, P, (long)nd, T1 + (size_t)D * (w - 1), (long)D * w, y, c->stream));
+  c->hc_applies++;
+}
+
 // H_C (effective.jl:72-76): y = sum_k GL_k x GR_k with GL = leftenv(site+1), GR = rightenv(site).
 // With identity structural levels, GL_0 x = x (copied into level 0 of T1) and the level-(w-1)
 // term is T1_{w-1} itself (copied into y, onto which the GEMM over the other levels adds).
@@ -406,6 +455,10 @@ void hc_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
   const int sl = wrap(c, site + 1);
   const double* GL = c->GL[sl];
   const double* GR = c->GR[site];
+  if (prec == PREC_BF16X3 && hc_planes_ok(c, site)) {
+    hc_apply_planes(c, site, x, y);
+    return;
+  }
   double* T1 = ws(c, "hc_T1", (size_t)D * w * D);
   const bool skip = w >= 2 && c->gl_id[sl] && c->gr_id[site];
   const int lv = skip ? 1 : 0;
@@ -1424,6 +1477,15 @@ EigOut eigsolve_ac(imps_ctx* c, int site, double tol, int kdim, int maxiter, boo
 EigOut eigsolve_c(imps_ctx* c, int site, double tol, int kdim, int maxiter, bool eager, int prec) {
   const size_t n = (size_t)c->D * c->D;
   EigOut o;
+  site = wrap(c, site);
+  struct PlanesScope {
+    imps_ctx* c;
+    ~PlanesScope() { c->hc_planes_site = -1; }
+  } scope{c};
+  if (prec == PREC_BF16X3 && hc_planes_ok(c, site)) {
+    hc_planes_prepare(c, site);
+    c->hc_planes_site = site;
+  }
   Op op = [&](const double* x, double* y) { hc_apply(c, site, x, y, prec); };
   o.lambda = lanczos(c, op, n, c->C[site], c->pC[site], tol, kdim, maxiter, eager, &o.info);
   return o;

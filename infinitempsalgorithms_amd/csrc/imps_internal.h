@@ -58,6 +58,17 @@ hipError_t launch_split_planes(int rows, int cols, const double* in, long ldi, b
 hipError_t launch_gemm_planes(int M, int N, int K, const uint16_t* Ah, const uint16_t* Al, long lda,
                               const uint16_t* Bh, const uint16_t* Bl, long ldb, void* C, long ldc, bool c_double,
                               double alpha, double beta, hipStream_t s);
+// For each level z < nlev: out[z*out_z + r*ldo + c] = split(in[z*in_z + r + ldi*c]) from FP32
+// (rows % 64 == cols % 64 == 0).
+hipError_t launch_split_planes_levels(int rows, int cols, int nlev, const float* in, long ldi, long in_z,
+                                      uint16_t* hi, uint16_t* lo, long ldo, long out_z, hipStream_t s);
+// Split-K plane GEMM: S partial products over K/S each, P[z*M*N + m + M*n] (FP32);
+// K % (64 S) == 0, M % 128 == 0, N % 64 == 0.
+hipError_t launch_gemm_planes_splitk(int M, int N, int K, int S, const uint16_t* Ah, const uint16_t* Al, long lda,
+                                     const uint16_t* Bh, const uint16_t* Bl, long ldb, float* P, hipStream_t s);
+// y[a + D*b] = sum_{z<S} P[z*pstride + a + D*b] + add[a + add_ld*b] (D x D, FP64 out)
+hipError_t launch_planes_reduce(int D, int S, const float* P, long pstride, const float* add, long add_ld, double* y,
+                                hipStream_t s);
 // The H_AC mix of launch_wmix for identity structural levels on planes: T1 in FP32 (levels
 // 1..w-1 at b + D*(i + w*(s + d*bb))), level 0 from X0; channel j = w-1 to Y (FP64), channels
 // j < w-1 as planes Th/Tl[(b + D*u)*D*(w-1) + D*j + bb].  D % 32 == 0, 2 <= w <= 8, d <= 4.

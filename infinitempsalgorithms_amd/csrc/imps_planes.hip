@@ -58,13 +58,18 @@ __global__ void __launch_bounds__(256) split_rows_kernel(int rows, int cols, con
   }
 }
 
-// out[r ldo + c] = split(in[r + ldi c]) through a 64 x 64 LDS tile; rows % 64 == cols % 64 == 0
-__global__ void __launch_bounds__(256) split_trans_kernel(const double* __restrict__ in, long ldi,
+// out[r ldo + c] = split(in[r + ldi c]) through a 64 x 64 LDS tile; rows % 64 == cols % 64 == 0;
+// blockIdx.z selects a level: in += z in_z, out += z out_z
+template <typename TI>
+__global__ void __launch_bounds__(256) split_trans_kernel(const TI* __restrict__ in, long ldi, long in_z,
                                                           __bf16* __restrict__ hi, __bf16* __restrict__ lo,
-                                                          long ldo) {
+                                                          long ldo, long out_z) {
   __shared__ float T[64][65];
   const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
   const int t = threadIdx.x;
+  in += blockIdx.z * in_z;
+  hi += blockIdx.z * out_z;
+  lo += blockIdx.z * out_z;
   // read: lanes along r (contiguous in `in`)
   for (int e = 0; e < 16; ++e) {
     const int c = (t >> 6) + 4 * e, r = t & 63;
@@ -99,6 +104,7 @@ struct PlaneGemm {
   long ldc;
   float alpha;
   double beta;
+  long zstride;  // split-K: blockIdx.z covers K [z K, (z + 1) K) and writes C + z zstride
 };
 
 // Block = 4 waves (2 x 2), tile BM x BN, K stage BK (multiple of 32).  Each thread moves
@@ -127,10 +133,11 @@ __global__ void __launch_bounds__(256, 2) gemm_planes_kernel(PlaneGemm g) {
   const int crow = tid / CH, ck = 8 * (tid % CH);
 
   // wave-uniform stage bases (bytes) plus one 32-bit offset per chunk (checked on the host)
-  const char* pah = (const char*)(g.Ah + (long)m0 * g.lda);
-  const char* pal = (const char*)(g.Al + (long)m0 * g.lda);
-  const char* pbh = (const char*)(g.Bh + (long)n0 * g.ldb);
-  const char* pbl = (const char*)(g.Bl + (long)n0 * g.ldb);
+  const long kz = (long)blockIdx.z * g.K;
+  const char* pah = (const char*)(g.Ah + (long)m0 * g.lda + kz);
+  const char* pal = (const char*)(g.Al + (long)m0 * g.lda + kz);
+  const char* pbh = (const char*)(g.Bh + (long)n0 * g.ldb + kz);
+  const char* pbl = (const char*)(g.Bl + (long)n0 * g.ldb + kz);
   unsigned offA[AE], offB[BE];
 #pragma unroll
   for (int e = 0; e < AE; ++e) offA[e] = (unsigned)(2 * ((long)(crow + e * RSTEP) * g.lda + ck));
@@ -208,7 +215,7 @@ __global__ void __launch_bounds__(256, 2) gemm_planes_kernel(PlaneGemm g) {
   }
   // accumulator register r of lane l: m = l % 16, n = 4 (l / 16) + r within the 16 x 16 tile.
   // With beta the old values of a row of tiles are loaded together before any store.
-  TO* C = (TO*)g.C;
+  TO* C = (TO*)g.C + blockIdx.z * g.zstride;
   const bool with_beta = sizeof(TO) == 8 && g.beta != 0.0;
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
@@ -483,6 +490,18 @@ __global__ void __launch_bounds__(256) wmix_planes_kernel(int D, int w, int d, c
   }
 }
 
+// y[a + D b] = sum_s P[s pstride + a + D b] + add[a + add_ld b]  (FP32 parts, FP64 sum)
+__global__ void __launch_bounds__(256) planes_reduce_kernel(int D, int S, const float* __restrict__ P, long pstride,
+                                                            const float* __restrict__ add, long add_ld,
+                                                            double* __restrict__ y) {
+  const int a = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
+  if (a >= D) return;
+  const long i = a + (long)D * b;
+  double v = (double)add[a + add_ld * b];
+  for (int z = 0; z < S; ++z) v += (double)P[z * pstride + i];
+  y[i] = v;
+}
+
 bool al16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 }  // namespace
@@ -493,8 +512,8 @@ hipError_t launch_split_planes(int rows, int cols, const double* in, long ldi, b
   if (!al16(hi) || !al16(lo) || ldo % 8 != 0 || ldo < cols) return hipErrorInvalidValue;
   if (trans) {
     if (rows % 64 || cols % 64 || ldi < rows) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(split_trans_kernel, dim3(rows / 64, cols / 64), dim3(256), 0, s, in, ldi, (__bf16*)hi,
-                       (__bf16*)lo, ldo);
+    hipLaunchKernelGGL(split_trans_kernel<double>, dim3(rows / 64, cols / 64), dim3(256), 0, s, in, ldi, 0L,
+                       (__bf16*)hi, (__bf16*)lo, ldo, 0L);
   } else {
     if (cols % 8 || ldi % 2 || ldi < cols || !al16(in)) return hipErrorInvalidValue;
     const long chunks = (long)rows * (cols / 8);
@@ -510,7 +529,7 @@ hipError_t launch_gemm_planes(int M, int N, int K, const uint16_t* Ah, const uin
                               double alpha, double beta, hipStream_t s) {
   if (M <= 0 || N <= 0) return hipSuccess;
   PlaneGemm g{M, N, K, (const __bf16*)Ah, (const __bf16*)Al, lda, (const __bf16*)Bh, (const __bf16*)Bl, ldb,
-              C, ldc, (float)alpha, beta};
+              C, ldc, (float)alpha, beta, 0};
   constexpr int BK = 64;
   // full tiles, 16-byte chunks and 32-bit chunk offsets (rows of one tile times the stride)
   if (K % BK || lda % 8 || ldb % 8 || lda < K || ldb < K) return hipErrorInvalidValue;
@@ -519,8 +538,9 @@ hipError_t launch_gemm_planes(int M, int N, int K, const uint16_t* Ah, const uin
   const long t128 = (long)(M / 128) * (N / 128);
   const bool full128 = M % 128 == 0 && N % 128 == 0;
   const bool big = full128 && t128 >= 512;
-  // long K on about one workgroup per CU: the LDS-ring kernel keeps three stages in flight
-  const bool ring = full128 && !big && K >= 4 * 32;
+  // at most one workgroup per CU (long K on a small grid): the LDS-ring kernel keeps three
+  // stages in flight
+  const bool ring = full128 && t128 <= 256 && K >= 4 * 32;
   const int BM = 128, BN = (big || ring) ? 128 : 64;
   if (M % BM || N % BN || ldc < M) return hipErrorInvalidValue;
   if ((long)BM * lda * 2 >= (1L << 31) || (long)BN * ldb * 2 >= (1L << 31)) return hipErrorInvalidValue;
@@ -541,6 +561,38 @@ hipError_t launch_gemm_planes(int M, int N, int K, const uint16_t* Ah, const uin
     else
       hipLaunchKernelGGL((gemm_planes_kernel<128, 64, BK, float>), grid, dim3(256), 0, s, g);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_split_planes_levels(int rows, int cols, int nlev, const float* in, long ldi, long in_z,
+                                      uint16_t* hi, uint16_t* lo, long ldo, long out_z, hipStream_t s) {
+  if (rows <= 0 || cols <= 0 || nlev <= 0) return hipSuccess;
+  if (rows % 64 || cols % 64 || ldi < rows || ldo < cols || ldo % 8 || out_z % 8) return hipErrorInvalidValue;
+  if (!al16(hi) || !al16(lo)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(split_trans_kernel<float>, dim3(rows / 64, cols / 64, nlev), dim3(256), 0, s, in, ldi, in_z,
+                     (__bf16*)hi, (__bf16*)lo, ldo, out_z);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_planes_splitk(int M, int N, int K, int S, const uint16_t* Ah, const uint16_t* Al, long lda,
+                                     const uint16_t* Bh, const uint16_t* Bl, long ldb, float* P, hipStream_t s) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  constexpr int BK = 64;
+  if (S < 1 || K % (S * BK) || M % 128 || N % 64) return hipErrorInvalidValue;
+  if (lda % 8 || ldb % 8 || lda < K || ldb < K) return hipErrorInvalidValue;
+  if (!al16(Ah) || !al16(Al) || !al16(Bh) || !al16(Bl)) return hipErrorInvalidValue;
+  if (128L * lda * 2 >= (1L << 31) || 64L * ldb * 2 >= (1L << 31)) return hipErrorInvalidValue;
+  PlaneGemm g{M, N, K / S, (const __bf16*)Ah, (const __bf16*)Al, lda, (const __bf16*)Bh, (const __bf16*)Bl, ldb,
+              P, M, 1.0f, 0.0, (long)M * N};
+  hipLaunchKernelGGL((gemm_planes_kernel<128, 64, BK, float>), dim3(M / 128, N / 64, S), dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_planes_reduce(int D, int S, const float* P, long pstride, const float* add, long add_ld, double* y,
+                                hipStream_t s) {
+  if (D <= 0) return hipSuccess;
+  hipLaunchKernelGGL(planes_reduce_kernel, dim3((D + 255) / 256, D), dim3(256), 0, s, D, S, P, pstride, add, add_ld,
+                     y);
   return hipGetLastError();
 }
 

@@ -463,6 +463,22 @@ def test_hac_bf16x3_planes_match_oracle(imps, ctx, oracle, model, D):
         assert rel(lhs, rhs) < 2e-5
 
 
+@pytest.mark.parametrize("model,D", [("hubbard", 128), ("xxz", 256), ("tfim", 128)])
+def test_hc_bf16x3_planes_match_oracle(imps, ctx, oracle, model, D):
+    # H_C on planes: GEMM 1 over the non-identity GL levels, transposing level splits, split-K
+    # GEMM 2 and the reduction that adds the identity level of GR (rel ~6e-6 in an emulation)
+    Wfn = {"tfim": oracle.tfim_mpo, "xxz": oracle.xxz_mpo, "hubbard": oracle.hubbard_mpo}[model]
+    H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, Wfn, D, 2, seed=D + 3, eye_levels=True)
+    envs = imps.GPUCache.__new__(imps.GPUCache)
+    envs.ctx, envs.ket, envs.operator = ctx, None, H
+    rng = np.random.default_rng(23)
+    for site in range(2):
+        x = rng.standard_normal((D, D))
+        ref = oracle.hc_apply(GLs[(site + 1) % 2], GRs[site], x)
+        yb = imps.C_hamiltonian(site, None, H, None, envs, prec=2)(x)
+        assert 1e-8 < rel(yb, ref) < 2e-5
+
+
 def test_hac_bf16x3_planes_full_size_hubbard(imps, ctx, oracle):
     D = 1024
     H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 1, seed=2, eye_levels=True)
@@ -471,6 +487,9 @@ def test_hac_bf16x3_planes_full_size_hubbard(imps, ctx, oracle):
     x = np.random.default_rng(3).standard_normal((D, 4, D))
     y = imps.AC_hamiltonian(0, None, H, None, envs, prec=2)(x)
     assert 1e-8 < rel(y, oracle.hac_apply_dense(GLs[0], H.Ws[0], GRs[0], x)) < 2e-5
+    c = np.random.default_rng(5).standard_normal((D, D))
+    yc = imps.C_hamiltonian(0, None, H, None, envs, prec=2)(c)
+    assert 1e-8 < rel(yc, oracle.hc_apply(GLs[0], GRs[0], c)) < 2e-5
 
 
 def test_vumps_bf16x3_planes_refined_matches_fp64(imps, ctx, oracle):
