@@ -89,8 +89,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the split-BF16 H_AC line of an FP64 run")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the path would shard "
-              "over several GPUs is described in DESIGN.md and is not part of this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the library runs on a "
+              "single GPU and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
