@@ -387,9 +387,9 @@ __global__ void __launch_bounds__(256, 1) gemm_planes_ring_kernel(PlaneGemm g) {
   }
 }
 
-// The MPO mix of H_AC on planes.  Tile: 16 consecutive a x 32 consecutive bb per block; thread t
-// owns a = a0 + t % 16 and bb = bb0 + t / 16 + 16 r (r = 0, 1) with all w d input channels in
-// registers:
+// The MPO mix of H_AC on planes.  Tile: 32 consecutive a x 32 consecutive bb per block of 512
+// threads (FP32 reads of whole 128-byte lines); thread t owns a = a0 + t % 32 and
+// bb = bb0 + t / 32 + 16 r (r = 0, 1) with all w d input channels in registers:
 //   in(i, s) = X0[a + D (s + d bb)] for level i = 0 (the identity level of GL), else
 //              T1[a + D i + D w (s + d bb)] (FP32 output of GEMM 1)
 //   out(u, j) = sum_{i,s} W[(u + d j) + w d (i + w s)] in(i, s), skipping all-zero d x d blocks
@@ -397,16 +397,16 @@ __global__ void __launch_bounds__(256, 1) gemm_planes_ring_kernel(PlaneGemm g) {
 // are staged in LDS and written as planes Th/Tl[(a + D u) KT + D j + bb], KT = D (w - 1), with
 // 4 lanes per 32-element row segment (the transpose that makes GEMM 2's A operand K-contiguous).
 template <int MW, int MD>
-__global__ void __launch_bounds__(256) wmix_planes_kernel(int D, int w, int d, const double* __restrict__ Wm,
+__global__ void __launch_bounds__(512) wmix_planes_kernel(int D, int w, int d, const double* __restrict__ Wm,
                                                           const float* __restrict__ T1,
                                                           const double* __restrict__ X0, __bf16* __restrict__ Th,
                                                           __bf16* __restrict__ Tl, double* __restrict__ Y) {
   __shared__ float Ws[MW * MD * MW * MD];
   __shared__ int nzb[MW * MW];
-  __shared__ float Lt[MD][32][17];
+  __shared__ float Lt[MD][32][33];
   const int wd = w * d;
   const int t = threadIdx.x;
-  for (int e = t; e < wd * wd; e += 256) Ws[e] = (float)Wm[e];
+  for (int e = t; e < wd * wd; e += 512) Ws[e] = (float)Wm[e];
   __syncthreads();
   if (t < w * w) {
     const int j = t / w, i = t % w;
@@ -416,8 +416,8 @@ __global__ void __launch_bounds__(256) wmix_planes_kernel(int D, int w, int d, c
     nzb[j * MW + i] = nz;
   }
   __syncthreads();
-  const int a0 = blockIdx.x * 16, bb0 = blockIdx.y * 32;
-  const int ta = t & 15, tb = t >> 4;
+  const int a0 = blockIdx.x * 32, bb0 = blockIdx.y * 32;
+  const int ta = t & 31, tb = t >> 5;
   const int a = a0 + ta;
   float in[2][MW][MD];
 #pragma unroll
@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(256) wmix_planes_kernel(int D, int w, int d, c
   }
   const long KT = (long)D * (w - 1);
   // transposed write: row (a0 + ra, u = ru) of the tile, 8 bb from 8 q
-  const int ra = (t >> 2) & 15, ru = t >> 6, q = t & 3;
+  const int ra = (t >> 2) & 31, ru = t >> 7, q = t & 3;
   for (int j = 0; j < w; ++j) {
     float o[2][MD];
 #pragma unroll
@@ -600,7 +600,7 @@ hipError_t launch_wmix_planes(int D, int w, int d, const double* Wm, const float
                               uint16_t* Th, uint16_t* Tl, double* Y, hipStream_t s) {
   if (D % 32 || w < 2 || w > 8 || d > 4 || d < 1) return hipErrorInvalidValue;
   if (!al16(Th) || !al16(Tl)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((wmix_planes_kernel<8, 4>), dim3(D / 16, D / 32), dim3(256), 0, s, D, w, d, Wm, T1, X0,
+  hipLaunchKernelGGL((wmix_planes_kernel<8, 4>), dim3(D / 32, D / 32), dim3(512), 0, s, D, w, d, Wm, T1, X0,
                      (__bf16*)Th, (__bf16*)Tl, Y);
   return hipGetLastError();
 }
