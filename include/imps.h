@@ -282,6 +282,11 @@ int imps_bench_hac(imps_ctx* ctx, int site, int reps, int prec, imps_hac_timing*
 int imps_gemm_dev(imps_ctx* ctx, char transa, char transb, int m, int n, int k, double alpha,
                   const double* d_a, int lda, const double* d_b, int ldb, double beta, double* d_c,
                   int ldc, int prec);
+/* GEMM tile autotuning, process-wide, on by default (no counterpart in the reference, whose
+ * contractions go to BLAS): the first FP64/FP32 GEMM of each shape times the library's
+ * bit-identical tile variants once and keeps the fastest; 0 switches back to the fixed
+ * shape heuristic.  Results do not depend on the setting. */
+int imps_set_gemm_autotune(int on);
 /* Device-to-device copy of `bytes` bytes with the library's copy kernel on the
  * context's stream (also the known-size reference for HBM traffic counters). */
 int imps_dev_copy(imps_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);

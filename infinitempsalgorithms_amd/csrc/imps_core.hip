@@ -17,7 +17,9 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <stdexcept>
+#include <tuple>
 #include <string>
 #include <thread>
 #include <vector>
@@ -235,10 +237,75 @@ double trace_prod(imps_ctx* c, int n, const double* l, const double* v) {
 
 // ------------------------------------------------------------------ GEMM wrappers
 
+// GEMM tile autotuning (process-wide, on by default): the first GEMM of each shape, stride and
+// alignment class times every bit-identical tile variant (one warm-up and two timed launches each,
+// into a scratch output with beta = 0) on the caller's stream and keeps the fastest for the rest
+// of the process.  Shapes for which launch_gemm splits K keep its choice.
+std::atomic<bool> g_gemm_autotune{true};
+
+typedef std::tuple<int, int, int, int, int, long, long, long, long, long, long, int> TuneKey;
+
+struct TuneTable {
+  std::mutex mu;
+  std::map<TuneKey, int> best;
+};
+
+TuneTable& tune_table() {
+  static TuneTable t;
+  return t;
+}
+
+int tuned_variant(imps_ctx* c, const GemmDesc& g, int prec, const int* ids, int nv) {
+  const int align = (((uintptr_t)g.A & 15) ? 1 : 0) | (((uintptr_t)g.B & 15) ? 2 : 0);
+  const TuneKey key(prec, g.M, g.N, g.K, g.Kc, g.sAm, g.sAk, g.sAc, g.sBk, g.sBn, g.sBc, align);
+  TuneTable& tt = tune_table();
+  {
+    std::lock_guard<std::mutex> lk(tt.mu);
+    auto it = tt.best.find(key);
+    if (it != tt.best.end()) return it->second;
+  }
+  GemmDesc t = g;
+  t.C = ws(c, "tune_C", (size_t)g.M * g.N);
+  t.ldc = g.M;
+  t.beta = 0.0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int best = ids[0];
+  float best_ms = std::numeric_limits<float>::infinity();
+  for (int k = 0; k < nv; ++k) {
+    HIPCHK(launch_gemm_variant(t, prec, ids[k], c->stream));
+    HIPCHK(hipEventRecord(e0, c->stream));
+    HIPCHK(launch_gemm_variant(t, prec, ids[k], c->stream));
+    HIPCHK(launch_gemm_variant(t, prec, ids[k], c->stream));
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = ids[k];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(tt.mu);
+  tt.best.emplace(key, best);
+  return best;
+}
+
 void gemm(imps_ctx* c, const GemmDesc& g0, int prec) {
   GemmDesc g = g0;
   const size_t nw = gemm_workspace(g.M, g.N, g.K);
   if (nw > 0 && g.W == nullptr) g.W = ws(c, "gemm_splitk", nw);
+  if (g_gemm_autotune.load(std::memory_order_relaxed)) {
+    int ids[4];
+    const int nv = gemm_tile_variants(g, prec, ids);
+    if (nv > 1) {
+      HIPCHK(launch_gemm_variant(g, prec, tuned_variant(c, g, prec, ids, nv), c->stream));
+      return;
+    }
+  }
   HIPCHK(launch_gemm(g, prec, c->stream));
 }
 
@@ -2244,6 +2311,11 @@ int imps_gemm_dev(imps_ctx* c, char transa, char transb, int m, int n, int k, do
     require(ldb >= std::max(1, transb == 'N' ? k : n), IMPS_E_ARG, "imps_gemm_dev: ldb");
     gemm2(c, transa, transb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, prec);
   });
+}
+
+int imps_set_gemm_autotune(int on) {
+  g_gemm_autotune.store(on != 0);
+  return IMPS_OK;
 }
 
 int imps_dev_copy(imps_ctx* c, void* d_dst, const void* d_src, size_t bytes) {

@@ -34,6 +34,11 @@ size_t gemm_workspace(int M, int N, int K);
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
 
 hipError_t launch_gemm(const GemmDesc& g, int prec, hipStream_t s);
+// Bit-identical tile variants of the FP64/FP32 GEMM (no split-K): fills ids with the variants
+// valid for g and returns their number, or 0 when launch_gemm would split K for this shape (or
+// prec is split-BF16).  launch_gemm_variant runs one of them.
+int gemm_tile_variants(const GemmDesc& g, int prec, int* ids);
+hipError_t launch_gemm_variant(const GemmDesc& g, int prec, int variant, hipStream_t s);
 
 // out[a,u,b] = sum_s M[u,s] in[a,s,b] + beta*out[a,u,b]; in/out (D1, d, D2), M (d, d) on host.
 hipError_t launch_mix_mid(int D1, int d, int D2, const double* M, const double* in, double* out,

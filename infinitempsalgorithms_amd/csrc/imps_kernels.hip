@@ -502,15 +502,77 @@ static hipError_t gemm_select(const GemmDesc& g, bool amc, bool bkc, hipStream_t
   return gemm_dispatch_tile<TC, 64, 64, 16, 2, false>(g, amc, bkc, splitk, s);
 }
 
+// split-K depth gemm_select takes for this shape (1: none)
+static int gemm_select_splitk(const GemmDesc& g, bool amc, bool bkc) {
+  const long t64x128 = (long)((g.M + 63) / 64) * ((g.N + 127) / 128);
+  if (t64x128 >= 384 && gemm_vec_ok(g, amc, bkc, 64, 128)) return 1;
+  const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
+  if (t128 >= 192) return 1;
+  const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
+  if (g.W == nullptr || t64 >= 512) return 1;
+  const int ktiles = (g.K + 15) / 16;
+  int splitk = (int)std::min<long>(8, std::max<long>(1, 1024 / std::max<long>(t64, 1)));
+  return std::min(splitk, std::max(1, ktiles / 16));
+}
+
+// checks the descriptor and normalises Kc; false when it is invalid
+static bool gemm_normalise(GemmDesc& g, bool& amc, bool& bkc) {
+  if (g.Kc <= 0 || g.Kc >= g.K) g.Kc = std::max(g.K, 1);
+  if (g.Kc < g.K && (g.Kc % 16) != 0) return false;
+  amc = (g.sAm == 1);
+  bkc = (g.sBk == 1);
+  if (!amc && g.sAk != 1) return false;
+  if (!bkc && g.sBn != 1) return false;
+  return true;
+}
+
+// Tile variants of the FP64/FP32 GEMM without split-K.  They accumulate every output over K in
+// the same order (K tiles of 16, MFMA steps of 4), so their results are bit-identical and the
+// choice among them is a matter of speed only:
+//   0: 64 x 128, occupancy 4, paired loads   1: 64 x 64, occupancy 4, paired loads
+//   2: 128 x 128, occupancy 2                3: 64 x 64, occupancy 2
+template <typename TC>
+static hipError_t gemm_variant(const GemmDesc& g, bool amc, bool bkc, int v, hipStream_t s) {
+  switch (v) {
+    case 0: return gemm_dispatch_tile<TC, 64, 128, 16, 4, true>(g, amc, bkc, 1, s);
+    case 1: return gemm_dispatch_tile<TC, 64, 64, 16, 4, true>(g, amc, bkc, 1, s);
+    case 2: return gemm_dispatch_tile<TC, 128, 128, 16, 2, false>(g, amc, bkc, 1, s);
+    case 3: return gemm_dispatch_tile<TC, 64, 64, 16, 2, false>(g, amc, bkc, 1, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int gemm_tile_variants(const GemmDesc& g0, int prec, int* ids) {
+  GemmDesc g = g0;
+  bool amc, bkc;
+  if (prec != PREC_FP64 && prec != PREC_FP32) return 0;
+  if (g.M <= 0 || g.N <= 0 || !gemm_normalise(g, amc, bkc)) return 0;
+  if (gemm_select_splitk(g, amc, bkc) > 1) return 0;
+  int n = 0;
+  if (gemm_vec_ok(g, amc, bkc, 64, 128)) ids[n++] = 0;
+  if (gemm_vec_ok(g, amc, bkc, 64, 64)) ids[n++] = 1;
+  ids[n++] = 2;
+  ids[n++] = 3;
+  return n;
+}
+
+hipError_t launch_gemm_variant(const GemmDesc& g0, int prec, int v, hipStream_t s) {
+  GemmDesc g = g0;
+  bool amc, bkc;
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (!gemm_normalise(g, amc, bkc)) return hipErrorInvalidValue;
+  if ((v == 0 && !gemm_vec_ok(g, amc, bkc, 64, 128)) || (v == 1 && !gemm_vec_ok(g, amc, bkc, 64, 64)))
+    return hipErrorInvalidValue;
+  if (prec == PREC_FP32) return gemm_variant<float>(g, amc, bkc, v, s);
+  if (prec == PREC_FP64) return gemm_variant<double>(g, amc, bkc, v, s);
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_gemm(const GemmDesc& g0, int prec, hipStream_t s) {
   GemmDesc g = g0;
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
-  if (g.Kc <= 0 || g.Kc >= g.K) g.Kc = std::max(g.K, 1);
-  if (g.Kc < g.K && (g.Kc % 16) != 0) return hipErrorInvalidValue;
-  const bool amc = (g.sAm == 1);
-  const bool bkc = (g.sBk == 1);
-  if (!amc && g.sAk != 1) return hipErrorInvalidValue;
-  if (!bkc && g.sBn != 1) return hipErrorInvalidValue;
+  bool amc, bkc;
+  if (!gemm_normalise(g, amc, bkc)) return hipErrorInvalidValue;
   if (prec == PREC_BF16X3) {
     if (g.Kc < g.K && (g.Kc % 32) != 0) return hipErrorInvalidValue;
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);

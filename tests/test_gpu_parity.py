@@ -70,6 +70,32 @@ def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
     assert rel(got, ref) < 1e-13
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 2048, 1536, 64), ("T", "N", 1920, 1664, 80), ("N", "T", 640, 512, 96),
+                                         ("N", "N", 130, 77, 45), ("N", "N", 5120, 4096, 64)])
+def test_gemm_autotune_is_bit_identical(imps, ctx, prec, ta, tb, m, n, k):
+    # the autotuner picks among tile variants that sum over K in the same order: the output
+    # with autotuning equals the fixed heuristic's bit for bit
+    lib = imps._lib.load()
+    rng = np.random.default_rng(m + 2 * n + 3 * k + prec)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    C0 = rng.standard_normal((m, n))
+    outs = []
+    for on in (1, 1, 0):  # tune, cached choice, heuristic
+        assert lib.imps_set_gemm_autotune(on) == 0
+        dA, dB, dC = DevArray(ctx, A), DevArray(ctx, B), DevArray(ctx, C0)
+        ctx.call("imps_gemm_dev", ta.encode(), tb.encode(), m, n, k, 0.7, dA.p, A.shape[0], dB.p, B.shape[0],
+                 -0.3, dC.p, m, prec)
+        outs.append(dC.get())
+        for d in (dA, dB, dC):
+            d.free()
+    lib.imps_set_gemm_autotune(1)
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[2])
+    ref = 0.7 * (A if ta == "N" else A.T) @ (B if tb == "N" else B.T) - 0.3 * C0
+    assert rel(outs[0], ref) < (1e-13 if prec == 0 else 2e-6)
+
+
 @pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 130, 77, 45), ("T", "N", 1024, 1024, 4096), ("N", "T", 640, 512, 96),
                                          ("N", "N", 2048, 1536, 256)])
 def test_gemm_fp32_mfma(ctx, ta, tb, m, n, k):
