@@ -87,6 +87,7 @@ struct imps_ctx {
   // workspaces and pinned host scratch for reductions
   std::map<std::string, DevBuf> ws;
   double* hpart = nullptr;
+  double* hscal = nullptr;  // pinned scalar readbacks outside the Gram-Schmidt steps (trace_prod)
   double* dpart = nullptr;
   hipEvent_t ev_coef = nullptr;  // marks the arrival of Gram-Schmidt coefficients in hpart
   // counters
@@ -224,15 +225,18 @@ void cgs2_normalize(imps_ctx* c, size_t n, int nv, const double* V, double* w, d
 }
 
 // tr(l v) = sum l[a + ldl*b] v[b + ldv*a] for n x n (contiguous l and v)
+// sum_{a,b} l[a + n b] v[b + n a]: block partials reduced on the device in a fixed order, one
+// double read back into pinned memory (this runs inside regularised GMRES operators, while a
+// speculative Gram-Schmidt readback may be in flight into hpart, hence its own buffer)
 double trace_prod(imps_ctx* c, int n, const double* l, const double* v) {
   int np = 0;
-  double* part = ws(c, "tp_part", (size_t)((n + 31) / 32) * ((n + 31) / 32));
+  const size_t maxp = (size_t)((n + 31) / 32) * ((n + 31) / 32);
+  double* part = ws(c, "tp_part", maxp + 1);
   HIPCHK(launch_trace_prod_parts(n, l, v, part, &np, c->stream));
-  std::vector<double> h(np);
-  d2h(c, h.data(), part, np);
-  double s = 0.0;
-  for (double x : h) s += x;
-  return s;
+  HIPCHK(launch_partials_reduce(1, np, part, part + maxp, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hscal, part + maxp, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return c->hscal[0];
 }
 
 // ------------------------------------------------------------------ GEMM wrappers
@@ -1567,6 +1571,7 @@ imps_ctx* worker(imps_ctx* c, int k) {
     c->workers.push_back(w);
     HIPCHK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc((void**)&w->hpart, sizeof(double) * kMaxDotBlocks * kMaxVec, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&w->hscal, sizeof(double) * 8, hipHostMallocDefault));
     w->dpart = dalloc((size_t)kMaxDotBlocks * kMaxVec);
     HIPCHK(hipEventCreateWithFlags(&w->ev_coef, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
@@ -1598,6 +1603,7 @@ void destroy_worker(imps_ctx* w) {
   for (auto& kv : w->ws) dfree(kv.second.p);
   dfree(w->dpart);
   if (w->hpart) (void)hipHostFree(w->hpart);
+  if (w->hscal) (void)hipHostFree(w->hscal);
   if (w->ev_coef) (void)hipEventDestroy(w->ev_coef);
   if (w->ev_done) (void)hipEventDestroy(w->ev_done);
   if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -1689,6 +1695,7 @@ int imps_create(int device, imps_ctx** out) {
     RBCHK(rocblas_create_handle(&c->rb));
     RBCHK(rocblas_set_stream(c->rb, c->stream));
     HIPCHK(hipHostMalloc((void**)&c->hpart, sizeof(double) * kMaxDotBlocks * kMaxVec, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&c->hscal, sizeof(double) * 8, hipHostMallocDefault));
     c->dpart = dalloc((size_t)kMaxDotBlocks * kMaxVec);
     HIPCHK(hipEventCreateWithFlags(&c->ev_coef, hipEventDisableTiming));
   } catch (const ImpsError& e) {
@@ -1714,6 +1721,7 @@ int imps_destroy(imps_ctx* c) {
   for (auto& kv : c->ws) dfree(kv.second.p);
   dfree(c->dpart);
   if (c->hpart) (void)hipHostFree(c->hpart);
+  if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->ev_coef) (void)hipEventDestroy(c->ev_coef);
   if (c->rb) rocblas_destroy_handle(c->rb);
   if (c->stream) (void)hipStreamDestroy(c->stream);
