@@ -1077,3 +1077,100 @@ def find_groundstate(psi0, Ws, alg=None, envs=None, callback=None):
             converged = True
             break
     return psi, envs, IterativeConvergenceInfo(it, converged, losses, energies)
+
+
+# ---------------------------------------------------------------------------
+# Single-site IDMRG (src/algorithms/groundstates/idmrg.jl)
+# ---------------------------------------------------------------------------
+
+
+def push_env_left_mpo(L, W, A):
+    """MPO-channel left push, transfermatrix.jl:11-13 (real):
+    L'[a',w',b'] = A[a,u,a'] L[a,w,b] W[w,u,w',d] A[b,d,b']."""
+    Y = np.einsum("awb,bdc->awdc", L, A, optimize=True)
+    Y = np.einsum("wuvd,awdc->auvc", W, Y, optimize=True)
+    return np.einsum("aup,auvc->pvc", A, Y, optimize=True)
+
+
+def push_env_right_mpo(R, W, A):
+    """MPO-channel right push, transfermatrix.jl:31-35 (real):
+    R'[a',w',b'] = A[a',d,a] R[a,w,b] W[w',u,w,d] A[b',u,b]."""
+    Y = np.einsum("pda,awb->pdwb", A, R, optimize=True)
+    Y = np.einsum("vuwd,pdwb->pvub", W, Y, optimize=True)
+    return np.einsum("pvub,qub->pvq", Y, A, optimize=True)
+
+
+@dataclass
+class IDMRG:
+    """src/algorithms/algdefs.jl:72-82 with the Defaults of utility.jl:308-364 (Lanczos eigensolver
+    and gauge with DynamicTol; QRpos orthogonal factorisation)."""
+
+    tol: float = Defaults.tol
+    maxiter: int = Defaults.maxiter
+    verbosity: int = Defaults.verbosity
+    eigs_tolfactor: float = Defaults.eigs_tolfactor
+    gauge_tolfactor: float = Defaults.gauge_tolfactor
+    krylovdim: int = Defaults.krylovdim
+    eig_maxiter: int = Defaults.maxiter
+    gauge_maxiter: int = Defaults.maxiter
+
+
+def idmrg_sweep(psi, Ws, envs, tol_eig, alg):
+    """_localupdate_sweep_idmrg! (idmrg.jl:12-34): forward sweep (eigsolve, QRpos split into AL, C,
+    AC <- AL except at the last site, transfer_leftenv!), backward sweep (eigsolve, LQpos split into
+    C, AR, AC <- AR except at the first site, transfer_rightenv!).  transfer_*env! (envs.jl:591-604)
+    are the MPO-channel pushes without normalisation.  Returns (psi, envs, C_old, E)."""
+    N = len(psi)
+    C_old = psi.C[N - 1].copy()
+    for pos in range(N):
+        GL, GR = envs.leftenv(pos), envs.rightenv(pos)
+        _, AC, _ = lanczos_eigsolve(lambda x: hac_apply(GL, Ws[pos], GR, x), psi.AC[pos], tol_eig,
+                                    alg.krylovdim, alg.eig_maxiter)
+        D1, d, D2 = AC.shape
+        Q, R = qrpos(AC.reshape(D1 * d, D2, order="F"))
+        psi.AL[pos] = Q.reshape(D1, d, D2, order="F")
+        psi.C[pos] = R
+        psi.AC[pos] = psi.AL[pos].copy() if pos < N - 1 else AC
+        envs.lefts[(pos + 1) % N] = push_env_left_mpo(envs.lefts[pos], Ws[pos], psi.AL[pos])
+    E = None
+    for pos in range(N - 1, -1, -1):
+        GL, GR = envs.leftenv(pos), envs.rightenv(pos)
+        E, AC, _ = lanczos_eigsolve(lambda x: hac_apply(GL, Ws[pos], GR, x), psi.AC[pos], tol_eig,
+                                    alg.krylovdim, alg.eig_maxiter)
+        D1, d, D2 = AC.shape
+        L, Q = lqpos(AC.reshape(D1, d * D2, order="F"))
+        psi.C[(pos - 1) % N] = L
+        psi.AR[pos] = Q.reshape(D1, d, D2, order="F")
+        psi.AC[pos] = psi.AR[pos].copy() if pos > 0 else AC
+        envs.rights[(pos - 1) % N] = push_env_right_mpo(envs.rights[pos], Ws[pos], psi.AR[pos])
+    return psi, envs, C_old, E
+
+
+def find_groundstate_idmrg(psi0, Ws, alg=None, envs=None, callback=None):
+    """find_groundstate(psi0, H::SparseIMPO, alg::IDMRG, envs) (idmrg.jl:44-86): sweeps until
+    eps = ||C[bond 0] - C_old|| <= tol, then the state is rebuilt from AR (CanonicalIMPS with the
+    dynamic gauge tolerance) and the environments recalculated.  losses = [initial Galerkin
+    error, eps per sweep]; energies = the backward sweep's final eigenvalue per sweep."""
+    alg = alg or IDMRG()
+    if envs is None:
+        envs = dmrg_cache(psi0, Ws)
+    psi = psi0.copy()
+    eps = calc_galerkin(psi, Ws, envs)
+    losses, energies = [eps], []
+    converged = False
+    it = 0
+    for it in range(1, alg.maxiter + 1):
+        tol_eig = updatetol(alg.eigs_tolfactor, it - 1, eps)
+        psi, envs, C_old, E = idmrg_sweep(psi, Ws, envs, tol_eig, alg)
+        eps = float(np.linalg.norm(psi.C[len(psi) - 1] - C_old))
+        losses.append(eps)
+        energies.append(E)
+        if callback is not None:
+            callback(it, eps, E)
+        if eps <= alg.tol:
+            converged = True
+            break
+    tol_gauge = updatetol(alg.gauge_tolfactor, it, eps)
+    psi2 = canonical_from_tensors([a.copy() for a in psi.AR], tol=tol_gauge, maxiter=alg.gauge_maxiter)
+    recalculate(envs, psi2, Ws, Defaults.tol)
+    return psi2, envs, IterativeConvergenceInfo(it, converged, losses, energies)
