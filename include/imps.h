@@ -234,6 +234,19 @@ int imps_vumps_default_params(imps_vumps_params* p);
 int imps_vumps_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* info,
                    double* losses, int nloss, imps_iter_cb cb, void* user);
 
+/* find_groundstate(ψ₀, H::SparseIMPO, alg::IDMRG, envs) (src/algorithms/groundstates/idmrg.jl:44-86):
+ * single-site IDMRG sweeps (_localupdate_sweep_idmrg!, idmrg.jl:12-34: forward H_AC eigensolves
+ * with QRpos splits and MPO-channel left pushes, backward eigensolves with LQpos splits and right
+ * pushes), stopping when the bond-0 centre matrix moves by at most p->tol; the state is then
+ * rebuilt from AR (CanonicalIMPS with the dynamic gauge tolerance) and the environments
+ * recalculated.  Uses p's tol, maxiter, verbosity, tol_min/max, eigs/gauge tolerance factors,
+ * krylovdim and maxiters; FP64 throughout (prec, refine_below and eig_floor are ignored).
+ * `losses` receives the initial Galerkin error, then the centre drift of every sweep; `cb`
+ * receives (iter, drift, lowest H_AC eigenvalue of the sweep).  info->energy is
+ * expectationvalue(ψ, H, envs) of the rebuilt state. */
+int imps_idmrg_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* info,
+                   double* losses, int nloss, imps_iter_cb cb, void* user);
+
 /* ---- batched sweeps ------------------------------------------------------------------ */
 
 /* One instance of a parameter sweep: its MPO, raw ψ₀ tensors (canonicalised on the device

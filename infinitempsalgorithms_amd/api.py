@@ -411,6 +411,32 @@ class VUMPS:
 
 
 @dataclass
+class IDMRG:
+    """IDMRG(; D, tol, maxiter, verbosity, ...) (algdefs.jl:72-82): single-site infinite DMRG with
+    the same DynamicTol eigensolver and gauge defaults as VUMPS (utility.jl:308-364)."""
+
+    D: int = 64
+    tol: float = 1e-10
+    maxiter: int = 300
+    verbosity: int = 0
+    tol_min: float = 1e-14
+    tol_max: float = 1e-4
+    eigs_tolfactor: float = 1e-3
+    gauge_tolfactor: float = 1e-6
+    envs_tolfactor: float = 1e-4
+    krylovdim: int = 30
+    eig_maxiter: int = 300
+    gauge_maxiter: int = 300
+    envs_maxiter: int = 300
+
+    def params(self):
+        return _lib.VumpsParams(self.tol, self.maxiter, self.verbosity, self.tol_min, self.tol_max,
+                                self.eigs_tolfactor, self.gauge_tolfactor, self.envs_tolfactor,
+                                self.krylovdim, self.eig_maxiter, self.gauge_maxiter, self.envs_maxiter,
+                                _lib.IMPS_PREC_FP64, 0.0, 0.0)
+
+
+@dataclass
 class IterativeConvergenceInfo:
     """IterativeConvergenceInfo (algdefs.jl:180-194) plus the library's timings."""
 
@@ -431,12 +457,14 @@ class IterativeConvergenceInfo:
         return self.losses[-1] if self.losses else float("nan")
 
 
-def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | None = None,
+def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | IDMRG | None = None,
                      envs: GPUCache | None = None, seed=0, ctx: Context | None = None, callback=None):
-    """find_groundstate(ψ₀, H::SparseIMPO, alg::VUMPS, envs) (vumps.jl:51-86).  With
-    psi0 = None a random state of bond dimension alg.D is generated (vumps.jl:96-102).
-    `callback(iter, eps, energy)` is called after every iteration (alg.finalize's slot);
-    a truthy return value stops the loop.  Returns (ψ, envs, info)."""
+    """find_groundstate(ψ₀, H::SparseIMPO, alg::VUMPS, envs) (vumps.jl:51-86), or with
+    alg::IDMRG the single-site IDMRG driver (idmrg.jl:44-86).  With psi0 = None a random state
+    of bond dimension alg.D is generated (vumps.jl:96-102).  `callback(iter, eps, energy)` is
+    called after every iteration (alg.finalize's slot); a truthy return value stops the loop.
+    Returns (ψ, envs, info); for IDMRG, info.losses holds the initial Galerkin error and then the
+    bond-0 centre drift of each sweep, info.energies the sweeps' lowest H_AC eigenvalues."""
     alg = alg or VUMPS()
     if envs is None:
         if psi0 is None:
@@ -456,7 +484,8 @@ def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | Non
         return 0
 
     cb = _lib.ITER_CB(_cb)
-    envs.ctx.call("imps_vumps_run", ctypes.byref(p), ctypes.byref(info),
+    entry = "imps_idmrg_run" if isinstance(alg, IDMRG) else "imps_vumps_run"
+    envs.ctx.call(entry, ctypes.byref(p), ctypes.byref(info),
                   losses.ctypes.data_as(_lib.P_double), len(losses), cb, None)
     psi = envs.ctx.get_state()
     envs.ket = psi

@@ -661,6 +661,33 @@ void slice_sum_right(imps_ctx* c, const std::vector<SliceTerm>& terms, const dou
   gemm2(c, 'N', 'T', D, D, d * D, 1.0, Z, D, A, D, 0.0, out, ldo);
 }
 
+// MPO-channel pushes (transfermatrix.jl:11-13, 31-35) as sums of slice pushes over the nonzero
+// d x d blocks of W: out[:, v, :] = sum_k push_slice_left(L[:, k, :], W[k, :, v, :], A) and
+// out[:, v, :] = sum_k push_slice_right(R[:, k, :], W[v, :, k, :], A).  out must not alias L / R.
+void push_mpo_left(imps_ctx* c, int site, const double* L, const double* A, double* out) {
+  const int D = c->D, w = c->w;
+  for (int v = 0; v < w; ++v) {
+    std::vector<SliceTerm> terms;
+    for (int k = 0; k < w; ++k) {
+      std::vector<double> M = wblock(c, site, k, v);
+      if (!is_zero(M)) terms.push_back(SliceTerm{L + (size_t)D * k, (long)D * w, M});
+    }
+    slice_sum_left(c, terms, A, out + (size_t)D * v, (long)D * w);
+  }
+}
+
+void push_mpo_right(imps_ctx* c, int site, const double* R, const double* A, double* out) {
+  const int D = c->D, w = c->w;
+  for (int v = 0; v < w; ++v) {
+    std::vector<SliceTerm> terms;
+    for (int k = 0; k < w; ++k) {
+      std::vector<double> M = wblock(c, site, v, k);
+      if (!is_zero(M)) terms.push_back(SliceTerm{R + (size_t)D * k, (long)D * w, M});
+    }
+    slice_sum_right(c, terms, A, out + (size_t)D * v, (long)D * w);
+  }
+}
+
 // ------------------------------------------------------------------ Krylov solvers
 
 typedef std::function<void(const double*, double*)> Op;
@@ -2197,6 +2224,99 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       info->gauge_sweeps = sweeps_total;
       info->gmres_ops = est.ops;
       info->qr_fallbacks = (int)(g_qr_fallbacks - qf0);
+    }
+  });
+}
+
+int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* info, double* losses, int nloss,
+                   imps_iter_cb cb, void* user) {
+  if (!c || !p) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    const int N = c->N, D = c->D, d = c->d, w = c->w;
+    const long hac0 = c->hac_applies;
+    auto t0 = std::chrono::steady_clock::now();
+    auto secs = [&] {
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    auto tolu = [&](double f, int it, double e) { return updatetol(f, it, e, p->tol_min, p->tol_max); };
+    const size_t nA = (size_t)D * d * D, nC = (size_t)D * D, nE = (size_t)D * w * D;
+    // envs = DMRGCache(psi0, H) at its default tolerance (idmrg.jl:46)
+    if (!c->envs_valid) envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, nullptr);
+    double eps = galerkin(c, PREC_FP64);
+    int nl = 0;
+    if (losses && nl < nloss) losses[nl++] = eps;
+    double* Cold = ws(c, "id_Cold", nC);
+    double* Env = ws(c, "id_env", nE);
+    double E = 0.0;
+    bool converged = false;
+    int it = 0;
+    for (it = 1; it <= p->maxiter; ++it) {
+      const double tol_eig = tolu(p->eigs_tolfactor, it - 1, eps);
+      // _localupdate_sweep_idmrg! (idmrg.jl:12-34)
+      d2d(c, Cold, c->C[N - 1], nC);
+      for (int pos = 0; pos < N; ++pos) {
+        eigsolve_ac(c, pos, tol_eig, p->krylovdim, p->eig_maxiter, true, PREC_FP64);
+        d2d(c, c->AL[pos], c->pAC[pos], nA);
+        qrpos(c, D * d, D, c->AL[pos], c->C[pos]);
+        d2d(c, c->AC[pos], pos < N - 1 ? c->AL[pos] : c->pAC[pos], nA);
+        // transfer_leftenv! (envs.jl:591-596): a pushed environment is no longer pinned
+        const int nxt = (pos + 1) % N;
+        push_mpo_left(c, pos, c->GL[pos], c->AL[pos], Env);
+        d2d(c, c->GL[nxt], Env, nE);
+        c->gl_id[nxt] = 0;
+      }
+      for (int pos = N - 1; pos >= 0; --pos) {
+        E = eigsolve_ac(c, pos, tol_eig, p->krylovdim, p->eig_maxiter, true, PREC_FP64).lambda;
+        const int prv = (pos - 1 + N) % N;
+        lqpos(c, D, d * D, c->pAC[pos], c->C[prv], c->AR[pos]);
+        d2d(c, c->AC[pos], pos > 0 ? c->AR[pos] : c->pAC[pos], nA);
+        // transfer_rightenv! (envs.jl:599-604)
+        push_mpo_right(c, pos, c->GR[pos], c->AR[pos], Env);
+        d2d(c, c->GR[prv], Env, nE);
+        c->gr_id[prv] = 0;
+      }
+      // error criterion: drift of the centre matrix on bond 0 (idmrg.jl:63)
+      HIPCHK(launch_axpby(nC, 1.0, c->C[N - 1], -1.0, Cold, c->stream));
+      eps = nrm2(c, nC, Cold);
+      if (losses && nl < nloss) losses[nl++] = eps;
+      if (p->verbosity > 0) {
+        printf("IDMRG iter %4d : eps = %.3e f = %.10g  [%.2fs]\n", it, eps, E, secs());
+        fflush(stdout);
+      }
+      if (eps <= p->tol) converged = true;
+      if (cb && cb(it, eps, E, user) != 0) break;
+      if (converged) break;
+    }
+    const int niter = std::min(it, p->maxiter);
+    // rebuild from AR with the dynamic gauge tolerance, then recalculate! (idmrg.jl:76-84)
+    const double tol_g = tolu(p->gauge_tolfactor, niter, eps);
+    std::vector<double*> raw(N);
+    for (int l = 0; l < N; ++l) {
+      raw[l] = c->pAC[l];
+      d2d(c, raw[l], c->AR[l], nA);
+    }
+    double* I = ws(c, "sr_I", nC);
+    HIPCHK(launch_set_identity(D, I, D, c->stream));
+    uniform_leftorth(c, raw, I, tol_g, p->gauge_maxiter);
+    double* C0 = ws(c, "sr_C0", nC);
+    d2d(c, C0, c->C[N - 1], nC);
+    uniform_rightorth(c, c->AL, C0, tol_g, p->gauge_maxiter);
+    for (int l = 0; l < N; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+    envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, nullptr);
+    c->envs_valid = true;
+    const double f = energy(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (info) {
+      *info = imps_vumps_info{};
+      info->niter = niter;
+      info->converged = converged ? 1 : 0;
+      info->itererr = eps;
+      info->energy = f;
+      info->seconds = secs();
+      info->t_galerkin_reached = converged ? info->seconds : -1.0;
+      info->hac_applies = (int)(c->hac_applies - hac0);
+      info->prec_switch_iter = -1;
     }
   });
 }
