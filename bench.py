@@ -15,6 +15,7 @@ Also reported:
   vumps         seconds per VUMPS iteration (and its phases) for a few iterations at
                 D = 1024 from a random canonical state (disable with --no-vumps);
   time_to_galerkin  with --ttg: the full run to Galerkin < 1e-8 (long);
+  idmrg         with --idmrg K: seconds per single-site IDMRG sweep over K sweeps;
   cpu_baseline  the FP64 CPU restatement of H_AC (numpy / OpenBLAS) on a bounded sample.
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -82,6 +83,7 @@ def main():
     ap.add_argument("--ttg-maxiter", type=int, default=2000)
     ap.add_argument("--ttg-seconds", type=float, default=1e9, help="stop the --ttg run after this wall time")
     ap.add_argument("--refine-below", type=float, default=1e-6, help="mixed precision: FP64 below this Galerkin error")
+    ap.add_argument("--idmrg", type=int, default=0, help="also time this many single-site IDMRG sweeps")
     ap.add_argument("--eig-floor", type=float, default=0.0,
                     help="mixed precision: eigensolver tolerance floor (0: library default)")
     ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
@@ -137,6 +139,9 @@ def main():
     # ---- VUMPS outer iterations on the same configuration
     if not args.no_vumps or args.ttg or args.vumps_only:
         vumps_iterations(args, imps, H, prec, out, D, N)
+
+    if args.idmrg > 0:
+        out["idmrg"] = idmrg_sweeps(args, imps, H, D, N)
 
     if not args.no_cpu and not args.vumps_only:
         log("bench: CPU baseline (numpy restatement)")
@@ -220,6 +225,27 @@ def hac_throughput(args, imps, _lib, H, pname, D, d, w, N, F):
         roof.update({"achieved": 3 * achieved, "frac": 3 * achieved / PEAKS[pname],
                      "unit": "TFLOP/s (bf16 MFMA: 3 products per FP64-equivalent flop)"})
     return {"value": F / dt / 1e12, "ms_per_step": dt * 1e3, "kernels": kernels, "roofline": roof}
+
+
+def idmrg_sweeps(args, imps, H, D, N):
+    """Single-site IDMRG (idmrg.jl:12-86) from the same random canonical state as the VUMPS
+    timing: seconds per sweep (two eigensolves and two MPO-channel pushes per site)."""
+    ctx = imps.Context(0)
+    psi, ctx = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=100)
+    envs = imps.GPUCache(psi, H, ctx)
+    marks = []
+    t0 = time.perf_counter()
+
+    def cb(it, eps, e):
+        marks.append((it, eps, time.perf_counter() - t0))
+        log(f"bench: IDMRG sweep {it} drift {eps:.3e} t {marks[-1][2]:.1f}s")
+
+    _, _, info = imps.find_groundstate(psi, H, imps.IDMRG(tol=1e-12, maxiter=args.idmrg), envs=envs, callback=cb)
+    ctx.close()
+    loop = marks[-1][2] if marks else 0.0
+    return {"sweeps": info.niter, "seconds_per_sweep": loop / max(len(marks), 1), "loop_seconds": loop,
+            "total_seconds": info.seconds, "drift": info.itererr, "energy_per_site": info.energy / N,
+            "hac_applies": info.hac_applies}
 
 
 def tfim_exact_energy(h, J=1.0, n=200000):
