@@ -1174,3 +1174,123 @@ def find_groundstate_idmrg(psi0, Ws, alg=None, envs=None, callback=None):
     psi2 = canonical_from_tensors([a.copy() for a in psi.AR], tol=tol_gauge, maxiter=alg.gauge_maxiter)
     recalculate(envs, psi2, Ws, Defaults.tol)
     return psi2, envs, IterativeConvergenceInfo(it, converged, losses, energies)
+
+
+# ---------------------------------------------------------------------------
+# TDVP time step (src/algorithms/timeevo/tdvp.jl), real dt (imaginary time)
+# ---------------------------------------------------------------------------
+
+
+def lanczos_exponentiate(op, t, x0, tol, krylovdim=Defaults.krylovdim, maxiter=Defaults.maxiter):
+    """exp(t op) x0 for a symmetric operator and real t: KrylovKit.exponentiate with Lanczos
+    (the `integrate` of tdvp.jl:14-19; KrylovKit is not vendored, so this restates its published
+    scheme).  A Lanczos basis (full reorthogonalisation) is grown from the current vector; with
+    T = U diag(theta) U^T the projected exponential y(tau) = U exp(tau theta) U^T e1 is accepted for
+    the whole remaining time when its residual estimate |beta_K y_K(tau)| <= tol; when the basis
+    reaches krylovdim first, the largest tau = remaining / 2^j with an accepted estimate is taken,
+    the vector advanced to V y(tau) and the basis rebuilt for the rest.  Returns (x, info)."""
+    n = x0.size
+    kmax = min(krylovdim, n)
+    scale = float(np.linalg.norm(x0))
+    v = x0.reshape(-1, order="F") / scale
+    remaining = float(t)
+    steps = 0
+    converged = True
+    while True:
+        V = np.zeros((n, kmax + 1))
+        V[:, 0] = v
+        T = np.zeros((kmax + 1, kmax + 1))
+        beta = 0.0
+        K = 0
+        accepted = None
+        while True:
+            w = op(V[:, K].reshape(x0.shape, order="F")).reshape(-1, order="F")
+            w, h = _cgs2(w, V[:, :K + 1])
+            beta = float(np.linalg.norm(w))
+            T[:K + 1, K] = h
+            T[K, :K + 1] = h
+            K += 1
+            if beta > 0:
+                V[:, K] = w / beta
+            theta, U = np.linalg.eigh(0.5 * (T[:K, :K] + T[:K, :K].T))
+
+            def y_of(tau):
+                return U @ (np.exp(tau * theta) * U[0, :])
+
+            y = y_of(remaining)
+            if abs(beta * y[K - 1]) <= tol or beta == 0.0:
+                accepted = (remaining, y)
+                break
+            if K == kmax:
+                tau = remaining
+                for _ in range(60):
+                    tau *= 0.5
+                    y = y_of(tau)
+                    if abs(beta * y[K - 1]) <= tol:
+                        accepted = (tau, y)
+                        break
+                break
+        steps += 1
+        if accepted is None:
+            converged = False
+            tau, y = remaining, y_of(remaining)
+        else:
+            tau, y = accepted
+        x = V[:, :K] @ y
+        ny = float(np.linalg.norm(x))
+        scale *= ny
+        v = x / ny
+        remaining -= tau
+        if remaining == 0.0 or not converged or steps >= maxiter:
+            converged = converged and remaining == 0.0
+            break
+    return (scale * v).reshape(x0.shape, order="F"), {"steps": steps, "converged": converged}
+
+
+@dataclass
+class TDVP:
+    """src/algorithms/algdefs.jl:167-173: integrator = alg_expsolve() (Lanczos, utility.jl:343-347),
+    alg_gauge = Defaults.alg_gauge() (tolgauge, maxiter)."""
+
+    tol: float = Defaults.tol
+    krylovdim: int = Defaults.krylovdim
+    maxiter: int = Defaults.maxiter
+    gauge_tol: float = Defaults.tolgauge
+    gauge_maxiter: int = Defaults.maxiter
+
+
+def tdvp_timestep(psi, Ws, envs, dt, alg=None):
+    """timestep(psi, H, dt, alg::TDVP, envs) (tdvp.jl:33-52) for real dt: every site's AC and C
+    evolved with exp(dt H_AC), exp(dt H_C); AL from regauge! (ortho.jl:214-219); the state
+    rebuilt as CanonicalIMPS(ALs, C[end]) (canonicalmps.jl:180-195: gaugefix!(:R)); recalculate!."""
+    alg = alg or TDVP()
+    N = len(psi)
+    ALs = []
+    for loc in range(N):
+        GL, GR = envs.leftenv(loc), envs.rightenv(loc)
+        tAC, _ = lanczos_exponentiate(lambda x: hac_apply(GL, Ws[loc], GR, x), dt, psi.AC[loc], alg.tol,
+                                      alg.krylovdim, alg.maxiter)
+        GLc = envs.leftenv(loc + 1)
+        tC, _ = lanczos_exponentiate(lambda x: hc_apply(GLc, GR, x), dt, psi.C[loc], alg.tol,
+                                     alg.krylovdim, alg.maxiter)
+        ALs.append(regauge(tAC, tC))
+    C0 = psi.C[N - 1].copy()
+    AR, C, _ = uniform_rightorth(ALs, C0, alg.gauge_tol, alg.gauge_maxiter)
+    AL = [a.copy() for a in ALs]
+    psi2 = CanonicalIMPS(AL, AR, C, [mul_AL_C(AL[l], C[l]) for l in range(N)])
+    recalculate(envs, psi2, Ws, Defaults.tol)
+    return psi2, envs
+
+
+def time_evolve(psi0, Ws, t_span, alg=None, envs=None, observer=None):
+    """time_evolve(psi0, H, t_span, alg::TDVP, envs) (tdvp.jl:91-112) for real steps; history holds
+    observer(psi, iter, t) or the cell energy at every point of t_span."""
+    alg = alg or TDVP()
+    if envs is None:
+        envs = dmrg_cache(psi0, Ws)
+    psi = psi0.copy()
+    hist = [expectation_energy(psi, Ws, envs) if observer is None else observer(psi, 0, t_span[0])]
+    for it in range(len(t_span) - 1):
+        psi, envs = tdvp_timestep(psi, Ws, envs, t_span[it + 1] - t_span[it], alg)
+        hist.append(expectation_energy(psi, Ws, envs) if observer is None else observer(psi, it + 1, t_span[it + 1]))
+    return psi, envs, hist

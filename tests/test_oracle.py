@@ -403,3 +403,33 @@ def test_idmrg_xxx_energy_matches_vumps(oracle):
     assert max(oracle.mixedcanonical_errors(psi_i)) < 1e-8
     # the loss is the bond-0 centre drift, shrinking over the sweeps
     assert info.losses[-1] < 1e-2 * info.losses[1]
+
+
+def test_lanczos_exponentiate_matches_expm(oracle):
+    # exp(t A) x against scipy's dense expm, with one Krylov space (krylovdim 30) and with the
+    # sub-stepping that a short space forces (krylovdim 8); real t of both signs
+    import scipy.linalg
+    rng = np.random.default_rng(3)
+    n = 120
+    A = rng.standard_normal((n, n))
+    A = 0.5 * (A + A.T)
+    x = rng.standard_normal(n)
+    for t in (-0.05, -1.0, 0.3):
+        ref = scipy.linalg.expm(t * A) @ x
+        for kd in (30, 8):
+            y, info = oracle.lanczos_exponentiate(lambda v: A @ v, t, x, 1e-10, kd, 300)
+            assert info["converged"]
+            assert np.linalg.norm(y - ref) < 1e-10 * np.linalg.norm(ref)
+
+
+def test_tdvp_imaginary_time_xxx(oracle):
+    # test/algorithms/tdvp.jl:28-35: imaginary time (dt = -0.05 up to t = 20) from a random D = 8
+    # state reaches 1/4 - ln 2 within 2e-3; the energy never rises along the way
+    W = oracle.xxz_mpo(1.0, 1.0)
+    psi0 = oracle.randomimps(8, [2, 2], 5)
+    t_span = -np.arange(0, 20.0001, 0.05)
+    psi, envs, hist = oracle.time_evolve(psi0, [W, W], t_span)
+    assert len(hist) == 401
+    assert abs(hist[-1] / 2 - (0.25 - math.log(2))) < 2e-3
+    assert all(hist[i + 1] <= hist[i] + 1e-10 for i in range(len(hist) - 1))
+    assert max(oracle.mixedcanonical_errors(psi)) < 1e-8
