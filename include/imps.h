@@ -247,6 +247,17 @@ int imps_vumps_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* i
 int imps_idmrg_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* info,
                    double* losses, int nloss, imps_iter_cb cb, void* user);
 
+/* timestep(ψ, H, dt, alg::TDVP, envs) (src/algorithms/timeevo/tdvp.jl:33-52) for real dt
+ * (imaginary time, dt = -τ; the library is real-valued, so real-time steps are not available):
+ * every site's AC and C become exp(dt H_AC) AC and exp(dt H_C) C (Lanczos exponential with
+ * tolerance tol, krylovdim, maxiter: KrylovKit.exponentiate as used by tdvp.jl:14-19), AL follows
+ * from regauge! (ortho.jl:214-219), the state is rebuilt as CanonicalIMPS(ALs, C[end]) with
+ * (gauge_tol, gauge_maxiter) and the environments are recalculated.  Builds DMRGCache first when
+ * no environments exist.  A non-converged exponential is reported on stderr, as the reference
+ * warns. */
+int imps_tdvp_timestep(imps_ctx* ctx, double dt, double tol, int krylovdim, int maxiter, double gauge_tol,
+                       int gauge_maxiter);
+
 /* ---- batched sweeps ------------------------------------------------------------------ */
 
 /* One instance of a parameter sweep: its MPO, raw ψ₀ tensors (canonicalised on the device

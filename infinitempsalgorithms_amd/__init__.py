@@ -6,16 +6,16 @@ host-side mirror of the reference's API (``find_groundstate``, ``VUMPS``,
 ``TransferMatrix``, ...).
 """
 from . import _lib
-from .api import (IDMRG, VUMPS, AC_hamiltonian, C_hamiltonian, CanonicalIMPS, Context, DMRGCache,
+from .api import (IDMRG, TDVP, VUMPS, AC_hamiltonian, C_hamiltonian, CanonicalIMPS, Context, DMRGCache,
                   GPUCache, IterativeConvergenceInfo, Lanczos, SparseIMPO, TransferMatrix,
                   calc_galerkin, canonicalimps, energy_gradient, expectationvalue, find_groundstate,
                   find_groundstates_batched, fixedpoint,
                   leftenv, random_tensors, randomimps, recalculate_, regauge_, regularize_,
-                  rightenv)
+                  rightenv, time_evolve, timestep)
 from . import models
 
 __all__ = [
-    "IDMRG", "VUMPS", "AC_hamiltonian", "C_hamiltonian", "CanonicalIMPS", "Context", "DMRGCache", "GPUCache",
+    "IDMRG", "TDVP", "time_evolve", "timestep", "VUMPS", "AC_hamiltonian", "C_hamiltonian", "CanonicalIMPS", "Context", "DMRGCache", "GPUCache",
     "IterativeConvergenceInfo", "Lanczos", "SparseIMPO", "TransferMatrix", "calc_galerkin",
     "canonicalimps", "energy_gradient", "expectationvalue", "find_groundstate", "find_groundstates_batched", "fixedpoint", "leftenv", "random_tensors",
     "randomimps", "recalculate_", "regauge_", "regularize_", "rightenv", "models",

@@ -96,6 +96,7 @@ SIGNATURES = [
                                c_int, ITER_CB, c_void_p]),
     ("imps_idmrg_run", c_int, [c_void_p, ctypes.POINTER(VumpsParams), ctypes.POINTER(VumpsInfo), P_double,
                                c_int, ITER_CB, c_void_p]),
+    ("imps_tdvp_timestep", c_int, [c_void_p, c_double, c_double, c_int, c_int, c_double, c_int]),
     ("imps_vumps_batch", c_int, [c_int, c_int, ctypes.POINTER(BatchInstance), ctypes.POINTER(VumpsParams), c_int,
                                  c_double]),
     ("imps_bench_hac", c_int, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(HacTiming)]),

@@ -499,6 +499,56 @@ def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | IDM
     return psi, envs, res
 
 
+@dataclass
+class TDVP:
+    """TDVP(; integrator, alg_gauge, alg_orth) (algdefs.jl:167-173): Lanczos exponential with the
+    alg_expsolve defaults (utility.jl:343-347) and the alg_gauge defaults (tolgauge, maxiter)."""
+
+    tol: float = 1e-10
+    krylovdim: int = 30
+    maxiter: int = 300
+    gauge_tol: float = 1e-13
+    gauge_maxiter: int = 300
+
+
+def timestep(psi: CanonicalIMPS, H: SparseIMPO, dt, alg: TDVP | None = None, envs: GPUCache | None = None):
+    """timestep(ψ, H, dt, alg::TDVP, envs) (tdvp.jl:33-52) for real dt (imaginary time,
+    dt = -τ): returns (ψ′, envs)."""
+    alg = alg or TDVP()
+    if isinstance(dt, complex) or np.iscomplexobj(dt):
+        raise ValueError("timestep: the library is real-valued; real-time steps (complex dt) are not supported")
+    if envs is None:
+        envs = DMRGCache(psi, H)
+    elif psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ctx.call("imps_envs_reset")
+    envs.ctx.call("imps_tdvp_timestep", float(dt), float(alg.tol), int(alg.krylovdim), int(alg.maxiter),
+                  float(alg.gauge_tol), int(alg.gauge_maxiter))
+    psi2 = envs.ctx.get_state()
+    envs.ket = psi2
+    return psi2, envs
+
+
+def time_evolve(psi0: CanonicalIMPS, H: SparseIMPO, t_span, alg: TDVP | None = None, envs: GPUCache | None = None,
+                observer=None):
+    """time_evolve(ψ₀, H, t_span, alg::TDVP, envs; observer) (tdvp.jl:91-112) for real steps
+    dt = t_span[i+1] - t_span[i]: returns (ψ, envs, history) with observer(ψ, iter, t) (default:
+    the cell energy) at every point of t_span."""
+    alg = alg or TDVP()
+    if envs is None:
+        envs = DMRGCache(psi0, H)
+    psi = psi0
+
+    def obs(p, it, t):
+        return expectationvalue(p, H, envs) if observer is None else observer(p, it, t)
+
+    history = [obs(psi, 0, t_span[0])]
+    for it in range(len(t_span) - 1):
+        psi, envs = timestep(psi, H, t_span[it + 1] - t_span[it], alg, envs)
+        history.append(obs(psi, it + 1, t_span[it + 1]))
+    return psi, envs, history
+
+
 def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, device=0, gauge_tol=1e-12):
     """Batched parameter sweep: find_groundstate (vumps.jl:51-86) for every
     (H::SparseIMPO, raw tensors As) in `problems`, the instances running concurrently on

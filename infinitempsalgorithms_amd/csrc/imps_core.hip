@@ -794,6 +794,96 @@ double lanczos(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xo
   return theta[0];
 }
 
+// exp(t op) x0 for a symmetric operator and real t: KrylovKit.exponentiate with Lanczos, the
+// integrator of TDVP (tdvp.jl:14-19).  A Lanczos basis (two-pass Gram-Schmidt on the device) grows
+// from the current vector; with T = U diag(theta) U^T on the host, y(tau) = U exp(tau theta) U^T e1
+// is accepted for the whole remaining time when |beta_K y_K(tau)| <= tol (or the space is
+// invariant); when the basis reaches krylovdim first, the largest tau = remaining / 2^j with an
+// accepted estimate advances the vector to V y(tau) and the basis restarts for the rest.  The
+// running norm is kept apart from the unit vector, so exp(t op) may grow or shrink the result.
+KInfo lanczos_expm(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xout, double t, double tol,
+                   int kdim, int maxiter) {
+  const int kmax = (int)std::min<size_t>(kdim, n);
+  require(kmax >= 1 && kmax < kMaxVec, IMPS_E_ARG, "exponentiate: krylovdim out of range");
+  double* V = ws(c, "lx_V", (size_t)(kmax + 1) * n);
+  double* tmp = ws(c, "lx_tmp", n);
+  auto Vc = [&](int k) { return V + (size_t)k * n; };
+  const int ldt = kmax + 1;
+  std::vector<double> T((size_t)ldt * ldt), h(kmax + 1);
+  double scale = nrm2(c, n, x0);
+  require(scale > 0 && std::isfinite(scale), IMPS_E_ARG, "exponentiate: zero or non-finite start vector");
+  axpby(c, n, 1.0 / scale, x0, 0.0, Vc(0));
+  double remaining = t;
+  KInfo info;
+  bool converged = true;
+  int steps = 0;
+  while (true) {
+    std::fill(T.begin(), T.end(), 0.0);
+    int K = 0;
+    double beta = 0.0, tau = remaining;
+    std::vector<double> y;
+    bool accepted = false;
+    while (true) {
+      op(Vc(K), Vc(K + 1));
+      info.numops++;
+      cgs2_enqueue(c, n, K + 1, V, Vc(K + 1));
+      cgs2_wait(c, K + 1, h.data(), &beta);
+      for (int i = 0; i <= K; ++i) {
+        T[i + (size_t)ldt * K] = h[i];
+        T[K + (size_t)ldt * i] = h[i];
+      }
+      K += 1;
+      std::vector<double> A((size_t)K * K), theta(K), U((size_t)K * K);
+      for (int j = 0; j < K; ++j)
+        for (int i = 0; i < K; ++i) A[i + (size_t)K * j] = 0.5 * (T[i + (size_t)ldt * j] + T[j + (size_t)ldt * i]);
+      sym_eig(K, A.data(), theta.data(), U.data());
+      auto y_of = [&](double tt) {
+        std::vector<double> r(K, 0.0);
+        for (int j = 0; j < K; ++j) {
+          const double f = std::exp(tt * theta[j]) * U[(size_t)K * j];
+          for (int i = 0; i < K; ++i) r[i] += U[i + (size_t)K * j] * f;
+        }
+        return r;
+      };
+      y = y_of(remaining);
+      if (std::fabs(beta * y[K - 1]) <= tol || beta == 0.0) {
+        accepted = true;
+        tau = remaining;
+        break;
+      }
+      if (K == kmax) {
+        double tt = remaining;
+        for (int j = 0; j < 60; ++j) {
+          tt *= 0.5;
+          std::vector<double> yt = y_of(tt);
+          if (std::fabs(beta * yt[K - 1]) <= tol) {
+            accepted = true;
+            tau = tt;
+            y = yt;
+            break;
+          }
+        }
+        break;
+      }
+    }
+    steps++;
+    if (!accepted) converged = false;
+    multiaxpy(c, n, K, V, n, y.data(), 0.0, tmp);
+    const double ny = nrm2(c, n, tmp);
+    scale *= ny;
+    axpby(c, n, 1.0 / ny, tmp, 0.0, Vc(0));
+    remaining -= tau;
+    if (remaining == 0.0 || !converged || steps >= maxiter) {
+      converged = converged && remaining == 0.0;
+      break;
+    }
+  }
+  axpby(c, n, scale, Vc(0), 0.0, xout);
+  info.converged = converged ? 1 : 0;
+  info.numiter = steps;
+  return info;
+}
+
 // Dominant eigenpair (which = :LM) of a real operator by Arnoldi (KrylovKit.schursolve
 // via gauge_fixedpoint, utility.jl:204-211; eigsolve(..., :LM) via dot, canonicalmps.jl:
 // 262-269); restarts from the dominant Ritz vector.
@@ -2318,6 +2408,33 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       info->hac_applies = (int)(c->hac_applies - hac0);
       info->prec_switch_iter = -1;
     }
+  });
+}
+
+int imps_tdvp_timestep(imps_ctx* c, double dt, double tol, int krylovdim, int maxiter, double gauge_tol,
+                       int gauge_maxiter) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    require(std::isfinite(dt), IMPS_E_ARG, "imps_tdvp_timestep: dt must be finite");
+    const int N = c->N, D = c->D, d = c->d;
+    // envs = DMRGCache(psi, H) when none exist (tdvp.jl:34)
+    if (!c->envs_valid) envs_recalculate(c, 1.0e-10, 300, 30, nullptr);
+    const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
+    bool ok = true;
+    for (int loc = 0; loc < N; ++loc) {
+      Op hac = [&](const double* x, double* y) { hac_apply(c, loc, x, y, PREC_FP64); };
+      Op hc = [&](const double* x, double* y) { hc_apply(c, loc, x, y, PREC_FP64); };
+      ok = lanczos_expm(c, hac, nA, c->AC[loc], c->pAC[loc], dt, tol, krylovdim, maxiter).converged && ok;
+      ok = lanczos_expm(c, hc, nC, c->C[loc], c->pC[loc], dt, tol, krylovdim, maxiter).converged && ok;
+    }
+    if (!ok) fprintf(stderr, "imps: TDVP integrate: exponentiate not converged (dt = %g)\n", dt);
+    // ALs = regauge!(ACs, Cs) (ortho.jl:214-219); CanonicalIMPS(ALs, C[end]) = gaugefix!(:R)
+    // (canonicalmps.jl:180-195); recalculate!(envs, psi', H)
+    for (int loc = 0; loc < N; ++loc) regauge_site(c, loc);
+    gauge_step(c, gauge_tol, gauge_maxiter, nullptr);
+    envs_recalculate(c, 1.0e-10, 300, 30, nullptr);
+    HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
 

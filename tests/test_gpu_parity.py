@@ -562,6 +562,29 @@ def test_idmrg_matches_oracle(imps, ctx, oracle, model, D, N, sweeps):
         assert abs(info.energy / N - (0.25 - math.log(2))) < 2e-4
 
 
+def test_tdvp_imaginary_time_matches_oracle(imps, ctx, oracle):
+    # TDVP time steps (tdvp.jl:33-52) with dt = -0.05 through imps_tdvp_timestep: the first ten
+    # steps' energies against the oracle's restatement from the same psi0, then the reference
+    # test's protocol (test/algorithms/tdvp.jl:28-35: t = 20, within 2e-3 of 1/4 - ln 2)
+    W = oracle.xxz_mpo(1.0, 1.0)
+    H = imps.SparseIMPO([W, W])
+    psi0 = oracle.randomimps(8, [2, 2], 5)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+    envs = imps.DMRGCache(st, H, ctx=ctx)
+    t10 = -np.arange(0, 0.50001, 0.05)
+    psi, envs, hist = imps.time_evolve(st, H, t10, imps.TDVP(), envs)
+    _, _, hist_o = oracle.time_evolve(psi0, [W, W], t10)
+    assert len(hist) == len(hist_o) == 11
+    assert np.allclose(hist, hist_o, rtol=0, atol=1e-10)
+    t_span = -np.arange(0, 20.0001, 0.05)
+    psi, envs, hist = imps.time_evolve(psi, H, t_span[10:], imps.TDVP(), envs)
+    assert abs(hist[-1] / 2 - (0.25 - math.log(2))) < 2e-3
+    assert all(hist[i + 1] <= hist[i] + 1e-10 for i in range(len(hist) - 1))
+    assert max(oracle.mixedcanonical_errors(oracle.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC))) < 1e-8
+    with pytest.raises(ValueError):
+        imps.timestep(psi, H, -0.01j, imps.TDVP(), envs)
+
+
 def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     # reverse H_AC sandwich against the oracle on a random state
     Wx = oracle.xxz_mpo(1.0, 0.8)
