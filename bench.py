@@ -16,6 +16,7 @@ Also reported:
                 D = 1024 from a random canonical state (disable with --no-vumps);
   time_to_galerkin  with --ttg: the full run to Galerkin < 1e-8 (long);
   idmrg         with --idmrg K: seconds per single-site IDMRG sweep over K sweeps;
+  tdvp          with --tdvp K: seconds per imaginary-time TDVP step over K steps;
   cpu_baseline  the FP64 CPU restatement of H_AC (numpy / OpenBLAS) on a bounded sample.
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -84,6 +85,7 @@ def main():
     ap.add_argument("--ttg-seconds", type=float, default=1e9, help="stop the --ttg run after this wall time")
     ap.add_argument("--refine-below", type=float, default=1e-6, help="mixed precision: FP64 below this Galerkin error")
     ap.add_argument("--idmrg", type=int, default=0, help="also time this many single-site IDMRG sweeps")
+    ap.add_argument("--tdvp", type=int, default=0, help="also time this many imaginary-time TDVP steps (dt = -0.05)")
     ap.add_argument("--eig-floor", type=float, default=0.0,
                     help="mixed precision: eigensolver tolerance floor (0: library default)")
     ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
@@ -142,6 +144,8 @@ def main():
 
     if args.idmrg > 0:
         out["idmrg"] = idmrg_sweeps(args, imps, H, D, N)
+    if args.tdvp > 0:
+        out["tdvp"] = tdvp_steps(args, imps, H, D, N)
 
     if not args.no_cpu and not args.vumps_only:
         log("bench: CPU baseline (numpy restatement)")
@@ -246,6 +250,25 @@ def idmrg_sweeps(args, imps, H, D, N):
     return {"sweeps": info.niter, "seconds_per_sweep": loop / max(len(marks), 1), "loop_seconds": loop,
             "total_seconds": info.seconds, "drift": info.itererr, "energy_per_site": info.energy / N,
             "hac_applies": info.hac_applies}
+
+
+def tdvp_steps(args, imps, H, D, N, dt=-0.05):
+    """Imaginary-time TDVP steps (tdvp.jl:33-52) from the random canonical state: seconds per
+    step (one Lanczos exponential per site for AC and for C, regauge!, gauge fix, recalculate!)."""
+    ctx = imps.Context(0)
+    psi, ctx = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=100)
+    envs = imps.DMRGCache(psi, H, ctx=ctx)
+    e0 = imps.expectationvalue(psi, H, envs) / N
+    times = []
+    for k in range(args.tdvp):
+        t0 = time.perf_counter()
+        psi, envs = imps.timestep(psi, H, dt, imps.TDVP(), envs)
+        times.append(time.perf_counter() - t0)
+        log(f"bench: TDVP step {k + 1} dt {dt} t {times[-1]:.2f}s")
+    e1 = imps.expectationvalue(psi, H, envs) / N
+    ctx.close()
+    return {"steps": args.tdvp, "dt": dt, "seconds_per_step": sum(times) / len(times),
+            "energy_per_site_before": e0, "energy_per_site_after": e1}
 
 
 def tfim_exact_energy(h, J=1.0, n=200000):
