@@ -1294,3 +1294,42 @@ def time_evolve(psi0, Ws, t_span, alg=None, envs=None, observer=None):
         psi, envs = tdvp_timestep(psi, Ws, envs, t_span[it + 1] - t_span[it], alg)
         hist.append(expectation_energy(psi, Ws, envs) if observer is None else observer(psi, it + 1, t_span[it + 1]))
     return psi, envs, hist
+
+
+# ---------------------------------------------------------------------------
+# Observables: correlator, fidelity (src/algorithms/correlators.jl, src/states/canonicalmps.jl)
+# ---------------------------------------------------------------------------
+
+
+def correlator(psi, O1, O2, i, js):
+    """<psi| O1[i] O2[j] |psi> / dot(psi, psi) for increasing j > i (correlators.jl:13-42):
+    V[b, c] = AC[i][a,u,b] O1[u,s] AC[i][a,s,c], pushed through AR of the sites between, closed
+    with AR[j][b,u,e] O2[u,s] AR[j][c,s,e].  Every value is the reference's single-j form
+    correlator(psi, O1, O2, i, j); for a range the sites already evaluated are pushed through
+    before the next j."""
+    N = len(psi)
+    js = list(js)
+    if not js or js[0] <= i or any(b <= a for a, b in zip(js, js[1:])):
+        raise ValueError("correlator: need i < j1 < j2 < ...")
+    AC = psi.AC[i % N]
+    V = np.einsum("aub,us,asc->bc", AC, O1, AC, optimize=True)
+    ctr = i
+    out = []
+    for j in js:
+        for l in range(ctr + 1, j):
+            V = push_env_left2(V, psi.AR[l % N], psi.AR[l % N])
+        AR = psi.AR[j % N]
+        out.append(float(np.einsum("bc,bue,us,cse->", V, AR, O2, AR, optimize=True)))
+        V = push_env_left2(V, AR, AR)
+        ctr = j
+    return np.array(out) / dot_psi(psi, psi)
+
+
+def norm_psi(psi):
+    """LinearAlgebra.norm(psi) = norm(psi.AC[1]) (canonicalmps.jl:238)."""
+    return float(np.linalg.norm(psi.AC[0]))
+
+
+def fidelity(psi1, psi2, krylovdim=Defaults.krylovdim):
+    """fidelity(psi1, psi2) = |dot(psi1, psi2)| / (norm(psi1) norm(psi2)) (canonicalmps.jl:281-282)."""
+    return abs(dot_psi(psi1, psi2, krylovdim)) / (norm_psi(psi1) * norm_psi(psi2))

@@ -433,3 +433,30 @@ def test_tdvp_imaginary_time_xxx(oracle):
     assert abs(hist[-1] / 2 - (0.25 - math.log(2))) < 2e-3
     assert all(hist[i + 1] <= hist[i] + 1e-10 for i in range(len(hist) - 1))
     assert max(oracle.mixedcanonical_errors(psi)) < 1e-8
+
+
+def test_correlator_and_fidelity(oracle):
+    # test/algorithms/observables.jl:6-13: |up up> gives <Z1 Zj> = 1 and <X1 Xj> = 0; Neel <Z1 Z2> = -1
+    Z = oracle.sigma_z()
+    X = oracle.sigma_x()
+    up = oracle.prodimps([2, 2], [0, 0])
+    assert np.allclose(oracle.correlator(up, Z, Z, 0, range(1, 5)), 1.0, atol=1e-12)
+    assert np.allclose(oracle.correlator(up, X, X, 0, range(1, 4)), 0.0, atol=1e-12)
+    neel = oracle.prodimps([2, 2], [0, 1])
+    assert abs(oracle.correlator(neel, Z, Z, 0, [1])[0] + 1.0) < 1e-12
+    # entangled state: the nearest-neighbour value is the two-site expectation value, a range
+    # equals its single-j values, and at long distance the correlator factorises
+    psi = oracle.randomimps(6, [2, 2], 8)
+    c = oracle.correlator(psi, Z, X, 0, range(1, 7))
+    # expval.jl:49 reshapes O12 column-major, so kron(A, B) puts B on site i and A on site i+1
+    assert abs(c[0] - oracle.local_expectation2(psi, 0, np.kron(X, Z))) < 1e-12
+    for k, j in enumerate(range(1, 7)):
+        assert abs(c[k] - oracle.correlator(psi, Z, X, 0, [j])[0]) < 1e-12
+    far = oracle.correlator(psi, Z, X, 0, [301])[0]
+    fact = oracle.local_expectation1(psi, 0, Z) * oracle.local_expectation1(psi, 1, X)
+    assert abs(far - fact) < 1e-8
+    # fidelity: 1 with itself and under rescaling, < 1 between different states, symmetric
+    assert abs(oracle.fidelity(psi, psi) - 1.0) < 1e-10
+    psi2 = oracle.randomimps(6, [2, 2], 9)
+    f12, f21 = oracle.fidelity(psi, psi2), oracle.fidelity(psi2, psi)
+    assert 0.0 <= f12 < 0.999 and abs(f12 - f21) < 1e-10
