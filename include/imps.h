@@ -258,6 +258,17 @@ int imps_idmrg_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* i
 int imps_tdvp_timestep(imps_ctx* ctx, double dt, double tol, int krylovdim, int maxiter, double gauge_tol,
                        int gauge_maxiter);
 
+/* correlator(ψ, O1, O2, i, js) (src/algorithms/correlators.jl:13-42) on the context's state:
+ * out[k] = ⟨ψ|O1[i] O2[js[k]]|ψ⟩ / dot(ψ, ψ) for i < js[0] < js[1] < ..., O1 and O2 d x d
+ * column-major on the host.  Every value equals the reference's single-j form; for a range the
+ * sites already evaluated are pushed through before the next j. */
+int imps_correlator(imps_ctx* ctx, const double* O1, const double* O2, int i, int nj, const int* js,
+                    double* out);
+/* dot(ψ₁, ψ₂) (src/states/canonicalmps.jl:262-269) with ψ₁ the context's state and ψ₂ given by
+ * its AL tensors (host, same unit cell and bond dimension): the dominant eigenvalue of
+ * TransferMatrix(ψ₂.AL, ψ₁.AL) from the identity (Arnoldi, tolerance 1e-10). */
+int imps_dot_state(imps_ctx* ctx, int nsites, const double* const* AL2, double* out);
+
 /* ---- batched sweeps ------------------------------------------------------------------ */
 
 /* One instance of a parameter sweep: its MPO, raw ψ₀ tensors (canonicalised on the device

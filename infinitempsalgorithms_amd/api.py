@@ -361,6 +361,46 @@ def expectationvalue(psi: CanonicalIMPS, op, envs: GPUCache):
     return val.value
 
 
+def correlator(psi: CanonicalIMPS, O1, O2, i: int, js, envs: GPUCache):
+    """correlator(ψ, O1, O2, i, j | js) (correlators.jl:13-42): ⟨ψ|O1[i] O2[j]|ψ⟩ / dot(ψ, ψ)
+    for j > i (0-based sites), computed on the device.  A scalar for an int j, an array for a
+    range of increasing js; each value equals the single-j form."""
+    if psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ket = psi
+    scalar = isinstance(js, (int, np.integer))
+    jl = [int(js)] if scalar else [int(j) for j in js]
+    O1, O2 = fortran(O1), fortran(O2)
+    jarr = np.array(jl, dtype=np.int32)
+    out = np.zeros(len(jl))
+    envs.ctx.call("imps_correlator", dptr(O1), dptr(O2), int(i), len(jl),
+                  jarr.ctypes.data_as(ctypes.c_void_p), dptr(out))
+    return float(out[0]) if scalar else out
+
+
+def dot(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
+    """dot(ψ₁, ψ₂) (canonicalmps.jl:262-269): dominant eigenvalue of the mixed AL transfer
+    matrix, on the device (ψ₁ becomes the context's state)."""
+    if psi1 is not envs.ket:
+        envs.ctx.set_state(psi1)
+        envs.ket = psi1
+    AL2 = [fortran(a) for a in psi2.AL]
+    pp, keep = ptr_array(AL2)
+    val = ctypes.c_double()
+    envs.ctx.call("imps_dot_state", len(AL2), pp, ctypes.byref(val))
+    return val.value
+
+
+def norm(psi: CanonicalIMPS):
+    """norm(ψ) = norm(ψ.AC[1]) (canonicalmps.jl:238)."""
+    return float(np.linalg.norm(psi.AC[0]))
+
+
+def fidelity(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
+    """fidelity(ψ₁, ψ₂) = |dot(ψ₁, ψ₂)| / (norm(ψ₁) norm(ψ₂)) (canonicalmps.jl:281-282)."""
+    return abs(dot(psi1, psi2, envs)) / (norm(psi1) * norm(psi2))
+
+
 def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
     """d e_cell / d W[site] at fixed ψ and environments (the reverse of the H_AC
     contraction, effective.jl:90-102, applied to the energy of expval.jl:93-112).  At a

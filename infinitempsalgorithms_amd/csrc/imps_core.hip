@@ -2438,6 +2438,77 @@ int imps_tdvp_timestep(imps_ctx* c, double dt, double tol, int krylovdim, int ma
   });
 }
 
+int imps_correlator(imps_ctx* c, const double* O1, const double* O2, int i, int nj, const int* js, double* out) {
+  if (!c || !O1 || !O2 || nj < 0 || (nj > 0 && (!js || !out))) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    const int N = (int)c->AL.size(), D = c->D, d = c->d;
+    for (int k = 0; k < nj; ++k)
+      require(js[k] > (k == 0 ? i : js[k - 1]), IMPS_E_ARG, "imps_correlator: need i < j1 < j2 < ...");
+    const size_t DD = (size_t)D * D;
+    double* V = ws(c, "co_V", DD);
+    double* V2 = ws(c, "co_V2", DD);
+    double* Z = ws(c, "co_Z", (size_t)D * d * D);
+    double* E = ws(c, "co_E", DD);
+    double* I = ws(c, "co_I", DD);
+    HIPCHK(launch_set_identity(D, I, D, c->stream));
+    // V[b, c] = AC[a,u,b] O1[u,s] AC[a,s,c] at site i (correlators.jl:28-30)
+    const double* AC = c->AC[wrap(c, i)];
+    HIPCHK(launch_mix_mid(D, d, D, O1, AC, Z, 0.0, c->stream));
+    gemm2(c, 'T', 'N', D, D, d * D, 1.0, AC, (long)d * D, Z, (long)d * D, 0.0, V, D);
+    std::vector<double> O2v(O2, O2 + (size_t)d * d);
+    auto push = [&](int l) {
+      const double* AR = c->AR[((l % N) + N) % N];
+      push_left2(c, V, D, AR, AR, V2, D);
+      std::swap(V, V2);
+    };
+    int ctr = i;
+    std::vector<double> vals(nj);
+    for (int k = 0; k < nj; ++k) {
+      const int j = js[k];
+      for (int l = ctr + 1; l < j; ++l) push(l);
+      // E = V[b, c] AR[b,u,e] O2[u,s] AR[c,s,e] = trace of the slice push of V with O2 (31-38)
+      slice_sum_left(c, {SliceTerm{V, (long)D, O2v}}, c->AR[((j % N) + N) % N], E, D);
+      vals[k] = trace_prod(c, D, I, E);
+      push(j);
+      ctr = j;
+    }
+    const double nrm = dot_psi(c);
+    for (int k = 0; k < nj; ++k) out[k] = vals[k] / nrm;
+  });
+}
+
+int imps_dot_state(imps_ctx* c, int nsites, const double* const* AL2, double* out) {
+  if (!c || !AL2 || !out) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    const int N = (int)c->AL.size(), D = c->D, d = c->d;
+    require(nsites == N, IMPS_E_ARG, "imps_dot_state: unit cells differ");
+    const size_t nA = (size_t)D * d * D, DD = (size_t)D * D;
+    std::vector<double*> A2(N);
+    for (int l = 0; l < N; ++l) {
+      A2[l] = ws(c, "ds_A2_" + std::to_string(l), nA);
+      h2d(c, A2[l], AL2[l], nA);
+    }
+    // dot(psi1, psi2) = dominant eigenvalue of TransferMatrix(psi2.AL, psi1.AL) from v0 = I
+    // (canonicalmps.jl:262-269); psi1 is the context's state
+    double* v0 = ws(c, "ds_v0", DD);
+    double* ev = ws(c, "ds_ev", DD);
+    HIPCHK(launch_set_identity(D, v0, D, c->stream));
+    Op tm = [&](const double* v, double* o) {
+      double* t0 = ws(c, "ds_t0", DD);
+      double* t1 = ws(c, "ds_t1", DD);
+      const double* cur = v;
+      for (int l = 0; l < N; ++l) {
+        double* dst = (l == N - 1) ? o : (l % 2 == 0 ? t0 : t1);
+        push_left2(c, cur, D, A2[l], c->AL[l], dst, D);
+        cur = dst;
+      }
+    };
+    *out = arnoldi_dominant(c, tm, DD, v0, ev, 1.0e-10, 30, 300, true, nullptr);
+  });
+}
+
 int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imps_vumps_params* p, int nworkers,
                      double gauge_tol) {
   if (ninst < 0 || (ninst > 0 && !inst) || !p || nworkers < 1) return IMPS_E_ARG;

@@ -585,6 +585,32 @@ def test_tdvp_imaginary_time_matches_oracle(imps, ctx, oracle):
         imps.timestep(psi, H, -0.01j, imps.TDVP(), envs)
 
 
+def test_correlator_and_fidelity_match_oracle(imps, ctx, oracle):
+    # correlator (correlators.jl:13-42) and dot/fidelity (canonicalmps.jl:262-282) on the device
+    # against the oracle on a random XXZ-cell state, plus the reference test's product states
+    W = oracle.xxz_mpo(1.0, 1.0)
+    H = imps.SparseIMPO([W, W])
+    Z, X = oracle.sigma_z(), oracle.sigma_x()
+    psi = oracle.randomimps(16, [2, 2], 8)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, H, ctx=ctx)
+    js = list(range(1, 9)) + [20, 57]
+    got = imps.correlator(st, Z, X, 0, js, envs)
+    ref = oracle.correlator(psi, Z, X, 0, js)
+    assert np.max(np.abs(got - ref)) < 1e-12
+    assert abs(imps.correlator(st, X, X, 1, 4, envs) - oracle.correlator(psi, X, X, 1, [4])[0]) < 1e-12
+    psi2 = oracle.randomimps(16, [2, 2], 9)
+    st2 = imps.CanonicalIMPS(psi2.AL, psi2.AR, psi2.C, psi2.AC)
+    assert abs(imps.dot(st, st2, envs) - oracle.dot_psi(psi, psi2)) < 1e-10 * abs(oracle.dot_psi(psi, psi2))
+    assert abs(imps.fidelity(st, st2, envs) - oracle.fidelity(psi, psi2)) < 1e-10
+    assert abs(imps.fidelity(st, st, envs) - 1.0) < 1e-10
+    up = oracle.prodimps([2, 2], [0, 0])
+    stu = imps.CanonicalIMPS(up.AL, up.AR, up.C, up.AC)
+    envs_u = imps.DMRGCache(stu, H, ctx=ctx)
+    assert np.allclose(imps.correlator(stu, Z, Z, 0, range(1, 5), envs_u), 1.0, atol=1e-12)
+    assert np.allclose(imps.correlator(stu, X, X, 0, range(1, 4), envs_u), 0.0, atol=1e-12)
+
+
 def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     # reverse H_AC sandwich against the oracle on a random state
     Wx = oracle.xxz_mpo(1.0, 0.8)
