@@ -401,6 +401,11 @@ def fidelity(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
     return abs(dot(psi1, psi2, envs)) / (norm(psi1) * norm(psi2))
 
 
+def infidelity(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
+    """infidelity(ψ₁, ψ₂) = 1 − fidelity(ψ₁, ψ₂) (canonicalmps.jl:283-284)."""
+    return 1.0 - fidelity(psi1, psi2, envs)
+
+
 def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
     """d e_cell / d W[site] at fixed ψ and environments (the reverse of the H_AC
     contraction, effective.jl:90-102, applied to the energy of expval.jl:93-112).  At a

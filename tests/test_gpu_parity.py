@@ -604,6 +604,13 @@ def test_correlator_and_fidelity_match_oracle(imps, ctx, oracle):
     assert abs(imps.dot(st, st2, envs) - oracle.dot_psi(psi, psi2)) < 1e-10 * abs(oracle.dot_psi(psi, psi2))
     assert abs(imps.fidelity(st, st2, envs) - oracle.fidelity(psi, psi2)) < 1e-10
     assert abs(imps.fidelity(st, st, envs) - 1.0) < 1e-10
+    assert abs(imps.infidelity(st, st2, envs) - (1.0 - oracle.fidelity(psi, psi2))) < 1e-10
+    # the reference's phase-invariance check (test/states/mps.jl:57-65) in real arithmetic: site 1's
+    # AL, AR and AC times -1 makes the dominant transfer eigenvalue negative, the fidelity stays 1
+    stp = imps.CanonicalIMPS([-psi.AL[0], psi.AL[1]], [-psi.AR[0], psi.AR[1]], psi.C,
+                             [-psi.AC[0], psi.AC[1]])
+    assert imps.dot(st, stp, envs) < 0
+    assert abs(imps.fidelity(st, stp, envs) - 1.0) < 1e-10
     up = oracle.prodimps([2, 2], [0, 0])
     stu = imps.CanonicalIMPS(up.AL, up.AR, up.C, up.AC)
     envs_u = imps.DMRGCache(stu, H, ctx=ctx)
