@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the split-BF16 H_AC line of an FP64 run")
+    ap.add_argument("--gemm-variant", type=int, default=-1,
+                    help="run FP64/FP32 GEMM tile variant V wherever valid instead of autotuning (-1)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the library runs on a "
@@ -102,6 +104,8 @@ def main():
     import infinitempsalgorithms_amd as imps
     from infinitempsalgorithms_amd import _lib
 
+    if args.gemm_variant >= 0:
+        _lib.check(None, _lib.load().imps_set_gemm_autotune(2 + args.gemm_variant), "imps_set_gemm_autotune")
     D, d, w, N = args.D, 4, 6, 2
     H = imps.models.fermi_hubbard(t=1.0, U=4.0, mu=2.0, nsites=N)
     prec = PREC[args.prec]

@@ -245,7 +245,9 @@ double trace_prod(imps_ctx* c, int n, const double* l, const double* v) {
 // alignment class times every bit-identical tile variant (one warm-up and two timed launches each,
 // into a scratch output with beta = 0) on the caller's stream and keeps the fastest for the rest
 // of the process.  Shapes for which launch_gemm splits K keep its choice.
-std::atomic<bool> g_gemm_autotune{true};
+// Mode: 0 fixed heuristic, 1 autotune, 2 + v tile variant v wherever it is valid (measurement
+// and tests; other shapes autotune).
+std::atomic<int> g_gemm_autotune{1};
 
 typedef std::tuple<int, int, int, int, int, long, long, long, long, long, long, int> TuneKey;
 
@@ -302,11 +304,16 @@ void gemm(imps_ctx* c, const GemmDesc& g0, int prec) {
   GemmDesc g = g0;
   const size_t nw = gemm_workspace(g.M, g.N, g.K);
   if (nw > 0 && g.W == nullptr) g.W = ws(c, "gemm_splitk", nw);
-  if (g_gemm_autotune.load(std::memory_order_relaxed)) {
+  const int mode = g_gemm_autotune.load(std::memory_order_relaxed);
+  if (mode != 0) {
     int ids[4];
     const int nv = gemm_tile_variants(g, prec, ids);
     if (nv > 1) {
-      HIPCHK(launch_gemm_variant(g, prec, tuned_variant(c, g, prec, ids, nv), c->stream));
+      int v = -1;
+      for (int k = 0; k < nv && mode >= 2; ++k)
+        if (ids[k] == mode - 2) v = ids[k];
+      if (v < 0) v = tuned_variant(c, g, prec, ids, nv);
+      HIPCHK(launch_gemm_variant(g, prec, v, c->stream));
       return;
     }
   }
@@ -2630,7 +2637,8 @@ int imps_gemm_dev(imps_ctx* c, char transa, char transb, int m, int n, int k, do
 }
 
 int imps_set_gemm_autotune(int on) {
-  g_gemm_autotune.store(on != 0);
+  if (on < 0 || on > 2 + 3) return IMPS_E_ARG;
+  g_gemm_autotune.store(on);
   return IMPS_OK;
 }
 

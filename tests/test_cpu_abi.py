@@ -113,3 +113,11 @@ def test_random_tensors_match_oracle(oracle):
     b = oracle.random_tensors(6, [2, 2], seed=3)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_gemm_autotune_mode_range(lib):
+    # 0 heuristic, 1 autotune, 2 + v a forced tile variant (v = 0..3); anything else is refused
+    for mode in (-1, 6, 100):
+        assert lib.imps_set_gemm_autotune(mode) != 0
+    for mode in (0, 5, 1):
+        assert lib.imps_set_gemm_autotune(mode) == 0

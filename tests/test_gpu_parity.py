@@ -82,7 +82,8 @@ def test_gemm_autotune_is_bit_identical(imps, ctx, prec, ta, tb, m, n, k):
     B = rng.standard_normal((k, n) if tb == "N" else (n, k))
     C0 = rng.standard_normal((m, n))
     outs = []
-    for on in (1, 1, 0):  # tune, cached choice, heuristic
+    # tune, cached choice, heuristic, then each tile variant forced where valid
+    for on in (1, 1, 0) + tuple(range(2, 6)):
         assert lib.imps_set_gemm_autotune(on) == 0
         dA, dB, dC = DevArray(ctx, A), DevArray(ctx, B), DevArray(ctx, C0)
         ctx.call("imps_gemm_dev", ta.encode(), tb.encode(), m, n, k, 0.7, dA.p, A.shape[0], dB.p, B.shape[0],
@@ -91,7 +92,8 @@ def test_gemm_autotune_is_bit_identical(imps, ctx, prec, ta, tb, m, n, k):
         for d in (dA, dB, dC):
             d.free()
     lib.imps_set_gemm_autotune(1)
-    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[2])
+    for o in outs:
+        assert np.array_equal(o, outs[2])
     ref = 0.7 * (A if ta == "N" else A.T) @ (B if tb == "N" else B.T) - 0.3 * C0
     assert rel(outs[0], ref) < (1e-13 if prec == 0 else 2e-6)
 
