@@ -242,9 +242,9 @@ double trace_prod(imps_ctx* c, int n, const double* l, const double* v) {
 // ------------------------------------------------------------------ GEMM wrappers
 
 // GEMM tile autotuning (process-wide, on by default): the first GEMM of each shape, stride and
-// alignment class times every bit-identical tile variant (one warm-up and two timed launches each,
-// into a scratch output with beta = 0) on the caller's stream and keeps the fastest for the rest
-// of the process.  Shapes for which launch_gemm splits K keep its choice.
+// alignment class times every bit-identical tile variant (one warm-up launch each, then the best of
+// three single launches, into a scratch output with beta = 0) on the caller's stream and keeps the
+// fastest for the rest of the process.  Shapes for which launch_gemm splits K keep its choice.
 // Mode: 0 fixed heuristic, 1 autotune, 2 + v tile variant v wherever it is valid (measurement
 // and tests; other shapes autotune).
 std::atomic<int> g_gemm_autotune{1};
@@ -277,22 +277,31 @@ int tuned_variant(imps_ctx* c, const GemmDesc& g, int prec, const int* ids, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  int best = ids[0];
-  float best_ms = std::numeric_limits<float>::infinity();
+  // one warm-up launch each, then three rounds over the variants in turn, one timed launch per
+  // variant and round; the minimum per variant decides (rounds in turn so that clock changes
+  // during tuning affect every variant alike)
+  float vms[8];
   for (int k = 0; k < nv; ++k) {
+    vms[k] = std::numeric_limits<float>::infinity();
     HIPCHK(launch_gemm_variant(t, prec, ids[k], c->stream));
-    HIPCHK(hipEventRecord(e0, c->stream));
-    HIPCHK(launch_gemm_variant(t, prec, ids[k], c->stream));
-    HIPCHK(launch_gemm_variant(t, prec, ids[k], c->stream));
-    HIPCHK(hipEventRecord(e1, c->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best_ms) {
-      best_ms = ms;
+  }
+  for (int round = 0; round < 3; ++round)
+    for (int k = 0; k < nv; ++k) {
+      HIPCHK(hipEventRecord(e0, c->stream));
+      HIPCHK(launch_gemm_variant(t, prec, ids[k], c->stream));
+      HIPCHK(hipEventRecord(e1, c->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      vms[k] = std::min(vms[k], ms);
+    }
+  int best = ids[0];
+  float best_ms = vms[0];
+  for (int k = 1; k < nv; ++k)
+    if (vms[k] < best_ms) {
+      best_ms = vms[k];
       best = ids[k];
     }
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lk(tt.mu);
