@@ -268,6 +268,16 @@ int imps_correlator(imps_ctx* ctx, const double* O1, const double* O2, int i, in
  * its AL tensors (host, same unit cell and bond dimension): the dominant eigenvalue of
  * TransferMatrix(ψ₂.AL, ψ₁.AL) from the identity (Arnoldi, tolerance 1e-10). */
 int imps_dot_state(imps_ctx* ctx, int nsites, const double* const* AL2, double* out);
+/* dominant_env(ψ; side) and dominant_env(W::DenseIMPO, ψ; side) (src/algorithms/groundstates/
+ * envs.jl:71-99, transfer maps 34-68) on the context's state: the dominant (:LM) eigenpair of the
+ * unit-cell transfer map, from the left over AL (side 0) or from the right over AR (side 1).
+ * W == NULL: identity channel (nw = 0, w = 1).  Otherwise nw DenseIMPO tensors (w, d, w, d)
+ * column-major on the host, W[l % nw] acting at site l.  Arnoldi from ones (eager; tol,
+ * krylovdim, maxiter as in the reference's keyword arguments).  lambda receives the eigenvalue,
+ * env the eigenvector (D, w, D) column-major, Frobenius norm 1, sign fixed by a non-negative
+ * entry sum.  IMPS_E_UNSUPPORTED when the dominant eigenvalue is complex (real arithmetic). */
+int imps_dominant_env(imps_ctx* ctx, int side, int nw, int w, int d, const double* const* W, double tol,
+                      int krylovdim, int maxiter, double* lambda, double* env);
 
 /* ---- batched sweeps ------------------------------------------------------------------ */
 

@@ -406,6 +406,67 @@ def infidelity(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
     return 1.0 - fidelity(psi1, psi2, envs)
 
 
+class DenseIMPO:
+    """DenseIMPO (src/operators/infinitempo.jl:12-30): per-site dense MPO tensors (wl, d, wr, d),
+    periodic over the unit cell; each right bond must match the next site's left bond.  The
+    library's device kernels take square tensors with one w on every site."""
+
+    def __init__(self, Ws):
+        Ws = [fortran(W) for W in Ws]
+        if not Ws:
+            raise ValueError("Ws must not be empty")
+        for l, W in enumerate(Ws):
+            if W.ndim != 4 or W.shape[1] != W.shape[3]:
+                raise ValueError(f"site {l} physical dims: {W.shape}")
+            if W.shape[2] != Ws[(l + 1) % len(Ws)].shape[0]:
+                raise ValueError(f"MPO bond mismatch at site {l}")
+        self.Ws = Ws
+
+    def __len__(self):
+        return len(self.Ws)
+
+    def __getitem__(self, i):
+        return self.Ws[i % len(self.Ws)]
+
+
+def dominant_env(psi: CanonicalIMPS, envs, W=None, side: str = "left", which: str = "LM",
+                 tol: float = 1e-10, krylovdim: int = 30, maxiter: int = 300):
+    """dominant_env(ψ; side, which) / dominant_env(W::DenseIMPO, ψ; ...) (envs.jl:71-99) on the
+    device: (λ, L) with L of shape (D, w, D) (w = 1 for the identity channel), ‖L‖ = 1.  ``envs``
+    is a GPUCache or a Context.  Only ``which = "LM"`` (the reference's default and the only value
+    its callers use); the eigenvector's sign is fixed by a non-negative entry sum."""
+    if which != "LM":
+        raise ValueError("dominant_env: only which='LM' is supported")
+    if side not in ("left", "right"):
+        raise ValueError("side must be 'left' or 'right'")
+    ctx = envs.ctx if hasattr(envs, "ctx") else envs
+    if hasattr(envs, "ket"):
+        if psi is not envs.ket:
+            ctx.set_state(psi)
+            envs.ket = psi
+    else:
+        ctx.set_state(psi)
+    D = psi.D
+    lam = ctypes.c_double()
+    if W is None:
+        out = np.zeros((D, 1, D), order="F")
+        ctx.call("imps_dominant_env", 0 if side == "left" else 1, 0, 1, psi.d, None, float(tol), int(krylovdim),
+                 int(maxiter), ctypes.byref(lam), dptr(out))
+        return lam.value, out
+    Ws = W.Ws if isinstance(W, DenseIMPO) else [fortran(x) for x in W]
+    w, d = Ws[0].shape[0], Ws[0].shape[1]
+    for x in Ws:
+        if x.shape != (w, d, w, d):
+            raise ValueError(f"dominant_env: every tensor must have shape {(w, d, w, d)}")
+    if d != psi.d:
+        raise ValueError("dominant_env: physical dimensions of W and ψ differ")
+    pp, keep = ptr_array(Ws)
+    out = np.zeros((D, w, D), order="F")
+    ctx.call("imps_dominant_env", 0 if side == "left" else 1, len(Ws), w, d, pp, float(tol), int(krylovdim),
+             int(maxiter), ctypes.byref(lam), dptr(out))
+    return lam.value, out
+
+
 def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
     """d e_cell / d W[site] at fixed ψ and environments (the reverse of the H_AC
     contraction, effective.jl:90-102, applied to the energy of expval.jl:93-112).  At a

@@ -602,15 +602,16 @@ void push_right2(imps_ctx* c, const double* X, long ldx, const double* above, co
   gemm2(c, 'N', 'T', D, D, d * D, 1.0, Y, D, below, D, 0.0, out, ldo, prec);
 }
 
-// MPO block W[l,:,i,:] of site as a (d x d) column-major matrix M[u + d*s]
-std::vector<double> wblock(imps_ctx* c, int site, int l, int i) {
-  const int w = c->w, d = c->d;
-  const std::vector<double>& W = c->Wh[site];
+// block W[l,:,i,:] of a (w, d, w, d) tensor as a (d x d) column-major matrix M[u + d*s]
+std::vector<double> wblock_of(const std::vector<double>& W, int w, int d, int l, int i) {
   std::vector<double> M((size_t)d * d);
   for (int s = 0; s < d; ++s)
     for (int u = 0; u < d; ++u) M[u + (size_t)d * s] = W[l + (size_t)w * (u + (size_t)d * (i + (size_t)w * s))];
   return M;
 }
+
+// MPO block W[l,:,i,:] of site
+std::vector<double> wblock(imps_ctx* c, int site, int l, int i) { return wblock_of(c->Wh[site], c->w, c->d, l, i); }
 
 bool is_zero(const std::vector<double>& M) {
   for (double v : M)
@@ -680,28 +681,37 @@ void slice_sum_right(imps_ctx* c, const std::vector<SliceTerm>& terms, const dou
 // MPO-channel pushes (transfermatrix.jl:11-13, 31-35) as sums of slice pushes over the nonzero
 // d x d blocks of W: out[:, v, :] = sum_k push_slice_left(L[:, k, :], W[k, :, v, :], A) and
 // out[:, v, :] = sum_k push_slice_right(R[:, k, :], W[v, :, k, :], A).  out must not alias L / R.
-void push_mpo_left(imps_ctx* c, int site, const double* L, const double* A, double* out) {
-  const int D = c->D, w = c->w;
+// W: a (w, d, w, d) tensor on the host; L, R, out: (D, w, D) on the device.
+void push_w_left(imps_ctx* c, const std::vector<double>& W, int w, const double* L, const double* A, double* out) {
+  const int D = c->D;
   for (int v = 0; v < w; ++v) {
     std::vector<SliceTerm> terms;
     for (int k = 0; k < w; ++k) {
-      std::vector<double> M = wblock(c, site, k, v);
+      std::vector<double> M = wblock_of(W, w, c->d, k, v);
       if (!is_zero(M)) terms.push_back(SliceTerm{L + (size_t)D * k, (long)D * w, M});
     }
     slice_sum_left(c, terms, A, out + (size_t)D * v, (long)D * w);
   }
 }
 
-void push_mpo_right(imps_ctx* c, int site, const double* R, const double* A, double* out) {
-  const int D = c->D, w = c->w;
+void push_w_right(imps_ctx* c, const std::vector<double>& W, int w, const double* R, const double* A, double* out) {
+  const int D = c->D;
   for (int v = 0; v < w; ++v) {
     std::vector<SliceTerm> terms;
     for (int k = 0; k < w; ++k) {
-      std::vector<double> M = wblock(c, site, v, k);
+      std::vector<double> M = wblock_of(W, w, c->d, v, k);
       if (!is_zero(M)) terms.push_back(SliceTerm{R + (size_t)D * k, (long)D * w, M});
     }
     slice_sum_right(c, terms, A, out + (size_t)D * v, (long)D * w);
   }
+}
+
+void push_mpo_left(imps_ctx* c, int site, const double* L, const double* A, double* out) {
+  push_w_left(c, c->Wh[site], c->w, L, A, out);
+}
+
+void push_mpo_right(imps_ctx* c, int site, const double* R, const double* A, double* out) {
+  push_w_right(c, c->Wh[site], c->w, R, A, out);
 }
 
 // ------------------------------------------------------------------ Krylov solvers
@@ -904,7 +914,7 @@ KInfo lanczos_expm(imps_ctx* c, const Op& op, size_t n, const double* x0, double
 // via gauge_fixedpoint, utility.jl:204-211; eigsolve(..., :LM) via dot, canonicalmps.jl:
 // 262-269); restarts from the dominant Ritz vector.
 double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xout,
-                        double tol, int kdim, int maxiter, bool eager, KInfo* info) {
+                        double tol, int kdim, int maxiter, bool eager, KInfo* info, bool* complex_dominant = nullptr) {
   const int kmax = (int)std::min<size_t>(kdim, n);
   require(kmax >= 1 && kmax < kMaxVec, IMPS_E_ARG, "arnoldi: krylovdim out of range");
   double* V = ws(c, "ar_V", (size_t)(kmax + 1) * n);
@@ -938,7 +948,8 @@ double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, d
       H[(K + 1) + (size_t)ldh * K] = beta;
       K++;
       if (K == kmax || beta <= tol || eager) {
-        hess_dominant(K, H.data(), ldh, &lam, u.data());
+        const int hrc = hess_dominant(K, H.data(), ldh, &lam, u.data());
+        if (complex_dominant) *complex_dominant = (hrc == 1);
         res = std::fabs(beta * u[K - 1]);
         if (res <= tol) {
           conv = true;
@@ -2522,6 +2533,59 @@ int imps_dot_state(imps_ctx* c, int nsites, const double* const* AL2, double* ou
       }
     };
     *out = arnoldi_dominant(c, tm, DD, v0, ev, 1.0e-10, 30, 300, true, nullptr);
+  });
+}
+
+int imps_dominant_env(imps_ctx* c, int side, int nw, int w, int d, const double* const* W, double tol,
+                      int krylovdim, int maxiter, double* lambda, double* env) {
+  if (!c || !lambda || !env || (side != 0 && side != 1)) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    const int N = (int)c->AL.size(), D = c->D;
+    const bool ident = (W == nullptr);
+    require(ident ? (w == 1 && nw == 0) : (nw >= 1 && w >= 1 && d == c->d), IMPS_E_ARG,
+            "imps_dominant_env: operator dimensions do not match the state");
+    require(tol > 0 && krylovdim >= 1 && maxiter >= 1, IMPS_E_ARG, "imps_dominant_env: bad solver parameters");
+    std::vector<std::vector<double>> Ws;
+    const size_t nW = (size_t)w * d * w * d;
+    for (int l = 0; l < nw; ++l) {
+      require(W[l] != nullptr, IMPS_E_ARG, "imps_dominant_env: null tensor");
+      Ws.emplace_back(W[l], W[l] + nW);
+    }
+    // dominant_env (envs.jl:83-99): the unit-cell transfer map of envs.jl:34-68, from the left
+    // over AL[0..N-1] or from the right over AR[N-1..0]; Arnoldi (:LM, eager) from ones
+    const size_t n = (size_t)D * w * D;
+    double* v0 = ws(c, "de_v0", n);
+    double* ev = ws(c, "de_ev", n);
+    HIPCHK(launch_fill(n, 1.0, v0, c->stream));
+    Op tm = [&](const double* v, double* o) {
+      double* t0 = ws(c, "de_t0", n);
+      double* t1 = ws(c, "de_t1", n);
+      const double* cur = v;
+      for (int k = 0; k < N; ++k) {
+        const int l = side == 0 ? k : N - 1 - k;
+        double* dst = (k == N - 1) ? o : (k % 2 == 0 ? t0 : t1);
+        if (ident && side == 0)
+          push_left2(c, cur, D, c->AL[l], c->AL[l], dst, D);
+        else if (ident)
+          push_right2(c, cur, D, c->AR[l], c->AR[l], dst, D);
+        else if (side == 0)
+          push_w_left(c, Ws[l % nw], w, cur, c->AL[l], dst);
+        else
+          push_w_right(c, Ws[l % nw], w, cur, c->AR[l], dst);
+        cur = dst;
+      }
+    };
+    bool cplx = false;
+    const double lam = arnoldi_dominant(c, tm, n, v0, ev, tol, krylovdim, maxiter, true, nullptr, &cplx);
+    require(!cplx, IMPS_E_UNSUPPORTED, "imps_dominant_env: the dominant eigenvalue is complex");
+    // L ./= norm(L), sign fixed by sum(L) >= 0
+    const double nr = nrm2(c, n, ev);
+    const double sum = dot(c, n, ev, v0);
+    require(nr > 0 && std::isfinite(nr), IMPS_E_SOLVER, "imps_dominant_env: zero eigenvector");
+    axpby(c, n, (sum < 0 ? -1.0 : 1.0) / nr, ev, 0.0, ev);
+    d2h(c, env, ev, n);
+    *lambda = lam;
   });
 }
 

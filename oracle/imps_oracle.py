@@ -1333,3 +1333,48 @@ def norm_psi(psi):
 def fidelity(psi1, psi2, krylovdim=Defaults.krylovdim):
     """fidelity(psi1, psi2) = |dot(psi1, psi2)| / (norm(psi1) norm(psi2)) (canonicalmps.jl:281-282)."""
     return abs(dot_psi(psi1, psi2, krylovdim)) / (norm_psi(psi1) * norm_psi(psi2))
+
+
+def dominant_env(psi, W=None, side="left", tol=Defaults.tol, krylovdim=Defaults.krylovdim,
+                 maxiter=Defaults.maxiter):
+    """dominant_env(psi; side) and dominant_env(W::DenseIMPO, psi; side) (envs.jl:71-99 with the
+    transfer maps of envs.jl:34-68): the dominant (:LM) eigenpair of the unit-cell transfer map,
+    L -> push_env_left(L, AL[l]) over l = 1..N (identity channel) or
+    L -> push_env_left(L, W[l], AL[l]) (DenseIMPO channel), and from the right with AR over
+    l = N..1; Arnoldi from v0 = ones, eager, as _eigsolve (utility.jl:379-393) calls it.  Returns
+    (lambda, L) with L of shape (D, w, D) (w = 1 for the identity channel), Frobenius norm 1.
+    The reference leaves the eigenvector's sign to KrylovKit; here it is fixed by
+    sum(L) >= 0."""
+    N = len(psi)
+    D = psi.AL[0].shape[0]
+    left = side == "left"
+    if side not in ("left", "right"):
+        raise ValueError("side must be 'left' or 'right'")
+    if W is None:
+        w = 1
+
+        def op(L):
+            X = L.reshape(D, D, order="F")
+            for k in range(N):
+                if left:
+                    X = push_env_left2(X, psi.AL[k], psi.AL[k])
+                else:
+                    X = push_env_right2(X, psi.AR[N - 1 - k], psi.AR[N - 1 - k])
+            return X.reshape(D, 1, D, order="F")
+    else:
+        w = W[0].shape[0]
+
+        def op(L):
+            X = L
+            for k in range(N):
+                if left:
+                    X = push_env_left_mpo(X, W[k % len(W)], psi.AL[k])
+                else:
+                    l = N - 1 - k
+                    X = push_env_right_mpo(X, W[l % len(W)], psi.AR[l])
+            return X
+    lam, x, _ = arnoldi_dominant(op, np.ones((D, w, D)), tol, krylovdim, maxiter, eager=True)
+    x = x / np.linalg.norm(x)
+    if x.sum() < 0:
+        x = -x
+    return lam, x

@@ -620,6 +620,45 @@ def test_correlator_and_fidelity_match_oracle(imps, ctx, oracle):
     assert np.allclose(imps.correlator(stu, X, X, 0, range(1, 4), envs_u), 0.0, atol=1e-12)
 
 
+def test_dominant_env_matches_oracle(imps, ctx, oracle):
+    # dominant_env (envs.jl:71-99) on the device against the oracle: identity channel (closed form
+    # lambda = 1, L = R = I / sqrt(D)) and a DenseIMPO channel with a real dominant eigenvalue (a unit
+    # level plus a random perturbation, w = 3); a channel whose dominant pair is complex is refused
+    W = oracle.xxz_mpo(1.0, 1.0)
+    H = imps.SparseIMPO([W, W])
+    psi = oracle.randomimps(12, [2, 2], 21)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, H, ctx=ctx)
+    for side in ("left", "right"):
+        lam, L = imps.dominant_env(st, envs, side=side)
+        lam_o, L_o = oracle.dominant_env(psi, side=side)
+        assert L.shape == (12, 1, 12)
+        assert abs(lam - 1) < 1e-9 and abs(lam - lam_o) < 1e-9
+        assert np.allclose(L[:, 0, :], np.eye(12) / np.sqrt(12), atol=1e-8)
+        assert np.linalg.norm(L - L_o) < 1e-8
+    rng = np.random.default_rng(8)
+    Ws = []
+    for _ in range(2):
+        Wd = 0.3 * rng.standard_normal((3, 2, 3, 2))
+        Wd[0, :, 0, :] += np.eye(2)
+        Ws.append(np.asfortranarray(Wd))
+    for side in ("left", "right"):
+        lam, L = imps.dominant_env(st, envs, W=imps.DenseIMPO(Ws), side=side)
+        lam_o, L_o = oracle.dominant_env(psi, Ws, side=side)
+        assert L.shape == (12, 3, 12)
+        assert abs(lam - lam_o) < 1e-9 * abs(lam_o)
+        assert abs(np.linalg.norm(L) - 1) < 1e-12
+        assert np.linalg.norm(L - L_o) < 1e-7
+    th = 0.7
+    R = np.array([[np.cos(th), np.sin(th)], [-np.sin(th), np.cos(th)]])
+    Wr = np.asfortranarray(np.einsum("ij,us->iujs", R, np.eye(2)))
+    with pytest.raises(imps._lib.ImpsError, match="complex"):
+        imps.dominant_env(st, envs, W=[Wr, Wr], side="left", maxiter=20)
+    # the context still works after the refused call
+    lam, _ = imps.dominant_env(st, envs, side="left")
+    assert abs(lam - 1) < 1e-9
+
+
 def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     # reverse H_AC sandwich against the oracle on a random state
     Wx = oracle.xxz_mpo(1.0, 0.8)

@@ -460,3 +460,45 @@ def test_correlator_and_fidelity(oracle):
     psi2 = oracle.randomimps(6, [2, 2], 9)
     f12, f21 = oracle.fidelity(psi, psi2), oracle.fidelity(psi2, psi)
     assert 0.0 <= f12 < 0.999 and abs(f12 - f21) < 1e-10
+
+
+def test_dominant_env_closed_forms_and_brute_force(oracle):
+    # identity channel (envs.jl:79-99): AL is left- and AR right-orthonormal, so the fixed points
+    # are lambda = 1 with L = R = I / sqrt(D) (the reference test's |lambda - 1| and norm checks)
+    psi = oracle.randomimps(6, [2, 2], 11)
+    for side in ("left", "right"):
+        lam, L = oracle.dominant_env(psi, side=side)
+        assert L.shape == (6, 1, 6)
+        assert abs(lam - 1) < 1e-10
+        assert np.allclose(L[:, 0, :], np.eye(6) / math.sqrt(6), atol=1e-8)
+    # a one-level DenseIMPO holding the identity is the identity channel
+    lam, L = oracle.dominant_env(psi, [np.eye(2).reshape(1, 2, 1, 2)] * 2, side="left")
+    assert abs(lam - 1) < 1e-10 and np.allclose(L[:, 0, :], np.eye(6) / math.sqrt(6), atol=1e-8)
+    # a one-level operator O on a product state |0 0>: lambda = O[0, 0]^2 per two-site cell
+    up = oracle.prodimps([2, 2], [0, 0])
+    O = np.array([[2.0, 0.3], [0.5, 0.7]])
+    lam, L = oracle.dominant_env(up, [O.reshape(1, 2, 1, 2)] * 2, side="right")
+    assert abs(lam - 4.0) < 1e-12 and abs(L[0, 0, 0] - 1.0) < 1e-12
+    # dense random W (non-symmetric blocks, w = 3): the dominant eigenpair of the explicit
+    # transfer matrix built column by column from the unit-cell map (up to the sign convention)
+    rng = np.random.default_rng(5)
+    psi = oracle.randomimps(4, [2, 2], 12)
+    Ws = [np.asfortranarray(rng.standard_normal((3, 2, 3, 2))) for _ in range(2)]
+    for side in ("left", "right"):
+        lam, L = oracle.dominant_env(psi, Ws, side=side, tol=1e-12)
+        n = 4 * 3 * 4
+        T = np.zeros((n, n))
+        for k in range(n):
+            e = np.zeros(n)
+            e[k] = 1.0
+            X = e.reshape(4, 3, 4, order="F")
+            for j in range(2):
+                X = (oracle.push_env_left_mpo(X, Ws[j], psi.AL[j]) if side == "left"
+                     else oracle.push_env_right_mpo(X, Ws[1 - j], psi.AR[1 - j]))
+            T[:, k] = X.reshape(-1, order="F")
+        vals, vecs = np.linalg.eig(T)
+        i = int(np.argmax(np.abs(vals)))
+        assert abs(vals[i].imag) < 1e-12 and abs(vals[i].real - lam) < 1e-9 * abs(lam)
+        v = vecs[:, i].real / np.linalg.norm(vecs[:, i].real)
+        v = v if v.sum() >= 0 else -v
+        assert np.linalg.norm(v - L.reshape(-1, order="F")) < 1e-7
