@@ -278,6 +278,16 @@ int imps_dot_state(imps_ctx* ctx, int nsites, const double* const* AL2, double* 
  * entry sum.  IMPS_E_UNSUPPORTED when the dominant eigenvalue is complex (real arithmetic). */
 int imps_dominant_env(imps_ctx* ctx, int side, int nw, int w, int d, const double* const* W, double tol,
                       int krylovdim, int maxiter, double* lambda, double* env);
+/* expectationvalue(ψ, W::DenseIMPO, DMRGCache(ψ, W)) (src/observables/expval.jl:69-77 with the
+ * environments of src/algorithms/groundstates/envs.jl:284-304 and normalize_envs! 559-581) on the
+ * context's state: dominant left environment at site 0 and right environment at site N-1
+ * (imps_dominant_env), MPO-channel pushes in between, unit-norm right environments and left
+ * environments scaled so that <C[i], H_C(i) C[i]> = 1; value = sum over the cell of the
+ * single-site sandwiches / dot(ψ, ψ).  W as for imps_dominant_env (nw >= 1).  GL_out / GR_out
+ * (optional, NULL or N host pointers, each NULL or D*w*D doubles) receive the environments,
+ * (bra, w, ket) for GL and (ket, w, bra) for GR, column-major. */
+int imps_dense_expectation(imps_ctx* ctx, int nw, int w, int d, const double* const* W, double tol, int krylovdim,
+                           int maxiter, double* value, double* const* GL_out, double* const* GR_out);
 
 /* ---- batched sweeps ------------------------------------------------------------------ */
 

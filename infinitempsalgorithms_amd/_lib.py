@@ -101,6 +101,8 @@ SIGNATURES = [
     ("imps_dot_state", c_int, [c_void_p, c_int, PP_double, P_double]),
     ("imps_dominant_env", c_int, [c_void_p, c_int, c_int, c_int, c_int, PP_double, c_double, c_int, c_int, P_double,
                                   P_double]),
+    ("imps_dense_expectation", c_int, [c_void_p, c_int, c_int, c_int, PP_double, c_double, c_int, c_int, P_double,
+                                       PP_double, PP_double]),
     ("imps_vumps_batch", c_int, [c_int, c_int, ctypes.POINTER(BatchInstance), ctypes.POINTER(VumpsParams), c_int,
                                  c_double]),
     ("imps_bench_hac", c_int, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(HacTiming)]),

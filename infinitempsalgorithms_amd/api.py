@@ -467,6 +467,37 @@ def dominant_env(psi: CanonicalIMPS, envs, W=None, side: str = "left", which: st
     return lam.value, out
 
 
+def dense_expectation(psi: CanonicalIMPS, W, envs, tol: float = 1e-10, krylovdim: int = 30, maxiter: int = 300,
+                      return_envs: bool = False):
+    """expectationvalue(ψ, W::DenseIMPO, DMRGCache(ψ, W)) (expval.jl:69-77; envs.jl:284-304,
+    559-581) on the device.  ``envs`` is a GPUCache or a Context whose state is ψ's model's.  With
+    ``return_envs`` also the DenseIMPO cache's environments (lefts, rights), each (D, w, D)."""
+    ctx = envs.ctx if hasattr(envs, "ctx") else envs
+    if hasattr(envs, "ket"):
+        if psi is not envs.ket:
+            ctx.set_state(psi)
+            envs.ket = psi
+    else:
+        ctx.set_state(psi)
+    Ws = W.Ws if isinstance(W, DenseIMPO) else [fortran(x) for x in W]
+    w, d = Ws[0].shape[0], Ws[0].shape[1]
+    for x in Ws:
+        if x.shape != (w, d, w, d):
+            raise ValueError(f"dense_expectation: every tensor must have shape {(w, d, w, d)}")
+    if d != psi.d:
+        raise ValueError("dense_expectation: physical dimensions of W and ψ differ")
+    pp, keep = ptr_array(Ws)
+    N, D = len(psi), psi.D
+    lefts = [np.zeros((D, w, D), order="F") for _ in range(N)]
+    rights = [np.zeros((D, w, D), order="F") for _ in range(N)]
+    pl, keep_l = ptr_array(lefts)
+    pr, keep_r = ptr_array(rights)
+    val = ctypes.c_double()
+    ctx.call("imps_dense_expectation", len(Ws), w, d, pp, float(tol), int(krylovdim), int(maxiter),
+             ctypes.byref(val), pl if return_envs else None, pr if return_envs else None)
+    return (val.value, lefts, rights) if return_envs else val.value
+
+
 def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
     """d e_cell / d W[site] at fixed ψ and environments (the reverse of the H_AC
     contraction, effective.jl:90-102, applied to the energy of expval.jl:93-112).  At a

@@ -2536,56 +2536,137 @@ int imps_dot_state(imps_ctx* c, int nsites, const double* const* AL2, double* ou
   });
 }
 
+// dominant_env (envs.jl:83-99) of the context's state: the unit-cell transfer map of envs.jl:34-68,
+// from the left over AL[0..N-1] (side 0) or from the right over AR[N-1..0] (side 1), identity
+// channel when Ws is empty, else DenseIMPO tensors Ws[l % nw] (w, d, w, d); Arnoldi (:LM, eager)
+// from ones.  out (D w D doubles on the device) receives the eigenvector with unit norm and a
+// non-negative entry sum; returns the eigenvalue.
+double dominant_env_dev(imps_ctx* c, int side, const std::vector<std::vector<double>>& Ws, int w, double tol,
+                        int krylovdim, int maxiter, double* out) {
+  const int N = (int)c->AL.size(), D = c->D;
+  const bool ident = Ws.empty();
+  const int nw = (int)Ws.size();
+  const size_t n = (size_t)D * w * D;
+  double* v0 = ws(c, "de_v0", n);
+  HIPCHK(launch_fill(n, 1.0, v0, c->stream));
+  Op tm = [&](const double* v, double* o) {
+    double* t0 = ws(c, "de_t0", n);
+    double* t1 = ws(c, "de_t1", n);
+    const double* cur = v;
+    for (int k = 0; k < N; ++k) {
+      const int l = side == 0 ? k : N - 1 - k;
+      double* dst = (k == N - 1) ? o : (k % 2 == 0 ? t0 : t1);
+      if (ident && side == 0)
+        push_left2(c, cur, D, c->AL[l], c->AL[l], dst, D);
+      else if (ident)
+        push_right2(c, cur, D, c->AR[l], c->AR[l], dst, D);
+      else if (side == 0)
+        push_w_left(c, Ws[l % nw], w, cur, c->AL[l], dst);
+      else
+        push_w_right(c, Ws[l % nw], w, cur, c->AR[l], dst);
+      cur = dst;
+    }
+  };
+  bool cplx = false;
+  const double lam = arnoldi_dominant(c, tm, n, v0, out, tol, krylovdim, maxiter, true, nullptr, &cplx);
+  require(!cplx, IMPS_E_UNSUPPORTED, "dominant_env: the dominant eigenvalue is complex");
+  // L ./= norm(L), sign fixed by sum(L) >= 0
+  const double nr = nrm2(c, n, out);
+  const double sum = dot(c, n, out, v0);
+  require(nr > 0 && std::isfinite(nr), IMPS_E_SOLVER, "dominant_env: zero eigenvector");
+  axpby(c, n, (sum < 0 ? -1.0 : 1.0) / nr, out, 0.0, out);
+  return lam;
+}
+
+// host copies of nw DenseIMPO tensors (w, d, w, d), checked against the context's state
+std::vector<std::vector<double>> dense_tensors(imps_ctx* c, int nw, int w, int d, const double* const* W,
+                                               const char* who) {
+  require(W != nullptr && nw >= 1 && w >= 1 && d == c->d, IMPS_E_ARG,
+          std::string(who) + ": operator dimensions do not match the state");
+  std::vector<std::vector<double>> Ws;
+  const size_t nW = (size_t)w * d * w * d;
+  for (int l = 0; l < nw; ++l) {
+    require(W[l] != nullptr, IMPS_E_ARG, std::string(who) + ": null tensor");
+    Ws.emplace_back(W[l], W[l] + nW);
+  }
+  return Ws;
+}
+
 int imps_dominant_env(imps_ctx* c, int side, int nw, int w, int d, const double* const* W, double tol,
                       int krylovdim, int maxiter, double* lambda, double* env) {
   if (!c || !lambda || !env || (side != 0 && side != 1)) return IMPS_E_ARG;
   return run_guarded(c, [&] {
     need_state(c);
-    const int N = (int)c->AL.size(), D = c->D;
-    const bool ident = (W == nullptr);
-    require(ident ? (w == 1 && nw == 0) : (nw >= 1 && w >= 1 && d == c->d), IMPS_E_ARG,
-            "imps_dominant_env: operator dimensions do not match the state");
     require(tol > 0 && krylovdim >= 1 && maxiter >= 1, IMPS_E_ARG, "imps_dominant_env: bad solver parameters");
     std::vector<std::vector<double>> Ws;
-    const size_t nW = (size_t)w * d * w * d;
-    for (int l = 0; l < nw; ++l) {
-      require(W[l] != nullptr, IMPS_E_ARG, "imps_dominant_env: null tensor");
-      Ws.emplace_back(W[l], W[l] + nW);
-    }
-    // dominant_env (envs.jl:83-99): the unit-cell transfer map of envs.jl:34-68, from the left
-    // over AL[0..N-1] or from the right over AR[N-1..0]; Arnoldi (:LM, eager) from ones
-    const size_t n = (size_t)D * w * D;
-    double* v0 = ws(c, "de_v0", n);
+    if (W == nullptr)
+      require(w == 1 && nw == 0, IMPS_E_ARG, "imps_dominant_env: the identity channel has w = 1, nw = 0");
+    else
+      Ws = dense_tensors(c, nw, w, d, W, "imps_dominant_env");
+    const size_t n = (size_t)c->D * w * c->D;
     double* ev = ws(c, "de_ev", n);
-    HIPCHK(launch_fill(n, 1.0, v0, c->stream));
-    Op tm = [&](const double* v, double* o) {
-      double* t0 = ws(c, "de_t0", n);
-      double* t1 = ws(c, "de_t1", n);
-      const double* cur = v;
-      for (int k = 0; k < N; ++k) {
-        const int l = side == 0 ? k : N - 1 - k;
-        double* dst = (k == N - 1) ? o : (k % 2 == 0 ? t0 : t1);
-        if (ident && side == 0)
-          push_left2(c, cur, D, c->AL[l], c->AL[l], dst, D);
-        else if (ident)
-          push_right2(c, cur, D, c->AR[l], c->AR[l], dst, D);
-        else if (side == 0)
-          push_w_left(c, Ws[l % nw], w, cur, c->AL[l], dst);
-        else
-          push_w_right(c, Ws[l % nw], w, cur, c->AR[l], dst);
-        cur = dst;
-      }
-    };
-    bool cplx = false;
-    const double lam = arnoldi_dominant(c, tm, n, v0, ev, tol, krylovdim, maxiter, true, nullptr, &cplx);
-    require(!cplx, IMPS_E_UNSUPPORTED, "imps_dominant_env: the dominant eigenvalue is complex");
-    // L ./= norm(L), sign fixed by sum(L) >= 0
-    const double nr = nrm2(c, n, ev);
-    const double sum = dot(c, n, ev, v0);
-    require(nr > 0 && std::isfinite(nr), IMPS_E_SOLVER, "imps_dominant_env: zero eigenvector");
-    axpby(c, n, (sum < 0 ? -1.0 : 1.0) / nr, ev, 0.0, ev);
+    *lambda = dominant_env_dev(c, side, Ws, w, tol, krylovdim, maxiter, ev);
     d2h(c, env, ev, n);
-    *lambda = lam;
+  });
+}
+
+int imps_dense_expectation(imps_ctx* c, int nw, int w, int d, const double* const* W, double tol, int krylovdim,
+                           int maxiter, double* value, double* const* GL_out, double* const* GR_out) {
+  if (!c || !value) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    require(tol > 0 && krylovdim >= 1 && maxiter >= 1, IMPS_E_ARG,
+            "imps_dense_expectation: bad solver parameters");
+    const std::vector<std::vector<double>> Ws = dense_tensors(c, nw, w, d, W, "imps_dense_expectation");
+    const int N = (int)c->AL.size(), D = c->D;
+    const size_t n = (size_t)D * w * D, DD = (size_t)D * D;
+    // DMRGCache(psi, W::DenseIMPO) (envs.jl:284-304): dominant environments at the cell's ends,
+    // MPO-channel pushes in between
+    std::vector<double*> GL(N), GR(N);
+    for (int l = 0; l < N; ++l) {
+      GL[l] = ws(c, "dn_GL_" + std::to_string(l), n);
+      GR[l] = ws(c, "dn_GR_" + std::to_string(l), n);
+    }
+    dominant_env_dev(c, 0, Ws, w, tol, krylovdim, maxiter, GL[0]);
+    dominant_env_dev(c, 1, Ws, w, tol, krylovdim, maxiter, GR[N - 1]);
+    for (int l = 1; l < N; ++l) push_w_left(c, Ws[(l - 1) % nw], w, GL[l - 1], c->AL[l - 1], GL[l]);
+    for (int l = N - 2; l >= 0; --l) push_w_right(c, Ws[(l + 1) % nw], w, GR[l + 1], c->AR[l + 1], GR[l]);
+    // normalize_envs! (envs.jl:559-581): unit-norm GR[i], then GL[i+1] /= <C[i], H_C(i) C[i]> with
+    // H_C(i) C = sum_w GL[i+1][:, w, :] C GR[i][:, w, :] (effective.jl:37-50)
+    double* Y = ws(c, "dn_Y", DD);
+    double* M = ws(c, "dn_M", DD);
+    for (int i = 0; i < N; ++i) {
+      const double nr = nrm2(c, n, GR[i]);
+      if (nr > 0) axpby(c, n, 1.0 / nr, GR[i], 0.0, GR[i]);
+      const int j = (i + 1) % N;
+      double lam = 0.0;
+      for (int k = 0; k < w; ++k) {
+        // M[b', b] = sum_{a, a'} C[a, b] GL[a, k, a'] C[a', b'] -> lam += sum M[b', b] GR[b', k, b]
+        // (elementwise: GR's first index is the ket side)
+        gemm2(c, 'N', 'N', D, D, D, 1.0, GL[j] + (size_t)D * k, (long)D * w, c->C[i], D, 0.0, Y, D);
+        gemm2(c, 'T', 'N', D, D, D, 1.0, Y, D, c->C[i], D, 0.0, M, D);
+        HIPCHK(launch_copy2d(D, D, GR[i] + (size_t)D * k, (long)D * w, Y, D, c->stream));
+        lam += dot(c, DD, M, Y);
+      }
+      if (std::fabs(lam) > 0) axpby(c, n, 1.0 / lam, GL[j], 0.0, GL[j]);
+    }
+    // expectationvalue(psi, W::DenseIMPO, envs) (expval.jl:60-77): per site
+    // sum_{w'} sum_{a, b} Z[a, w', b] GR[b, w', a] with Z the MPO-channel left push of GL through AC
+    double* Z = ws(c, "dn_Z", n);
+    double E = 0.0;
+    for (int l = 0; l < N; ++l) {
+      push_w_left(c, Ws[l % nw], w, GL[l], c->AC[l], Z);
+      for (int k = 0; k < w; ++k) {
+        HIPCHK(launch_copy2d(D, D, Z + (size_t)D * k, (long)D * w, Y, D, c->stream));
+        HIPCHK(launch_copy2d(D, D, GR[l] + (size_t)D * k, (long)D * w, M, D, c->stream));
+        E += trace_prod(c, D, Y, M);
+      }
+    }
+    *value = E / dot_psi(c);
+    for (int l = 0; l < N; ++l) {
+      if (GL_out && GL_out[l]) d2h(c, GL_out[l], GL[l], n);
+      if (GR_out && GR_out[l]) d2h(c, GR_out[l], GR[l], n);
+    }
   });
 }
 

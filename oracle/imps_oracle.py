@@ -1378,3 +1378,40 @@ def dominant_env(psi, W=None, side="left", tol=Defaults.tol, krylovdim=Defaults.
     if x.sum() < 0:
         x = -x
     return lam, x
+
+
+def dense_envs(psi, W, tol=Defaults.tol, krylovdim=Defaults.krylovdim, maxiter=Defaults.maxiter):
+    """DMRGCache(psi, W::DenseIMPO) (envs.jl:284-304): lefts[0] and rights[N-1] are the dominant
+    environments (dominant_env, left and right), the others follow by MPO-channel pushes through
+    AL (left) and AR (right); then normalize_envs! (envs.jl:559-581): for i = 0..N-1 the right
+    environment rights[i] gets unit norm and lefts[i+1] is divided by
+    lambda_i = <C[i], H_C(i) C[i]> (H_C(i) with lefts[i+1] and rights[i], effective.jl:37-50)."""
+    N = len(psi)
+    nw = len(W)
+    _, L0 = dominant_env(psi, W, "left", tol, krylovdim, maxiter)
+    _, R0 = dominant_env(psi, W, "right", tol, krylovdim, maxiter)
+    lefts = [None] * N
+    rights = [None] * N
+    lefts[0] = L0
+    for l in range(1, N):
+        lefts[l] = push_env_left_mpo(lefts[l - 1], W[(l - 1) % nw], psi.AL[l - 1])
+    rights[N - 1] = R0
+    for l in range(N - 2, -1, -1):
+        rights[l] = push_env_right_mpo(rights[l + 1], W[(l + 1) % nw], psi.AR[l + 1])
+    for i in range(N):
+        nr = np.linalg.norm(rights[i])
+        if nr > 0:
+            rights[i] = rights[i] / nr
+        j = (i + 1) % N
+        lam = float(np.sum(psi.C[i] * hc_apply(lefts[j], rights[i], psi.C[i])))
+        if abs(lam) > 0:
+            lefts[j] = lefts[j] / lam
+    return lefts, rights
+
+
+def expectation_dense(psi, W, envs=None):
+    """expectationvalue(psi, W::DenseIMPO, envs) (expval.jl:69-77): the sum over the unit cell of
+    contract_mpo_expval(AC[l], lefts[l], W[l], rights[l]), divided by dot(psi, psi)."""
+    lefts, rights = envs if envs is not None else dense_envs(psi, W)
+    E = sum(contract_mpo_expval(psi.AC[l], lefts[l], W[l % len(W)], rights[l]) for l in range(len(psi)))
+    return E / dot_psi(psi, psi)

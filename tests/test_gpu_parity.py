@@ -659,6 +659,36 @@ def test_dominant_env_matches_oracle(imps, ctx, oracle):
     assert abs(lam - 1) < 1e-9
 
 
+def test_dense_expectation_matches_oracle(imps, ctx, oracle):
+    # expectationvalue(psi, W::DenseIMPO) with DMRGCache(psi, W) (expval.jl:69-77; envs.jl:284-304,
+    # 559-581) on the device against the oracle, environments included; for a one-site cell the
+    # value is the dominant eigenvalue of the transfer map (AC = AL C and the left eigen-equation)
+    W = oracle.xxz_mpo(1.0, 1.0)
+    rng = np.random.default_rng(8)
+    Ws = []
+    for _ in range(2):
+        Wd = 0.3 * rng.standard_normal((3, 2, 3, 2))
+        Wd[0, :, 0, :] += np.eye(2)
+        Ws.append(np.asfortranarray(Wd))
+    psi = oracle.randomimps(12, [2, 2], 21)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, imps.SparseIMPO([W, W]), ctx=ctx)
+    E, lefts, rights = imps.dense_expectation(st, imps.DenseIMPO(Ws), envs, return_envs=True)
+    E_o = oracle.expectation_dense(psi, Ws)
+    lefts_o, rights_o = oracle.dense_envs(psi, Ws)
+    assert abs(E - E_o) < 1e-9 * abs(E_o)
+    for l in range(2):
+        assert np.linalg.norm(lefts[l] - lefts_o[l]) < 1e-7 * np.linalg.norm(lefts_o[l])
+        assert np.linalg.norm(rights[l] - rights_o[l]) < 1e-7
+    assert abs(imps.dense_expectation(st, [2.5 * x for x in Ws], envs) - 2.5 * E) < 1e-8 * abs(E)
+    psi1 = oracle.randomimps(8, [2], 31)
+    st1 = imps.CanonicalIMPS(psi1.AL, psi1.AR, psi1.C, psi1.AC)
+    envs1 = imps.DMRGCache(st1, imps.SparseIMPO([W]), ctx=ctx)
+    lam, _ = imps.dominant_env(st1, envs1, W=[Ws[0]])
+    assert abs(imps.dense_expectation(st1, [Ws[0]], envs1) - lam) < 1e-9 * abs(lam)
+    assert abs(lam - oracle.dominant_env(psi1, [Ws[0]], "left")[0]) < 1e-9 * abs(lam)
+
+
 def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     # reverse H_AC sandwich against the oracle on a random state
     Wx = oracle.xxz_mpo(1.0, 0.8)

@@ -502,3 +502,33 @@ def test_dominant_env_closed_forms_and_brute_force(oracle):
         v = vecs[:, i].real / np.linalg.norm(vecs[:, i].real)
         v = v if v.sum() >= 0 else -v
         assert np.linalg.norm(v - L.reshape(-1, order="F")) < 1e-7
+
+
+def test_dense_envs_and_expectation_identities(oracle):
+    # DMRGCache(psi, DenseIMPO) + expectationvalue (envs.jl:284-304, 559-581; expval.jl:69-77).
+    # For a one-site cell, AC = AL C and the left environment's eigen-equation give
+    # contract(AC, L, W, R) = Lambda <C, H_C C>, so the normalised value is the dominant eigenvalue
+    # Lambda of the cell's transfer map; a doubled cell gives 2 Lambda (this fixes the wrap of the
+    # normalisation); the value is linear in a scale of W; the identity MPO gives N.
+    rng = np.random.default_rng(8)
+    Wd = 0.3 * rng.standard_normal((3, 2, 3, 2))
+    Wd[0, :, 0, :] += np.eye(2)
+    Wd = np.asfortranarray(Wd)
+    psi1 = oracle.randomimps(6, [2], 31)
+    lam, _ = oracle.dominant_env(psi1, [Wd], "left")
+    E1 = oracle.expectation_dense(psi1, [Wd])
+    assert abs(E1 - lam) < 1e-9 * abs(lam)
+    psi2 = oracle.canonical_from_tensors([psi1.AL[0], psi1.AL[0]])
+    E2 = oracle.expectation_dense(psi2, [Wd, Wd])
+    assert abs(E2 - 2 * lam) < 1e-8 * abs(lam)
+    psi = oracle.randomimps(6, [2, 2], 32)
+    Ws = [Wd, np.asfortranarray(Wd[:, ::-1, :, :])]
+    E = oracle.expectation_dense(psi, Ws)
+    assert abs(oracle.expectation_dense(psi, [2.5 * x for x in Ws]) - 2.5 * E) < 1e-8 * abs(E)
+    assert abs(oracle.expectation_dense(psi, [np.eye(2).reshape(1, 2, 1, 2)] * 2) - 2.0) < 1e-10
+    lefts, rights = oracle.dense_envs(psi, Ws)
+    for i in range(2):
+        assert abs(np.linalg.norm(rights[i]) - 1) < 1e-12
+        j = (i + 1) % 2
+        lam_i = np.sum(psi.C[i] * oracle.hc_apply(lefts[j], rights[i], psi.C[i]))
+        assert abs(lam_i - 1) < 1e-10
