@@ -43,7 +43,8 @@ enum {
   IMPS_E_STATE = -2,      /* required model/state/environments not set */
   IMPS_E_HIP = -3,        /* HIP runtime failure */
   IMPS_E_SOLVER = -4,     /* rocSOLVER/rocBLAS failure */
-  IMPS_E_UNSUPPORTED = -5 /* configuration outside the supported scope */
+  IMPS_E_UNSUPPORTED = -5, /* configuration outside the supported scope */
+  IMPS_E_IO = -6           /* file missing, unreadable, of another kind, truncated or corrupted */
 };
 
 /* Arithmetic of the bulk contractions (H_AC, H_C, transfer pushes). */
@@ -75,6 +76,8 @@ int imps_set_mpo(imps_ctx* ctx, int nsites, int w, int d, const double* const* W
  * and C of shape (D, D) per site; C[l] is the bond right of site l. */
 int imps_set_state(imps_ctx* ctx, int nsites, int D, const double* const* AL,
                    const double* const* AR, const double* const* C, const double* const* AC);
+/* Unit cell, bond and physical dimension of the context's state (IMPS_E_STATE without one). */
+int imps_state_dims(imps_ctx* ctx, int* nsites, int* D, int* d);
 int imps_get_state(imps_ctx* ctx, double* const* AL, double* const* AR, double* const* C,
                    double* const* AC);
 
@@ -288,6 +291,27 @@ int imps_dominant_env(imps_ctx* ctx, int side, int nw, int w, int d, const doubl
  * (bra, w, ket) for GL and (ket, w, bra) for GR, column-major. */
 int imps_dense_expectation(imps_ctx* ctx, int nw, int w, int d, const double* const* W, double tol, int krylovdim,
                            int maxiter, double* value, double* const* GL_out, double* const* GR_out);
+
+/* State and model files (no counterpart in the reference, which has no file format; for
+ * checkpoint and resume of long runs and for exchanging initial states).  Little-endian: an
+ * 8-byte magic ("IMPSSTA1" state, "IMPSMPO1" MPO), uint32 version 1, uint32 0x01020304, int32
+ * nsites, D (or w), d, int32 element type 1 (float64), the column-major payload (per site AL, AR
+ * (D d D), C (D D), AC (D d D); or W (w d w d)), then a 64-bit FNV-1a hash of the payload.
+ * Writes go to "<path>.tmp", renamed onto path when complete.  Reads check the header, the
+ * requested dimensions (IMPS_E_ARG), the length and the hash (IMPS_E_IO) before writing the
+ * output.  Errors of these context-free calls are read with imps_last_error(NULL). */
+int imps_write_state_file(const char* path, int nsites, int D, int d, const double* const* AL,
+                          const double* const* AR, const double* const* C, const double* const* AC);
+int imps_state_file_info(const char* path, int* nsites, int* D, int* d);
+int imps_read_state_file(const char* path, int nsites, int D, int d, double* const* AL, double* const* AR,
+                         double* const* C, double* const* AC);
+int imps_write_mpo_file(const char* path, int nsites, int w, int d, const double* const* W);
+int imps_mpo_file_info(const char* path, int* nsites, int* w, int* d);
+int imps_read_mpo_file(const char* path, int nsites, int w, int d, double* const* W);
+/* The context's state to a state file / a state file into the context (its model must have the
+ * file's unit cell and physical dimension, as for imps_set_state). */
+int imps_checkpoint_state(imps_ctx* ctx, const char* path);
+int imps_resume_state(imps_ctx* ctx, const char* path);
 
 /* ---- batched sweeps ------------------------------------------------------------------ */
 

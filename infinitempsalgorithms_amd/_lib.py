@@ -14,6 +14,7 @@ import numpy as np
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libimps.so")
 
 IMPS_OK = 0
+IMPS_E_ARG, IMPS_E_STATE, IMPS_E_HIP, IMPS_E_SOLVER, IMPS_E_UNSUPPORTED, IMPS_E_IO = -1, -2, -3, -4, -5, -6
 IMPS_PREC_FP64, IMPS_PREC_FP32, IMPS_PREC_BF16X3 = 0, 1, 2
 IMPS_SIDE_LEFT, IMPS_SIDE_RIGHT = 0, 1
 
@@ -21,6 +22,7 @@ c_int, c_double, c_char, c_void_p, c_size_t = (ctypes.c_int, ctypes.c_double, ct
                                                ctypes.c_void_p, ctypes.c_size_t)
 P_double = ctypes.POINTER(ctypes.c_double)
 PP_double = ctypes.POINTER(P_double)
+P_int = ctypes.POINTER(ctypes.c_int)
 
 
 class KrylovParams(ctypes.Structure):
@@ -70,6 +72,17 @@ SIGNATURES = [
     ("imps_set_mpo", c_int, [c_void_p, c_int, c_int, c_int, PP_double]),
     ("imps_set_state", c_int, [c_void_p, c_int, c_int, PP_double, PP_double, PP_double, PP_double]),
     ("imps_get_state", c_int, [c_void_p, PP_double, PP_double, PP_double, PP_double]),
+    ("imps_state_dims", c_int, [c_void_p, P_int, P_int, P_int]),
+    ("imps_write_state_file", c_int, [ctypes.c_char_p, c_int, c_int, c_int, PP_double, PP_double, PP_double,
+                                      PP_double]),
+    ("imps_state_file_info", c_int, [ctypes.c_char_p, P_int, P_int, P_int]),
+    ("imps_read_state_file", c_int, [ctypes.c_char_p, c_int, c_int, c_int, PP_double, PP_double, PP_double,
+                                     PP_double]),
+    ("imps_write_mpo_file", c_int, [ctypes.c_char_p, c_int, c_int, c_int, PP_double]),
+    ("imps_mpo_file_info", c_int, [ctypes.c_char_p, P_int, P_int, P_int]),
+    ("imps_read_mpo_file", c_int, [ctypes.c_char_p, c_int, c_int, c_int, PP_double]),
+    ("imps_checkpoint_state", c_int, [c_void_p, ctypes.c_char_p]),
+    ("imps_resume_state", c_int, [c_void_p, ctypes.c_char_p]),
     ("imps_set_state_raw", c_int, [c_void_p, c_int, c_int, PP_double, c_double, c_int]),
     ("imps_hac_apply", c_int, [c_void_p, c_int, P_double, P_double, c_int]),
     ("imps_hac_apply_dev", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),

@@ -11,6 +11,7 @@ matrices ``C[a, b]`` (bond right of a site), MPO ``W[wl, u, wr, d]``, environmen
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -64,6 +65,17 @@ class Context:
         pp, keep = ptr_array(As)
         self.call("imps_set_state_raw", len(As), As[0].shape[0], pp, float(tol), int(maxiter))
         self.D = As[0].shape[0]
+
+    def checkpoint(self, path):
+        """Writes the context's state to a state file (imps_checkpoint_state)."""
+        self.call("imps_checkpoint_state", os.fsencode(path))
+
+    def resume(self, path):
+        """Loads a state file into the context (imps_resume_state); the model must match."""
+        self.call("imps_resume_state", os.fsencode(path))
+        n, D, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.call("imps_state_dims", ctypes.byref(n), ctypes.byref(D), ctypes.byref(d))
+        self.D = D.value
 
     def get_state(self):
         N, D, d = self.N, self.D, self.d
@@ -496,6 +508,57 @@ def dense_expectation(psi: CanonicalIMPS, W, envs, tol: float = 1e-10, krylovdim
     ctx.call("imps_dense_expectation", len(Ws), w, d, pp, float(tol), int(krylovdim), int(maxiter),
              ctypes.byref(val), pl if return_envs else None, pr if return_envs else None)
     return (val.value, lefts, rights) if return_envs else val.value
+
+
+def _file_call(name, *args):
+    L = _lib.load()
+    st = getattr(L, name)(*args)
+    if st != _lib.IMPS_OK:
+        msg = L.imps_last_error(None)
+        raise _lib.ImpsError(f"{name} failed with status {st}: {msg.decode() if msg else ''}")
+
+
+def save_state(psi: CanonicalIMPS, path):
+    """Writes ψ (AL, AR, C, AC of every site) to a state file (imps_write_state_file); the
+    library's own format, for checkpoint and resume (the reference has no file format)."""
+    AL, AR, C, AC = ([fortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
+    keep = [ptr_array(x) for x in (AL, AR, C, AC)]
+    _file_call("imps_write_state_file", os.fsencode(path), len(AL), psi.D, psi.d, *(k[0] for k in keep))
+
+
+def load_state(path) -> CanonicalIMPS:
+    """Reads a state file written by save_state or Context.checkpoint."""
+    n, D, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _file_call("imps_state_file_info", os.fsencode(path), ctypes.byref(n), ctypes.byref(D), ctypes.byref(d))
+    N, D, d = n.value, D.value, d.value
+    AL = [np.zeros((D, d, D), order="F") for _ in range(N)]
+    AR = [np.zeros((D, d, D), order="F") for _ in range(N)]
+    AC = [np.zeros((D, d, D), order="F") for _ in range(N)]
+    C = [np.zeros((D, D), order="F") for _ in range(N)]
+    keep = [ptr_array(x) for x in (AL, AR, C, AC)]
+    _file_call("imps_read_state_file", os.fsencode(path), N, D, d, *(k[0] for k in keep))
+    return CanonicalIMPS(AL, AR, C, AC)
+
+
+def save_mpo(H, path):
+    """Writes the per-site MPO tensors (w, d, w, d) of a SparseIMPO or DenseIMPO to an MPO file."""
+    Ws = [fortran(W) for W in H.Ws]
+    w, d = Ws[0].shape[0], Ws[0].shape[1]
+    for W in Ws:
+        if W.shape != (w, d, w, d):
+            raise ValueError(f"save_mpo: every tensor must have shape {(w, d, w, d)}")
+    pp, keep = ptr_array(Ws)
+    _file_call("imps_write_mpo_file", os.fsencode(path), len(Ws), w, d, pp)
+
+
+def load_mpo(path, dense: bool = False):
+    """Reads an MPO file as a SparseIMPO (or a DenseIMPO with dense=True)."""
+    n, w, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _file_call("imps_mpo_file_info", os.fsencode(path), ctypes.byref(n), ctypes.byref(w), ctypes.byref(d))
+    Ws = [np.zeros((w.value, d.value, w.value, d.value), order="F") for _ in range(n.value)]
+    pp, keep = ptr_array(Ws)
+    _file_call("imps_read_mpo_file", os.fsencode(path), n.value, w.value, d.value, pp)
+    return DenseIMPO(Ws) if dense else SparseIMPO(Ws)
 
 
 def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):

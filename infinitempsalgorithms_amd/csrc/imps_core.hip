@@ -102,6 +102,13 @@ struct imps_ctx {
 
 static thread_local std::string g_create_error;
 
+namespace imps {
+// message for imps_last_error(NULL): errors of calls that have no context
+void set_global_error(const std::string& msg) { g_create_error = msg; }
+// message for imps_last_error(c)
+void set_context_error(imps_ctx* c, const std::string& msg) { c->err = msg; }
+}  // namespace imps
+
 namespace {
 
 // ------------------------------------------------------------------ memory helpers
@@ -1945,6 +1952,16 @@ int imps_set_state(imps_ctx* c, int nsites, int D, const double* const* AL, cons
         mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_state_dims(imps_ctx* c, int* nsites, int* D, int* d) {
+  if (!c || !nsites || !D || !d) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    *nsites = (int)c->AL.size();
+    *D = c->D;
+    *d = c->d;
   });
 }
 

@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+typedef struct imps_ctx imps_ctx;
+
 namespace imps {
 
 // Strided GEMM on column-major data:
@@ -140,6 +142,11 @@ hipError_t launch_copy2d(int rows, int cols, const double* src, long lds, double
                          hipStream_t s);
 // deterministic pseudo-random fill in [-1, 1) (for measurement inputs)
 hipError_t launch_fill_random(size_t n, uint64_t seed, double* x, hipStream_t s);
+
+// message returned by imps_last_error(NULL) (calls without a context)
+void set_global_error(const std::string& msg);
+// message returned by imps_last_error(c)
+void set_context_error(imps_ctx* c, const std::string& msg);
 
 // ---- host small dense linear algebra (imps_small.cpp) ----
 // Symmetric eigen-decomposition of the n x n column-major A (destroyed): eigenvalues

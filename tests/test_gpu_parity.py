@@ -689,6 +689,41 @@ def test_dense_expectation_matches_oracle(imps, ctx, oracle):
     assert abs(lam - oracle.dominant_env(psi1, [Ws[0]], "left")[0]) < 1e-9 * abs(lam)
 
 
+def test_checkpoint_and_resume_state(imps, ctx, oracle, tmp_path):
+    # imps_checkpoint_state / imps_resume_state: the device state goes to a file and back into
+    # another context bit for bit, and gives the same energy there
+    W = oracle.xxz_mpo(1.0, 1.0)
+    H = imps.SparseIMPO([W, W])
+    psi = oracle.randomimps(10, [2, 2], 41)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    envs = imps.DMRGCache(st, H, ctx=ctx)
+    e0 = imps.expectationvalue(st, H, envs)
+    path = tmp_path / "ck.imps"
+    ctx.checkpoint(path)
+    c2 = imps.Context(0)
+    try:
+        c2.set_mpo(H)
+        c2.resume(path)
+        back = c2.get_state()
+        for xs, ys in zip((st.AL, st.AR, st.C, st.AC), (back.AL, back.AR, back.C, back.AC)):
+            for x, y in zip(xs, ys):
+                assert np.array_equal(x, y)
+        envs2 = imps.DMRGCache(back, H, ctx=c2)
+        assert abs(imps.expectationvalue(back, H, envs2) - e0) < 1e-12
+        assert np.array_equal(imps.load_state(path).AC[1], st.AC[1])
+    finally:
+        c2.close()
+    c3 = imps.Context(0)
+    try:
+        c3.set_mpo(imps.SparseIMPO([W]))
+        with pytest.raises(imps._lib.ImpsError):
+            c3.resume(path)
+        with pytest.raises(imps._lib.ImpsError, match="cannot open"):
+            c3.resume(tmp_path / "missing.imps")
+    finally:
+        c3.close()
+
+
 def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     # reverse H_AC sandwich against the oracle on a random state
     Wx = oracle.xxz_mpo(1.0, 0.8)
