@@ -48,6 +48,17 @@ Parity status of each part (how the restatement is pinned, see tests/test_oracle
   DMRGCache / energy              two-site RDM energy (api.jl:510-527), product state
                                   energy -1 (mpo.jl:83-85), recalculate! invariance
   find_groundstate (VUMPS)        TFIM -4/pi, XXX 1/4 - ln 2 closed forms
+  push_env_*_mpo                  slice sums and brute-force contractions
+  IDMRG                           XXX energy against VUMPS and 1/4 - ln 2
+  lanczos_exponentiate            dense scipy expm (both signs of t, with sub-steps)
+  TDVP (real dt)                  imaginary-time XXX run to 1/4 - ln 2 (the reference's test)
+  correlator / dot / fidelity     product states, a two-site expectation value as the
+                                  neighbouring correlator, factorisation at long distance,
+                                  self-fidelity 1
+  dominant_env                    identity channel: lambda = 1, L = R = I/sqrt(D); product
+                                  state closed form; explicit transfer-matrix eigenpair
+  dense_envs / expectation_dense  one-site cell value = dominant transfer eigenvalue; doubled
+                                  cell; linearity in W; identity MPO = N
   KrylovKit iteration trajectories: parity unpinned (no Julia in this environment).
 """
 from __future__ import annotations
