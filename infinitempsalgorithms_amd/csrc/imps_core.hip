@@ -322,7 +322,7 @@ void gemm(imps_ctx* c, const GemmDesc& g0, int prec) {
   if (nw > 0 && g.W == nullptr) g.W = ws(c, "gemm_splitk", nw);
   const int mode = g_gemm_autotune.load(std::memory_order_relaxed);
   if (mode != 0) {
-    int ids[4];
+    int ids[8];
     const int nv = gemm_tile_variants(g, prec, ids);
     if (nv > 1) {
       int v = -1;
@@ -2808,7 +2808,7 @@ int imps_gemm_dev(imps_ctx* c, char transa, char transb, int m, int n, int k, do
 }
 
 int imps_set_gemm_autotune(int on) {
-  if (on < 0 || on > 2 + 3) return IMPS_E_ARG;
+  if (on < 0 || on > 2 + 4) return IMPS_E_ARG;
   g_gemm_autotune.store(on);
   return IMPS_OK;
 }

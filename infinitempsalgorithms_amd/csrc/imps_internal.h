@@ -37,7 +37,7 @@ enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
 
 hipError_t launch_gemm(const GemmDesc& g, int prec, hipStream_t s);
 // Bit-identical tile variants of the FP64/FP32 GEMM (no split-K): fills ids with the variants
-// valid for g and returns their number, or 0 when launch_gemm would split K for this shape (or
+// valid for g (at most 8) and returns their number, or 0 when launch_gemm would split K for this shape (or
 // prec is split-BF16).  launch_gemm_variant runs one of them.
 int gemm_tile_variants(const GemmDesc& g, int prec, int* ids);
 hipError_t launch_gemm_variant(const GemmDesc& g, int prec, int variant, hipStream_t s);

@@ -47,10 +47,35 @@ struct MfmaTraits<float> {
   static __device__ inline int row(int lk, int r) { return 4 * lk + r; }
 };
 
+// Output tile of workgroup (bx, by) in an MT x NT grid.  GM = 0: tile (bx, by).  GM > 0: the
+// hardware hands consecutive workgroup ids to the 8 XCDs in turn, so id i runs on XCD i % 8; the
+// tiles are renumbered so that each XCD owns a contiguous range of tile numbers (when 8 divides
+// MT NT), and numbers run over groups of GM tile rows, m fastest inside a group, so that the
+// workgroups resident on one XCD share A and B panels in its L2.  A bijection of the tiles, so
+// every output is computed exactly as in the identity order.
+template <int GM>
+__device__ inline void gemm_tile_of(int bx, int by, int MT, int NT, int& mt, int& nt) {
+  if (GM == 0) {
+    mt = bx;
+    nt = by;
+    return;
+  }
+  const int T = MT * NT;
+  const int lin = bx + MT * by;
+  const int L = (T % 8 == 0) ? (lin % 8) * (T / 8) + lin / 8 : lin;
+  const int per = GM * NT;
+  const int grp = L / per;
+  const int first = grp * GM;
+  const int gsz = (MT - first < GM) ? MT - first : GM;
+  const int r = L - grp * per;
+  mt = first + r % gsz;
+  nt = r / gsz;
+}
+
 // VEC: every tile is full (M % BM, N % BN, K % BK, Kc % BK all zero) and each thread moves
 // 16-byte pairs along the contiguous direction of A and B (even strides, 16-byte aligned bases),
-// so the tile loads need no bounds checks.
-template <typename TC, int BM, int BN, int BK, bool AMC, bool BKC, int OCC, bool VEC>
+// so the tile loads need no bounds checks.  GM: tile order (gemm_tile_of).
+template <typename TC, int BM, int BN, int BK, bool AMC, bool BKC, int OCC, bool VEC, int GM = 0>
 __global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
   typedef MfmaTraits<TC> MT;
   typedef typename MT::v4 v4;
@@ -65,8 +90,10 @@ __global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int m0 = blockIdx.x * BM;
-  const int n0 = blockIdx.y * BN;
+  int mt, nt;
+  gemm_tile_of<GM>(blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, mt, nt);
+  const int m0 = mt * BM;
+  const int n0 = nt * BN;
   const int wm = (wave & 1) * WM;
   const int wn = (wave >> 1) * WN;
   const int M = g.M, N = g.N, K = g.K, Kc = g.Kc;
@@ -420,18 +447,18 @@ __global__ void splitk_reduce_kernel(int M, int N, int nz, const double* __restr
   *p = (beta == 0.0) ? alpha * s : alpha * s + beta * *p;
 }
 
-template <typename TC, int BM, int BN, int BK, int OCC, bool VEC>
+template <typename TC, int BM, int BN, int BK, int OCC, bool VEC, int GM = 0>
 static hipError_t gemm_dispatch_tile(const GemmDesc& g, bool amc, bool bkc, int splitk, hipStream_t s) {
   dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, splitk);
   dim3 block(256);
   if (amc && bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, true, OCC, VEC>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, true, OCC, VEC, GM>), grid, block, 0, s, g);
   else if (amc && !bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, false, OCC, VEC>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, false, OCC, VEC, GM>), grid, block, 0, s, g);
   else if (!amc && bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, true, OCC, VEC>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, true, OCC, VEC, GM>), grid, block, 0, s, g);
   else
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, false, OCC, VEC>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, false, OCC, VEC, GM>), grid, block, 0, s, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || splitk == 1) return e;
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3((g.M + 255) / 256, g.N), dim3(256), 0, s, g.M, g.N, splitk,
@@ -486,7 +513,7 @@ static hipError_t gemm_select(const GemmDesc& g, bool amc, bool bkc, hipStream_t
   const long t64x128 = (long)((g.M + 63) / 64) * ((g.N + 127) / 128);
   // large shapes: 64 x 128 tiles at four workgroups per CU with paired loads
   if (t64x128 >= 384 && gemm_vec_ok(g, amc, bkc, 64, 128))
-    return gemm_dispatch_tile<TC, 64, 128, 16, 4, true>(g, amc, bkc, 1, s);
+    return gemm_dispatch_tile<TC, 64, 128, 16, 4, true, 8>(g, amc, bkc, 1, s);
   const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
   if (t128 >= 192) return gemm_dispatch_tile<TC, 128, 128, 16, 2, false>(g, amc, bkc, 1, s);
   const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
@@ -529,15 +556,17 @@ static bool gemm_normalise(GemmDesc& g, bool& amc, bool& bkc) {
 // Tile variants of the FP64/FP32 GEMM without split-K.  They accumulate every output over K in
 // the same order (K tiles of 16, MFMA steps of 4), so their results are bit-identical and the
 // choice among them is a matter of speed only:
-//   0: 64 x 128, occupancy 4, paired loads   1: 64 x 64, occupancy 4, paired loads
+//   0: 64 x 128, occupancy 4, paired loads, XCD-grouped tile order (groups of 8 tile rows)
+//   1: 64 x 64, occupancy 4, paired loads   4: variant 0 in the identity tile order
 //   2: 128 x 128, occupancy 2                3: 64 x 64, occupancy 2
 template <typename TC>
 static hipError_t gemm_variant(const GemmDesc& g, bool amc, bool bkc, int v, hipStream_t s) {
   switch (v) {
-    case 0: return gemm_dispatch_tile<TC, 64, 128, 16, 4, true>(g, amc, bkc, 1, s);
+    case 0: return gemm_dispatch_tile<TC, 64, 128, 16, 4, true, 8>(g, amc, bkc, 1, s);
     case 1: return gemm_dispatch_tile<TC, 64, 64, 16, 4, true>(g, amc, bkc, 1, s);
     case 2: return gemm_dispatch_tile<TC, 128, 128, 16, 2, false>(g, amc, bkc, 1, s);
     case 3: return gemm_dispatch_tile<TC, 64, 64, 16, 2, false>(g, amc, bkc, 1, s);
+    case 4: return gemm_dispatch_tile<TC, 64, 128, 16, 4, true, 0>(g, amc, bkc, 1, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -549,10 +578,12 @@ int gemm_tile_variants(const GemmDesc& g0, int prec, int* ids) {
   if (g.M <= 0 || g.N <= 0 || !gemm_normalise(g, amc, bkc)) return 0;
   if (gemm_select_splitk(g, amc, bkc) > 1) return 0;
   int n = 0;
-  if (gemm_vec_ok(g, amc, bkc, 64, 128)) ids[n++] = 0;
+  const bool v128 = gemm_vec_ok(g, amc, bkc, 64, 128);
+  if (v128) ids[n++] = 0;
   if (gemm_vec_ok(g, amc, bkc, 64, 64)) ids[n++] = 1;
   ids[n++] = 2;
   ids[n++] = 3;
+  if (v128) ids[n++] = 4;
   return n;
 }
 
@@ -561,7 +592,7 @@ hipError_t launch_gemm_variant(const GemmDesc& g0, int prec, int v, hipStream_t 
   bool amc, bkc;
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (!gemm_normalise(g, amc, bkc)) return hipErrorInvalidValue;
-  if ((v == 0 && !gemm_vec_ok(g, amc, bkc, 64, 128)) || (v == 1 && !gemm_vec_ok(g, amc, bkc, 64, 64)))
+  if (((v == 0 || v == 4) && !gemm_vec_ok(g, amc, bkc, 64, 128)) || (v == 1 && !gemm_vec_ok(g, amc, bkc, 64, 64)))
     return hipErrorInvalidValue;
   if (prec == PREC_FP32) return gemm_variant<float>(g, amc, bkc, v, s);
   if (prec == PREC_FP64) return gemm_variant<double>(g, amc, bkc, v, s);

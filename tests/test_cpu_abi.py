@@ -116,8 +116,8 @@ def test_random_tensors_match_oracle(oracle):
 
 
 def test_gemm_autotune_mode_range(lib):
-    # 0 heuristic, 1 autotune, 2 + v a forced tile variant (v = 0..3); anything else is refused
-    for mode in (-1, 6, 100):
+    # 0 heuristic, 1 autotune, 2 + v a forced tile variant (v = 0..4); anything else is refused
+    for mode in (-1, 7, 100):
         assert lib.imps_set_gemm_autotune(mode) != 0
-    for mode in (0, 5, 1):
+    for mode in (0, 6, 1):
         assert lib.imps_set_gemm_autotune(mode) == 0
