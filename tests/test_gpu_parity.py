@@ -724,6 +724,36 @@ def test_checkpoint_and_resume_state(imps, ctx, oracle, tmp_path):
         c3.close()
 
 
+def test_python_example_sequence(imps, tmp_path):
+    # the Python example of the integration notes at D=16: VUMPS, observables, IDMRG and a TDVP
+    # step on one GPUCache, then a checkpoint; each call sees the state the previous one left
+    H = imps.models.fermi_hubbard(t=1.0, U=4.0, mu=2.0, nsites=2)
+    psi0, c = imps.randomimps(16, H, seed=2)
+    try:
+        envs = imps.GPUCache(psi0, H, c)
+        psi, envs, info = imps.find_groundstate(psi0, H, imps.VUMPS(tol=1e-6, maxiter=60), envs=envs)
+        assert np.isfinite(info.energy) and info.niter >= 1
+        Z = np.diag([1.0, -1.0, -1.0, 1.0])
+        cz = imps.correlator(psi, Z, Z, 0, range(1, 9), envs)
+        assert cz.shape == (8,) and np.all(np.abs(cz) <= 1 + 1e-9)
+        f = imps.fidelity(psi, psi0, envs)
+        assert 0.0 <= f <= 1.0 + 1e-9
+        assert abs(imps.fidelity(psi, psi, envs) - 1) < 1e-9
+        lam, L = imps.dominant_env(psi, envs, side="left")
+        assert abs(lam - 1) < 1e-8 and np.allclose(L[:, 0, :], np.eye(16) / 4, atol=1e-6)
+        e_v = imps.expectationvalue(psi, H, envs)
+        psi2, envs, info2 = imps.find_groundstate(psi, H, imps.IDMRG(tol=1e-6, maxiter=20), envs=envs)
+        assert np.isfinite(info2.energy)
+        psi3, envs = imps.timestep(psi2, H, -0.05, imps.TDVP(), envs)
+        e3 = imps.expectationvalue(psi3, H, envs)
+        assert np.isfinite(e3) and e3 < e_v + 1e-3 * abs(e_v)
+        c.checkpoint(tmp_path / "psi.imps")
+        psi4 = imps.load_state(tmp_path / "psi.imps")
+        assert np.array_equal(psi4.AC[0], c.get_state().AC[0])
+    finally:
+        c.close()
+
+
 def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     # reverse H_AC sandwich against the oracle on a random state
     Wx = oracle.xxz_mpo(1.0, 0.8)
