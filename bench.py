@@ -134,6 +134,8 @@ def main():
                        "D": D, "d": d, "w": w, "flop_per_step": F, "parallelism": "single GPU"},
             "roofline": r["roofline"], "kernels": r["kernels"],
         })
+        if r["krylov"]:
+            out["krylov"] = r["krylov"]
         if args.prec == "fp64" and not args.no_mixed:
             # the mixed-precision bulk path of the same H_AC (split-BF16 planes, FP32 accumulation)
             m = hac_throughput(args, imps, _lib, H, "bf16x3", D, d, w, N, F)
@@ -207,6 +209,20 @@ def hac_throughput(args, imps, _lib, H, pname, D, d, w, N, F):
     ctx.call("imps_bench_hac", 0, max(10, args.steps // 2), prec, ctypes.byref(t))
     ctx.call("imps_dev_free", x)
     ctx.call("imps_dev_free", y)
+    krylov = None
+    if pname == "fp64":
+        # the Krylov solvers' Gram-Schmidt sweeps (HBM-bound) on vectors of the H_AC length
+        # against a 20-vector basis, operands rotated through three bases
+        kt = _lib.KrylovTiming()
+        nv = 20
+        ctx.call("imps_bench_krylov", n, nv, 20, ctypes.byref(kt))
+
+        def rate(ms, b):
+            return {"ms": ms, "bytes": b, "gbs": b / (ms * 1e-3) / 1e9}
+        krylov = {"n": n, "basis_vectors": nv,
+                  "dot_sweep": rate(kt.ms_multidot, kt.bytes_multidot),
+                  "second_pass_sweep": rate(kt.ms_axpy_dots, kt.bytes_axpy),
+                  "norm_sweep": rate(kt.ms_axpy_norm, kt.bytes_axpy)}
     ctx.close()
     # per-launch algorithmic flops of the GEMMs as executed: 2 (w-1) d D^3 each, the identity
     # levels being copies (F keeps the full 4 w d D^3 of the H_AC definition)
@@ -233,7 +249,7 @@ def hac_throughput(args, imps, _lib, H, pname, D, d, w, N, F):
         # three bf16 MFMA products per FP64-equivalent multiply-add, against the dense bf16 peak
         roof.update({"achieved": 3 * achieved, "frac": 3 * achieved / PEAKS[pname],
                      "unit": "TFLOP/s (bf16 MFMA: 3 products per FP64-equivalent flop)"})
-    return {"value": F / dt / 1e12, "ms_per_step": dt * 1e3, "kernels": kernels, "roofline": roof}
+    return {"value": F / dt / 1e12, "ms_per_step": dt * 1e3, "kernels": kernels, "roofline": roof, "krylov": krylov}
 
 
 def idmrg_sweeps(args, imps, H, D, N):

@@ -356,6 +356,21 @@ typedef struct imps_hac_timing {
 } imps_hac_timing;
 int imps_bench_hac(imps_ctx* ctx, int site, int reps, int prec, imps_hac_timing* t);
 
+/* Average times of the Krylov solvers' Gram-Schmidt kernels on synthetic vectors of length n
+ * against a basis of nv (1..32) vectors, `reps` launches each with HIP events on the context's
+ * stream, rotating over three bases so that the operands come from HBM:
+ * X_v . w (the dot sweep), w -= X c followed by X_v . w (the second-pass sweep) and w -= X c
+ * followed by w . w (the norm sweep).  Algorithmic bytes: (nv + 1) n 8 for the dot sweep,
+ * (nv + 2) n 8 for the others (the basis, and w read and written). */
+typedef struct imps_krylov_timing {
+  double ms_multidot;
+  double ms_axpy_dots;
+  double ms_axpy_norm;
+  double bytes_multidot;
+  double bytes_axpy;
+} imps_krylov_timing;
+int imps_bench_krylov(imps_ctx* ctx, long n, int nv, int reps, imps_krylov_timing* t);
+
 /* General strided GEMM of the library on device pointers (column-major):
  * C = alpha*op(A)*op(B) + beta*C, op = 'N' or 'T'.  Exposed for tests. */
 int imps_gemm_dev(imps_ctx* ctx, char transa, char transb, int m, int n, int k, double alpha,

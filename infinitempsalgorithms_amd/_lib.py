@@ -25,6 +25,11 @@ PP_double = ctypes.POINTER(P_double)
 P_int = ctypes.POINTER(ctypes.c_int)
 
 
+class KrylovTiming(ctypes.Structure):
+    _fields_ = [("ms_multidot", c_double), ("ms_axpy_dots", c_double), ("ms_axpy_norm", c_double),
+                ("bytes_multidot", c_double), ("bytes_axpy", c_double)]
+
+
 class KrylovParams(ctypes.Structure):
     _fields_ = [("tol", c_double), ("krylovdim", c_int), ("maxiter", c_int), ("eager", c_int),
                 ("prec", c_int)]
@@ -73,6 +78,7 @@ SIGNATURES = [
     ("imps_set_state", c_int, [c_void_p, c_int, c_int, PP_double, PP_double, PP_double, PP_double]),
     ("imps_get_state", c_int, [c_void_p, PP_double, PP_double, PP_double, PP_double]),
     ("imps_state_dims", c_int, [c_void_p, P_int, P_int, P_int]),
+    ("imps_bench_krylov", c_int, [c_void_p, ctypes.c_long, c_int, c_int, ctypes.POINTER(KrylovTiming)]),
     ("imps_write_state_file", c_int, [ctypes.c_char_p, c_int, c_int, c_int, PP_double, PP_double, PP_double,
                                       PP_double]),
     ("imps_state_file_info", c_int, [ctypes.c_char_p, P_int, P_int, P_int]),

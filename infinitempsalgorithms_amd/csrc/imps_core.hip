@@ -2731,6 +2731,49 @@ int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imp
   return worst.load();
 }
 
+int imps_bench_krylov(imps_ctx* c, long n, int nv, int reps, imps_krylov_timing* t) {
+  if (!c || !t || n < 1 || nv < 1 || nv > 32 || reps < 1) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    // three bases of nv vectors plus the vector they act on, used in turn, so that repeated
+    // launches read their operands from HBM rather than from the last level of cache
+    const int NB = 3;
+    const size_t per = (size_t)(nv + 1) * (size_t)n;
+    double* B = ws(c, "bk_B", NB * per);
+    HIPCHK(launch_fill_random(NB * per, 777, B, c->stream));
+    double* coef = ws(c, "bk_coef", 64);
+    HIPCHK(launch_fill(64, 1e-3, coef, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double ms[3];
+    for (int k = 0; k < 3; ++k) {
+      auto run = [&](int r) {
+        double* X = B + (size_t)(r % NB) * per;
+        double* y = X + (size_t)nv * n;
+        if (k == 0)
+          HIPCHK(launch_multidot((size_t)n, nv, X, (size_t)n, y, c->dpart, c->stream));
+        else
+          HIPCHK(launch_axpy_reduce((size_t)n, nv, X, (size_t)n, coef, y, c->dpart, k == 1, c->stream));
+      };
+      run(0);
+      HIPCHK(hipEventRecord(e0, c->stream));
+      for (int r = 1; r <= reps; ++r) run(r);
+      HIPCHK(hipEventRecord(e1, c->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float el = 0.f;
+      HIPCHK(hipEventElapsedTime(&el, e0, e1));
+      ms[k] = el / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    t->ms_multidot = ms[0];
+    t->ms_axpy_dots = ms[1];
+    t->ms_axpy_norm = ms[2];
+    t->bytes_multidot = 8.0 * (double)(nv + 1) * (double)n;
+    t->bytes_axpy = 8.0 * (double)(nv + 2) * (double)n;
+  });
+}
+
 int imps_bench_hac(imps_ctx* c, int site, int reps, int prec, imps_hac_timing* t) {
   if (!c || !t || reps <= 0) return IMPS_E_ARG;
   return run_guarded(c, [&] {
