@@ -44,6 +44,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def oracle_ttg(D, maxiter):
+    """The CPU baseline of the time-to-Galerkin runs: the oracle's find_groundstate (numpy
+    restatement of vumps.jl:51-86) on the JW Hubbard chain (t=1, U=4, mu=2, N=2) from
+    randomimps(D, [4, 4], seed 2), VUMPS tol 1e-8, on this host's cores."""
+    from oracle import imps_oracle as O
+    W = O.hubbard_mpo(1.0, 4.0, 2.0)
+    psi = O.randomimps(D, [4, 4], 2)
+    t0 = time.perf_counter()
+
+    def cb(it, eps, f):
+        if it % 10 == 0 or eps < 1e-8:
+            log(f"oracle: VUMPS iter {it} eps {eps:.3e} e/site {f / 2:.12f} t {time.perf_counter() - t0:.1f}s")
+    _, _, info = O.find_groundstate(psi, [W, W], O.VUMPS(tol=1e-8, maxiter=maxiter), callback=cb)
+    return {"D": D, "iterations": info.niter, "converged": bool(info.converged), "itererr": info.losses[-1],
+            "seconds": time.perf_counter() - t0, "threads": os.environ.get("OPENBLAS_NUM_THREADS", "default"),
+            "kind": "port"}
+
+
 def cpu_baseline(D, d, w, seconds=15.0):
     import numpy as np
     from oracle import imps_oracle as oracle
@@ -92,6 +110,9 @@ def main():
     ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the split-BF16 H_AC line of an FP64 run")
+    ap.add_argument("--oracle-ttg", action="store_true",
+                    help="CPU only: the oracle's VUMPS run to Galerkin < 1e-8 at --D (JW Hubbard, "
+                         "PCG64 seed 2; long: minutes at D=64)")
     ap.add_argument("--gemm-variant", type=int, default=-1,
                     help="run FP64/FP32 GEMM tile variant V wherever valid instead of autotuning (-1)")
     args = ap.parse_args()
@@ -101,6 +122,11 @@ def main():
         sys.exit(2)
 
     import numpy as np
+
+    if args.oracle_ttg:
+        print(json.dumps({"metric": METRIC, "oracle_time_to_galerkin": oracle_ttg(args.D, args.ttg_maxiter)}),
+              flush=True)
+        return
 
     import infinitempsalgorithms_amd as imps
     from infinitempsalgorithms_amd import _lib
