@@ -10,11 +10,14 @@
 //         32..    payload, column-major float64: per site AL, AR (D d D each), C (D D), AC (D d D)
 //                 for a state; W (w d w d) for an MPO
 //   last 8 bytes  uint64 FNV-1a hash of the payload bytes
+// Reads check the file's size against the size its header announces before allocating anything.
 // A file is written to "<path>.tmp" and renamed onto <path> once complete, so an interrupted
 // write leaves any earlier file at <path> intact.
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -98,6 +101,19 @@ int write_file(const char* path, const char* magic, int nsites, int dim, int d,
   return IMPS_OK;
 }
 
+// payload doubles of a file of this kind and these dimensions; false when it would not fit in
+// 2^60 bytes (no real file, so the header is not trusted)
+bool payload_doubles(const char* magic, int64_t nsites, int64_t dim, int64_t d, uint64_t& out) {
+  const long double per = (magic == kStateMagic) ? 3.0L * dim * d * dim + (long double)dim * dim
+                                                 : (long double)dim * d * dim * d;
+  const long double total = per * nsites;
+  if (!(total * 8.0L < 1152921504606846976.0L)) return false;
+  out = (magic == kStateMagic) ? (uint64_t)nsites * (3ull * dim * d * dim + (uint64_t)dim * dim)
+                               : (uint64_t)nsites * ((uint64_t)dim * d * dim * d);
+  return true;
+}
+
+// checks the header and that the file holds exactly the payload it announces plus the hash
 int read_header(FILE* f, const char* path, const char* magic, Header& h) {
   if (std::fread(&h, sizeof h, 1, f) != 1) return fail(IMPS_E_IO, std::string(path) + ": truncated header");
   if (std::memcmp(h.magic, magic, 8) != 0)
@@ -106,6 +122,15 @@ int read_header(FILE* f, const char* path, const char* magic, Header& h) {
   if (h.order != kOrder) return fail(IMPS_E_IO, std::string(path) + ": written with another byte order");
   if (h.dtype != 1) return fail(IMPS_E_IO, std::string(path) + ": element type is not float64");
   if (h.nsites <= 0 || h.dim <= 0 || h.d <= 0) return fail(IMPS_E_IO, std::string(path) + ": bad dimensions");
+  uint64_t nd = 0;
+  if (!payload_doubles(magic, h.nsites, h.dim, h.d, nd))
+    return fail(IMPS_E_IO, std::string(path) + ": dimensions in the header are not plausible");
+  const long here = std::ftell(f);
+  if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) return fail(IMPS_E_IO, std::string(path) + ": cannot seek");
+  const long size = std::ftell(f);
+  if (size < 0 || std::fseek(f, here, SEEK_SET) != 0) return fail(IMPS_E_IO, std::string(path) + ": cannot seek");
+  if ((uint64_t)size < sizeof(Header) + 8 || (uint64_t)size - sizeof(Header) - 8 != nd * 8)
+    return fail(IMPS_E_IO, std::string(path) + ": file size does not match its header (truncated or extended)");
   return IMPS_OK;
 }
 
@@ -157,114 +182,147 @@ int read_file(const char* path, const char* magic, int nsites, int dim, int d,
 
 }  // namespace
 
+// runs f, turning an exception (allocation failure) into an error code: nothing may unwind
+// through the C interface
+template <typename F>
+int guarded(const char* who, F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(IMPS_E_IO, std::string(who) + ": out of host memory");
+  } catch (const std::exception& e) {
+    return fail(IMPS_E_IO, std::string(who) + ": " + e.what());
+  } catch (...) {
+    return fail(IMPS_E_IO, std::string(who) + ": unexpected error");
+  }
+}
+
 extern "C" {
 
 int imps_write_state_file(const char* path, int nsites, int D, int d, const double* const* AL,
                           const double* const* AR, const double* const* C, const double* const* AC) {
-  if (!path || nsites <= 0 || D <= 0 || d <= 0 || !AL || !AR || !C || !AC)
-    return fail(IMPS_E_ARG, "imps_write_state_file: bad arguments");
-  const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
-  std::vector<std::pair<const double*, size_t>> blocks;
-  for (int l = 0; l < nsites; ++l) {
-    if (!AL[l] || !AR[l] || !C[l] || !AC[l]) return fail(IMPS_E_ARG, "imps_write_state_file: null tensor");
-    blocks.push_back({AL[l], nA});
-    blocks.push_back({AR[l], nA});
-    blocks.push_back({C[l], nC});
-    blocks.push_back({AC[l], nA});
-  }
-  return write_file(path, kStateMagic, nsites, D, d, blocks);
+  return guarded("imps_write_state_file", [&]() -> int {
+    if (!path || nsites <= 0 || D <= 0 || d <= 0 || !AL || !AR || !C || !AC)
+      return fail(IMPS_E_ARG, "imps_write_state_file: bad arguments");
+    const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
+    std::vector<std::pair<const double*, size_t>> blocks;
+    for (int l = 0; l < nsites; ++l) {
+      if (!AL[l] || !AR[l] || !C[l] || !AC[l]) return fail(IMPS_E_ARG, "imps_write_state_file: null tensor");
+      blocks.push_back({AL[l], nA});
+      blocks.push_back({AR[l], nA});
+      blocks.push_back({C[l], nC});
+      blocks.push_back({AC[l], nA});
+    }
+    return write_file(path, kStateMagic, nsites, D, d, blocks);
+  });
 }
 
 int imps_state_file_info(const char* path, int* nsites, int* D, int* d) {
-  return info_file(path, kStateMagic, nsites, D, d);
+  return guarded("imps_state_file_info", [&]() -> int {
+    return info_file(path, kStateMagic, nsites, D, d);
+  });
 }
 
 int imps_read_state_file(const char* path, int nsites, int D, int d, double* const* AL, double* const* AR,
                          double* const* C, double* const* AC) {
-  if (!path || nsites <= 0 || D <= 0 || d <= 0 || !AL || !AR || !C || !AC)
-    return fail(IMPS_E_ARG, "imps_read_state_file: bad arguments");
-  const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
-  std::vector<std::pair<double*, size_t>> blocks;
-  for (int l = 0; l < nsites; ++l) {
-    if (!AL[l] || !AR[l] || !C[l] || !AC[l]) return fail(IMPS_E_ARG, "imps_read_state_file: null tensor");
-    blocks.push_back({AL[l], nA});
-    blocks.push_back({AR[l], nA});
-    blocks.push_back({C[l], nC});
-    blocks.push_back({AC[l], nA});
-  }
-  return read_file(path, kStateMagic, nsites, D, d, blocks);
+  return guarded("imps_read_state_file", [&]() -> int {
+    if (!path || nsites <= 0 || D <= 0 || d <= 0 || !AL || !AR || !C || !AC)
+      return fail(IMPS_E_ARG, "imps_read_state_file: bad arguments");
+    const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
+    std::vector<std::pair<double*, size_t>> blocks;
+    for (int l = 0; l < nsites; ++l) {
+      if (!AL[l] || !AR[l] || !C[l] || !AC[l]) return fail(IMPS_E_ARG, "imps_read_state_file: null tensor");
+      blocks.push_back({AL[l], nA});
+      blocks.push_back({AR[l], nA});
+      blocks.push_back({C[l], nC});
+      blocks.push_back({AC[l], nA});
+    }
+    return read_file(path, kStateMagic, nsites, D, d, blocks);
+  });
 }
 
 int imps_write_mpo_file(const char* path, int nsites, int w, int d, const double* const* W) {
-  if (!path || nsites <= 0 || w <= 0 || d <= 0 || !W) return fail(IMPS_E_ARG, "imps_write_mpo_file: bad arguments");
-  const size_t nW = (size_t)w * d * w * d;
-  std::vector<std::pair<const double*, size_t>> blocks;
-  for (int l = 0; l < nsites; ++l) {
-    if (!W[l]) return fail(IMPS_E_ARG, "imps_write_mpo_file: null tensor");
-    blocks.push_back({W[l], nW});
-  }
-  return write_file(path, kMpoMagic, nsites, w, d, blocks);
+  return guarded("imps_write_mpo_file", [&]() -> int {
+    if (!path || nsites <= 0 || w <= 0 || d <= 0 || !W) return fail(IMPS_E_ARG, "imps_write_mpo_file: bad arguments");
+    const size_t nW = (size_t)w * d * w * d;
+    std::vector<std::pair<const double*, size_t>> blocks;
+    for (int l = 0; l < nsites; ++l) {
+      if (!W[l]) return fail(IMPS_E_ARG, "imps_write_mpo_file: null tensor");
+      blocks.push_back({W[l], nW});
+    }
+    return write_file(path, kMpoMagic, nsites, w, d, blocks);
+  });
 }
 
-int imps_mpo_file_info(const char* path, int* nsites, int* w, int* d) { return info_file(path, kMpoMagic, nsites, w, d); }
+int imps_mpo_file_info(const char* path, int* nsites, int* w, int* d) {
+  return guarded("imps_mpo_file_info", [&]() -> int {
+    return info_file(path, kMpoMagic, nsites, w, d);
+  });
+}
 
 int imps_read_mpo_file(const char* path, int nsites, int w, int d, double* const* W) {
-  if (!path || nsites <= 0 || w <= 0 || d <= 0 || !W) return fail(IMPS_E_ARG, "imps_read_mpo_file: bad arguments");
-  const size_t nW = (size_t)w * d * w * d;
-  std::vector<std::pair<double*, size_t>> blocks;
-  for (int l = 0; l < nsites; ++l) {
-    if (!W[l]) return fail(IMPS_E_ARG, "imps_read_mpo_file: null tensor");
-    blocks.push_back({W[l], nW});
-  }
-  return read_file(path, kMpoMagic, nsites, w, d, blocks);
+  return guarded("imps_read_mpo_file", [&]() -> int {
+    if (!path || nsites <= 0 || w <= 0 || d <= 0 || !W) return fail(IMPS_E_ARG, "imps_read_mpo_file: bad arguments");
+    const size_t nW = (size_t)w * d * w * d;
+    std::vector<std::pair<double*, size_t>> blocks;
+    for (int l = 0; l < nsites; ++l) {
+      if (!W[l]) return fail(IMPS_E_ARG, "imps_read_mpo_file: null tensor");
+      blocks.push_back({W[l], nW});
+    }
+    return read_file(path, kMpoMagic, nsites, w, d, blocks);
+  });
 }
 
 int imps_checkpoint_state(imps_ctx* ctx, const char* path) {
-  if (!ctx || !path) return IMPS_E_ARG;
-  int N = 0, D = 0, d = 0;
-  int st = imps_state_dims(ctx, &N, &D, &d);
-  if (st != IMPS_OK) return st;
-  const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
-  std::vector<std::vector<double>> AL(N, std::vector<double>(nA)), AR = AL, AC = AL;
-  std::vector<std::vector<double>> C(N, std::vector<double>(nC));
-  std::vector<double*> pl(N), pr(N), pc(N), pa(N);
-  for (int l = 0; l < N; ++l) {
-    pl[l] = AL[l].data();
-    pr[l] = AR[l].data();
-    pc[l] = C[l].data();
-    pa[l] = AC[l].data();
-  }
-  st = imps_get_state(ctx, pl.data(), pr.data(), pc.data(), pa.data());
-  if (st != IMPS_OK) return st;
-  st = imps_write_state_file(path, N, D, d, pl.data(), pr.data(), pc.data(), pa.data());
-  if (st != IMPS_OK) imps::set_context_error(ctx, imps_last_error(nullptr));
-  return st;
+  return guarded("imps_checkpoint_state", [&]() -> int {
+    if (!ctx || !path) return IMPS_E_ARG;
+    int N = 0, D = 0, d = 0;
+    int st = imps_state_dims(ctx, &N, &D, &d);
+    if (st != IMPS_OK) return st;
+    const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
+    std::vector<std::vector<double>> AL(N, std::vector<double>(nA)), AR = AL, AC = AL;
+    std::vector<std::vector<double>> C(N, std::vector<double>(nC));
+    std::vector<double*> pl(N), pr(N), pc(N), pa(N);
+    for (int l = 0; l < N; ++l) {
+      pl[l] = AL[l].data();
+      pr[l] = AR[l].data();
+      pc[l] = C[l].data();
+      pa[l] = AC[l].data();
+    }
+    st = imps_get_state(ctx, pl.data(), pr.data(), pc.data(), pa.data());
+    if (st != IMPS_OK) return st;
+    st = imps_write_state_file(path, N, D, d, pl.data(), pr.data(), pc.data(), pa.data());
+    if (st != IMPS_OK) imps::set_context_error(ctx, imps_last_error(nullptr));
+    return st;
+  });
 }
 
 int imps_resume_state(imps_ctx* ctx, const char* path) {
-  if (!ctx || !path) return IMPS_E_ARG;
-  int N = 0, D = 0, d = 0;
-  int st = imps_state_file_info(path, &N, &D, &d);
-  if (st != IMPS_OK) {
-    imps::set_context_error(ctx, imps_last_error(nullptr));
-    return st;
-  }
-  const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
-  std::vector<std::vector<double>> AL(N, std::vector<double>(nA)), AR = AL, AC = AL;
-  std::vector<std::vector<double>> C(N, std::vector<double>(nC));
-  std::vector<double*> pl(N), pr(N), pc(N), pa(N);
-  for (int l = 0; l < N; ++l) {
-    pl[l] = AL[l].data();
-    pr[l] = AR[l].data();
-    pc[l] = C[l].data();
-    pa[l] = AC[l].data();
-  }
-  st = imps_read_state_file(path, N, D, d, pl.data(), pr.data(), pc.data(), pa.data());
-  if (st != IMPS_OK) {
-    imps::set_context_error(ctx, imps_last_error(nullptr));
-    return st;
-  }
-  return imps_set_state(ctx, N, D, pl.data(), pr.data(), pc.data(), pa.data());
+  return guarded("imps_resume_state", [&]() -> int {
+    if (!ctx || !path) return IMPS_E_ARG;
+    int N = 0, D = 0, d = 0;
+    int st = imps_state_file_info(path, &N, &D, &d);
+    if (st != IMPS_OK) {
+      imps::set_context_error(ctx, imps_last_error(nullptr));
+      return st;
+    }
+    const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
+    std::vector<std::vector<double>> AL(N, std::vector<double>(nA)), AR = AL, AC = AL;
+    std::vector<std::vector<double>> C(N, std::vector<double>(nC));
+    std::vector<double*> pl(N), pr(N), pc(N), pa(N);
+    for (int l = 0; l < N; ++l) {
+      pl[l] = AL[l].data();
+      pr[l] = AR[l].data();
+      pc[l] = C[l].data();
+      pa[l] = AC[l].data();
+    }
+    st = imps_read_state_file(path, N, D, d, pl.data(), pr.data(), pc.data(), pa.data());
+    if (st != IMPS_OK) {
+      imps::set_context_error(ctx, imps_last_error(nullptr));
+      return st;
+    }
+    return imps_set_state(ctx, N, D, pl.data(), pr.data(), pc.data(), pa.data());
+  });
 }
 
 }  // extern "C"

@@ -93,3 +93,27 @@ def test_bad_files_are_refused_and_outputs_untouched(oracle, tmp_path):
         imps.save_state(st, tmp_path / "no" / "such" / "dir.imps")
     with pytest.raises(_lib.ImpsError, match="checksum"):
         imps.load_state(tmp_path / "flip.imps")
+
+
+def test_headers_with_wrong_or_absurd_dimensions_are_refused(oracle, tmp_path):
+    # the reader compares the file size with the size the header announces before allocating,
+    # so a damaged header cannot make it allocate or read past the payload
+    import struct
+    st = _state(oracle)
+    path = tmp_path / "psi.imps"
+    imps.save_state(st, path)
+    raw = bytearray(path.read_bytes())
+    for nsites, D, d in ((1_000_000_000, 100_000, 4), (2, 6, 3), (3, 5, 3), (2, 5, 2)):
+        bad = bytearray(raw)
+        struct.pack_into("<iii", bad, 16, nsites, D, d)
+        p = tmp_path / f"hdr_{nsites}_{D}_{d}.imps"
+        p.write_bytes(bytes(bad))
+        with pytest.raises(_lib.ImpsError, match="not plausible|does not match"):
+            imps.load_state(p)
+    H = imps.models.fermi_hubbard(t=1.0, U=4.0, mu=2.0, nsites=2)
+    imps.save_mpo(H, tmp_path / "h.impo")
+    bad = bytearray((tmp_path / "h.impo").read_bytes())
+    struct.pack_into("<i", bad, 16, 2_000_000_000)
+    (tmp_path / "hbad.impo").write_bytes(bytes(bad))
+    with pytest.raises(_lib.ImpsError, match="not plausible|does not match"):
+        imps.load_mpo(tmp_path / "hbad.impo")
