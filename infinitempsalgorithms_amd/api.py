@@ -56,12 +56,20 @@ class Context:
 
     def set_state(self, psi: "CanonicalIMPS"):
         AL, AR, C, AC = ([fortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
+        D = psi.D
+        # (without a model the library itself refuses the call)
+        if self.d and (not (len(AL) == len(AR) == len(C) == len(AC)) or
+                       any(x.shape != (D, self.d, D) for x in AL + AR + AC) or any(x.shape != (D, D) for x in C)):
+            raise ValueError(f"set_state: every AL, AR, AC must be ({D}, {self.d}, {D}) and every C ({D}, {D})")
         keep = [ptr_array(x) for x in (AL, AR, C, AC)]
         self.call("imps_set_state", len(AL), psi.D, *(k[0] for k in keep))
         self.D = psi.D
 
     def set_state_raw(self, As, tol, maxiter):
         As = [fortran(a) for a in As]
+        D = As[0].shape[0]
+        if self.d and any(a.shape != (D, self.d, D) for a in As):
+            raise ValueError(f"set_state_raw: every tensor must be ({D}, {self.d}, {D})")
         pp, keep = ptr_array(As)
         self.call("imps_set_state_raw", len(As), As[0].shape[0], pp, float(tol), int(maxiter))
         self.D = As[0].shape[0]
@@ -369,6 +377,8 @@ def expectationvalue(psi: CanonicalIMPS, op, envs: GPUCache):
     else:
         site, O = op
         O = fortran(O)
+        if O.shape != (psi.d, psi.d):
+            raise ValueError(f"expectationvalue: the operator must be {psi.d} x {psi.d}")
         envs.ctx.call("imps_local_expectation", int(site), dptr(O), ctypes.byref(val))
     return val.value
 
@@ -383,6 +393,8 @@ def correlator(psi: CanonicalIMPS, O1, O2, i: int, js, envs: GPUCache):
     scalar = isinstance(js, (int, np.integer))
     jl = [int(js)] if scalar else [int(j) for j in js]
     O1, O2 = fortran(O1), fortran(O2)
+    if O1.shape != (psi.d, psi.d) or O2.shape != (psi.d, psi.d):
+        raise ValueError(f"correlator: operators must be {psi.d} x {psi.d}")
     jarr = np.array(jl, dtype=np.int32)
     out = np.zeros(len(jl))
     envs.ctx.call("imps_correlator", dptr(O1), dptr(O2), int(i), len(jl),
@@ -393,6 +405,9 @@ def correlator(psi: CanonicalIMPS, O1, O2, i: int, js, envs: GPUCache):
 def dot(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
     """dot(ψ₁, ψ₂) (canonicalmps.jl:262-269): dominant eigenvalue of the mixed AL transfer
     matrix, on the device (ψ₁ becomes the context's state)."""
+    if len(psi2) != len(psi1) or any(a.shape != b.shape for a, b in zip(psi1.AL, psi2.AL)):
+        raise ValueError("dot: the library takes states with the same unit cell, bond and physical "
+                         "dimensions")
     if psi1 is not envs.ket:
         envs.ctx.set_state(psi1)
         envs.ket = psi1

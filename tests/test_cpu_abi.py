@@ -121,3 +121,25 @@ def test_gemm_autotune_mode_range(lib):
         assert lib.imps_set_gemm_autotune(mode) != 0
     for mode in (0, 6, 1):
         assert lib.imps_set_gemm_autotune(mode) == 0
+
+
+def test_host_arrays_are_checked_before_the_library_reads_them(oracle):
+    # the C interface takes plain pointers, so the Python mirror refuses arrays of the wrong shape
+    # before a call could read past them (no GPU needed: the checks come first)
+    import numpy as np
+    import infinitempsalgorithms_amd as imps
+    from infinitempsalgorithms_amd import api
+    psi = oracle.randomimps(4, [2, 2], 3)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    other = oracle.randomimps(5, [2, 2], 4)
+    st5 = imps.CanonicalIMPS(other.AL, other.AR, other.C, other.AC)
+
+    class FakeEnvs:
+        ket = st
+        ctx = None
+    with pytest.raises(ValueError, match="same unit cell"):
+        api.dot(st, st5, FakeEnvs())
+    with pytest.raises(ValueError, match="2 x 2"):
+        api.correlator(st, np.eye(3), np.eye(2), 0, [1], FakeEnvs())
+    with pytest.raises(ValueError, match="2 x 2"):
+        api.expectationvalue(st, (0, np.eye(3)), FakeEnvs())
