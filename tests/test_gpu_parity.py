@@ -754,6 +754,72 @@ def test_python_example_sequence(imps, tmp_path):
         c.close()
 
 
+def _pad_all_bonds(imps, psi, n, seed=0):
+    """Every bond of psi grown by n with zero weight (the reference's _padbond!, test/testhelpers.jl:
+    125-146, applied to all bonds): AL gets n orthonormal completing columns, AR n orthonormal
+    completing rows, C and AC are zero-extended; AL C = AC = C AR still holds."""
+    rng = np.random.default_rng(seed)
+
+    def complete(M):
+        R = rng.standard_normal((M.shape[0], n))
+        for _ in range(2):
+            R -= M @ (M.T @ R)
+        return np.hstack([M, np.linalg.qr(R)[0]])
+    D, d = psi.AL[0].shape[0], psi.AL[0].shape[1]
+    Dn = D + n
+    AL, AR, C, AC = [], [], [], []
+    for l in range(len(psi.AL)):
+        A = np.zeros((Dn, d, D))
+        A[:D] = psi.AL[l]
+        AL.append(np.asfortranarray(complete(A.reshape(Dn * d, D, order="F")).reshape(Dn, d, Dn, order="F")))
+        B = np.zeros((D, d, Dn))
+        B[:, :, :D] = psi.AR[l]
+        AR.append(np.asfortranarray(complete(B.reshape(D, d * Dn, order="F").T).T.reshape(Dn, d, Dn, order="F")))
+        Cp = np.zeros((Dn, Dn))
+        Cp[:D, :D] = psi.C[l]
+        C.append(np.asfortranarray(Cp))
+        X = np.zeros((Dn, d, Dn))
+        X[:D, :, :D] = psi.AC[l]
+        AC.append(np.asfortranarray(X))
+    return imps.CanonicalIMPS(AL, AR, C, AC)
+
+
+def test_zero_padding_leaves_energy_and_fidelity_unchanged(imps, ctx, oracle):
+    # test/algorithms/envs.jl:62-67: the same physical state with zero-weight bond directions
+    # (here on every bond; the library keeps one bond dimension) has the same energy to 1e-10,
+    # with the environments solved on the device around a singular C
+    W = oracle.xxz_mpo(1.0, 1.0)
+    H = imps.SparseIMPO([W, W])
+    psi = oracle.randomimps(6, [2, 2], 5)
+    st = imps.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC)
+    e0 = imps.expectationvalue(st, H, imps.DMRGCache(st, H, ctx=ctx))
+    pad = _pad_all_bonds(imps, st, 2)
+    envs = imps.DMRGCache(pad, H, ctx=ctx)
+    assert abs(imps.expectationvalue(pad, H, envs) - e0) < 1e-10
+    assert abs(e0 - oracle.expectation_energy(psi, [W, W], oracle.dmrg_cache(psi, [W, W]))) < 1e-10
+    assert abs(imps.fidelity(pad, pad, envs) - 1) < 1e-10
+
+
+def test_xxx_ground_state_observables(imps, ctx, oracle):
+    # test/algorithms/observables.jl:20-32 and twosite.jl:66-73 on a ground state found on the
+    # device: <Sz^2> = 1/4, antiferromagnetic nearest-neighbour correlator, all correlators below
+    # 1/4, and the SU(2) relation <Sz Sz> = e0 / 3 within the reference's 5e-3
+    H = imps.models.heisenberg_hamiltonian(J=1.0, Delta=1.0, nsites=2)
+    psi0, c = imps.randomimps(12, H, seed=1)
+    try:
+        envs = imps.GPUCache(psi0, H, c)
+        psi, envs, info = imps.find_groundstate(psi0, H, imps.VUMPS(tol=1e-9, maxiter=200), envs=envs)
+        e0 = 0.25 - np.log(2)
+        assert abs(info.energy / 2 - e0) < 2e-4
+        Sz = np.diag([0.5, -0.5])
+        assert abs(imps.expectationvalue(psi, (0, Sz @ Sz), envs) - 0.25) < 1e-12
+        czz = imps.correlator(psi, Sz, Sz, 0, range(1, 6), envs)
+        assert czz[0] < 0 and np.all(czz < 0.26)
+        assert abs(czz[0] - e0 / 3) < 5e-3
+    finally:
+        c.close()
+
+
 def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     # reverse H_AC sandwich against the oracle on a random state
     Wx = oracle.xxz_mpo(1.0, 0.8)
