@@ -9,7 +9,9 @@ from __future__ import annotations
 
 import numpy as np
 
-from .api import SparseIMPO
+from types import SimpleNamespace
+
+from .api import DenseIMPO, SparseIMPO
 
 
 def sigma_x():
@@ -49,6 +51,13 @@ def mpohamiltonian(h1, pairs):
     return W
 
 
+def bulk_mpo(h1, pairs):
+    """Schur bulk tensor of an on-site + nearest-neighbour Hamiltonian (src/models.jl:35-60):
+    the cell of unit levels, h1 in the corner and one channel per pair, as the dense
+    (w, d, w, d) tensor that tompotensor gives (the same tensor as mpohamiltonian's)."""
+    return mpohamiltonian(h1, pairs)
+
+
 def tfim_hamiltonian(J=1.0, h=1.0, nsites=1):
     """Transverse-field Ising model H = -J sum sx sx - h sum sz (src/models.jl:119-122)."""
     W = mpohamiltonian(-h * sigma_z(), [(-J, sigma_x(), sigma_x())])
@@ -85,3 +94,21 @@ def fermi_hubbard(t=1.0, U=4.0, mu=2.0, nsites=1):
     h1 = U * (n_up @ n_dn) - mu * (n_up + n_dn)
     pairs = [(-t, c_up.T @ F, c_up), (-t, c_dn.T @ F, c_dn), (-t, F @ c_up, c_up.T), (-t, F @ c_dn, c_dn.T)]
     return SparseIMPO([mpohamiltonian(h1, pairs)] * nsites)
+
+
+def tfim(J=1.0, h=1.0):
+    """tfim(; J, h) (src/models.jl:150-155): (mpo, bulk, hamiltonian) with mpo the one-site
+    periodic DenseIMPO of the Schur bulk, bulk that tensor and hamiltonian the SparseIMPO of
+    tfim_hamiltonian (the reference's note: energies come from the SparseIMPO)."""
+    W = bulk_mpo(-h * sigma_z(), [(-J, sigma_x(), sigma_x())])
+    return SimpleNamespace(mpo=DenseIMPO([W]), bulk=W, hamiltonian=tfim_hamiltonian(J=J, h=h))
+
+
+def heisenberg_xxz(J=1.0, Delta=1.0, h=0.0):
+    """heisenberg_xxz(; J, Delta, h) (src/models.jl:135-139): (mpo, bulk, hamiltonian) as for
+    tfim.  The reference's bulk takes the channels Sx Sx, Sy Sy, Sz Sz (complex, through Sy);
+    this real library takes the S+- form of the same operator with the same 5 levels."""
+    sp = np.array([[0.0, 1.0], [0.0, 0.0]])
+    sm = sp.T.copy()
+    W = bulk_mpo(-h * Sz(), [(J / 2, sp, sm), (J / 2, sm, sp), (J * Delta, Sz(), Sz())])
+    return SimpleNamespace(mpo=DenseIMPO([W]), bulk=W, hamiltonian=heisenberg_hamiltonian(J=J, Delta=Delta, h=h))

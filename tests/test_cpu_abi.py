@@ -143,3 +143,22 @@ def test_host_arrays_are_checked_before_the_library_reads_them(oracle):
         api.correlator(st, np.eye(3), np.eye(2), 0, [1], FakeEnvs())
     with pytest.raises(ValueError, match="2 x 2"):
         api.expectationvalue(st, (0, np.eye(3)), FakeEnvs())
+
+
+def test_dense_model_constructors(oracle):
+    # tfim / heisenberg_xxz (src/models.jl:135-155): a one-site DenseIMPO of the Schur bulk and
+    # the SparseIMPO of the same model; the XXZ bulk has 5 levels (test/operators/mpo.jl:6-10)
+    import numpy as np
+    from infinitempsalgorithms_amd import models, DenseIMPO, SparseIMPO
+    m = models.tfim(J=1.0, h=0.7)
+    assert isinstance(m.mpo, DenseIMPO) and len(m.mpo) == 1 and isinstance(m.hamiltonian, SparseIMPO)
+    assert np.array_equal(m.bulk, m.hamiltonian.Ws[0]) and m.bulk.shape == (3, 2, 3, 2)
+    assert np.allclose(m.bulk, oracle.tfim_mpo(1.0, 0.7))
+    x = models.heisenberg_xxz(J=1.0, Delta=0.5)
+    assert x.bulk.shape == (5, 2, 5, 2) and len(x.mpo) == 1
+    assert np.array_equal(x.bulk, x.hamiltonian.Ws[0])
+    # the S+- channels give the same two-site operator as Sx Sx + Sy Sy + Delta Sz Sz
+    w = x.bulk
+    h2 = sum(np.kron(w[0, :, k, :], w[k, :, 4, :]) for k in range(1, 4))
+    sx = np.array([[0, 0.5], [0.5, 0]]); sy = np.array([[0, -0.5j], [0.5j, 0]]); sz = np.diag([0.5, -0.5])
+    assert np.allclose(h2, (np.kron(sx, sx) + np.kron(sy, sy) + 0.5 * np.kron(sz, sz)).real)
