@@ -162,6 +162,8 @@ def main():
         })
         if r["krylov"]:
             out["krylov"] = r["krylov"]
+        if r["hc"]:
+            out["hc"] = r["hc"]
         if args.prec == "fp64" and not args.no_mixed:
             # the mixed-precision bulk path of the same H_AC (split-BF16 planes, FP32 accumulation)
             m = hac_throughput(args, imps, _lib, H, "bf16x3", D, d, w, N, F)
@@ -235,6 +237,28 @@ def hac_throughput(args, imps, _lib, H, pname, D, d, w, N, F):
     ctx.call("imps_bench_hac", 0, max(10, args.steps // 2), prec, ctypes.byref(t))
     ctx.call("imps_dev_free", x)
     ctx.call("imps_dev_free", y)
+    hc = None
+    if pname == "fp64":
+        # H_C (effective.jl:72-76) on the same environments: C-sized vectors, F_HC = 4 w D^3
+        # (the identity levels included, as in F_HAC); wall clock over device-resident data
+        nc = D * D
+        xc, yc = ctypes.c_void_p(), ctypes.c_void_p()
+        ctx.call("imps_dev_alloc", nc * 8, ctypes.byref(xc))
+        ctx.call("imps_dev_alloc", nc * 8, ctypes.byref(yc))
+        xch = np.asfortranarray(rng.standard_normal(nc))
+        ctx.call("imps_memcpy_h2d", xc, xch.ctypes.data_as(ctypes.c_void_p), nc * 8)
+        for _ in range(args.warmup):
+            ctx.call("imps_hc_apply_dev", 0, xc, yc, prec)
+        ctx.call("imps_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.call("imps_hc_apply_dev", 0, xc, yc, prec)
+        ctx.call("imps_synchronize")
+        dtc = (time.perf_counter() - t0) / args.steps
+        fhc = 4.0 * w * D ** 3
+        hc = {"ms": dtc * 1e3, "flop": fhc, "tflops": fhc / dtc / 1e12, "frac_of_fp64_peak": fhc / dtc / 1e12 / PEAKS["fp64"]}
+        ctx.call("imps_dev_free", xc)
+        ctx.call("imps_dev_free", yc)
     krylov = None
     if pname == "fp64":
         # the Krylov solvers' Gram-Schmidt sweeps (HBM-bound) on vectors of the H_AC length
@@ -275,7 +299,8 @@ def hac_throughput(args, imps, _lib, H, pname, D, d, w, N, F):
         # three bf16 MFMA products per FP64-equivalent multiply-add, against the dense bf16 peak
         roof.update({"achieved": 3 * achieved, "frac": 3 * achieved / PEAKS[pname],
                      "unit": "TFLOP/s (bf16 MFMA: 3 products per FP64-equivalent flop)"})
-    return {"value": F / dt / 1e12, "ms_per_step": dt * 1e3, "kernels": kernels, "roofline": roof, "krylov": krylov}
+    return {"value": F / dt / 1e12, "ms_per_step": dt * 1e3, "kernels": kernels, "roofline": roof, "krylov": krylov,
+            "hc": hc}
 
 
 def idmrg_sweeps(args, imps, H, D, N):
