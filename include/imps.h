@@ -118,7 +118,7 @@ int imps_envs_recalculate(imps_ctx* ctx, double tol, int maxiter, int krylovdim)
 /* Drops the environments so the next recalculation starts from zero vectors. */
 int imps_envs_reset(imps_ctx* ctx);
 
-/* leftenv(envs, site) / rightenv(envs, site) (src/environments.jl:230-233):
+/* leftenv(envs, site) / rightenv(envs, site) (src/environments.jl:36-39):
  * (D, w, D) arrays.  imps_set_env overrides one environment (for tests/drivers). */
 int imps_get_env(imps_ctx* ctx, int site, int side, double* out);
 int imps_set_env(imps_ctx* ctx, int site, int side, const double* in);
@@ -261,7 +261,7 @@ int imps_idmrg_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* i
 int imps_tdvp_timestep(imps_ctx* ctx, double dt, double tol, int krylovdim, int maxiter, double gauge_tol,
                        int gauge_maxiter);
 
-/* correlator(ψ, O1, O2, i, js) (src/algorithms/correlators.jl:13-42) on the context's state:
+/* correlator(ψ, O1, O2, i, js) (src/observables/correlators.jl:13-42) on the context's state:
  * out[k] = ⟨ψ|O1[i] O2[js[k]]|ψ⟩ / dot(ψ, ψ) for i < js[0] < js[1] < ..., O1 and O2 d x d
  * column-major on the host.  Every value equals the reference's single-j form; for a range the
  * sites already evaluated are pushed through before the next j. */
@@ -269,7 +269,10 @@ int imps_correlator(imps_ctx* ctx, const double* O1, const double* O2, int i, in
                     double* out);
 /* dot(ψ₁, ψ₂) (src/states/canonicalmps.jl:262-269) with ψ₁ the context's state and ψ₂ given by
  * its AL tensors (host, same unit cell and bond dimension): the dominant eigenvalue of
- * TransferMatrix(ψ₂.AL, ψ₁.AL) from the identity (Arnoldi, tolerance 1e-10). */
+ * TransferMatrix(ψ₂.AL, ψ₁.AL) from the identity (Arnoldi, tolerance 1e-10).  The eigenvalue of
+ * a real transfer matrix may be complex, as the reference's is: out[0] receives its real part and
+ * out[1] its imaginary part (0 for a real dominant eigenvalue).  IMPS_E_SOLVER when the
+ * eigensolver does not converge. */
 int imps_dot_state(imps_ctx* ctx, int nsites, const double* const* AL2, double* out);
 /* dominant_env(ψ; side) and dominant_env(W::DenseIMPO, ψ; side) (src/algorithms/groundstates/
  * envs.jl:71-99, transfer maps 34-68) on the context's state: the dominant (:LM) eigenpair of the

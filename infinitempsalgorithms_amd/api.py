@@ -10,8 +10,10 @@ matrices ``C[a, b]`` (bond right of a site), MPO ``W[wl, u, wr, d]``, environmen
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -20,8 +22,25 @@ from . import _lib
 from ._lib import check, dptr, fortran, ptr_array
 
 
+# Contexts still open when the interpreter exits are closed by an atexit hook, which runs before
+# the interpreter tears modules down and before the C runtime's exit handlers (the HIP runtime's and
+# a profiler's among them), so no stream, event or pinned buffer of the library is released from a
+# finaliser after the runtime has begun to shut down.
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    for c in list(_LIVE):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Context:
-    """One libimps context (a HIP stream plus device-resident state) on `device`."""
+    """One libimps context (a HIP stream plus device-resident state) on `device`.  Close it with
+    close() (or use it as a context manager); contexts left open are closed at interpreter exit."""
 
     def __init__(self, device=0):
         L = _lib.load()
@@ -32,6 +51,14 @@ class Context:
         self.h = h
         self.L = L
         self.N = self.w = self.d = self.D = 0
+        _LIVE.add(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def close(self):
         if self.h is not None and self.h.value:
@@ -231,12 +258,12 @@ def recalculate_(envs: GPUCache, psi: CanonicalIMPS, H: SparseIMPO | None = None
 
 
 def leftenv(envs: GPUCache, site):
-    """leftenv(envs, ℓ) (src/environments.jl:230): (D, w, D), periodic site index."""
+    """leftenv(envs, ℓ) (src/environments.jl:36): (D, w, D), periodic site index."""
     return envs.ctx.get_env(site, _lib.IMPS_SIDE_LEFT)
 
 
 def rightenv(envs: GPUCache, site):
-    """rightenv(envs, ℓ) (src/environments.jl:233)."""
+    """rightenv(envs, ℓ) (src/environments.jl:39)."""
     return envs.ctx.get_env(site, _lib.IMPS_SIDE_RIGHT)
 
 
@@ -404,7 +431,8 @@ def correlator(psi: CanonicalIMPS, O1, O2, i: int, js, envs: GPUCache):
 
 def dot(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
     """dot(ψ₁, ψ₂) (canonicalmps.jl:262-269): dominant eigenvalue of the mixed AL transfer
-    matrix, on the device (ψ₁ becomes the context's state)."""
+    matrix, on the device (ψ₁ becomes the context's state).  A float, or a complex number when the
+    dominant eigenvalue is complex (as the reference returns it)."""
     if len(psi2) != len(psi1) or any(a.shape != b.shape for a, b in zip(psi1.AL, psi2.AL)):
         raise ValueError("dot: the library takes states with the same unit cell, bond and physical "
                          "dimensions")
@@ -413,9 +441,9 @@ def dot(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
         envs.ket = psi1
     AL2 = [fortran(a) for a in psi2.AL]
     pp, keep = ptr_array(AL2)
-    val = ctypes.c_double()
-    envs.ctx.call("imps_dot_state", len(AL2), pp, ctypes.byref(val))
-    return val.value
+    val = np.zeros(2)
+    envs.ctx.call("imps_dot_state", len(AL2), pp, dptr(val))
+    return complex(val[0], val[1]) if val[1] != 0.0 else float(val[0])
 
 
 def norm(psi: CanonicalIMPS):

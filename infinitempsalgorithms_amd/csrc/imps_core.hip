@@ -87,7 +87,12 @@ struct imps_ctx {
   // workspaces and pinned host scratch for reductions
   std::map<std::string, DevBuf> ws;
   double* hpart = nullptr;
-  double* hscal = nullptr;  // pinned scalar readbacks outside the Gram-Schmidt steps (trace_prod)
+  // pinned buffer of kSmallPinned doubles for small readbacks outside the Gram-Schmidt steps
+  // (trace_prod, Cholesky breakdown flags, gradient sums)
+  double* hscal = nullptr;
+  // pinned staging of host<->device copies of caller arrays (allocated on first use), so that no
+  // DMA ever targets pageable memory
+  double* hstage = nullptr;
   double* dpart = nullptr;
   hipEvent_t ev_coef = nullptr;  // marks the arrival of Gram-Schmidt coefficients in hpart
   // counters
@@ -107,11 +112,20 @@ namespace imps {
 void set_global_error(const std::string& msg) { g_create_error = msg; }
 // message for imps_last_error(c)
 void set_context_error(imps_ctx* c, const std::string& msg) { c->err = msg; }
+bool context_model(const imps_ctx* c, int* nsites, int* d) {
+  if (c->N <= 0 || c->w <= 0) return false;
+  *nsites = c->N;
+  *d = c->d;
+  return true;
+}
 }  // namespace imps
 
 namespace {
 
 // ------------------------------------------------------------------ memory helpers
+
+constexpr size_t kSmallPinned = 4096;        // doubles in imps_ctx::hscal
+constexpr size_t kStageDoubles = 1u << 19;   // 4 MiB pinned staging chunk
 
 double* dalloc(size_t n) {
   void* p = nullptr;
@@ -135,13 +149,47 @@ double* ws(imps_ctx* c, const std::string& name, size_t n) {
   return b.p;
 }
 
-void h2d(imps_ctx* c, double* dst, const double* src, size_t n) {
-  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+double* stage(imps_ctx* c) {
+  if (!c->hstage) HIPCHK(hipHostMalloc((void**)&c->hstage, kStageDoubles * sizeof(double), hipHostMallocDefault));
+  return c->hstage;
 }
 
-void d2h(imps_ctx* c, double* dst, const double* src, size_t n) {
-  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+// Caller (pageable) host memory <-> device, through the pinned staging chunk: the host copies
+// into / out of pinned memory and the DMA engine only ever sees pinned pages.  Both return with
+// the copy complete (the staging chunk is reused).
+void h2d_bytes(imps_ctx* c, void* dst, const void* src, size_t bytes) {
+  char* st = (char*)stage(c);
+  const size_t chunk = kStageDoubles * sizeof(double);
+  for (size_t off = 0; off < bytes; off += chunk) {
+    const size_t m = std::min(chunk, bytes - off);
+    HIPCHK(hipStreamSynchronize(c->stream));  // the previous chunk has left the staging buffer
+    std::memcpy(st, (const char*)src + off, m);
+    HIPCHK(hipMemcpyAsync((char*)dst + off, st, m, hipMemcpyHostToDevice, c->stream));
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void d2h_bytes(imps_ctx* c, void* dst, const void* src, size_t bytes) {
+  char* st = (char*)stage(c);
+  const size_t chunk = kStageDoubles * sizeof(double);
+  for (size_t off = 0; off < bytes; off += chunk) {
+    const size_t m = std::min(chunk, bytes - off);
+    HIPCHK(hipMemcpyAsync(st, (const char*)src + off, m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::memcpy((char*)dst + off, st, m);
+  }
+}
+
+void h2d(imps_ctx* c, double* dst, const double* src, size_t n) { h2d_bytes(c, dst, src, n * sizeof(double)); }
+void d2h(imps_ctx* c, double* dst, const double* src, size_t n) { d2h_bytes(c, dst, src, n * sizeof(double)); }
+
+// n <= kSmallPinned doubles from the device into the context's pinned scalar buffer (synchronous);
+// returns the pinned copy
+const double* d2h_small(imps_ctx* c, const void* src, size_t bytes) {
+  require(bytes <= kSmallPinned * sizeof(double), IMPS_E_ARG, "d2h_small: readback too large");
+  HIPCHK(hipMemcpyAsync(c->hscal, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return c->hscal;
 }
 
 void d2d(imps_ctx* c, double* dst, const double* src, size_t n) {
@@ -231,19 +279,26 @@ void cgs2_normalize(imps_ctx* c, size_t n, int nv, const double* V, double* w, d
   cgs2_wait(c, nv, h, beta);
 }
 
-// tr(l v) = sum l[a + ldl*b] v[b + ldv*a] for n x n (contiguous l and v)
-// sum_{a,b} l[a + n b] v[b + n a]: block partials reduced on the device in a fixed order, one
-// double read back into pinned memory (this runs inside regularised GMRES operators, while a
-// speculative Gram-Schmidt readback may be in flight into hpart, hence its own buffer)
+// tr(l v) = sum_{a,b} l[a + n b] v[b + n a] for n x n contiguous l and v: block partials
+// reduced on the device in a fixed order, one double read back into the pinned scalar buffer
+// (not hpart, where a speculative Gram-Schmidt readback may be in flight)
 double trace_prod(imps_ctx* c, int n, const double* l, const double* v) {
   int np = 0;
   const size_t maxp = (size_t)((n + 31) / 32) * ((n + 31) / 32);
   double* part = ws(c, "tp_part", maxp + 1);
-  HIPCHK(launch_trace_prod_parts(n, l, v, part, &np, c->stream));
+  HIPCHK(launch_trace_prod_parts(n, l, n, v, n, part, &np, c->stream));
   HIPCHK(launch_partials_reduce(1, np, part, part + maxp, c->stream));
-  HIPCHK(hipMemcpyAsync(c->hscal, part + maxp, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return c->hscal[0];
+  return d2h_small(c, part + maxp, sizeof(double))[0];
+}
+
+// regularize!(v, l, I) (envs.jl:28-32) with the identity as rvec: v -= tr(l v) I, entirely on the
+// device (two launches, no host round trip), v (n x n, leading dimension ldv), l contiguous
+void regularize_dev(imps_ctx* c, int n, double* v, long ldv, const double* l) {
+  int np = 0;
+  const size_t maxp = (size_t)((n + 31) / 32) * ((n + 31) / 32);
+  double* part = ws(c, "rgd_part", maxp);
+  HIPCHK(launch_trace_prod_parts(n, l, n, v, ldv, part, &np, c->stream));
+  HIPCHK(launch_reduce_add_diag(np, part, -1.0, v, ldv, n, nullptr, c->stream));
 }
 
 // ------------------------------------------------------------------ GEMM wrappers
@@ -491,6 +546,7 @@ void hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec, hipE
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   c->hac_applies++;
 }
+
 
 // Split-BF16 planes path of H_C (identity structural levels, D % 128 == 0): with T1_k = GL_k x,
 // y = T1_{w-1} + sum_{k < w-1} T1_k GR_k.  GEMM 1 makes T1_k for k >= 1 in FP32; T1_0 = x and the
@@ -919,9 +975,13 @@ KInfo lanczos_expm(imps_ctx* c, const Op& op, size_t n, const double* x0, double
 
 // Dominant eigenpair (which = :LM) of a real operator by Arnoldi (KrylovKit.schursolve
 // via gauge_fixedpoint, utility.jl:204-211; eigsolve(..., :LM) via dot, canonicalmps.jl:
-// 262-269); restarts from the dominant Ritz vector.
+// 262-269); restarts from the dominant Ritz vector.  The residual is |beta x_K| with x the unit
+// (complex, for a complex pair) eigenvector of the Hessenberg matrix; for a complex dominant pair
+// xout is the real part of the phase-fixed Ritz vector (a vector of the pair's real invariant
+// plane, from which the next restart spans it), the return value Re(lambda) and *lam_im Im(lambda).
 double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xout,
-                        double tol, int kdim, int maxiter, bool eager, KInfo* info, bool* complex_dominant = nullptr) {
+                        double tol, int kdim, int maxiter, bool eager, KInfo* info, bool* complex_dominant = nullptr,
+                        double* lam_im = nullptr) {
   const int kmax = (int)std::min<size_t>(kdim, n);
   require(kmax >= 1 && kmax < kMaxVec, IMPS_E_ARG, "arnoldi: krylovdim out of range");
   double* V = ws(c, "ar_V", (size_t)(kmax + 1) * n);
@@ -930,7 +990,7 @@ double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, d
   std::vector<double> H((size_t)ldh * kmax), h(kmax + 1), u(kmax);
   int numiter = 1, numops = 0, K = 0;
   bool conv = false;
-  double lam = 0.0, res = 0.0;
+  double lam = 0.0, lim = 0.0, res = 0.0;
   const double nx = nrm2(c, n, x0);
   require(nx > 0 && std::isfinite(nx), IMPS_E_ARG, "arnoldi: zero or non-finite start vector");
   axpby(c, n, 1.0 / nx, x0, 0.0, Vc(0));
@@ -955,9 +1015,12 @@ double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, d
       H[(K + 1) + (size_t)ldh * K] = beta;
       K++;
       if (K == kmax || beta <= tol || eager) {
-        const int hrc = hess_dominant(K, H.data(), ldh, &lam, u.data());
+        double last = 0.0;
+        const int hrc = hess_dominant_c(K, H.data(), ldh, &lam, &lim, u.data(), &last);
+        require(hrc != 2, IMPS_E_SOLVER, "arnoldi: Hessenberg eigenvalue iteration failed");
         if (complex_dominant) *complex_dominant = (hrc == 1);
-        res = std::fabs(beta * u[K - 1]);
+        if (hrc == 0) lim = 0.0;  // a real eigenvalue: drop the solver's rounding-level imaginary part
+        res = std::fabs(beta) * last;
         if (res <= tol) {
           conv = true;
           break;
@@ -977,6 +1040,7 @@ double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, d
     info->numops = numops;
     info->normres = res;
   }
+  if (lam_im) *lam_im = lim;
   return lam;
 }
 
@@ -1096,8 +1160,7 @@ void chol_async(imps_ctx* c, int n, double* G, bool upper, int* fail) {
 
 int read_flag(imps_ctx* c, const int* flag) {
   int h = 0;
-  HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  std::memcpy(&h, d2h_small(c, flag, sizeof(int)), sizeof(int));
   return h;
 }
 
@@ -1416,12 +1479,7 @@ void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* s
     }
     HIPCHK(launch_copy2d(D, D, x, D, slice(G, i), ldg, c->stream));
   };
-  auto regularize_slice = [&](double* v, long ld, const double* l) {
-    double* tmp = ws(c, "env_reg", DD);
-    HIPCHK(launch_copy2d(D, D, v, ld, tmp, D, c->stream));
-    const double cc = trace_prod(c, D, l, tmp);
-    HIPCHK(launch_add_diag(D, -cc, v, ld, c->stream));
-  };
+  auto regularize_slice = [&](double* v, long ld, const double* l) { regularize_dev(c, D, v, ld, l); };
   auto ring_left = [&](const double* v, double* out, double* rh) {
     double* t0 = ws(c, "env_t0", DD);
     double* t1 = ws(c, "env_t1", DD);
@@ -1579,8 +1637,8 @@ void energy_gradient(imps_ctx* c, int site, double* out) {
   gemm2(c, 'N', 'T', D * d, D * w, D, 1.0, c->AC[site], (long)D * d, c->GR[site], (long)D * w, 0.0, T3, (long)D * d);
   HIPCHK(launch_gsand(D, w, d, T1, T3, Gp, c->stream));
   HIPCHK(launch_partials_reduce(wd * wd, D, Gp, G, c->stream));
-  std::vector<double> h((size_t)wd * wd);
-  d2h(c, h.data(), G, h.size());
+  const double* hp = d2h_small(c, G, sizeof(double) * wd * wd);
+  std::vector<double> h(hp, hp + (size_t)wd * wd);
   const double nrm = dot_psi(c);
   for (int i = 0; i < w; ++i)
     for (int u = 0; u < d; ++u)
@@ -1722,7 +1780,7 @@ imps_ctx* worker(imps_ctx* c, int k) {
     c->workers.push_back(w);
     HIPCHK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc((void**)&w->hpart, sizeof(double) * kMaxDotBlocks * kMaxVec, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&w->hscal, sizeof(double) * 8, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&w->hscal, sizeof(double) * kSmallPinned, hipHostMallocDefault));
     w->dpart = dalloc((size_t)kMaxDotBlocks * kMaxVec);
     HIPCHK(hipEventCreateWithFlags(&w->ev_coef, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
@@ -1755,6 +1813,7 @@ void destroy_worker(imps_ctx* w) {
   dfree(w->dpart);
   if (w->hpart) (void)hipHostFree(w->hpart);
   if (w->hscal) (void)hipHostFree(w->hscal);
+  if (w->hstage) (void)hipHostFree(w->hstage);
   if (w->ev_coef) (void)hipEventDestroy(w->ev_coef);
   if (w->ev_done) (void)hipEventDestroy(w->ev_done);
   if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -1846,7 +1905,7 @@ int imps_create(int device, imps_ctx** out) {
     RBCHK(rocblas_create_handle(&c->rb));
     RBCHK(rocblas_set_stream(c->rb, c->stream));
     HIPCHK(hipHostMalloc((void**)&c->hpart, sizeof(double) * kMaxDotBlocks * kMaxVec, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&c->hscal, sizeof(double) * 8, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&c->hscal, sizeof(double) * kSmallPinned, hipHostMallocDefault));
     c->dpart = dalloc((size_t)kMaxDotBlocks * kMaxVec);
     HIPCHK(hipEventCreateWithFlags(&c->ev_coef, hipEventDisableTiming));
   } catch (const ImpsError& e) {
@@ -1873,6 +1932,7 @@ int imps_destroy(imps_ctx* c) {
   dfree(c->dpart);
   if (c->hpart) (void)hipHostFree(c->hpart);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->ev_coef) (void)hipEventDestroy(c->ev_coef);
   if (c->rb) rocblas_destroy_handle(c->rb);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2549,7 +2609,12 @@ int imps_dot_state(imps_ctx* c, int nsites, const double* const* AL2, double* ou
         cur = dst;
       }
     };
-    *out = arnoldi_dominant(c, tm, DD, v0, ev, 1.0e-10, 30, 300, true, nullptr);
+    KInfo ki;
+    double im = 0.0;
+    const double re = arnoldi_dominant(c, tm, DD, v0, ev, 1.0e-10, 30, 300, true, &ki, nullptr, &im);
+    require(ki.converged != 0, IMPS_E_SOLVER, "dot: the transfer-matrix eigensolver did not converge");
+    out[0] = re;
+    out[1] = im;
   });
 }
 
@@ -2885,18 +2950,12 @@ int imps_dev_free(imps_ctx* c, void* d_ptr) {
 
 int imps_memcpy_h2d(imps_ctx* c, void* d_dst, const void* h_src, size_t bytes) {
   if (!c) return IMPS_E_ARG;
-  return run_guarded(c, [&] {
-    HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-  });
+  return run_guarded(c, [&] { h2d_bytes(c, d_dst, h_src, bytes); });
 }
 
 int imps_memcpy_d2h(imps_ctx* c, void* h_dst, const void* d_src, size_t bytes) {
   if (!c) return IMPS_E_ARG;
-  return run_guarded(c, [&] {
-    HIPCHK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-  });
+  return run_guarded(c, [&] { d2h_bytes(c, h_dst, d_src, bytes); });
 }
 
 int imps_synchronize(imps_ctx* c) {

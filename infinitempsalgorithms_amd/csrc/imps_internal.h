@@ -116,9 +116,13 @@ hipError_t launch_set_identity(int n, double* x, long ld, hipStream_t s);
 // out (cols x rows) = in^T, in (rows x cols) column-major with leading dims
 hipError_t launch_transpose(int rows, int cols, const double* in, long ldin, double* out,
                             long ldout, hipStream_t s);
-// partial sums of sum_{a,b} l[a + n*b] * v[b + n*a]
-hipError_t launch_trace_prod_parts(int n, const double* l, const double* v, double* part,
+// partial sums of sum_{a,b} l[a + ldl*b] * v[b + ldv*a] (n x n), one per 32 x 32 tile
+hipError_t launch_trace_prod_parts(int n, const double* l, long ldl, const double* v, long ldv, double* part,
                                    int* nparts, hipStream_t s);
+// c = sum_{b < np} part[b] on the device (fixed order), then x[i + ld*i] += factor * c for i < n
+// and *out = c when out != nullptr (one workgroup)
+hipError_t launch_reduce_add_diag(int np, const double* part, double factor, double* x, long ld, int n, double* out,
+                                  hipStream_t s);
 // column/row sign fix for QRpos: Q[:,i] *= s_i, R[i,:] *= s_i with s_i = sign(R[i,i]);
 // sbuf: n doubles of device scratch
 hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr, double* sbuf,
@@ -147,6 +151,8 @@ hipError_t launch_fill_random(size_t n, uint64_t seed, double* x, hipStream_t s)
 void set_global_error(const std::string& msg);
 // message returned by imps_last_error(c)
 void set_context_error(imps_ctx* c, const std::string& msg);
+// unit cell and physical dimension of the context's model; false when no model is set
+bool context_model(const imps_ctx* c, int* nsites, int* d);
 
 // ---- host small dense linear algebra (imps_small.cpp) ----
 // Symmetric eigen-decomposition of the n x n column-major A (destroyed): eigenvalues
@@ -156,8 +162,11 @@ void sym_eig(int n, double* A, double* w, double* V);
 // Returns 0 on success.
 int hess_eigvals(int n, const double* H, int ldh, double* wr, double* wi);
 // Dominant (largest modulus) eigenpair of the n x n upper-Hessenberg H.  For a complex
-// dominant pair the real part of the phase-fixed eigenvector is returned.  u is unit norm.
-// Returns 0 on success, 1 if the dominant value is complex.
+// dominant pair the real part of the phase-fixed eigenvector is returned and *lam its real part.
+// u is unit norm.  Returns 0 on success, 1 if the dominant value is complex.
 int hess_dominant(int n, const double* H, int ldh, double* lam, double* u);
+// The same with the eigenvalue's imaginary part (lam_im, may be NULL) and |x_{n-1}| of the unit
+// complex eigenvector x (last, may be NULL; the Arnoldi residual is |beta| times it).
+int hess_dominant_c(int n, const double* H, int ldh, double* lam_re, double* lam_im, double* u, double* last);
 
 }  // namespace imps

@@ -306,6 +306,20 @@ int imps_resume_state(imps_ctx* ctx, const char* path) {
       imps::set_context_error(ctx, imps_last_error(nullptr));
       return st;
     }
+    // the file must fit the context's model before any buffer is sized from it: imps_set_state
+    // reads D*d*D doubles per tensor with the model's d
+    int mN = 0, md = 0;
+    if (!imps::context_model(ctx, &mN, &md)) {
+      imps::set_context_error(ctx, "imps_resume_state: MPO not set");
+      return IMPS_E_STATE;
+    }
+    if (N != mN || d != md) {
+      imps::set_context_error(ctx, "imps_resume_state: the file's unit cell (" + std::to_string(N) +
+                                       ") or physical dimension (" + std::to_string(d) +
+                                       ") differs from the model's (" + std::to_string(mN) + ", " +
+                                       std::to_string(md) + ")");
+      return IMPS_E_ARG;
+    }
     const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
     std::vector<std::vector<double>> AL(N, std::vector<double>(nA)), AR = AL, AC = AL;
     std::vector<std::vector<double>> C(N, std::vector<double>(nC));

@@ -1120,23 +1120,23 @@ hipError_t launch_transpose(int rows, int cols, const double* in, long ldin, dou
   return hipGetLastError();
 }
 
-// sum_{a,b} l[a + n*b] * v[b + n*a], one partial per 32 x 32 tile
-__global__ void trace_prod_kernel(int n, const double* __restrict__ l, const double* __restrict__ v,
-                                  double* __restrict__ part) {
+// sum_{a,b} l[a + ldl*b] * v[b + ldv*a], one partial per 32 x 32 tile
+__global__ void trace_prod_kernel(int n, const double* __restrict__ l, long ldl, const double* __restrict__ v,
+                                  long ldv, double* __restrict__ part) {
   __shared__ double vt[32][33];
   __shared__ double red[4];
   const int a0 = blockIdx.x * 32, b0 = blockIdx.y * 32;
   const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
-  // v[b + n*a] for b in tile rows (contiguous), a in tile columns
+  // v[b + ldv*a] for b in tile rows (contiguous), a in tile columns
   for (int k = ty; k < 32; k += 8) {
     const int b = b0 + tx, a = a0 + k;
-    vt[k][tx] = (a < n && b < n) ? v[b + (long)n * a] : 0.0;  // vt[a-a0][b-b0]
+    vt[k][tx] = (a < n && b < n) ? v[b + ldv * a] : 0.0;  // vt[a-a0][b-b0]
   }
   __syncthreads();
   double acc = 0.0;
   for (int k = ty; k < 32; k += 8) {
     const int a = a0 + tx, b = b0 + k;
-    if (a < n && b < n) acc += l[a + (long)n * b] * vt[tx][k];
+    if (a < n && b < n) acc += l[a + ldl * b] * vt[tx][k];
   }
   const int t = ty * 32 + tx;
   acc = wave_sum(acc);
@@ -1145,11 +1145,36 @@ __global__ void trace_prod_kernel(int n, const double* __restrict__ l, const dou
   if (t == 0) part[blockIdx.x + gridDim.x * blockIdx.y] = red[0] + red[1] + red[2] + red[3];
 }
 
-hipError_t launch_trace_prod_parts(int n, const double* l, const double* v, double* part,
+hipError_t launch_trace_prod_parts(int n, const double* l, long ldl, const double* v, long ldv, double* part,
                                    int* nparts, hipStream_t s) {
   dim3 grid((n + 31) / 32, (n + 31) / 32);
   *nparts = (int)(grid.x * grid.y);
-  hipLaunchKernelGGL(trace_prod_kernel, grid, dim3(32, 8), 0, s, n, l, v, part);
+  hipLaunchKernelGGL(trace_prod_kernel, grid, dim3(32, 8), 0, s, n, l, ldl, v, ldv, part);
+  return hipGetLastError();
+}
+
+// One workgroup: c = sum_b part[b] (the fixed order of partials_reduce_kernel), then
+// x[i + ld*i] += factor * c for i < n; c is stored to *out when out != nullptr.
+__global__ void __launch_bounds__(256) reduce_add_diag_kernel(int np, const double* __restrict__ part, double factor,
+                                                             double* __restrict__ x, long ld, int n,
+                                                             double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < np; b += 256) s += part[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double c = red[0];
+  if (out != nullptr && threadIdx.x == 0) *out = c;
+  for (int i = threadIdx.x; i < n; i += 256) x[(size_t)i * ld + i] += factor * c;
+}
+
+hipError_t launch_reduce_add_diag(int np, const double* part, double factor, double* x, long ld, int n, double* out,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(reduce_add_diag_kernel, dim3(1), dim3(256), 0, s, np, part, factor, x, ld, n, out);
   return hipGetLastError();
 }
 

@@ -191,7 +191,7 @@ static void shifted_solve(int n, const double* H, int ldh, cplx lam, std::vector
   }
 }
 
-int hess_dominant(int n, const double* H, int ldh, double* lam_out, double* u) {
+int hess_dominant_c(int n, const double* H, int ldh, double* lam_re, double* lam_im, double* u, double* last) {
   std::vector<double> wr(n), wi(n);
   if (hess_eigvals(n, H, ldh, wr.data(), wi.data()) != 0) return 2;
   int best = 0;
@@ -212,6 +212,8 @@ int hess_dominant(int n, const double* H, int ldh, double* lam_out, double* u) {
     nr = std::sqrt(nr);
     for (int i = 0; i < n; ++i) x[i] /= nr;
   }
+  // |x_{n-1}| of the unit (complex) eigenvector: the Arnoldi residual is |beta| times this
+  if (last) *last = std::abs(x[n - 1]);
   int k = 0;
   for (int i = 1; i < n; ++i)
     if (std::abs(x[i]) > std::abs(x[k])) k = i;
@@ -223,8 +225,13 @@ int hess_dominant(int n, const double* H, int ldh, double* lam_out, double* u) {
   }
   nr = std::sqrt(nr);
   for (int i = 0; i < n; ++i) u[i] /= nr;
-  *lam_out = lam.real();
+  *lam_re = lam.real();
+  if (lam_im) *lam_im = lam.imag();
   return (std::fabs(lam.imag()) > 1e-12 * std::max(1.0, std::abs(lam))) ? 1 : 0;
+}
+
+int hess_dominant(int n, const double* H, int ldh, double* lam_out, double* u) {
+  return hess_dominant_c(n, H, ldh, lam_out, nullptr, u, nullptr);
 }
 
 }  // namespace imps

@@ -449,18 +449,23 @@ def lanczos_eigsolve(op, x0, tol, krylovdim=Defaults.krylovdim, maxiter=Defaults
 
 
 def _dominant_ritz(H):
-    """Dominant (largest magnitude) eigenpair of a small matrix; real part only."""
+    """Dominant (largest modulus) eigenpair of a small real matrix.  Returns (lam, u, last): lam
+    the eigenvalue (complex for a complex dominant pair, else float), u the real part of the
+    phase-fixed unit eigenvector x, normalised (a vector of the pair's real invariant plane when
+    the pair is complex), and last = |x[-1]|, so that the Arnoldi residual is |beta| * last."""
     lam, U = np.linalg.eig(H)
     i = int(np.argmax(np.abs(lam)))
-    u = U[:, i]
-    if abs(lam[i].imag) > 1e-12 * max(1.0, abs(lam[i])):
-        warnings.warn("dominant Ritz value is complex; taking the real part")
+    x = U[:, i] / np.linalg.norm(U[:, i])
+    last = float(abs(x[-1]))
     # rotate the phase so the largest entry is real, then take the real part
-    k = int(np.argmax(np.abs(u)))
-    u = u * (abs(u[k]) / u[k])
-    u = np.real(u)
+    k = int(np.argmax(np.abs(x)))
+    x = x * (abs(x[k]) / x[k])
+    u = np.real(x)
     u = u / np.linalg.norm(u)
-    return float(np.real(lam[i])), u
+    li = lam[i]
+    if abs(li.imag) <= 1e-12 * max(1.0, abs(li)):
+        return float(np.real(li)), u, last
+    return complex(li), u, last
 
 
 def arnoldi_dominant(op, x0, tol, krylovdim=Defaults.krylovdim, maxiter=Defaults.maxiter,
@@ -471,7 +476,9 @@ def arnoldi_dominant(op, x0, tol, krylovdim=Defaults.krylovdim, maxiter=Defaults
     ``gauge_fixedpoint`` (src/utility.jl:204-211) and KrylovKit.eigsolve(..., :LM;
     ishermitian=false) as called from ``dot(psi1, psi2)`` (src/states/canonicalmps.jl:262-269).
     For howmany = 1 with a real dominant eigenvalue the first Schur vector is the
-    normalised eigenvector.
+    normalised eigenvector.  A complex dominant pair is returned as a complex eigenvalue (as
+    KrylovKit's eigsolve returns it) with the real part of the Ritz vector; restarts from that
+    vector span the pair's invariant plane.
     """
     shape = x0.shape
     n = x0.size
@@ -496,8 +503,8 @@ def arnoldi_dominant(op, x0, tol, krylovdim=Defaults.krylovdim, maxiter=Defaults
             if beta > 0:
                 V[:, K] = w / beta
             if K == kmax or beta <= tol or eager:
-                lam, u = _dominant_ritz(H[:K, :K])
-                res = abs(beta * u[K - 1])
+                lam, u, last = _dominant_ritz(H[:K, :K])
+                res = abs(beta) * last
                 if res <= tol:
                     conv = True
                     break
@@ -754,11 +761,11 @@ class Envs:
     stats: dict = field(default_factory=dict)
 
     def leftenv(self, l):
-        """src/environments.jl:230."""
+        """src/environments.jl:36."""
         return self.lefts[l % len(self.lefts)]
 
     def rightenv(self, l):
-        """src/environments.jl:233."""
+        """src/environments.jl:39."""
         return self.rights[l % len(self.rights)]
 
 
@@ -919,7 +926,8 @@ def recalculate(envs, psi, Ws, tol):
 
 def dot_psi(psi1, psi2, krylovdim=Defaults.krylovdim):
     """dot(psi1, psi2): dominant eigenvalue of TransferMatrix(psi2.AL, psi1.AL),
-    canonicalmps.jl:262-269 (Arnoldi, :LM, tol = Defaults.tol)."""
+    canonicalmps.jl:262-269 (Arnoldi, :LM, tol = Defaults.tol); complex when the dominant
+    eigenvalue of the real transfer matrix is complex."""
     D1 = psi1.AL[0].shape[0]
     D2 = psi2.AL[0].shape[0]
     v0 = np.eye(D1, D2)
@@ -1308,7 +1316,7 @@ def time_evolve(psi0, Ws, t_span, alg=None, envs=None, observer=None):
 
 
 # ---------------------------------------------------------------------------
-# Observables: correlator, fidelity (src/algorithms/correlators.jl, src/states/canonicalmps.jl)
+# Observables: correlator, fidelity (src/observables/correlators.jl, src/states/canonicalmps.jl)
 # ---------------------------------------------------------------------------
 
 
@@ -1385,6 +1393,8 @@ def dominant_env(psi, W=None, side="left", tol=Defaults.tol, krylovdim=Defaults.
                     X = push_env_right_mpo(X, W[l % len(W)], psi.AR[l])
             return X
     lam, x, _ = arnoldi_dominant(op, np.ones((D, w, D)), tol, krylovdim, maxiter, eager=True)
+    if isinstance(lam, complex):
+        raise ValueError("dominant_env: the dominant eigenvalue is complex (real arithmetic only)")
     x = x / np.linalg.norm(x)
     if x.sum() < 0:
         x = -x

@@ -715,11 +715,24 @@ def test_checkpoint_and_resume_state(imps, ctx, oracle, tmp_path):
         c2.close()
     c3 = imps.Context(0)
     try:
+        with pytest.raises(imps._lib.ImpsError, match="MPO not set"):
+            c3.resume(path)
         c3.set_mpo(imps.SparseIMPO([W]))
-        with pytest.raises(imps._lib.ImpsError):
+        with pytest.raises(imps._lib.ImpsError, match="unit cell"):
             c3.resume(path)
         with pytest.raises(imps._lib.ImpsError, match="cannot open"):
             c3.resume(tmp_path / "missing.imps")
+        # a d = 2 file into a d = 4 model (and the reverse) is refused before any buffer is sized
+        c3.set_mpo(imps.models.fermi_hubbard(t=1.0, U=4.0, mu=2.0, nsites=2))
+        with pytest.raises(imps._lib.ImpsError, match="physical dimension"):
+            c3.resume(path)
+        hub = oracle.randomimps(6, [4, 4], 42)
+        c3.set_state(imps.CanonicalIMPS(hub.AL, hub.AR, hub.C, hub.AC))
+        path4 = tmp_path / "ck4.imps"
+        c3.checkpoint(path4)
+        c3.set_mpo(H)
+        with pytest.raises(imps._lib.ImpsError, match="physical dimension"):
+            c3.resume(path4)
     finally:
         c3.close()
 

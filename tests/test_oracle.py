@@ -462,6 +462,51 @@ def test_correlator_and_fidelity(oracle):
     assert 0.0 <= f12 < 0.999 and abs(f12 - f21) < 1e-10
 
 
+def _explicit_mixed_transfer(oracle, psi1, psi2):
+    # TransferMatrix(psi2.AL, psi1.AL) as an explicit D^2 x D^2 matrix, built column by column
+    # from single-site pushes of unit vectors
+    D = psi1.AL[0].shape[0]
+    T = np.eye(D * D)
+    for a1, a2 in zip(psi1.AL, psi2.AL):
+        M = np.column_stack([oracle.transfer_left([a2], [a1], e) for e in np.eye(D * D)])
+        T = M @ T
+    return T
+
+
+@pytest.mark.parametrize("N,D,s1,s2", [(1, 4, 1003, 2003), (2, 3, 1001, 2001), (2, 6, 8, 9), (1, 5, 40, 41)])
+def test_dot_matches_explicit_transfer_matrix(oracle, N, D, s1, s2):
+    # dot(psi1, psi2) (canonicalmps.jl:262-269) is the dominant eigenvalue of the mixed transfer
+    # matrix; the first two pairs have a complex dominant pair, which dot must return as the
+    # complex eigenvalue (fidelity |lambda|), the others a real one
+    psi1, psi2 = oracle.randomimps(D, [2] * N, s1), oracle.randomimps(D, [2] * N, s2)
+    ev = np.linalg.eigvals(_explicit_mixed_transfer(oracle, psi1, psi2))
+    lam = ev[np.argmax(np.abs(ev))]
+    got = oracle.dot_psi(psi1, psi2)
+    if abs(lam.imag) > 1e-12:
+        assert isinstance(got, complex)
+        assert abs(got.real - lam.real) < 1e-10 and abs(abs(got.imag) - abs(lam.imag)) < 1e-10
+    else:
+        assert abs(got - lam.real) < 1e-10
+    assert abs(oracle.fidelity(psi1, psi2) - abs(lam)) < 1e-10
+
+
+def test_arnoldi_complex_dominant_pair(oracle):
+    # a real operator whose dominant eigenvalues are the pair 1.2 exp(+-0.7i): Arnoldi returns
+    # the complex value, converges, and its vector lies in the pair's invariant plane
+    rng = np.random.default_rng(12)
+    n = 50
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    B = np.diag(0.8 * rng.uniform(-1, 1, n))
+    r, th = 1.2, 0.7
+    B[:2, :2] = [[r * np.cos(th), -r * np.sin(th)], [r * np.sin(th), r * np.cos(th)]]
+    A = Q @ B @ Q.T
+    lam, v, info = oracle.arnoldi_dominant(lambda x: A @ x, rng.standard_normal(n), 1e-12, krylovdim=12)
+    assert info.converged and isinstance(lam, complex)
+    assert abs(lam.real - r * np.cos(th)) < 1e-10 and abs(abs(lam.imag) - r * np.sin(th)) < 1e-10
+    P = Q[:, :2] @ Q[:, :2].T
+    assert np.linalg.norm(v - P @ v) < 1e-9 * np.linalg.norm(v)
+
+
 def test_dominant_env_closed_forms_and_brute_force(oracle):
     # identity channel (envs.jl:79-99): AL is left- and AR right-orthonormal, so the fixed points
     # are lambda = 1 with L = R = I / sqrt(D) (the reference test's |lambda - 1| and norm checks)
@@ -521,8 +566,13 @@ def test_dense_envs_and_expectation_identities(oracle):
     psi2 = oracle.canonical_from_tensors([psi1.AL[0], psi1.AL[0]])
     E2 = oracle.expectation_dense(psi2, [Wd, Wd])
     assert abs(E2 - 2 * lam) < 1e-8 * abs(lam)
-    psi = oracle.randomimps(6, [2, 2], 32)
+    # two-site cell with a real dominant eigenvalue (-0.657 against 0.405 next, explicit spectrum);
+    # with the state of seed 32 the same channel's dominant pair is complex (-0.350 +- 0.722i),
+    # which the real-valued dominant environment refuses
     Ws = [Wd, np.asfortranarray(Wd[:, ::-1, :, :])]
+    with pytest.raises(ValueError, match="complex"):
+        oracle.expectation_dense(oracle.randomimps(6, [2, 2], 32), Ws)
+    psi = oracle.randomimps(6, [2, 2], 38)
     E = oracle.expectation_dense(psi, Ws)
     assert abs(oracle.expectation_dense(psi, [2.5 * x for x in Ws]) - 2.5 * E) < 1e-8 * abs(E)
     assert abs(oracle.expectation_dense(psi, [np.eye(2).reshape(1, 2, 1, 2)] * 2) - 2.0) < 1e-10
