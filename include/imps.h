@@ -94,6 +94,17 @@ int imps_set_state_raw(imps_ctx* ctx, int nsites, int D, const double* const* A,
  * GR = rightenv(site).  x, y: (D, d, D). */
 int imps_hac_apply(imps_ctx* ctx, int site, const double* x, double* y, int prec);
 int imps_hac_apply_dev(imps_ctx* ctx, int site, const double* d_x, double* d_y, int prec);
+/* No counterpart in the reference: imps_hac_apply_dev for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * calls no hipMalloc/hipFree and no allocating library routine: the H_AC intermediates and the
+ * GEMM split-K workspace live in the library's file-scope device scratch of 2^24 doubles (128 MiB,
+ * one copy on each GPU the library runs on, held until the process ends), and GEMM tiles are the
+ * autotuned ones already known or the fixed heuristic (never newly tuned), so the result equals
+ * imps_hac_apply_dev's bit for bit.  Limit: 2 D w d D plus the split-K workspace must fit the scratch
+ * (e.g. D = 512 with w d = 24), else IMPS_E_ARG.  prec: FP64 or FP32 (IMPS_E_UNSUPPORTED for
+ * split-BF16).  Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (several contexts or streams at once) race. */
+int imps_hac_apply_noalloc(imps_ctx* ctx, int site, const double* d_x, double* d_y, int prec);
 
 /* Replaces C_hamiltonian(site, ...)(x) (effective.jl:52-54, 72-76):
  * y[α,β] = Σ GL[α,w,α'] x[α',β'] GR[β',w,β] with GL = leftenv(site+1),

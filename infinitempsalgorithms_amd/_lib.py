@@ -92,6 +92,7 @@ SIGNATURES = [
     ("imps_set_state_raw", c_int, [c_void_p, c_int, c_int, PP_double, c_double, c_int]),
     ("imps_hac_apply", c_int, [c_void_p, c_int, P_double, P_double, c_int]),
     ("imps_hac_apply_dev", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
+    ("imps_hac_apply_noalloc", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     ("imps_hc_apply", c_int, [c_void_p, c_int, P_double, P_double, c_int]),
     ("imps_hc_apply_dev", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     ("imps_transfer_apply", c_int, [c_void_p, c_int, P_double, P_double]),

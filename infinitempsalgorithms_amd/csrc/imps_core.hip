@@ -107,6 +107,13 @@ struct imps_ctx {
 
 static thread_local std::string g_create_error;
 
+// Device scratch of imps_hac_apply_noalloc (T1, T2 and the split-K workspace of its GEMMs), for
+// callers that cannot allocate: every GPU the library runs on holds one copy, 128 MiB, until the
+// process ends.  Host code never uses the symbol's name as an address: the per-device address
+// comes from hipGetSymbolAddress (scratch_ptr) and only that pointer reaches kernels.
+constexpr size_t kScratchDoubles = (size_t)1 << 24;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace imps {
 // message for imps_last_error(NULL): errors of calls that have no context
 void set_global_error(const std::string& msg) { g_create_error = msg; }
@@ -371,6 +378,41 @@ int tuned_variant(imps_ctx* c, const GemmDesc& g, int prec, const int* ids, int 
   return best;
 }
 
+// the autotuned variant of a GEMM class when one is already known, else -1 (never tunes)
+int cached_variant(const GemmDesc& g, int prec) {
+  const int align = (((uintptr_t)g.A & 15) ? 1 : 0) | (((uintptr_t)g.B & 15) ? 2 : 0);
+  const TuneKey key(prec, g.M, g.N, g.K, g.Kc, g.sAm, g.sAk, g.sAc, g.sBk, g.sBn, g.sBc, align);
+  TuneTable& tt = tune_table();
+  std::lock_guard<std::mutex> lk(tt.mu);
+  auto it = tt.best.find(key);
+  return it == tt.best.end() ? -1 : it->second;
+}
+
+// GEMM without any allocation: the split-K workspace (when the shape takes one) is W, sized by
+// the caller with gemm_workspace; the tile is a forced variant, the autotuned one when it is already
+// known, or the fixed heuristic -- all of which sum over K in the same order, so the result equals
+// gemm()'s bit for bit.
+void gemm_prealloc(imps_ctx* c, const GemmDesc& g0, int prec, double* W) {
+  GemmDesc g = g0;
+  g.W = gemm_workspace(g.M, g.N, g.K) > 0 ? W : nullptr;
+  const int mode = g_gemm_autotune.load(std::memory_order_relaxed);
+  if (mode != 0) {
+    int ids[8];
+    const int nv = gemm_tile_variants(g, prec, ids);
+    if (nv > 1) {
+      int v = -1;
+      for (int k = 0; k < nv && mode >= 2; ++k)
+        if (ids[k] == mode - 2) v = ids[k];
+      if (v < 0) v = cached_variant(g, prec);
+      if (v >= 0) {
+        HIPCHK(launch_gemm_variant(g, prec, v, c->stream));
+        return;
+      }
+    }
+  }
+  HIPCHK(launch_gemm(g, prec, c->stream));
+}
+
 void gemm(imps_ctx* c, const GemmDesc& g0, int prec) {
   GemmDesc g = g0;
   const size_t nw = gemm_workspace(g.M, g.N, g.K);
@@ -497,18 +539,39 @@ void hac_apply_planes(imps_ctx* c, int site, const double* x, double* y, hipEven
 // level-(w-1) channel straight into y and GEMM 2 accumulates the other levels onto it, so
 // both GEMMs shrink by one level, 4 (w-1) d D^3 flop instead of 4 w d D^3, same result.
 // `ev` (optional, 4 events) brackets the three kernels for timing.
+// Buffers of the GEMM path of H_AC: T1 and T2 (D w d D doubles each) and, when `fixed`, the
+// split-K workspace W (gemm_workspace doubles of the larger GEMM, may be null when neither takes
+// one) with GEMMs that allocate nothing (gemm_prealloc); otherwise the GEMMs are gemm()'s.
+struct HacBufs {
+  double* T1;
+  double* T2;
+  double* W;
+  bool fixed;
+};
+
+void hac_apply_bufs(imps_ctx* c, int site, const double* x, double* y, int prec, const HacBufs& b,
+                    hipEvent_t* ev);
+
 void hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec, hipEvent_t* ev = nullptr) {
   const int D = c->D, w = c->w, d = c->d;
   site = wrap(c, site);
-  const double* GL = c->GL[site];
-  const double* GR = c->GR[site];
-  double* T1 = ws(c, "hac_T1", (size_t)D * w * d * D);
-  double* T2 = ws(c, "hac_T2", (size_t)D * w * d * D);
-  const bool skip = w >= 2 && c->gl_id[site] && c->gr_id[site];
   if (prec == PREC_BF16X3 && hac_planes_ok(c, site)) {
     hac_apply_planes(c, site, x, y, ev);
     return;
   }
+  HacBufs b{ws(c, "hac_T1", (size_t)D * w * d * D), ws(c, "hac_T2", (size_t)D * w * d * D), nullptr, false};
+  hac_apply_bufs(c, site, x, y, prec, b, ev);
+}
+
+// the GEMM -> W mix -> GEMM chain of hac_apply on the buffers b (site already wrapped)
+void hac_apply_bufs(imps_ctx* c, int site, const double* x, double* y, int prec, const HacBufs& b,
+                    hipEvent_t* ev) {
+  const int D = c->D, w = c->w, d = c->d;
+  const double* GL = c->GL[site];
+  const double* GR = c->GR[site];
+  double* T1 = b.T1;
+  double* T2 = b.T2;
+  const bool skip = w >= 2 && c->gl_id[site] && c->gr_id[site];
   const int lv = skip ? 1 : 0;  // levels handled as copies at each end
   GemmDesc g1;
   g1.M = D * (w - lv);
@@ -524,7 +587,10 @@ void hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec, hipE
   g1.C = T1 + (size_t)D * lv;
   g1.ldc = (long)D * w;
   if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
-  gemm(c, g1, prec);
+  if (b.fixed)
+    gemm_prealloc(c, g1, prec, b.W);
+  else
+    gemm(c, g1, prec);
   if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
   HIPCHK(launch_wmix(D, w, d, c->Wm[site], T1, T2, skip ? x : nullptr, skip ? y : nullptr, c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
@@ -542,11 +608,31 @@ void hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec, hipE
   g2.C = y;
   g2.ldc = (long)D * d;
   g2.beta = skip ? 1.0 : 0.0;
-  gemm(c, g2, prec);
+  if (b.fixed)
+    gemm_prealloc(c, g2, prec, b.W);
+  else
+    gemm(c, g2, prec);
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   c->hac_applies++;
 }
 
+// Device address of this GPU's copy of `scratch`, taken once per device with that device current
+// (imps_create, or the first no-allocation call on a device) and kept; -- the symbol's name is a
+// host-side stand-in and is never passed to a kernel.
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[64] = {};
+
+double* scratch_ptr(int device) {
+  require(device >= 0 && device < 64, IMPS_E_ARG, "scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, IMPS_E_HIP, "scratch: hipGetSymbolAddress returned no address");
+    g_scratch_ptr[device] = (double*)p;
+  }
+  return g_scratch_ptr[device];
+}
 
 // Split-BF16 planes path of H_C (identity structural levels, D % 128 == 0): with T1_k = GL_k x,
 // y = T1_{w-1} + sum_{k < w-1} T1_k GR_k.  GEMM 1 makes T1_k for k >= 1 in FP32; T1_0 = x and the
@@ -1908,6 +1994,7 @@ int imps_create(int device, imps_ctx** out) {
     HIPCHK(hipHostMalloc((void**)&c->hscal, sizeof(double) * kSmallPinned, hipHostMallocDefault));
     c->dpart = dalloc((size_t)kMaxDotBlocks * kMaxVec);
     HIPCHK(hipEventCreateWithFlags(&c->ev_coef, hipEventDisableTiming));
+    (void)scratch_ptr(device);  // this device's scratch address, for imps_hac_apply_noalloc
   } catch (const ImpsError& e) {
     g_create_error = e.what();
     if (c->rb) rocblas_destroy_handle(c->rb);
@@ -2087,6 +2174,28 @@ int imps_hac_apply_dev(imps_ctx* c, int site, const double* d_x, double* d_y, in
     need_match(c);
     need_envs(c);
     hac_apply(c, site, d_x, d_y, prec);
+  });
+}
+
+int imps_hac_apply_noalloc(imps_ctx* c, int site, const double* d_x, double* d_y, int prec) {
+  if (!c || !d_x || !d_y) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    require(prec == PREC_FP64 || prec == PREC_FP32, IMPS_E_UNSUPPORTED,
+            "imps_hac_apply_noalloc: FP64 and FP32 only (the split-BF16 path needs per-call plane buffers)");
+    site = wrap(c, site);
+    const int D = c->D, w = c->w, d = c->d;
+    const int lv = (w >= 2 && c->gl_id[site] && c->gr_id[site]) ? 1 : 0;
+    // T1 and T2 rounded up to 256 bytes, so T2 and the workspace keep the buffers' alignment
+    const size_t nT = (((size_t)D * w * d * D) + 31) & ~(size_t)31;
+    const size_t nW = std::max(gemm_workspace(D * (w - lv), d * D, D), gemm_workspace(D * d, D, D * (w - lv)));
+    const size_t need = 2 * nT + nW;
+    require(need <= kScratchDoubles, IMPS_E_ARG,
+            "imps_hac_apply_noalloc: needs " + std::to_string(need) + " doubles of scratch (2 D w d D plus the "
+            "split-K workspace); the maximum is " + std::to_string(kScratchDoubles) + " (128 MiB)");
+    double* S = scratch_ptr(c->device);
+    hac_apply_bufs(c, site, d_x, d_y, prec, HacBufs{S, S + nT, nW ? S + 2 * nT : nullptr, true}, nullptr);
   });
 }
 
