@@ -14,7 +14,13 @@ Also reported:
                 per-application cost inside an eigensolve, kernels and roofline;
   vumps         seconds per VUMPS iteration (and its phases) for a few iterations at
                 D = 1024 from a random canonical state (disable with --no-vumps);
-  time_to_galerkin  with --ttg: the full run to Galerkin < 1e-8 (long);
+  time_to_galerkin  (default; --no-ttg skips it) wall seconds of VUMPS from a random canonical
+                state (PCG64 seed 2, canonicalised on the device), FP64 against the mixed bulk
+                arithmetics with FP64 refinement: full runs to Galerkin < 1e-8 at D = 128 (a bond
+                dimension at which all three converge for this model) and the time to Galerkin
+                <= 1e-4 at D = 1024 (where the iteration settles into a limit cycle near 2e-5
+                instead of converging);
+  time_to_galerkin_long  with --ttg: the run at --D to Galerkin < 1e-8 (long);
   idmrg         with --idmrg K: seconds per single-site IDMRG sweep over K sweeps;
   tdvp          with --tdvp K: seconds per imaginary-time TDVP step over K steps;
   cpu_baseline  the FP64 CPU restatement of H_AC (numpy / OpenBLAS) on a bounded sample.
@@ -92,6 +98,7 @@ def main():
     ap.add_argument("--prec", default="fp64", choices=sorted(PREC))
     ap.add_argument("--D", type=int, default=1024)
     ap.add_argument("--no-vumps", action="store_true", help="skip the VUMPS iteration timing")
+    ap.add_argument("--no-ttg", action="store_true", help="skip the default time-to-Galerkin runs")
     ap.add_argument("--vumps-only", action="store_true", help="only the VUMPS iterations (for kernel traces)")
     ap.add_argument("--sweep", action="store_true",
                     help="batched parameter sweep: 64 TFIM instances, h in U[0.5,1.5], D in {32,64,128,256}")
@@ -102,6 +109,11 @@ def main():
     ap.add_argument("--ttg", action="store_true", help="run VUMPS to Galerkin < 1e-8 (long)")
     ap.add_argument("--ttg-maxiter", type=int, default=2000)
     ap.add_argument("--ttg-seconds", type=float, default=1e9, help="stop the --ttg run after this wall time")
+    ap.add_argument("--stop-below", type=float, default=0.0,
+                    help="VUMPS runs: stop once the Galerkin error is at or below this value")
+    ap.add_argument("--state-out", default="", help="VUMPS runs: write the final state to this state file")
+    ap.add_argument("--state-in", default="",
+                    help="VUMPS runs: start from this state file instead of the random canonical state")
     ap.add_argument("--refine-below", type=float, default=1e-6, help="mixed precision: FP64 below this Galerkin error")
     ap.add_argument("--idmrg", type=int, default=0, help="also time this many single-site IDMRG sweeps")
     ap.add_argument("--tdvp", type=int, default=0, help="also time this many imaginary-time TDVP steps (dt = -0.05)")
@@ -159,6 +171,9 @@ def main():
                        if D == 1024 else f"H_AC apply, JW Fermi-Hubbard, N=2, d=4, w=6, D={D}",
                        "D": D, "d": d, "w": w, "flop_per_step": F, "parallelism": "single GPU"},
             "roofline": r["roofline"], "kernels": r["kernels"],
+            # F_HAC counts the identity levels' products, which run as copies: the rate of the
+            # flops actually executed, 4 (w-1) d D^3 per step
+            "value_executed_flops": F * (w - 1) / w / (r["ms_per_step"] * 1e-3) / 1e12,
         })
         if r["krylov"]:
             out["krylov"] = r["krylov"]
@@ -176,6 +191,9 @@ def main():
     # ---- VUMPS outer iterations on the same configuration
     if not args.no_vumps or args.ttg or args.vumps_only:
         vumps_iterations(args, imps, H, prec, out, D, N)
+
+    if not args.no_ttg and not args.vumps_only and not args.ttg:
+        out["time_to_galerkin"] = time_to_galerkin(imps, H, N)
 
     if args.idmrg > 0:
         out["idmrg"] = idmrg_sweeps(args, imps, H, D, N)
@@ -255,8 +273,13 @@ def hac_throughput(args, imps, _lib, H, pname, D, d, w, N, F):
             ctx.call("imps_hc_apply_dev", 0, xc, yc, prec)
         ctx.call("imps_synchronize")
         dtc = (time.perf_counter() - t0) / args.steps
+        # executed flops: the identity levels run as copies, so the GEMMs do 4 (w-1) D^3; the dense
+        # form 4 w D^3 (the H_C definition's work) is kept as a separately labelled rate
         fhc = 4.0 * w * D ** 3
-        hc = {"ms": dtc * 1e3, "flop": fhc, "tflops": fhc / dtc / 1e12, "frac_of_fp64_peak": fhc / dtc / 1e12 / PEAKS["fp64"]}
+        fex = 4.0 * (w - 1) * D ** 3
+        hc = {"ms": dtc * 1e3, "flop_executed": fex, "tflops_executed": fex / dtc / 1e12,
+              "frac_of_fp64_peak": fex / dtc / 1e12 / PEAKS["fp64"],
+              "flop_dense_form": fhc, "tflops_dense_form": fhc / dtc / 1e12}
         ctx.call("imps_dev_free", xc)
         ctx.call("imps_dev_free", yc)
     krylov = None
@@ -301,6 +324,46 @@ def hac_throughput(args, imps, _lib, H, pname, D, d, w, N, F):
                      "unit": "TFLOP/s (bf16 MFMA: 3 products per FP64-equivalent flop)"})
     return {"value": F / dt / 1e12, "ms_per_step": dt * 1e3, "kernels": kernels, "roofline": roof, "krylov": krylov,
             "hc": hc}
+
+
+def time_to_galerkin(imps, H, N):
+    """VUMPS from randomimps(D, seed 2) (canonicalised on the device) with each bulk arithmetic:
+    full runs to Galerkin < 1e-8 at D = 128 and the time to Galerkin <= 1e-4 at D = 1024.  Wall
+    seconds are the library's loop time (info.t_galerkin_reached, or the callback's clock for the
+    1e-4 mark); the mixed runs refine in FP64 below 1e-6 or once the reduced arithmetic stalls."""
+    out = {}
+    for D, target, maxiter in ((128, 1e-8, 600), (1024, 1e-4, 60)):
+        key = f"D{D}_to_{target:.0e}"
+        rec = {}
+        precs = ("fp64", "fp32", "bf16x3") if D <= 128 else ("fp64", "bf16x3")
+        for pname in precs:
+            ctx = imps.Context(0)
+            psi, ctx = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=100)
+            envs = imps.GPUCache(psi, H, ctx)
+            marks = []
+            t0 = time.perf_counter()
+
+            def cb(it, eps, f):
+                marks.append((it, eps, time.perf_counter() - t0))
+                return eps <= target
+
+            prec = PREC[pname]
+            alg = imps.VUMPS(tol=target, maxiter=maxiter, prec=prec, refine_below=1e-6 if prec else 0.0)
+            _, _, info = imps.find_groundstate(psi, H, alg, envs=envs, callback=cb)
+            hit = next(((it, el) for it, eps, el in marks if eps <= target), None)
+            rec[pname] = {"seconds": hit[1] if hit else None, "iterations": hit[0] if hit else info.niter,
+                          "reached": hit is not None, "itererr": info.itererr,
+                          "energy_per_site": info.energy / N, "prec_switch_iter": info.prec_switch_iter,
+                          "hac_applies": info.hac_applies, "phases_seconds": info.timings}
+            log(f"bench: time to Galerkin {target:.0e} at D={D} ({pname}): "
+                f"{rec[pname]['seconds']} s, {rec[pname]['iterations']} iterations")
+            ctx.close()
+        for pname in precs[1:]:
+            a, b = rec["fp64"]["seconds"], rec[pname]["seconds"]
+            rec[pname]["speedup_vs_fp64"] = a / b if a and b else None
+            rec[pname]["energy_diff_vs_fp64"] = abs(rec[pname]["energy_per_site"] - rec["fp64"]["energy_per_site"])
+        out[key] = rec
+    return out
 
 
 def idmrg_sweeps(args, imps, H, D, N):
@@ -435,10 +498,16 @@ def gradients(imps):
 
 
 def vumps_iterations(args, imps, H, prec, out, D, N):
-    log(f"bench: canonicalising a random D={D} state (PCG64 seed 2) on the device")
     ctx = imps.Context(0)
     t0 = time.perf_counter()
-    psi, ctx = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=100)
+    if args.state_in:
+        log(f"bench: starting from the state file {args.state_in}")
+        ctx.set_mpo(H)
+        ctx.resume(args.state_in)
+        psi = ctx.get_state()
+    else:
+        log(f"bench: canonicalising a random D={D} state (PCG64 seed 2) on the device")
+        psi, ctx = imps.randomimps(D, H, seed=2, ctx=ctx, tol=1e-12, maxiter=100)
     t_setup = time.perf_counter() - t0
     envs = imps.GPUCache(psi, H, ctx)
     maxit = args.ttg_maxiter if args.ttg else args.vumps_iters
@@ -450,7 +519,7 @@ def vumps_iterations(args, imps, H, prec, out, D, N):
         el = time.perf_counter() - tstart
         trace.append((it, eps, el))
         log(f"bench: VUMPS iter {it} eps {eps:.3e} e/site {f / N:.12f} t {el:.1f}s")
-        return args.ttg and el > args.ttg_seconds
+        return (args.ttg and el > args.ttg_seconds) or eps <= args.stop_below
 
     alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec, refine_below=args.refine_below, eig_floor=args.eig_floor)
     _, _, info = imps.find_groundstate(psi, H, alg, envs=envs, callback=cb)
@@ -464,12 +533,15 @@ def vumps_iterations(args, imps, H, prec, out, D, N):
             for thr in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8):
                 if eps <= thr and thr not in marks:
                     marks[thr] = {"iter": it, "seconds": el}
-        out["time_to_galerkin"] = {args.prec: {**rec, "seconds": info.t_galerkin_reached if info.converged else None,
+        out["time_to_galerkin_long"] = {args.prec: {**rec, "seconds": info.t_galerkin_reached if info.converged else None,
                                                "converged": info.converged,
                                                "prec_switch_iter": info.prec_switch_iter,
                                                "first_reached": {f"{k:.0e}": v for k, v in marks.items()}}}
     else:
         out["vumps"] = rec
+    if args.state_out:
+        ctx.checkpoint(args.state_out)
+        log(f"bench: state written to {args.state_out}")
     ctx.close()
 
 

@@ -191,26 +191,51 @@ int imps_local_expectation(imps_ctx* ctx, int site, const double* O, double* val
 
 /* ---- the VUMPS driver ------------------------------------------------------------ */
 
-/* VUMPS(; tol, maxiter, verbosity, alg_gauge, alg_eigsolve, alg_environments)
- * (src/algorithms/algdefs.jl:26-35) with the DynamicTol defaults of
- * src/utility.jl:308-364. */
+/* finalize(iter, ψ, operator, envs) -> (ψ, envs) of VUMPS/IDMRG (algdefs.jl:34, vumps.jl:75,
+ * idmrg.jl:60): called once per iteration with the context, after recalculate! (VUMPS) or the sweep
+ * (IDMRG) and before the iteration's error is computed.  It may read or replace the state and the
+ * environments through this ABI (imps_get_state / imps_set_state / imps_set_env ...); environments
+ * it leaves invalid (imps_set_state drops them) are recalculated at the iteration's tolerance, warm
+ * starts lost.  Return IMPS_OK to continue, a positive value to stop after this iteration, or a
+ * negative IMPS_E_* code to abort the run with that status.  The unit cell must not change. */
+typedef int (*imps_finalize_cb)(int iter, imps_ctx* ctx, void* user);
+
+/* VUMPS(; tol, maxiter, verbosity, alg_gauge, alg_eigsolve, alg_environments, finalize)
+ * (src/algorithms/algdefs.jl:26-35) with the Defaults of src/utility.jl:308-364.  Each of
+ * alg_eigsolve, alg_gauge and alg_environments is either a DynamicTol (utility.jl:136-171; its
+ * tolerance becomes clamp(eps * factor / sqrt(max(iter, 1)), tol_min, tol_max) every iteration) or,
+ * with *_dynamic = 0, a fixed-tolerance solver (updatetol is then the identity, utility.jl:151). */
 typedef struct imps_vumps_params {
   double tol;             /* Defaults.tol = 1e-10 (Galerkin target) */
   int maxiter;            /* Defaults.maxiter = 300 */
   int verbosity;          /* 0: silent, >0: one line per iteration on stdout */
-  double tol_min;         /* 1e-14 */
-  double tol_max;         /* 1e-4 */
+  double tol_min;         /* alg_eigsolve DynamicTol tol_min = 1e-14 */
+  double tol_max;         /* alg_eigsolve DynamicTol tol_max = 1e-4 */
   double eigs_tolfactor;  /* 1e-3 */
   double gauge_tolfactor; /* 1e-6 */
   double envs_tolfactor;  /* 1e-4 */
-  int krylovdim;          /* 30 */
-  int eig_maxiter;        /* 300 */
-  int gauge_maxiter;      /* 300 */
-  int envs_maxiter;       /* 300 */
+  int krylovdim;          /* alg_eigsolve Lanczos krylovdim = 30 */
+  int eig_maxiter;        /* alg_eigsolve Lanczos maxiter = 300 */
+  int gauge_maxiter;      /* alg_gauge maxiter = 300 */
+  int envs_maxiter;       /* DMRGCache maxiter = 300 (recalculate! passes only tol) */
   int prec;               /* IMPS_PREC_* of the bulk contractions */
   double refine_below;    /* mixed precision: switch to FP64 once eps < refine_below */
   double eig_floor;       /* mixed precision: eigensolver tolerance floor relative to |lambda| in the
                              reduced-precision phase (0: 1e-6 for FP32, 1e-4 for split-BF16) */
+  int eig_dynamic;        /* 1: alg_eigsolve is a DynamicTol (default); 0: fixed tolerance eig_tol */
+  int eig_eager;          /* Lanczos eager = 1 */
+  double eig_tol;         /* fixed eigensolver tolerance (Defaults.tol = 1e-10) */
+  int gauge_dynamic;      /* 1: alg_gauge is a DynamicTol (default) */
+  double gauge_tol;       /* fixed gauge tolerance (Defaults.tolgauge = 1e-13) */
+  double gauge_tol_min;   /* 1e-14 */
+  double gauge_tol_max;   /* 1e-4 */
+  int envs_dynamic;       /* 1: alg_environments is a DynamicTol (default) */
+  double envs_tol;        /* fixed environment tolerance (Defaults.tol = 1e-10) */
+  double envs_tol_min;    /* 1e-14 */
+  double envs_tol_max;    /* 1e-4 */
+  int envs_krylovdim;     /* DMRGCache krylovdim = 30 */
+  imps_finalize_cb finalize; /* NULL: Defaults._finalize (identity) */
+  void* finalize_user;
 } imps_vumps_params;
 
 /* IterativeConvergenceInfo (algdefs.jl:180-194) plus timings. */

@@ -6,7 +6,7 @@ host-side mirror of the reference's API (``find_groundstate``, ``VUMPS``,
 ``TransferMatrix``, ...).
 """
 from . import _lib
-from .api import (IDMRG, TDVP, VUMPS, correlator, load_mpo, load_state, save_mpo, save_state, dense_expectation, dominant_env, DenseIMPO, dot, fidelity, infidelity, norm, AC_hamiltonian, C_hamiltonian, CanonicalIMPS, Context, DMRGCache,
+from .api import (IDMRG, TDVP, VUMPS, DynamicTol, SolverTol, alg_eigsolve, alg_environments, alg_gauge, correlator, load_mpo, load_state, save_mpo, save_state, dense_expectation, dominant_env, DenseIMPO, dot, fidelity, infidelity, norm, AC_hamiltonian, C_hamiltonian, CanonicalIMPS, Context, DMRGCache,
                   GPUCache, IterativeConvergenceInfo, Lanczos, SparseIMPO, TransferMatrix,
                   calc_galerkin, canonicalimps, energy_gradient, expectationvalue, find_groundstate,
                   find_groundstates_batched, fixedpoint,
@@ -15,7 +15,7 @@ from .api import (IDMRG, TDVP, VUMPS, correlator, load_mpo, load_state, save_mpo
 from . import models
 
 __all__ = [
-    "correlator", "load_mpo", "load_state", "save_mpo", "save_state", "dense_expectation", "dominant_env", "DenseIMPO", "dot", "fidelity", "infidelity", "norm", "IDMRG", "TDVP", "time_evolve", "timestep", "VUMPS", "AC_hamiltonian", "C_hamiltonian", "CanonicalIMPS", "Context", "DMRGCache", "GPUCache",
+    "DynamicTol", "SolverTol", "alg_eigsolve", "alg_environments", "alg_gauge", "correlator", "load_mpo", "load_state", "save_mpo", "save_state", "dense_expectation", "dominant_env", "DenseIMPO", "dot", "fidelity", "infidelity", "norm", "IDMRG", "TDVP", "time_evolve", "timestep", "VUMPS", "AC_hamiltonian", "C_hamiltonian", "CanonicalIMPS", "Context", "DMRGCache", "GPUCache",
     "IterativeConvergenceInfo", "Lanczos", "SparseIMPO", "TransferMatrix", "calc_galerkin",
     "canonicalimps", "energy_gradient", "expectationvalue", "find_groundstate", "find_groundstates_batched", "fixedpoint", "leftenv", "random_tensors",
     "randomimps", "recalculate_", "regauge_", "regularize_", "rightenv", "models",

@@ -39,12 +39,20 @@ class KrylovInfo(ctypes.Structure):
     _fields_ = [("converged", c_int), ("numiter", c_int), ("numops", c_int), ("normres", c_double)]
 
 
+FINALIZE_CB = ctypes.CFUNCTYPE(c_int, c_int, c_void_p, c_void_p)
+
+
 class VumpsParams(ctypes.Structure):
     _fields_ = [("tol", c_double), ("maxiter", c_int), ("verbosity", c_int), ("tol_min", c_double),
                 ("tol_max", c_double), ("eigs_tolfactor", c_double), ("gauge_tolfactor", c_double),
                 ("envs_tolfactor", c_double), ("krylovdim", c_int), ("eig_maxiter", c_int),
                 ("gauge_maxiter", c_int), ("envs_maxiter", c_int), ("prec", c_int),
-                ("refine_below", c_double), ("eig_floor", c_double)]
+                ("refine_below", c_double), ("eig_floor", c_double),
+                ("eig_dynamic", c_int), ("eig_eager", c_int), ("eig_tol", c_double),
+                ("gauge_dynamic", c_int), ("gauge_tol", c_double), ("gauge_tol_min", c_double),
+                ("gauge_tol_max", c_double), ("envs_dynamic", c_int), ("envs_tol", c_double),
+                ("envs_tol_min", c_double), ("envs_tol_max", c_double), ("envs_krylovdim", c_int),
+                ("finalize", FINALIZE_CB), ("finalize_user", c_void_p)]
 
 
 class VumpsInfo(ctypes.Structure):

@@ -307,7 +307,8 @@ def C_hamiltonian(site, below, operator, above, envs: GPUCache, prec=_lib.IMPS_P
 
 @dataclass
 class Lanczos:
-    """KrylovKit.Lanczos parameters as used by fixedpoint (utility.jl:180-186)."""
+    """KrylovKit.Lanczos(; tol, krylovdim, maxiter, eager) as used by fixedpoint (utility.jl:180-186)
+    and inside alg_eigsolve."""
 
     tol: float = 1e-10
     krylovdim: int = 30
@@ -618,16 +619,102 @@ def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
     return out
 
 
+# ---------------------------------------------------------------------------- solver objects
+
+
+@dataclass
+class SolverTol:
+    """The NamedTuple (; tol, maxiter) form of alg_gauge and alg_environments (utility.jl:350-364)."""
+
+    tol: float = 1e-10
+    maxiter: int = 300
+
+
+@dataclass
+class DynamicTol:
+    """DynamicTol(alg; tol_min, tol_max, tol_factor) (utility.jl:136-148): every iteration the inner
+    solver's tolerance becomes clamp(eps * tol_factor / sqrt(max(iter, 1)), tol_min, tol_max)."""
+
+    alg: object
+    tol_min: float = 1e-6
+    tol_max: float = 1e-2
+    tol_factor: float = 0.1
+
+    def __post_init__(self):
+        if not 0 <= self.tol_min <= self.tol_max:
+            raise ValueError("DynamicTol requires 0 ≤ tol_min ≤ tol_max")
+
+
+def alg_eigsolve(tol=1e-10, maxiter=300, eager=True, krylovdim=30, dynamic_tols=True, tol_min=1e-14,
+                 tol_max=1e-4, tol_factor=1e-3):
+    """Defaults.alg_eigsolve (utility.jl:322-333), Hermitian case."""
+    alg = Lanczos(tol=tol, krylovdim=krylovdim, maxiter=maxiter, eager=eager)
+    return DynamicTol(alg, tol_min, tol_max, tol_factor) if dynamic_tols else alg
+
+
+def alg_gauge(tol=1e-13, maxiter=300, dynamic_tols=True, tol_min=1e-14, tol_max=1e-4, tol_factor=1e-6):
+    """Defaults.alg_gauge (utility.jl:350-354)."""
+    alg = SolverTol(tol, maxiter)
+    return DynamicTol(alg, tol_min, tol_max, tol_factor) if dynamic_tols else alg
+
+
+def alg_environments(tol=1e-10, maxiter=300, dynamic_tols=True, tol_min=1e-14, tol_max=1e-4, tol_factor=1e-4):
+    """Defaults.alg_environments (utility.jl:359-364)."""
+    alg = SolverTol(tol, maxiter)
+    return DynamicTol(alg, tol_min, tol_max, tol_factor) if dynamic_tols else alg
+
+
+def _set_solver(p, which, obj):
+    """Writes alg_eigsolve / alg_gauge / alg_environments into the parameter struct."""
+    dyn = isinstance(obj, DynamicTol)
+    inner = obj.alg if dyn else obj
+    if which == "eig":
+        if not isinstance(inner, Lanczos):
+            raise TypeError("alg_eigsolve must be a Lanczos or a DynamicTol(Lanczos)")
+        p.eig_dynamic, p.eig_tol, p.eig_maxiter = int(dyn), float(inner.tol), int(inner.maxiter)
+        p.krylovdim, p.eig_eager = int(inner.krylovdim), int(bool(inner.eager))
+        if dyn:
+            p.tol_min, p.tol_max, p.eigs_tolfactor = obj.tol_min, obj.tol_max, obj.tol_factor
+    elif which == "gauge":
+        p.gauge_dynamic, p.gauge_tol, p.gauge_maxiter = int(dyn), float(inner.tol), int(inner.maxiter)
+        if dyn:
+            p.gauge_tol_min, p.gauge_tol_max, p.gauge_tolfactor = obj.tol_min, obj.tol_max, obj.tol_factor
+    else:
+        # recalculate! forwards only tol (vumps.jl:58, 73); maxiter stays the DMRGCache default
+        p.envs_dynamic, p.envs_tol = int(dyn), float(inner.tol)
+        if dyn:
+            p.envs_tol_min, p.envs_tol_max, p.envs_tolfactor = obj.tol_min, obj.tol_max, obj.tol_factor
+
+
+def _params(alg, prec, refine_below, eig_floor):
+    p = _lib.VumpsParams()
+    _lib.check(None, _lib.load().imps_vumps_default_params(ctypes.byref(p)), "imps_vumps_default_params")
+    p.tol, p.maxiter, p.verbosity = alg.tol, alg.maxiter, alg.verbosity
+    p.tol_min, p.tol_max = alg.tol_min, alg.tol_max
+    p.eigs_tolfactor, p.gauge_tolfactor, p.envs_tolfactor = alg.eigs_tolfactor, alg.gauge_tolfactor, alg.envs_tolfactor
+    p.krylovdim, p.eig_maxiter, p.gauge_maxiter, p.envs_maxiter = (alg.krylovdim, alg.eig_maxiter, alg.gauge_maxiter,
+                                                                   alg.envs_maxiter)
+    p.prec, p.refine_below, p.eig_floor = prec, refine_below, eig_floor
+    for which, obj in (("eig", alg.alg_eigsolve), ("gauge", alg.alg_gauge), ("envs", alg.alg_environments)):
+        if obj is not None:
+            _set_solver(p, which, obj)
+    return p
+
+
 # ---------------------------------------------------------------------------- VUMPS
 
 
 @dataclass
 class VUMPS:
-    """VUMPS(; D, tol, maxiter, verbosity, ...) (algdefs.jl:26-35) with the DynamicTol
-    defaults of utility.jl:308-364.  `prec` selects the arithmetic of the bulk
-    contractions; with a mixed precision the loop refines in FP64 once the Galerkin
-    error drops below `refine_below`; `eig_floor` (relative to |λ|) bounds the eigensolver
-    tolerance of the reduced-precision phase (0: the library's default per precision)."""
+    """VUMPS(; D, tol, maxiter, verbosity, alg_gauge, alg_eigsolve, alg_environments, finalize)
+    (algdefs.jl:26-35) with the Defaults of utility.jl:308-364.  The flat fields are the default
+    solver objects' parameters; alg_eigsolve / alg_gauge / alg_environments (Lanczos, SolverTol,
+    optionally wrapped in DynamicTol) override them when given.  finalize(iter, ψ, H, envs) ->
+    (ψ, envs) runs after recalculate! every iteration (vumps.jl:75); a returned state other than
+    the one passed in becomes the context's state (its environments are then rebuilt).  `prec`
+    selects the arithmetic of the bulk contractions; with a mixed precision the loop refines in
+    FP64 once the Galerkin error drops below `refine_below`; `eig_floor` (relative to |λ|) bounds
+    the eigensolver tolerance of the reduced-precision phase (0: the library's default)."""
 
     D: int = 64
     tol: float = 1e-10
@@ -645,18 +732,20 @@ class VUMPS:
     prec: int = _lib.IMPS_PREC_FP64
     refine_below: float = 0.0
     eig_floor: float = 0.0
+    alg_eigsolve: object = None
+    alg_gauge: object = None
+    alg_environments: object = None
+    finalize: object = None
 
     def params(self):
-        return _lib.VumpsParams(self.tol, self.maxiter, self.verbosity, self.tol_min, self.tol_max,
-                                self.eigs_tolfactor, self.gauge_tolfactor, self.envs_tolfactor,
-                                self.krylovdim, self.eig_maxiter, self.gauge_maxiter, self.envs_maxiter,
-                                self.prec, self.refine_below, self.eig_floor)
+        return _params(self, self.prec, self.refine_below, self.eig_floor)
 
 
 @dataclass
 class IDMRG:
-    """IDMRG(; D, tol, maxiter, verbosity, ...) (algdefs.jl:72-82): single-site infinite DMRG with
-    the same DynamicTol eigensolver and gauge defaults as VUMPS (utility.jl:308-364)."""
+    """IDMRG(; D, tol, maxiter, verbosity, alg_gauge, alg_environments, alg_eigsolve, finalize)
+    (algdefs.jl:72-82): single-site infinite DMRG with the same solver defaults and objects as VUMPS
+    (utility.jl:308-364); finalize runs after every sweep (idmrg.jl:60) and must keep D."""
 
     D: int = 64
     tol: float = 1e-10
@@ -671,12 +760,13 @@ class IDMRG:
     eig_maxiter: int = 300
     gauge_maxiter: int = 300
     envs_maxiter: int = 300
+    alg_eigsolve: object = None
+    alg_gauge: object = None
+    alg_environments: object = None
+    finalize: object = None
 
     def params(self):
-        return _lib.VumpsParams(self.tol, self.maxiter, self.verbosity, self.tol_min, self.tol_max,
-                                self.eigs_tolfactor, self.gauge_tolfactor, self.envs_tolfactor,
-                                self.krylovdim, self.eig_maxiter, self.gauge_maxiter, self.envs_maxiter,
-                                _lib.IMPS_PREC_FP64, 0.0, 0.0)
+        return _params(self, _lib.IMPS_PREC_FP64, 0.0, 0.0)
 
 
 @dataclass
@@ -727,9 +817,33 @@ def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | IDM
         return 0
 
     cb = _lib.ITER_CB(_cb)
+    failure = []
+
+    def _fin(it, h, user):
+        # alg.finalize(iter, ψ, H, envs) -> (ψ, envs) on the host copy of the current state
+        try:
+            cur = envs.ctx.get_state()
+            envs.ket = cur
+            psi2, _ = alg.finalize(it, cur, H, envs)
+            if psi2 is not cur:
+                envs.ctx.set_state(psi2)
+                envs.ket = psi2
+            return 0
+        except Exception as e:  # reported after the run returns
+            failure.append(e)
+            return _lib.IMPS_E_ARG
+
+    if alg.finalize is not None:
+        fin = _lib.FINALIZE_CB(_fin)
+        p.finalize = fin
     entry = "imps_idmrg_run" if isinstance(alg, IDMRG) else "imps_vumps_run"
-    envs.ctx.call(entry, ctypes.byref(p), ctypes.byref(info),
-                  losses.ctypes.data_as(_lib.P_double), len(losses), cb, None)
+    try:
+        envs.ctx.call(entry, ctypes.byref(p), ctypes.byref(info),
+                      losses.ctypes.data_as(_lib.P_double), len(losses), cb, None)
+    except _lib.ImpsError:
+        if failure:
+            raise failure[0]
+        raise
     psi = envs.ctx.get_state()
     envs.ket = psi
     nl = info.niter + 1

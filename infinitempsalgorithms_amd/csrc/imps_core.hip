@@ -1911,7 +1911,7 @@ void destroy_worker(imps_ctx* w) {
 // stream each (their device work overlaps; each solve's arithmetic is the same as when run
 // alone).  Then regauge! per site on this context's stream.
 void local_update(imps_ctx* c, const std::vector<double>& tol_ac, const std::vector<double>& tol_c, int kdim,
-                  int maxiter, int prec, std::vector<double>& lam_ac, std::vector<double>& lam_c) {
+                  int maxiter, bool eager, int prec, std::vector<double>& lam_ac, std::vector<double>& lam_c) {
   const int N = c->N;
   if (!c->ev_ready) HIPCHK(hipEventCreateWithFlags(&c->ev_ready, hipEventDisableTiming));
   HIPCHK(hipEventRecord(c->ev_ready, c->stream));
@@ -1931,9 +1931,9 @@ void local_update(imps_ctx* c, const std::vector<double>& tol_ac, const std::vec
         HIPCHK(hipSetDevice(w->device));
         const int site = k / 2;
         if (k % 2 == 0)
-          lam_ac[site] = eigsolve_ac(w, site, tol_ac[site], kdim, maxiter, true, prec).lambda;
+          lam_ac[site] = eigsolve_ac(w, site, tol_ac[site], kdim, maxiter, eager, prec).lambda;
         else
-          lam_c[site] = eigsolve_c(w, site, tol_c[site], kdim, maxiter, true, prec).lambda;
+          lam_c[site] = eigsolve_c(w, site, tol_c[site], kdim, maxiter, eager, prec).lambda;
         HIPCHK(hipEventRecord(w->ev_done, w->stream));
       } catch (const ImpsError& e) {
         code[k] = e.code;
@@ -2425,14 +2425,74 @@ int imps_vumps_default_params(imps_vumps_params* p) {
   p->prec = IMPS_PREC_FP64;
   p->refine_below = 0.0;
   p->eig_floor = 0.0;
+  p->eig_dynamic = 1;
+  p->eig_eager = 1;
+  p->eig_tol = 1.0e-10;
+  p->gauge_dynamic = 1;
+  p->gauge_tol = 1.0e-13;
+  p->gauge_tol_min = 1.0e-14;
+  p->gauge_tol_max = 1.0e-4;
+  p->envs_dynamic = 1;
+  p->envs_tol = 1.0e-10;
+  p->envs_tol_min = 1.0e-14;
+  p->envs_tol_max = 1.0e-4;
+  p->envs_krylovdim = 30;
+  p->finalize = nullptr;
+  p->finalize_user = nullptr;
   return IMPS_OK;
 }
+
+namespace {
+// updatetol(alg_*, iter, eps) (utility.jl:150-158) for the three solvers of imps_vumps_params
+struct SolverTols {
+  const imps_vumps_params* p;
+  double eig(int it, double eps) const {
+    return p->eig_dynamic ? updatetol(p->eigs_tolfactor, it, eps, p->tol_min, p->tol_max) : p->eig_tol;
+  }
+  double gauge(int it, double eps) const {
+    return p->gauge_dynamic ? updatetol(p->gauge_tolfactor, it, eps, p->gauge_tol_min, p->gauge_tol_max)
+                            : p->gauge_tol;
+  }
+  double envs(int it, double eps) const {
+    return p->envs_dynamic ? updatetol(p->envs_tolfactor, it, eps, p->envs_tol_min, p->envs_tol_max)
+                           : p->envs_tol;
+  }
+};
+
+void check_params(const imps_vumps_params* p) {
+  require(p->tol >= 0 && p->maxiter >= 0 && p->krylovdim >= 1 && p->eig_maxiter >= 1 && p->gauge_maxiter >= 1 &&
+              p->envs_maxiter >= 1 && p->envs_krylovdim >= 1,
+          IMPS_E_ARG, "VUMPS parameters: negative tolerance or non-positive solver sizes");
+  require(0 <= p->tol_min && p->tol_min <= p->tol_max && 0 <= p->gauge_tol_min &&
+              p->gauge_tol_min <= p->gauge_tol_max && 0 <= p->envs_tol_min && p->envs_tol_min <= p->envs_tol_max,
+          IMPS_E_ARG, "DynamicTol requires 0 <= tol_min <= tol_max");
+  require(p->eig_dynamic || p->eig_tol > 0, IMPS_E_ARG, "VUMPS parameters: fixed eigensolver tolerance must be > 0");
+  require(p->gauge_dynamic || p->gauge_tol > 0, IMPS_E_ARG, "VUMPS parameters: fixed gauge tolerance must be > 0");
+  require(p->envs_dynamic || p->envs_tol > 0, IMPS_E_ARG, "VUMPS parameters: fixed environment tolerance must be > 0");
+}
+
+// finalize (vumps.jl:75, idmrg.jl:60): the callback may replace the state and environments
+// through the ABI; environments it invalidated are rebuilt at tol_envs.  Returns true to stop.
+bool run_finalize(imps_ctx* c, const imps_vumps_params* p, int it, double tol_envs, EnvStats* est) {
+  if (!p->finalize) return false;
+  const int N = c->N;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int fr = p->finalize(it, c, p->finalize_user);
+  require(fr >= 0, fr, "finalize callback returned " + std::to_string(fr) + (c->err.empty() ? "" : ": " + c->err));
+  require(c->N == N && (int)c->AL.size() == N, IMPS_E_ARG, "finalize must not change the unit cell");
+  need_match(c);
+  if (!c->envs_valid) envs_recalculate(c, tol_envs, p->envs_maxiter, p->envs_krylovdim, est);
+  return fr > 0;
+}
+}  // namespace
 
 int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* info, double* losses,
                    int nloss, imps_iter_cb cb, void* user) {
   if (!c || !p) return IMPS_E_ARG;
   return run_guarded(c, [&] {
     need_match(c);
+    check_params(p);
+    const SolverTols stol{p};
     const int N = c->N;
     const long hac0 = c->hac_applies, hc0 = c->hc_applies;
     auto t0 = std::chrono::steady_clock::now();
@@ -2449,7 +2509,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       HIPCHK(hipStreamSynchronize(c->stream));
       acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     };
-    if (!c->envs_valid) lap(te, [&] { envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, &est); });
+    if (!c->envs_valid) lap(te, [&] { envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->envs_krylovdim, &est); });
     int prec = p->prec;
     int switch_iter = -1;
     std::vector<double> lam_ac(N, 1.0), lam_c(N, 1.0);
@@ -2457,8 +2517,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     bool stalled = false;
     double eps = 0.0;
     lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
-    auto tolu = [&](double f, int it, double e) { return updatetol(f, it, e, p->tol_min, p->tol_max); };
-    lap(te, [&] { envs_recalculate(c, tolu(p->envs_tolfactor, 0, eps), p->envs_maxiter, p->krylovdim, &est); });
+    lap(te, [&] { envs_recalculate(c, stol.envs(0, eps), p->envs_maxiter, p->envs_krylovdim, &est); });
     int nl = 0;
     if (losses && nl < nloss) losses[nl++] = eps;
     bool converged = false;
@@ -2473,7 +2532,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
         prec = PREC_FP64;
         switch_iter = it;
       }
-      double tol_eig = tolu(p->eigs_tolfactor, siter, eps);
+      double tol_eig = stol.eig(siter, eps);
       // a reduced-precision operator cannot resolve residuals below its own rounding
       // level: the eigensolver tolerance is floored there until the FP64 refinement
       const double floor_rel = prec == PREC_FP64 ? 0.0
@@ -2485,12 +2544,15 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
           ta[site] = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_ac[site])));
           tc[site] = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_c[site])));
         }
-        local_update(c, ta, tc, p->krylovdim, p->eig_maxiter, prec, lam_ac, lam_c);
+        local_update(c, ta, tc, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, prec, lam_ac, lam_c);
       });
       int sw = 0;
-      lap(tg, [&] { gauge_step(c, tolu(p->gauge_tolfactor, siter, eps), p->gauge_maxiter, &sw); });
+      lap(tg, [&] { gauge_step(c, stol.gauge(siter, eps), p->gauge_maxiter, &sw); });
       sweeps_total += sw;
-      lap(te, [&] { envs_recalculate(c, tolu(p->envs_tolfactor, siter, eps), p->envs_maxiter, p->krylovdim, &est); });
+      lap(te, [&] { envs_recalculate(c, stol.envs(siter, eps), p->envs_maxiter, p->envs_krylovdim, &est); });
+      // finalize (vumps.jl:75), between recalculate! and calc_galerkin
+      bool fstop = false;
+      lap(te, [&] { fstop = run_finalize(c, p, it, stol.envs(siter, eps), &est); });
       lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
       stalled = eps > 0.9 * best_eps;
       best_eps = std::min(best_eps, eps);
@@ -2506,7 +2568,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
         t_reach = secs();
       }
       if (cb && cb(it, eps, f, user) != 0) break;
-      if (converged) break;
+      if (converged || fstop) break;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     if (info) {
@@ -2542,10 +2604,11 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     auto secs = [&] {
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
-    auto tolu = [&](double f, int it, double e) { return updatetol(f, it, e, p->tol_min, p->tol_max); };
+    check_params(p);
+    const SolverTols stol{p};
     const size_t nA = (size_t)D * d * D, nC = (size_t)D * D, nE = (size_t)D * w * D;
     // envs = DMRGCache(psi0, H) at its default tolerance (idmrg.jl:46)
-    if (!c->envs_valid) envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, nullptr);
+    if (!c->envs_valid) envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->envs_krylovdim, nullptr);
     double eps = galerkin(c, PREC_FP64);
     int nl = 0;
     if (losses && nl < nloss) losses[nl++] = eps;
@@ -2555,11 +2618,11 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     bool converged = false;
     int it = 0;
     for (it = 1; it <= p->maxiter; ++it) {
-      const double tol_eig = tolu(p->eigs_tolfactor, it - 1, eps);
+      const double tol_eig = stol.eig(it - 1, eps);
       // _localupdate_sweep_idmrg! (idmrg.jl:12-34)
       d2d(c, Cold, c->C[N - 1], nC);
       for (int pos = 0; pos < N; ++pos) {
-        eigsolve_ac(c, pos, tol_eig, p->krylovdim, p->eig_maxiter, true, PREC_FP64);
+        eigsolve_ac(c, pos, tol_eig, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, PREC_FP64);
         d2d(c, c->AL[pos], c->pAC[pos], nA);
         qrpos(c, D * d, D, c->AL[pos], c->C[pos]);
         d2d(c, c->AC[pos], pos < N - 1 ? c->AL[pos] : c->pAC[pos], nA);
@@ -2570,7 +2633,7 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
         c->gl_id[nxt] = 0;
       }
       for (int pos = N - 1; pos >= 0; --pos) {
-        E = eigsolve_ac(c, pos, tol_eig, p->krylovdim, p->eig_maxiter, true, PREC_FP64).lambda;
+        E = eigsolve_ac(c, pos, tol_eig, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, PREC_FP64).lambda;
         const int prv = (pos - 1 + N) % N;
         lqpos(c, D, d * D, c->pAC[pos], c->C[prv], c->AR[pos]);
         d2d(c, c->AC[pos], pos > 0 ? c->AR[pos] : c->pAC[pos], nA);
@@ -2579,6 +2642,9 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
         d2d(c, c->GR[prv], Env, nE);
         c->gr_id[prv] = 0;
       }
+      // finalize (idmrg.jl:60), before the error criterion
+      const bool fstop = run_finalize(c, p, it, stol.envs(it - 1, eps), nullptr);
+      require(c->D == D, IMPS_E_ARG, "IDMRG: finalize must not change the bond dimension");
       // error criterion: drift of the centre matrix on bond 0 (idmrg.jl:63)
       HIPCHK(launch_axpby(nC, 1.0, c->C[N - 1], -1.0, Cold, c->stream));
       eps = nrm2(c, nC, Cold);
@@ -2589,11 +2655,11 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       }
       if (eps <= p->tol) converged = true;
       if (cb && cb(it, eps, E, user) != 0) break;
-      if (converged) break;
+      if (converged || fstop) break;
     }
     const int niter = std::min(it, p->maxiter);
     // rebuild from AR with the dynamic gauge tolerance, then recalculate! (idmrg.jl:76-84)
-    const double tol_g = tolu(p->gauge_tolfactor, niter, eps);
+    const double tol_g = stol.gauge(niter, eps);
     std::vector<double*> raw(N);
     for (int l = 0; l < N; ++l) {
       raw[l] = c->pAC[l];
@@ -2606,7 +2672,7 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     d2d(c, C0, c->C[N - 1], nC);
     uniform_rightorth(c, c->AL, C0, tol_g, p->gauge_maxiter);
     for (int l = 0; l < N; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
-    envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->krylovdim, nullptr);
+    envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->envs_krylovdim, nullptr);
     c->envs_valid = true;
     const double f = energy(c);
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -43,6 +43,29 @@ def test_default_params_match_reference_defaults(lib):
     assert (p.eigs_tolfactor, p.gauge_tolfactor, p.envs_tolfactor) == (1e-3, 1e-6, 1e-4)
     # plain FP64, no mixed-precision switch or floor
     assert (p.prec, p.refine_below, p.eig_floor) == (0, 0.0, 0.0)
+    # Defaults.alg_eigsolve / alg_gauge / alg_environments (utility.jl:322-364): DynamicTol-wrapped
+    # eager Lanczos, (tol = tolgauge, maxiter) and (tol, maxiter); identity finalize
+    assert (p.eig_dynamic, p.eig_eager, p.eig_tol) == (1, 1, 1e-10)
+    assert (p.gauge_dynamic, p.gauge_tol, p.gauge_tol_min, p.gauge_tol_max) == (1, 1e-13, 1e-14, 1e-4)
+    assert (p.envs_dynamic, p.envs_tol, p.envs_tol_min, p.envs_tol_max, p.envs_krylovdim) == (1, 1e-10, 1e-14, 1e-4, 30)
+    assert not p.finalize and p.finalize_user is None
+
+
+def test_vumps_solver_objects_map_to_params(lib):
+    # VUMPS(alg_eigsolve, alg_gauge, alg_environments) (algdefs.jl:26-35): a DynamicTol carries its
+    # own bounds and factor, a bare solver a fixed tolerance (updatetol is then the identity)
+    import infinitempsalgorithms_amd as imps
+    p = imps.VUMPS(alg_eigsolve=imps.alg_eigsolve(dynamic_tols=False, tol=1e-7, krylovdim=20, eager=False),
+                   alg_gauge=imps.alg_gauge(tol_factor=1e-5, tol_min=1e-13, tol_max=1e-3, maxiter=50),
+                   alg_environments=imps.SolverTol(tol=1e-11)).params()
+    assert (p.eig_dynamic, p.eig_tol, p.krylovdim, p.eig_eager) == (0, 1e-7, 20, 0)
+    assert (p.gauge_dynamic, p.gauge_tolfactor, p.gauge_tol_min, p.gauge_tol_max, p.gauge_maxiter) == \
+        (1, 1e-5, 1e-13, 1e-3, 50)
+    assert (p.envs_dynamic, p.envs_tol) == (0, 1e-11)
+    d = imps.VUMPS().params()
+    assert (d.eig_dynamic, d.gauge_dynamic, d.envs_dynamic) == (1, 1, 1)
+    with pytest.raises(ValueError):
+        imps.DynamicTol(imps.Lanczos(), tol_min=1e-3, tol_max=1e-4)
 
 
 def test_host_symmetric_eig(lib):

@@ -1,0 +1,241 @@
+"""GPU runs at the benchmark configurations, through the C ABI, against the oracle, golden
+fixtures written by the oracle (tests/golden/) and closed forms.
+
+  XXZ Delta=1, N=2, D=256 (FP64): the oracle's trajectory from the same psi0.  A rounding-level
+      perturbation of psi0 moves the oracle's own Galerkin errors by 2e-7 relative at iteration 5,
+      2e-6 at 6 and 1e-3 by iteration 10 (the fixture records it), so per-iteration errors are
+      compared for iterations 0-5 (1e-6 relative) and the energy per site for 8 iterations (1e-10).
+  JW Hubbard, N=2, d=4, D=1024: the FP32 H_AC against the FP64 oracle (rel between 1e-9 and 5e-6,
+      so the reduced precision ran); at D=128, FP64, FP32-bulk and split-BF16-bulk runs with FP64
+      refinement against the oracle's converged ground state.  (On this model VUMPS at a fixed D
+      can settle into a limit cycle, depending on rounding along the trajectory -- at D=48, 64 and
+      96 one or more of the three arithmetics cycle from this psi0; at D=128 all three converge):
+      energy per site within 1e-9, Galerkin <= 1e-8, density and double occupancy within 1e-7, the
+      local moment's magnitude within 1e-6.
+  64 TFIM instances, h ~ U[0.5, 1.5], D in {32, 64, 128, 256} (PCG64(3)), FP64 and FP32-bulk:
+      every energy against the free-fermion closed form.
+  Gradients: TFIM D=128 de/dh against -<sigma_z>, the closed-form derivative and central finite
+      differences; XXZ D=512 de/dDelta against central finite differences (step 1e-5).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def imps():
+    import infinitempsalgorithms_amd as m
+    return m
+
+
+@pytest.fixture(scope="module")
+def ctx(imps):
+    c = imps.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def tfim_energy(h):
+    """Ground-state energy per site of -sum sx sx - h sum sz: -(1/pi) int_0^pi sqrt(1 + h^2 - 2 h cos k) dk."""
+    return -integrate.quad(lambda k: math.sqrt(1 + h * h - 2 * h * math.cos(k)), 0, math.pi, epsabs=1e-14,
+                           epsrel=1e-14)[0] / math.pi
+
+
+def tfim_energy_dh(h):
+    """d/dh of tfim_energy: -(1/pi) int_0^pi (h - cos k) / sqrt(1 + h^2 - 2 h cos k) dk."""
+    return -integrate.quad(lambda k: (h - math.cos(k)) / math.sqrt(1 + h * h - 2 * h * math.cos(k)), 0, math.pi,
+                           epsabs=1e-14, epsrel=1e-14)[0] / math.pi
+
+
+def test_xxz_d256_trajectory_matches_oracle_fixture(imps, ctx, oracle):
+    with open(os.path.join(GOLDEN, "c2_xxz_d256_trajectory.json")) as f:
+        g = json.load(f)
+    W = oracle.xxz_mpo(1.0, 1.0)
+    H = imps.SparseIMPO([W, W])
+    psi0 = oracle.randomimps(256, [2, 2], seed=1)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+    envs = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, info = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-14, maxiter=8), envs=envs)
+    assert info.niter == 8
+    for it in range(6):
+        assert abs(info.losses[it] - g["losses"][it]) <= 1e-6 * g["losses"][it], it
+    for it in range(8):
+        assert abs(info.energies[it] / 2 - g["energy_per_site"][it]) < 1e-10, it
+    # the fixture's own record of the amplification: a rounding-level change of psi0 moves the
+    # oracle's errors by less than 1e-6 up to iteration 5 and by more than that from iteration 7
+    for mode in ("scale", "noise"):
+        split = g["perturbed"][mode]["rel_loss_split"]
+        assert max(split[:6]) < 1e-6 and max(split[7:]) > 1e-6
+
+
+def test_hac_fp32_full_size_hubbard(imps, ctx, oracle):
+    # FP32 MFMA H_AC at the D = 1024 Hubbard shape (w = 6, d = 4) against the FP64 oracle, with the
+    # environments' structural levels the identity (the copy path) and without
+    from test_gpu_parity import _setup_envs
+    for eye in (True, False):
+        D = 1024
+        H, psi, GLs, GRs = _setup_envs(imps, ctx, oracle, oracle.hubbard_mpo, D, 1, seed=2, eye_levels=eye)
+        envs = imps.GPUCache.__new__(imps.GPUCache)
+        envs.ctx, envs.ket, envs.operator = ctx, None, H
+        x = np.random.default_rng(3).standard_normal((D, 4, D))
+        y32 = imps.AC_hamiltonian(0, None, H, None, envs, prec=1)(x)
+        r = rel(y32, oracle.hac_apply_dense(GLs[0], H.Ws[0], GRs[0], x))
+        assert 1e-9 < r < 5e-6, r
+
+
+@pytest.mark.parametrize("prec,label", [(0, "fp64"), (1, "fp32"), (2, "bf16x3")])
+def test_hubbard_d128_mixed_refined_matches_oracle_ground_state(imps, ctx, oracle, prec, label):
+    with open(os.path.join(GOLDEN, "c3_hubbard_d128_groundstate.json")) as f:
+        g = json.load(f)
+    H = imps.models.fermi_hubbard(t=1.0, U=4.0, mu=2.0, nsites=2)
+    psi0 = oracle.randomimps(128, [4, 4], seed=2)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+    envs = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    alg = imps.VUMPS(tol=1e-8, maxiter=500, prec=prec, refine_below=1e-6 if prec else 0.0)
+    psi, envs, info = imps.find_groundstate(st, H, alg, envs=envs)
+    assert info.converged and info.itererr <= 1e-8
+    if prec:
+        assert info.prec_switch_iter > 1  # the reduced-precision phase ran, then FP64
+    assert abs(info.energy / 2 - g["energy_per_site"]) < 1e-9
+    c_up, c_dn, _, n_up, n_dn = oracle.hubbard_local_ops()
+    for k, M in (("density", n_up + n_dn), ("double_occupancy", n_up @ n_dn)):
+        for s in range(2):
+            assert abs(imps.expectationvalue(psi, (s, M), envs) - g["local"][k][s]) < 1e-7, (k, s)
+    # the state breaks the spin symmetry and rotations about the y axis are real: only the
+    # in-plane magnitude of the local moment is fixed (two oracle runs that differ in rounding
+    # alone put <S^z> 4e-8 apart)
+    sp = c_up.T @ c_dn
+    Sz, Sx = 0.5 * (n_up - n_dn), 0.5 * (sp + sp.T)
+    for s in range(2):
+        m = math.hypot(imps.expectationvalue(psi, (s, Sz), envs), imps.expectationvalue(psi, (s, Sx), envs))
+        assert abs(m - math.hypot(g["local"]["sz"][s], g["local"]["sx"][s])) < 1e-6, s
+
+
+def _sweep_problems(imps):
+    rng = np.random.Generator(np.random.PCG64(3))
+    hs = rng.uniform(0.5, 1.5, 64)
+    Ds = rng.choice([32, 64, 128, 256], 64)
+    return hs, Ds, [(imps.models.tfim_hamiltonian(J=1.0, h=float(h), nsites=1), imps.random_tensors(int(D), [2], seed=k))
+                    for k, (h, D) in enumerate(zip(hs, Ds))]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_tfim_sweep_64_instances_full_config(imps, prec):
+    hs, Ds, problems = _sweep_problems(imps)
+    assert sorted(set(int(D) for D in Ds)) == [32, 64, 128, 256]
+    alg = imps.VUMPS(tol=1e-8, maxiter=200, prec=prec, refine_below=1e-6 if prec else 0.0)
+    res = imps.find_groundstates_batched(problems, alg, nworkers=8)
+    assert len(res) == 64
+    for h, r in zip(hs, res):
+        assert r["converged"] and r["itererr"] <= 1e-8
+        assert abs(r["energy"] - tfim_energy(float(h))) < 1e-10, (h, r["energy"])
+
+
+def test_tfim_d128_gradient_hellmann_feynman_closed_form_fd(imps, oracle):
+    h, dh = 1.3, 1e-5
+
+    def solve(hh, psi0=None):
+        H = imps.models.tfim_hamiltonian(J=1.0, h=hh, nsites=1)
+        c = imps.Context(0)
+        if psi0 is None:
+            psi0, c = imps.randomimps(128, H, seed=0, ctx=c, tol=1e-12, maxiter=100)
+        envs = imps.GPUCache(psi0, H, c)
+        psi, envs, info = imps.find_groundstate(psi0, H, imps.VUMPS(tol=1e-11, maxiter=1000), envs=envs)
+        return H, psi, envs, info, c
+
+    H, psi, envs, info, c = solve(h)
+    try:
+        assert info.converged
+        Hp = imps.models.tfim_hamiltonian(J=1.0, h=h + 1.0, nsites=1)
+        G = imps.energy_gradient(psi, H, envs)
+        g = sum(float(np.sum(a * (Wp - W))) for a, W, Wp in zip(G, H.Ws, Hp.Ws))
+        sz = imps.expectationvalue(psi, (0, oracle.sigma_z()), envs)
+        assert abs(g + sz) < 1e-12
+        assert abs(g - tfim_energy_dh(h)) < 1e-9
+        assert abs(info.energy - tfim_energy(h)) < 1e-12
+    finally:
+        c.close()
+    ep = solve(h + dh, psi)
+    em = solve(h - dh, psi)
+    try:
+        fd = (ep[3].energy - em[3].energy) / (2 * dh)
+        assert abs(g - fd) < 1e-8
+    finally:
+        ep[4].close()
+        em[4].close()
+
+
+@pytest.mark.timeout(900)
+def test_xxz_d512_gradient_matches_finite_differences(imps):
+    delta = float(np.random.Generator(np.random.PCG64(5)).uniform(0.5, 1.5))
+    dd = 1e-5
+
+    def solve(dl, psi0=None):
+        H = imps.models.heisenberg_hamiltonian(J=1.0, Delta=dl, nsites=2)
+        c = imps.Context(0)
+        if psi0 is None:
+            psi0, c = imps.randomimps(512, H, seed=1, ctx=c, tol=1e-12, maxiter=100)
+        envs = imps.GPUCache(psi0, H, c)
+        psi, envs, info = imps.find_groundstate(psi0, H, imps.VUMPS(tol=1e-9, maxiter=300), envs=envs)
+        return H, psi, envs, info, c
+
+    H, psi, envs, info, c = solve(delta)
+    try:
+        assert info.itererr < 1e-8
+        Hp = imps.models.heisenberg_hamiltonian(J=1.0, Delta=delta + 1.0, nsites=2)
+        G = imps.energy_gradient(psi, H, envs)
+        g = sum(float(np.sum(a * (Wp - W))) for a, W, Wp in zip(G, H.Ws, Hp.Ws))
+    finally:
+        c.close()
+    rp = solve(delta + dd, psi)
+    rm = solve(delta - dd, psi)
+    try:
+        fd = (rp[3].energy - rm[3].energy) / (2 * dd)
+        assert abs(g - fd) < 1e-8, (g, fd)
+    finally:
+        rp[4].close()
+        rm[4].close()
+
+
+def _explicit_mixed_transfer(oracle, psi1, psi2):
+    D = psi1.AL[0].shape[0]
+    T = np.eye(D * D)
+    for a1, a2 in zip(psi1.AL, psi2.AL):
+        T = np.column_stack([oracle.transfer_left([a2], [a1], e) for e in np.eye(D * D)]) @ T
+    return T
+
+
+@pytest.mark.parametrize("N,D,s1,s2", [(1, 4, 1003, 2003), (2, 3, 1001, 2001), (2, 16, 8, 9)])
+def test_dot_matches_explicit_transfer_matrix(imps, ctx, oracle, N, D, s1, s2):
+    # dot (canonicalmps.jl:262-269) against the dominant eigenvalue of the explicit D^2 x D^2 mixed
+    # transfer matrix: the first two pairs have a complex dominant pair (dot returns the complex
+    # value, fidelity its modulus), the third a real one
+    W = oracle.tfim_mpo()
+    H = imps.SparseIMPO([W] * N)
+    p1, p2 = oracle.randomimps(D, [2] * N, s1), oracle.randomimps(D, [2] * N, s2)
+    st1 = imps.CanonicalIMPS(p1.AL, p1.AR, p1.C, p1.AC)
+    st2 = imps.CanonicalIMPS(p2.AL, p2.AR, p2.C, p2.AC)
+    envs = imps.GPUCache(st1, H, ctx)
+    ev = np.linalg.eigvals(_explicit_mixed_transfer(oracle, p1, p2))
+    lam = ev[np.argmax(np.abs(ev))]
+    got = imps.dot(st1, st2, envs)
+    if abs(lam.imag) > 1e-12:
+        assert isinstance(got, complex)
+        assert abs(got.real - lam.real) < 1e-9 and abs(abs(got.imag) - abs(lam.imag)) < 1e-9
+    else:
+        assert isinstance(got, float) and abs(got - lam.real) < 1e-9 * abs(lam)
+    assert abs(imps.fidelity(st1, st2, envs) - abs(lam) / (imps.norm(st1) * imps.norm(st2))) < 1e-9

@@ -861,3 +861,63 @@ def test_energy_gradient_matches_oracle_and_fd(imps, ctx, oracle):
     fd = (run(d0 + dd)[0][2].energy - run(d0 - dd)[0][2].energy) / (2 * dd)
     assert info.itererr < 1e-9
     assert abs(grad - fd) < 1e-6
+
+
+def test_vumps_finalize_and_solver_objects(imps, ctx, oracle):
+    # VUMPS(finalize, alg_eigsolve, alg_gauge, alg_environments) (algdefs.jl:26-35, vumps.jl:51-86):
+    # an identity finalize leaves the trajectory unchanged bit for bit and runs once per iteration;
+    # a finalize that hands back a sign-regauged copy of the state (the same physical state) makes
+    # the library rebuild the environments and still converges to the same energy; fixed-tolerance
+    # solver objects converge to the oracle's energy; an exception inside finalize propagates
+    W = oracle.tfim_mpo()
+    H = imps.SparseIMPO([W] * 2)
+    psi0 = oracle.randomimps(12, [2, 2], seed=0)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+
+    def run(alg):
+        envs = imps.GPUCache(st, H, ctx)
+        ctx.call("imps_envs_reset")
+        return imps.find_groundstate(st, H, alg, envs=envs)
+
+    _, _, base = run(imps.VUMPS(tol=1e-10))
+    calls = []
+
+    def ident(it, psi, HH, envs):
+        calls.append(it)
+        assert HH is H and len(psi) == 2
+        return psi, envs
+
+    _, _, i1 = run(imps.VUMPS(tol=1e-10, finalize=ident))
+    assert calls == list(range(1, i1.niter + 1))
+    assert i1.losses == base.losses
+
+    def flip(it, psi, HH, envs):
+        if it == 3:
+            psi = imps.CanonicalIMPS([-psi.AL[0], psi.AL[1]], [-psi.AR[0], psi.AR[1]], psi.C,
+                                     [-psi.AC[0], psi.AC[1]])
+        return psi, envs
+
+    _, _, i2 = run(imps.VUMPS(tol=1e-10, finalize=flip))
+    assert i2.converged and abs(i2.energy - base.energy) < 1e-9
+    # the rebuilt environments differ from the warm-started ones at the solver tolerance
+    assert abs(i2.losses[3] - base.losses[3]) < 1e-3 * base.losses[3]
+    fixed = imps.VUMPS(tol=1e-10, alg_eigsolve=imps.alg_eigsolve(dynamic_tols=False, tol=1e-12),
+                       alg_gauge=imps.alg_gauge(dynamic_tols=False, tol=1e-13),
+                       alg_environments=imps.alg_environments(dynamic_tols=False, tol=1e-12))
+    _, _, i3 = run(fixed)
+    _, _, ref = oracle.find_groundstate(psi0, [W] * 2, oracle.VUMPS(tol=1e-10))
+    assert i3.converged and abs(i3.energy - ref.energies[-1]) < 1e-9
+
+    def boom(it, psi, HH, envs):
+        raise RuntimeError("finalize failed on purpose")
+
+    with pytest.raises(RuntimeError, match="on purpose"):
+        run(imps.VUMPS(tol=1e-10, finalize=boom))
+    # IDMRG's finalize runs after every sweep
+    seen = []
+    envs = imps.GPUCache(st, H, ctx)
+    ctx.call("imps_envs_reset")
+    _, _, ii = imps.find_groundstate(st, H, imps.IDMRG(tol=1e-8, maxiter=5,
+                                                         finalize=lambda it, p, h, e: (seen.append(it), (p, e))[1]),
+                                     envs=envs)
+    assert seen == list(range(1, ii.niter + 1))
