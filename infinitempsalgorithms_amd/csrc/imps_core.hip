@@ -1224,12 +1224,14 @@ void qrpos_householder(imps_ctx* c, int m, int n, double* A, double* R) {
 // `upper`, so that G then also holds R = L^T).  Enqueued only: a non-positive pivot sets
 // *fail (substituting 1 so the factorisation continues with finite values); the caller
 // clears *fail beforehand and reads it afterwards.
-void chol_async(imps_ctx* c, int n, double* G, bool upper, int* fail) {
-  double* Ut = ws(c, "chol_ut", 64 * 64);
+// Ut receives the inverted diagonal blocks U_j = L_jj^{-T} (64 x 64 each, ld 64, block j at
+// Ut + 4096 j), which the triangular solves below reuse.
+void chol_async(imps_ctx* c, int n, double* G, bool upper, int* fail, double* Ut_all) {
   double* X = ws(c, "chol_panel", (size_t)n * 64);
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int nb = std::min(64, n - j0);
     double* Djj = G + j0 + (size_t)n * j0;
+    double* Ut = Ut_all + (size_t)4096 * (j0 / 64);
     HIPCHK(launch_chol_diag(nb, Djj, n, Ut, fail, c->stream));
     const int rows = n - j0 - nb;
     if (rows > 0) {
@@ -1244,6 +1246,43 @@ void chol_async(imps_ctx* c, int n, double* G, bool upper, int* fail) {
   if (upper) HIPCHK(launch_mirror_lower(n, G, n, c->stream));
 }
 
+// Triangular solves of the Cholesky QR on the library's GEMM, recursive halving so that the
+// updates are large GEMMs and only the 64-column leaves use the inverted diagonal blocks of
+// chol_async (Ut_all, U_j = L_jj^{-T}, block j at Ut_all + 4096 j); results go out of place.
+//
+// Q (m x n, ld m) <- A R^{-1}, R = L^T with L the lower factor in G (ld ldg).  A's columns are
+// updated in place (A is destroyed).  Columns [c0, c0 + nc):
+//   X_1 = A_1 R_11^{-1};  A_2 -= X_1 R_12 (R_12[k, j] = L[j, k]);  X_2 = A_2 R_22^{-1}.
+void trsm_right_upper(imps_ctx* c, int m, double* A, double* Q, const double* G, long ldg, const double* Ut_all,
+                      int c0, int nc) {
+  const int blocks = (nc + 63) / 64;
+  if (blocks <= 1) {
+    gemm2(c, 'N', 'N', m, nc, nc, 1.0, A + (size_t)m * c0, m, Ut_all + (size_t)4096 * (c0 / 64), 64, 0.0,
+          Q + (size_t)m * c0, m);
+    return;
+  }
+  const int h = 64 * ((blocks + 1) / 2);
+  trsm_right_upper(c, m, A, Q, G, ldg, Ut_all, c0, h);
+  gemm2(c, 'N', 'T', m, nc - h, h, -1.0, Q + (size_t)m * c0, m, G + (c0 + h) + ldg * c0, ldg, 1.0,
+        A + (size_t)m * (c0 + h), m);
+  trsm_right_upper(c, m, A, Q, G, ldg, Ut_all, c0 + h, nc - h);
+}
+
+// Q (m x n, ld m) <- L^{-1} M with L the lower factor in G (ld ldg); M's rows are updated in place.
+// Rows [r0, r0 + nr): Y_1 = L_11^{-1} M_1;  M_2 -= L_21 Y_1;  Y_2 = L_22^{-1} M_2;  L_ii^{-1} = U_i^T.
+void trsm_left_lower(imps_ctx* c, int m, int n, double* M, double* Q, const double* G, long ldg,
+                     const double* Ut_all, int r0, int nr) {
+  const int blocks = (nr + 63) / 64;
+  if (blocks <= 1) {
+    gemm2(c, 'T', 'N', nr, n, nr, 1.0, Ut_all + (size_t)4096 * (r0 / 64), 64, M + r0, m, 0.0, Q + r0, m);
+    return;
+  }
+  const int h = 64 * ((blocks + 1) / 2);
+  trsm_left_lower(c, m, n, M, Q, G, ldg, Ut_all, r0, h);
+  gemm2(c, 'N', 'N', nr - h, n, h, -1.0, G + (r0 + h) + ldg * r0, ldg, Q + r0, m, 1.0, M + r0 + h, m);
+  trsm_left_lower(c, m, n, M, Q, G, ldg, Ut_all, r0 + h, nr - h);
+}
+
 int read_flag(imps_ctx* c, const int* flag) {
   int h = 0;
   std::memcpy(&h, d2h_small(c, flag, sizeof(int)), sizeof(int));
@@ -1254,33 +1293,44 @@ int read_flag(imps_ctx* c, const int* flag) {
 // is overwritten by Q, R (n x n) gets the upper factor with a positive diagonal, which is
 // the QRpos factorisation.  Returns false (A restored) if a Cholesky factorisation breaks
 // down.
-bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R) {
+// With `defer` the breakdown flag is the caller's (cleared by the caller, read later) and A is not
+// backed up: the call returns true at once and the caller redoes its work on a breakdown.
+bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullptr) {
   const size_t nn = (size_t)n * n, mn = (size_t)m * n;
   double* G = ws(c, "cq_G", nn);
   double* R2 = ws(c, "cq_R2", nn);
-  double* bak = ws(c, "cq_bak", mn);
-  int* fail = (int*)ws(c, "chol_info", 1);
-  d2d(c, bak, A, mn);
-  HIPCHK(launch_fill_int(1, 0, fail, c->stream));
-  const double one = 1.0;
+  double* bak = defer ? nullptr : ws(c, "cq_bak", mn);
+  int* fail = defer ? defer : (int*)ws(c, "chol_info", 1);
+  double* Ut = ws(c, "chol_ut", (size_t)4096 * ((n + 63) / 64));
+  double* Rp = ws(c, "cq_Rp", nn);
+  double* W = ws(c, "cq_W", mn);
+  if (!defer) {
+    d2d(c, bak, A, mn);
+    HIPCHK(launch_fill_int(1, 0, fail, c->stream));
+  }
+  // the passes alternate between A and W (each triangular solve writes out of place)
+  double* cur = A;
+  double* nxt = W;
   for (int pass = 0; pass < 3; ++pass) {
-    gemm2(c, 'T', 'N', n, n, m, 1.0, A, m, A, m, 0.0, G, n);
+    gemm2(c, 'T', 'N', n, n, m, 1.0, cur, m, cur, m, 0.0, G, n);
     if (pass == 0)
       HIPCHK(launch_add_diag_trace(n, 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16, G, n,
                                    c->stream));
-    chol_async(c, n, G, true, fail);
-    RBCHK(rocblas_dtrsm(c->rb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                        rocblas_diagonal_non_unit, m, n, &one, G, n, A, m));
-    if (pass == 0) {
-      // potrf leaves the strictly lower triangle of G untouched: clear it before R is
-      // used as a general matrix by trmm
-      d2d(c, R, G, nn);
-      HIPCHK(launch_tri_keep(n, R, n, true, c->stream));
-    } else {
-      RBCHK(rocblas_dtrmm(c->rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                          rocblas_diagonal_non_unit, n, n, &one, G, n, R, n, R2, n));
+    chol_async(c, n, G, true, fail, Ut);
+    trsm_right_upper(c, m, cur, nxt, G, n, Ut, 0, n);  // nxt <- cur R_pass^{-1}
+    std::swap(cur, nxt);
+    // R_pass = L^T, the upper triangle of G (its strict lower triangle holds L)
+    HIPCHK(launch_copy2d(n, n, G, n, pass == 0 ? R : Rp, n, c->stream));
+    HIPCHK(launch_tri_keep(n, pass == 0 ? R : Rp, n, true, c->stream));
+    if (pass > 0) {
+      gemm2(c, 'N', 'N', n, n, n, 1.0, Rp, n, R, n, 0.0, R2, n);  // R <- R_pass R
       d2d(c, R, R2, nn);
     }
+  }
+  d2d(c, A, cur, mn);  // no-op when the result is already in A
+  if (defer) {
+    HIPCHK(launch_tri_keep(n, R, n, true, c->stream));
+    return true;
   }
   // a breakdown in any pass: restore A (the passes after it ran on substituted pivots)
   if (read_flag(c, fail) != 0) {
@@ -1293,31 +1343,41 @@ bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R) {
 
 // LQ counterpart: M (m x n, ld m, m <= n) = L Q computed as Q (in place of M) and L (m x m,
 // lower, positive diagonal).
-bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L) {
+bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L, int* defer = nullptr) {
   const size_t mm = (size_t)m * m, mn = (size_t)m * n;
   double* G = ws(c, "cq_G", mm);
   double* L2 = ws(c, "cq_R2", mm);
-  double* bak = ws(c, "cq_bak", mn);
-  int* fail = (int*)ws(c, "chol_info", 1);
-  d2d(c, bak, M, mn);
-  HIPCHK(launch_fill_int(1, 0, fail, c->stream));
-  const double one = 1.0;
+  double* bak = defer ? nullptr : ws(c, "cq_bak", mn);
+  int* fail = defer ? defer : (int*)ws(c, "chol_info", 1);
+  double* Ut = ws(c, "chol_ut", (size_t)4096 * ((m + 63) / 64));
+  double* Lp = ws(c, "cq_Rp", mm);
+  double* W = ws(c, "cq_W", mn);
+  if (!defer) {
+    d2d(c, bak, M, mn);
+    HIPCHK(launch_fill_int(1, 0, fail, c->stream));
+  }
+  double* cur = M;
+  double* nxt = W;
   for (int pass = 0; pass < 3; ++pass) {
-    gemm2(c, 'N', 'T', m, m, n, 1.0, M, m, M, m, 0.0, G, m);
+    gemm2(c, 'N', 'T', m, m, n, 1.0, cur, m, cur, m, 0.0, G, m);
     if (pass == 0)
       HIPCHK(launch_add_diag_trace(m, 11.0 * ((double)m * n + (double)m * (m + 1)) * 1.1102230246251565e-16, G, m,
                                    c->stream));
-    chol_async(c, m, G, false, fail);
-    RBCHK(rocblas_dtrsm(c->rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                        rocblas_diagonal_non_unit, m, n, &one, G, m, M, m));
-    if (pass == 0) {
-      d2d(c, L, G, mm);
-      HIPCHK(launch_tri_keep(m, L, m, false, c->stream));
-    } else {
-      RBCHK(rocblas_dtrmm(c->rb, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
-                          rocblas_diagonal_non_unit, m, m, &one, G, m, L, m, L2, m));
+    chol_async(c, m, G, false, fail, Ut);
+    trsm_left_lower(c, m, n, cur, nxt, G, m, Ut, 0, m);  // nxt <- L_pass^{-1} cur
+    std::swap(cur, nxt);
+    // L_pass: the lower triangle of G
+    HIPCHK(launch_copy2d(m, m, G, m, pass == 0 ? L : Lp, m, c->stream));
+    HIPCHK(launch_tri_keep(m, pass == 0 ? L : Lp, m, false, c->stream));
+    if (pass > 0) {
+      gemm2(c, 'N', 'N', m, m, m, 1.0, L, m, Lp, m, 0.0, L2, m);  // L <- L L_pass
       d2d(c, L, L2, mm);
     }
+  }
+  d2d(c, M, cur, mn);
+  if (defer) {
+    HIPCHK(launch_tri_keep(m, L, m, false, c->stream));
+    return true;
   }
   if (read_flag(c, fail) != 0) {
     d2d(c, M, bak, mn);
@@ -1331,7 +1391,11 @@ std::atomic<long> g_qr_fallbacks(0);
 
 // A (m x n, lda = m, m >= n) is overwritten by Q; R (n x n) receives the factor with a
 // non-negative diagonal (FiniteMPSAlgorithms' QRpos).
-void qrpos(imps_ctx* c, int m, int n, double* A, double* R) {
+void qrpos(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullptr) {
+  if (defer) {
+    cholqr3(c, m, n, A, R, defer);
+    return;
+  }
   if (cholqr3(c, m, n, A, R)) return;
   g_qr_fallbacks++;
   qrpos_householder(c, m, n, A, R);
@@ -1339,8 +1403,12 @@ void qrpos(imps_ctx* c, int m, int n, double* A, double* R) {
 
 // M (m x n, ld m, m <= n) = L Q with L (m x m) lower, non-negative diagonal and Q (m x n)
 // row-orthonormal (LQpos).  Q may alias M.
-void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q) {
+void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q, int* defer = nullptr) {
   d2d(c, Q, M, (size_t)m * n);
+  if (defer) {
+    cholqr3_lq(c, m, n, Q, L, defer);
+    return;
+  }
   if (cholqr3_lq(c, m, n, Q, L)) return;
   g_qr_fallbacks++;
   double* Mt = ws(c, "lq_Mt", (size_t)m * n);
@@ -1352,6 +1420,32 @@ void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q) {
 }
 
 // ------------------------------------------------------------------ gauge fixing
+
+// x /= ||x|| on the device (the norm reduced in a fixed order, no host round trip)
+void normalize_dev(imps_ctx* c, size_t n, double* x) {
+  double* sq = ws(c, "nd_sq", 1);
+  HIPCHK(launch_multidot(n, 1, x, n, x, c->dpart, c->stream));
+  HIPCHK(launch_partials_reduce(1, dot_blocks(n), c->dpart, sq, c->stream));
+  HIPCHK(launch_scale_inv_sqrt(n, x, sq, c->stream));
+}
+
+// The end of one gauge sweep (ortho.jl:176-201 / 135-163): Cn /= ||Cn||, then eps = ||Cn - Cpre||
+// (Cpre is overwritten), read back together with the sweep's Cholesky breakdown flag in one
+// host round trip.  Returns eps; *broke receives the flag.
+double sweep_end(imps_ctx* c, size_t n, double* Cn, double* Cpre, const int* flag, bool* broke) {
+  normalize_dev(c, n, Cn);
+  HIPCHK(launch_axpby(n, 1.0, Cn, -1.0, Cpre, c->stream));
+  double* e2 = ws(c, "sw_e2", 1);
+  HIPCHK(launch_multidot(n, 1, Cpre, n, Cpre, c->dpart, c->stream));
+  HIPCHK(launch_partials_reduce(1, dot_blocks(n), c->dpart, e2, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hscal, e2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hscal + 1, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int f = 0;
+  std::memcpy(&f, c->hscal + 1, sizeof(int));
+  *broke = f != 0;
+  return std::sqrt(std::max(0.0, c->hscal[0]));
+}
 
 constexpr double kTolMin = 1.0e-14, kTolMax = 1.0e-4;
 
@@ -1367,11 +1461,13 @@ int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* 
   const size_t DD = (size_t)D * D;
   double* Cn = c->C[N - 1];
   double* Cpre = ws(c, "ro_Cpre", DD);
+  double* Csave = ws(c, "ro_Csave", DD);
   double* M = ws(c, "ro_M", (size_t)D * d * D);
   double* ev = ws(c, "ro_ev", DD);
+  int* sflag = (int*)ws(c, "ro_flag", 1);
   d2d(c, Cpre, C0, DD);
   d2d(c, Cn, Cpre, DD);
-  scal(c, DD, 1.0 / nrm2(c, DD, Cn), Cn);
+  normalize_dev(c, DD, Cn);
   int it = 0;
   double eps = std::numeric_limits<double>::infinity();
   while (true) {
@@ -1392,13 +1488,27 @@ int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* 
       lqpos(c, D, D, ev, Cn, Ldum);
     }
     d2d(c, Cpre, Cn, DD);
+    d2d(c, Csave, Cn, DD);
+    // the sweep runs with the Cholesky breakdown checks deferred to its end (one host round trip
+    // per sweep); on a breakdown it is redone from its input with per-site checks and the
+    // Householder fallback
+    HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
     for (int i = N - 1; i >= 0; --i) {
       gemm2(c, 'N', 'N', D * d, D, D, 1.0, A[i], (long)D * d, c->C[i], D, 0.0, M, (long)D * d);
-      lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i]);
+      lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i], sflag);
     }
-    scal(c, DD, 1.0 / nrm2(c, DD, Cn), Cn);
-    axpby(c, DD, 1.0, Cn, -1.0, Cpre);
-    eps = nrm2(c, DD, Cpre);
+    bool broke = false;
+    eps = sweep_end(c, DD, Cn, Cpre, sflag, &broke);
+    if (broke) {
+      d2d(c, Cn, Csave, DD);
+      d2d(c, Cpre, Csave, DD);
+      for (int i = N - 1; i >= 0; --i) {
+        gemm2(c, 'N', 'N', D * d, D, D, 1.0, A[i], (long)D * d, c->C[i], D, 0.0, M, (long)D * d);
+        lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i]);
+      }
+      HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
+      eps = sweep_end(c, DD, Cn, Cpre, sflag, &broke);
+    }
     it++;
     if (eps < tol) return it;
     if (it > maxiter) {
@@ -1415,10 +1525,12 @@ int uniform_leftorth(imps_ctx* c, const std::vector<double*>& A, const double* C
   const size_t DD = (size_t)D * D;
   double* Cn = c->C[N - 1];
   double* Cpre = ws(c, "lo_Cpre", DD);
+  double* Csave = ws(c, "lo_Csave", DD);
   double* ev = ws(c, "lo_ev", DD);
+  int* sflag = (int*)ws(c, "lo_flag", 1);
   d2d(c, Cpre, C0, DD);
   d2d(c, Cn, Cpre, DD);
-  scal(c, DD, 1.0 / nrm2(c, DD, Cn), Cn);
+  normalize_dev(c, DD, Cn);
   int it = 0;
   double eps = std::numeric_limits<double>::infinity();
   while (true) {
@@ -1440,14 +1552,26 @@ int uniform_leftorth(imps_ctx* c, const std::vector<double*>& A, const double* C
       d2d(c, Cn, Rf, DD);
     }
     d2d(c, Cpre, Cn, DD);
+    d2d(c, Csave, Cn, DD);
+    HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
     for (int i = 0; i < N; ++i) {
-      // M = C[i-1] A[i] written straight into AL[i], then QR in place
+      // M = C[i-1] A[i] written straight into AL[i], then QR in place (checks deferred as in
+      // uniform_rightorth)
       gemm2(c, 'N', 'N', D, d * D, D, 1.0, c->C[(i - 1 + N) % N], D, A[i], D, 0.0, c->AL[i], D);
-      qrpos(c, D * d, D, c->AL[i], c->C[i]);
+      qrpos(c, D * d, D, c->AL[i], c->C[i], sflag);
     }
-    scal(c, DD, 1.0 / nrm2(c, DD, Cn), Cn);
-    axpby(c, DD, 1.0, Cn, -1.0, Cpre);
-    eps = nrm2(c, DD, Cpre);
+    bool broke = false;
+    eps = sweep_end(c, DD, Cn, Cpre, sflag, &broke);
+    if (broke) {
+      d2d(c, Cn, Csave, DD);
+      d2d(c, Cpre, Csave, DD);
+      for (int i = 0; i < N; ++i) {
+        gemm2(c, 'N', 'N', D, d * D, D, 1.0, c->C[(i - 1 + N) % N], D, A[i], D, 0.0, c->AL[i], D);
+        qrpos(c, D * d, D, c->AL[i], c->C[i]);
+      }
+      HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
+      eps = sweep_end(c, DD, Cn, Cpre, sflag, &broke);
+    }
     it++;
     if (eps < tol) return it;
     if (it > maxiter) {
