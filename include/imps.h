@@ -236,6 +236,10 @@ typedef struct imps_vumps_params {
   int envs_krylovdim;     /* DMRGCache krylovdim = 30 */
   imps_finalize_cb finalize; /* NULL: Defaults._finalize (identity) */
   void* finalize_user;
+  int refine_eig;         /* mixed precision, once the FP64 refinement starts: 1 (default) the
+                             eigensolves keep the reduced-precision operator for Jacobi-Davidson
+                             corrections (MINRES) of FP64 residuals, until |H x - theta x| <= tol;
+                             0: FP64 Lanczos */
 } imps_vumps_params;
 
 /* IterativeConvergenceInfo (algdefs.jl:180-194) plus timings. */
@@ -258,6 +262,7 @@ typedef struct imps_vumps_info {
   int gauge_sweeps;  /* LQ sweeps summed over iterations */
   int gmres_ops;     /* environment-solver operator applications */
   int qr_fallbacks;  /* Cholesky-QR breakdowns answered by Householder QR */
+  int refine_fallbacks; /* refined eigensolves that fell back to FP64 Lanczos */
 } imps_vumps_info;
 
 /* Per-iteration callback: return nonzero to stop the loop. */

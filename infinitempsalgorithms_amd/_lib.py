@@ -52,7 +52,7 @@ class VumpsParams(ctypes.Structure):
                 ("gauge_dynamic", c_int), ("gauge_tol", c_double), ("gauge_tol_min", c_double),
                 ("gauge_tol_max", c_double), ("envs_dynamic", c_int), ("envs_tol", c_double),
                 ("envs_tol_min", c_double), ("envs_tol_max", c_double), ("envs_krylovdim", c_int),
-                ("finalize", FINALIZE_CB), ("finalize_user", c_void_p)]
+                ("finalize", FINALIZE_CB), ("finalize_user", c_void_p), ("refine_eig", c_int)]
 
 
 class VumpsInfo(ctypes.Structure):
@@ -60,7 +60,8 @@ class VumpsInfo(ctypes.Structure):
                 ("seconds", c_double), ("t_galerkin_reached", c_double), ("hac_applies", c_int),
                 ("hc_applies", c_int), ("prec_switch_iter", c_int), ("t_local", c_double),
                 ("t_gauge", c_double), ("t_envs", c_double), ("t_galerkin", c_double), ("t_energy", c_double),
-                ("gauge_sweeps", c_int), ("gmres_ops", c_int), ("qr_fallbacks", c_int)]
+                ("gauge_sweeps", c_int), ("gmres_ops", c_int), ("qr_fallbacks", c_int),
+                ("refine_fallbacks", c_int)]
 
 
 class HacTiming(ctypes.Structure):

@@ -695,6 +695,7 @@ def _params(alg, prec, refine_below, eig_floor):
     p.krylovdim, p.eig_maxiter, p.gauge_maxiter, p.envs_maxiter = (alg.krylovdim, alg.eig_maxiter, alg.gauge_maxiter,
                                                                    alg.envs_maxiter)
     p.prec, p.refine_below, p.eig_floor = prec, refine_below, eig_floor
+    p.refine_eig = int(bool(getattr(alg, "refine_eig", True)))
     for which, obj in (("eig", alg.alg_eigsolve), ("gauge", alg.alg_gauge), ("envs", alg.alg_environments)):
         if obj is not None:
             _set_solver(p, which, obj)
@@ -714,7 +715,9 @@ class VUMPS:
     the one passed in becomes the context's state (its environments are then rebuilt).  `prec`
     selects the arithmetic of the bulk contractions; with a mixed precision the loop refines in
     FP64 once the Galerkin error drops below `refine_below`; `eig_floor` (relative to |λ|) bounds
-    the eigensolver tolerance of the reduced-precision phase (0: the library's default)."""
+    the eigensolver tolerance of the reduced-precision phase (0: the library's default);
+    `refine_eig` (default) keeps the reduced-precision operator after the switch for Jacobi-Davidson
+    corrections of FP64 residuals instead of switching the eigensolves to FP64 Lanczos."""
 
     D: int = 64
     tol: float = 1e-10
@@ -732,6 +735,7 @@ class VUMPS:
     prec: int = _lib.IMPS_PREC_FP64
     refine_below: float = 0.0
     eig_floor: float = 0.0
+    refine_eig: bool = True
     alg_eigsolve: object = None
     alg_gauge: object = None
     alg_environments: object = None
@@ -852,7 +856,7 @@ def find_groundstate(psi0: CanonicalIMPS | None, H: SparseIMPO, alg: VUMPS | IDM
                                    info.hc_applies, info.prec_switch_iter, energies,
                                    {k: getattr(info, k) for k in ("t_local", "t_gauge", "t_envs", "t_galerkin",
                                                                   "t_energy", "gauge_sweeps", "gmres_ops",
-                                                                  "qr_fallbacks")})
+                                                                  "qr_fallbacks", "refine_fallbacks")})
     return psi, envs, res
 
 

@@ -18,6 +18,7 @@
 #include <limits>
 #include <map>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <tuple>
 #include <string>
@@ -103,6 +104,7 @@ struct imps_ctx {
   std::vector<imps_ctx*> workers;
   hipEvent_t ev_ready = nullptr;  // parent: state and environments enqueued
   hipEvent_t ev_done = nullptr;   // worker: its solve enqueued
+  std::set<long> pretuned;        // (D, w, d, prec) whose H_AC/H_C GEMM classes were tuned uncontended
 };
 
 static thread_local std::string g_create_error;
@@ -969,6 +971,106 @@ double lanczos(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xo
   return theta[0];
 }
 
+// x /= ||x|| on the device (the norm reduced in a fixed order, no host round trip)
+void normalize_dev(imps_ctx* c, size_t n, double* x) {
+  double* sq = ws(c, "nd_sq", 1);
+  HIPCHK(launch_multidot(n, 1, x, n, x, c->dpart, c->stream));
+  HIPCHK(launch_partials_reduce(1, dot_blocks(n), c->dpart, sq, c->stream));
+  HIPCHK(launch_scale_inv_sqrt(n, x, sq, c->stream));
+}
+
+// Lowest eigenpair refined from a good start vector with a reduced-precision operator (the FP64
+// refinement of the mixed-precision VUMPS eigensolves).  Outer loop in FP64: theta = <x, H x>,
+// r = H x - theta x, stop when |r| <= tol (the residual Lanczos' eager test bounds); else solve the
+// Jacobi-Davidson correction equation P (H_low - theta) P t = -r, t _|_ x (P = I - x x^T) with
+// MINRES to relative accuracy max(tol / (2 |r|), 1e-6), and x <- (x + t) / |x + t|.  The reduced
+// precision only enters the correction, whose relative error it bounds, so the FP64 residual keeps
+// falling below the reduced operator's own rounding level.  MINRES runs entirely on the device
+// (launch_minres_*: no Gram-Schmidt against a basis, no host round trip per step); the host reads
+// its residual estimate every kCheck steps.  Returns false (xout then holds the last iterate) when
+// tol is not reached within max_outer corrections or a correction does not converge in max_inner
+// steps; the caller falls back to FP64 Lanczos.  *lambda receives theta of the last FP64 residual.
+bool jd_refine(imps_ctx* c, const Op& op64, const Op& oplow, size_t n, const double* x0, double* xout, double tol,
+               int max_outer, int max_inner, double* lambda, KInfo* info) {
+  constexpr int kCheck = 8;
+  double* y = ws(c, "jd_y", n);
+  double* r = ws(c, "jd_r", n);
+  double* t = ws(c, "jd_t", n);
+  double* vb[3] = {ws(c, "jd_v0", n), ws(c, "jd_v1", n), ws(c, "jd_v2", n)};
+  double* wb[2] = {ws(c, "jd_w0", n), ws(c, "jd_w1", n)};
+  double* S = ws(c, "jd_S", MR_NS);
+  const int nb = dot_blocks(n);
+  double* part = ws(c, "jd_part", 2 * (size_t)nb);
+  const double nx = nrm2(c, n, x0);
+  require(nx > 0 && std::isfinite(nx), IMPS_E_ARG, "refine: zero or non-finite start vector");
+  axpby(c, n, 1.0 / nx, x0, 0.0, xout);
+  int nops = 0, outer = 0;
+  double res = 0.0, theta = 0.0;
+  bool ok = false;
+  for (outer = 0; outer <= max_outer; ++outer) {
+    op64(xout, y);
+    nops++;
+    theta = dot(c, n, xout, y);
+    axpby(c, n, 1.0, y, 0.0, r);
+    axpby(c, n, -theta, xout, 1.0, r);
+    res = nrm2(c, n, r);
+    if (res <= tol) {
+      ok = true;
+      break;
+    }
+    if (outer == max_outer || !std::isfinite(res)) break;
+    // MINRES on P (H_low - theta) P t = -r from t = 0: v_1 = -r / |r|
+    const double rtol = std::max(0.5 * tol / res, 1e-6);
+    double* vold = vb[0];
+    double* v = vb[1];
+    double* z = vb[2];
+    double* wold = wb[0];
+    double* w = wb[1];
+    fill(c, n, 0.0, vold);
+    fill(c, n, 0.0, wold);
+    fill(c, n, 0.0, w);
+    fill(c, n, 0.0, t);
+    axpby(c, n, -1.0 / res, r, 0.0, v);
+    HIPCHK(launch_minres_init(S, res, c->stream));
+    bool inner_ok = false;
+    for (int k = 1; k <= max_inner; ++k) {
+      oplow(v, y);
+      nops++;
+      HIPCHK(launch_dot2(n, v, xout, y, part, c->stream));
+      HIPCHK(launch_partials_reduce(2, nb, part, S + MR_DELTA, c->stream));
+      HIPCHK(launch_minres_z(n, y, xout, v, vold, S, z, part, c->stream));
+      HIPCHK(launch_partials_reduce(1, nb, part, S + MR_ZZ, c->stream));
+      HIPCHK(launch_minres_scalar(S, theta, c->stream));
+      HIPCHK(launch_minres_update(n, v, wold, w, t, z, S, c->stream));
+      // rotate: w_new sits in wold; the normalised z is the next v
+      std::swap(wold, w);
+      double* vn = z;
+      z = vold;
+      vold = v;
+      v = vn;
+      if (k % kCheck == 0 || k == max_inner) {
+        const double* hs = d2h_small(c, S, sizeof(double) * MR_NS);
+        if (std::fabs(hs[MR_ETA]) <= rtol * res || hs[MR_DONE] != 0.0) {
+          inner_ok = true;
+          break;
+        }
+        if (!std::isfinite(hs[MR_ETA])) break;
+      }
+    }
+    if (!inner_ok) break;
+    axpby(c, n, 1.0, t, 1.0, xout);
+    normalize_dev(c, n, xout);
+  }
+  if (lambda) *lambda = theta;
+  if (info) {
+    info->converged = ok ? 1 : 0;
+    info->numiter = outer + 1;
+    info->numops = nops;
+    info->normres = res;
+  }
+  return ok;
+}
+
 // exp(t op) x0 for a symmetric operator and real t: KrylovKit.exponentiate with Lanczos, the
 // integrator of TDVP (tdvp.jl:14-19).  A Lanczos basis (two-pass Gram-Schmidt on the device) grows
 // from the current vector; with T = U diag(theta) U^T on the host, y(tau) = U exp(tau theta) U^T e1
@@ -1421,13 +1523,6 @@ void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q, int
 
 // ------------------------------------------------------------------ gauge fixing
 
-// x /= ||x|| on the device (the norm reduced in a fixed order, no host round trip)
-void normalize_dev(imps_ctx* c, size_t n, double* x) {
-  double* sq = ws(c, "nd_sq", 1);
-  HIPCHK(launch_multidot(n, 1, x, n, x, c->dpart, c->stream));
-  HIPCHK(launch_partials_reduce(1, dot_blocks(n), c->dpart, sq, c->stream));
-  HIPCHK(launch_scale_inv_sqrt(n, x, sq, c->stream));
-}
 
 // The end of one gauge sweep (ortho.jl:176-201 / 135-163): Cn /= ||Cn||, then eps = ||Cn - Cpre||
 // (Cpre is overwritten), read back together with the sweep's Cholesky breakdown flag in one
@@ -1964,6 +2059,41 @@ EigOut eigsolve_ac(imps_ctx* c, int site, double tol, int kdim, int maxiter, boo
   return o;
 }
 
+// The mixed-precision refinement of eigsolve_ac / eigsolve_c: jd_refine with the FP64 operator for
+// the residual and the `prec` operator for the corrections; FP64 Lanczos when it does not converge.
+std::atomic<long> g_refine_fallbacks(0);
+
+EigOut eigsolve_refined(imps_ctx* c, int site, bool ac, double tol, int kdim, int maxiter, bool eager, int prec) {
+  site = wrap(c, site);
+  const size_t n = ac ? (size_t)c->D * c->d * c->D : (size_t)c->D * c->D;
+  struct PlanesScope {
+    imps_ctx* c;
+    ~PlanesScope() {
+      c->planes_site = -1;
+      c->hc_planes_site = -1;
+    }
+  } scope{c};
+  if (prec == PREC_BF16X3 && ac && hac_planes_ok(c, site)) {
+    hac_planes_prepare(c, site);
+    c->planes_site = site;
+  }
+  if (prec == PREC_BF16X3 && !ac && hc_planes_ok(c, site)) {
+    hc_planes_prepare(c, site);
+    c->hc_planes_site = site;
+  }
+  Op op64 = ac ? Op([&](const double* x, double* y) { hac_apply(c, site, x, y, PREC_FP64); })
+               : Op([&](const double* x, double* y) { hc_apply(c, site, x, y, PREC_FP64); });
+  Op oplow = ac ? Op([&](const double* x, double* y) { hac_apply(c, site, x, y, prec); })
+                : Op([&](const double* x, double* y) { hc_apply(c, site, x, y, prec); });
+  const double* x0 = ac ? c->AC[site] : c->C[site];
+  double* xout = ac ? c->pAC[site] : c->pC[site];
+  EigOut o;
+  if (jd_refine(c, op64, oplow, n, x0, xout, tol, 4, 2000, &o.lambda, &o.info)) return o;
+  g_refine_fallbacks++;
+  o.lambda = lanczos(c, op64, n, x0, xout, tol, kdim, maxiter, eager, &o.info);
+  return o;
+}
+
 EigOut eigsolve_c(imps_ctx* c, int site, double tol, int kdim, int maxiter, bool eager, int prec) {
   const size_t n = (size_t)c->D * c->D;
   EigOut o;
@@ -2034,9 +2164,33 @@ void destroy_worker(imps_ctx* w) {
 // current state and environments, so the 2N solves run concurrently, one host thread and one
 // stream each (their device work overlaps; each solve's arithmetic is the same as when run
 // alone).  Then regauge! per site on this context's stream.
+// The GEMM classes of H_AC and H_C are autotuned on this context's stream, with nothing else
+// running, before the concurrent eigensolves would tune them on contended worker streams (one
+// application of each operator per site, once per state shape and precision).
+void pretune_local_ops(imps_ctx* c, int prec) {
+  if (g_gemm_autotune.load(std::memory_order_relaxed) == 0 || (prec != PREC_FP64 && prec != PREC_FP32)) return;
+  const long key = (((long)c->D * 64 + c->w) * 64 + c->d) * 4 + prec;
+  if (c->pretuned.count(key)) return;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t nA = (size_t)c->D * c->d * c->D, nC = (size_t)c->D * c->D;
+  double* y = ws(c, "pt_y", nA);
+  for (int site = 0; site < c->N; ++site) {
+    hac_apply(c, site, c->AC[site], y, prec);
+    hc_apply(c, site, c->C[site], y, prec);
+  }
+  (void)nC;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->hac_applies -= c->N;
+  c->hc_applies -= c->N;
+  c->pretuned.insert(key);
+}
+
+// refine: the eigensolves are eigsolve_refined with `prec` as the correction operator (FP64 residuals)
 void local_update(imps_ctx* c, const std::vector<double>& tol_ac, const std::vector<double>& tol_c, int kdim,
-                  int maxiter, bool eager, int prec, std::vector<double>& lam_ac, std::vector<double>& lam_c) {
+                  int maxiter, bool eager, int prec, std::vector<double>& lam_ac, std::vector<double>& lam_c,
+                  bool refine = false) {
   const int N = c->N;
+  pretune_local_ops(c, refine ? PREC_FP64 : prec);
   if (!c->ev_ready) HIPCHK(hipEventCreateWithFlags(&c->ev_ready, hipEventDisableTiming));
   HIPCHK(hipEventRecord(c->ev_ready, c->stream));
   const int njobs = 2 * N;
@@ -2054,7 +2208,11 @@ void local_update(imps_ctx* c, const std::vector<double>& tol_ac, const std::vec
       try {
         HIPCHK(hipSetDevice(w->device));
         const int site = k / 2;
-        if (k % 2 == 0)
+        if (refine)
+          (k % 2 == 0 ? lam_ac : lam_c)[site] =
+              eigsolve_refined(w, site, k % 2 == 0, k % 2 == 0 ? tol_ac[site] : tol_c[site], kdim, maxiter, eager, prec)
+                  .lambda;
+        else if (k % 2 == 0)
           lam_ac[site] = eigsolve_ac(w, site, tol_ac[site], kdim, maxiter, eager, prec).lambda;
         else
           lam_c[site] = eigsolve_c(w, site, tol_c[site], kdim, maxiter, eager, prec).lambda;
@@ -2563,6 +2721,7 @@ int imps_vumps_default_params(imps_vumps_params* p) {
   p->envs_krylovdim = 30;
   p->finalize = nullptr;
   p->finalize_user = nullptr;
+  p->refine_eig = 1;
   return IMPS_OK;
 }
 
@@ -2635,6 +2794,8 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     };
     if (!c->envs_valid) lap(te, [&] { envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->envs_krylovdim, &est); });
     int prec = p->prec;
+    bool refining = false;  // refinement phase with JD corrections in p->prec (refine_eig)
+    const long rf0 = g_refine_fallbacks;
     int switch_iter = -1;
     std::vector<double> lam_ac(N, 1.0), lam_c(N, 1.0);
     double best_eps = std::numeric_limits<double>::infinity();
@@ -2651,15 +2812,20 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       const int siter = it - 1;
       // FP64 refinement: below refine_below, or as soon as the reduced-precision phase
       // stops improving (an iteration after the third that gains less than 10 % on the best
-      // Galerkin error so far: the operator's rounding level has been reached)
-      if (prec != PREC_FP64 && (eps < p->refine_below || (it > 4 && stalled))) {
-        prec = PREC_FP64;
+      // Galerkin error so far: the operator's rounding level has been reached).  With refine_eig
+      // the eigensolves from then on are FP64-residual refinements whose corrections keep using
+      // the reduced precision (eigsolve_refined); otherwise they become FP64 Lanczos.
+      if (prec != PREC_FP64 && !refining && (eps < p->refine_below || (it > 4 && stalled))) {
         switch_iter = it;
+        if (p->refine_eig)
+          refining = true;
+        else
+          prec = PREC_FP64;
       }
       double tol_eig = stol.eig(siter, eps);
       // a reduced-precision operator cannot resolve residuals below its own rounding
       // level: the eigensolver tolerance is floored there until the FP64 refinement
-      const double floor_rel = prec == PREC_FP64 ? 0.0
+      const double floor_rel = (prec == PREC_FP64 || refining) ? 0.0
                                : p->eig_floor > 0.0 ? p->eig_floor
                                                     : (prec == PREC_FP32 ? 1e-6 : 1e-4);
       lap(tl, [&] {
@@ -2668,7 +2834,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
           ta[site] = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_ac[site])));
           tc[site] = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_c[site])));
         }
-        local_update(c, ta, tc, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, prec, lam_ac, lam_c);
+        local_update(c, ta, tc, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, prec, lam_ac, lam_c, refining);
       });
       int sw = 0;
       lap(tg, [&] { gauge_step(c, stol.gauge(siter, eps), p->gauge_maxiter, &sw); });
@@ -2713,6 +2879,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       info->gauge_sweeps = sweeps_total;
       info->gmres_ops = est.ops;
       info->qr_fallbacks = (int)(g_qr_fallbacks - qf0);
+      info->refine_fallbacks = (int)(g_refine_fallbacks - rf0);
     }
   });
 }

@@ -94,6 +94,23 @@ hipError_t launch_multidot(size_t n, int nv, const double* X, size_t ldx, const 
 // per-block shares of X_v . w (dots) or of w . w (!dots), laid out as launch_multidot's
 hipError_t launch_axpy_reduce(size_t n, int nv, const double* X, size_t ldx, const double* coef, double* w,
                               double* part, bool dots, hipStream_t s);
+// MINRES on the Jacobi-Davidson correction equation with its scalars on the device: S holds
+// MR_NS doubles (the recurrence state and the coefficients of the next vector update).
+enum {
+  MR_DELTA = 0, MR_GAMMA, MR_ZZ, MR_BETA, MR_COLD, MR_C, MR_SOLD, MR_S, MR_ETA,
+  MR_KV, MR_KWOLD, MR_KW, MR_KT, MR_INVB, MR_DONE, MR_NS
+};
+// part[0..nb) <- block shares of a . y, part[nb..2nb) <- of b . y (nb = dot_blocks(n))
+hipError_t launch_dot2(size_t n, const double* a, const double* b, const double* y, double* part, hipStream_t s);
+// z = y - S[GAMMA] x - S[DELTA] v - S[BETA] vold; part <- block shares of z . z
+hipError_t launch_minres_z(size_t n, const double* y, const double* x, const double* v, const double* vold,
+                           const double* S, double* z, double* part, hipStream_t s);
+// one MINRES Givens step from alpha = S[DELTA] - theta and beta = sqrt(S[ZZ])
+hipError_t launch_minres_scalar(double* S, double theta, hipStream_t s);
+// wold <- K_V v + K_WOLD wold + K_W w; t += K_T wold; z *= INV_BETA
+hipError_t launch_minres_update(size_t n, const double* v, double* wold, const double* w, double* t, double* z,
+                                const double* S, hipStream_t s);
+hipError_t launch_minres_init(double* S, double beta1, hipStream_t s);
 // y = alpha*y + sum_v coef[v] * X[v]  (nv <= 64 coefficients passed by value)
 hipError_t launch_multiaxpy(size_t n, int nv, const double* X, size_t ldx, const double* coef,
                             double alpha, double* y, hipStream_t s);

@@ -902,6 +902,127 @@ hipError_t launch_multidot(size_t n, int nv, const double* X, size_t ldx, const 
   return hipGetLastError();
 }
 
+static dim3 ew_grid(size_t n);
+
+// ---------------------------------------------------------------------------------
+// MINRES on the Jacobi-Davidson correction equation P (H - theta) P t = -r (P = I - x x^T),
+// with every scalar of the short recurrence kept on the device (S, see imps_internal.h), so
+// that a step needs no host round trip.
+// ---------------------------------------------------------------------------------
+
+// part[b] = block b's share of a . y, part[nb + b] = block b's share of b . y
+__global__ void __launch_bounds__(RED_THREADS) dot2_kernel(size_t n, const double* __restrict__ a,
+                                                           const double* __restrict__ b,
+                                                           const double* __restrict__ y, double* __restrict__ part) {
+  const int nb = gridDim.x;
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)nb * RED_THREADS) {
+    const double yi = y[i];
+    acc[0] += a[i] * yi;
+    acc[1] += b[i] * yi;
+  }
+  block_sum_store<2>(acc, 2, part, nb, blockIdx.x);
+}
+
+hipError_t launch_dot2(size_t n, const double* a, const double* b, const double* y, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(dot2_kernel, dim3(dot_blocks(n)), dim3(RED_THREADS), 0, s, n, a, b, y, part);
+  return hipGetLastError();
+}
+
+// z = y - S[GAMMA] x - S[DELTA] v - S[BETA] v_old (the new Lanczos vector of the projected,
+// shifted operator before normalisation), part <- block shares of z . z
+__global__ void __launch_bounds__(RED_THREADS) minres_z_kernel(size_t n, const double* __restrict__ y,
+                                                               const double* __restrict__ x,
+                                                               const double* __restrict__ v,
+                                                               const double* __restrict__ vold,
+                                                               const double* __restrict__ S, double* __restrict__ z,
+                                                               double* __restrict__ part) {
+  const int nb = gridDim.x;
+  const double g = S[MR_GAMMA], dl = S[MR_DELTA], bo = S[MR_BETA];
+  double acc[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)nb * RED_THREADS) {
+    const double zi = y[i] - g * x[i] - dl * v[i] - bo * vold[i];
+    z[i] = zi;
+    acc[0] += zi * zi;
+  }
+  block_sum_store<1>(acc, 1, part, nb, blockIdx.x);
+}
+
+hipError_t launch_minres_z(size_t n, const double* y, const double* x, const double* v, const double* vold,
+                           const double* S, double* z, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(minres_z_kernel, dim3(dot_blocks(n)), dim3(RED_THREADS), 0, s, n, y, x, v, vold, S, z, part);
+  return hipGetLastError();
+}
+
+// One thread: the Givens step of MINRES (Paige-Saunders) from alpha = S[DELTA] - theta and
+// beta = sqrt(S[ZZ]); leaves the coefficients of the vector update in S
+__global__ void minres_scalar_kernel(double* S, double theta) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double alpha = S[MR_DELTA] - theta;
+  const double bn = sqrt(fmax(S[MR_ZZ], 0.0));
+  const double bo = S[MR_BETA], co = S[MR_COLD], c = S[MR_C], so = S[MR_SOLD], sv = S[MR_S], eta = S[MR_ETA];
+  const double rho0 = c * alpha - co * sv * bo;
+  const double rho1 = hypot(rho0, bn);
+  const double rho2 = sv * alpha + co * c * bo;
+  const double rho3 = so * bo;
+  if (!(rho1 > 0.0) || S[MR_DONE] != 0.0) {
+    S[MR_KV] = S[MR_KWOLD] = S[MR_KW] = S[MR_KT] = S[MR_INVB] = 0.0;
+    S[MR_DONE] = 1.0;
+    return;
+  }
+  const double cn = rho0 / rho1, sn = bn / rho1;
+  S[MR_KV] = 1.0 / rho1;
+  S[MR_KWOLD] = -rho3 / rho1;
+  S[MR_KW] = -rho2 / rho1;
+  S[MR_KT] = cn * eta;
+  S[MR_ETA] = -sn * eta;
+  S[MR_COLD] = c;
+  S[MR_C] = cn;
+  S[MR_SOLD] = sv;
+  S[MR_S] = sn;
+  S[MR_BETA] = bn;
+  S[MR_INVB] = bn > 0.0 ? 1.0 / bn : 0.0;
+  if (!(bn > 0.0)) S[MR_DONE] = 1.0;  // invariant subspace: t is exact after this step
+}
+
+hipError_t launch_minres_scalar(double* S, double theta, hipStream_t s) {
+  hipLaunchKernelGGL(minres_scalar_kernel, dim3(1), dim3(64), 0, s, S, theta);
+  return hipGetLastError();
+}
+
+// w_new = K_V v + K_WOLD w_old + K_W w (into w_old), t += K_T w_new, z *= INV_BETA (the next v)
+__global__ void minres_update_kernel(size_t n, const double* __restrict__ v, double* __restrict__ wold,
+                                     const double* __restrict__ w, double* __restrict__ t, double* __restrict__ z,
+                                     const double* __restrict__ S) {
+  const double kv = S[MR_KV], kwo = S[MR_KWOLD], kw = S[MR_KW], kt = S[MR_KT], ib = S[MR_INVB];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double wn = kv * v[i] + kwo * wold[i] + kw * w[i];
+    wold[i] = wn;
+    t[i] += kt * wn;
+    z[i] *= ib;
+  }
+}
+
+hipError_t launch_minres_update(size_t n, const double* v, double* wold, const double* w, double* t, double* z,
+                                const double* S, hipStream_t s) {
+  hipLaunchKernelGGL(minres_update_kernel, ew_grid(n), dim3(256), 0, s, n, v, wold, w, t, z, S);
+  return hipGetLastError();
+}
+
+// S <- the initial MINRES state for a right-hand side of norm beta1
+__global__ void minres_init_kernel(double* S, double beta1) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int k = 0; k < MR_NS; ++k) S[k] = 0.0;
+  S[MR_COLD] = 1.0;
+  S[MR_C] = 1.0;
+  S[MR_ETA] = beta1;
+}
+
+hipError_t launch_minres_init(double* S, double beta1, hipStream_t s) {
+  hipLaunchKernelGGL(minres_init_kernel, dim3(1), dim3(64), 0, s, S, beta1);
+  return hipGetLastError();
+}
+
 // Fused Gram-Schmidt sweeps over w (nv <= 32 basis vectors, coefficients on the device):
 //   DOTS: w -= X coef, then part <- block shares of X_v . w (the next pass's coefficients)
 //   NORM: w -= X coef, then part <- block shares of w . w

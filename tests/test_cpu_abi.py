@@ -49,6 +49,7 @@ def test_default_params_match_reference_defaults(lib):
     assert (p.gauge_dynamic, p.gauge_tol, p.gauge_tol_min, p.gauge_tol_max) == (1, 1e-13, 1e-14, 1e-4)
     assert (p.envs_dynamic, p.envs_tol, p.envs_tol_min, p.envs_tol_max, p.envs_krylovdim) == (1, 1e-10, 1e-14, 1e-4, 30)
     assert not p.finalize and p.finalize_user is None
+    assert p.refine_eig == 1  # mixed runs refine with reduced-precision corrections of FP64 residuals
 
 
 def test_vumps_solver_objects_map_to_params(lib):
