@@ -427,6 +427,11 @@ int imps_gemm_dev(imps_ctx* ctx, char transa, char transb, int m, int n, int k, 
  * elsewhere (for tests and measurement).  Results do not depend on the setting.  Returns
  * IMPS_E_ARG for other values. */
 int imps_set_gemm_autotune(int on);
+/* Graph capture of fixed kernel sequences, process-wide, on by default (no counterpart in the
+ * reference): the gauge sweeps (uniform_rightorth!, ortho.jl:166-202) run each LQ sweep as a
+ * captured hipGraph replayed from the second sweep on; 0 runs them eagerly (same kernels, same
+ * order, bit-identical results; for tests and measurement).  IMPS_E_ARG for other values. */
+int imps_set_graph_capture(int on);
 /* Device-to-device copy of `bytes` bytes with the library's copy kernel on the
  * context's stream (also the known-size reference for HBM traffic counters). */
 int imps_dev_copy(imps_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);

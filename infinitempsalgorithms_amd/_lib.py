@@ -136,6 +136,7 @@ SIGNATURES = [
                                  c_double]),
     ("imps_bench_hac", c_int, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(HacTiming)]),
     ("imps_set_gemm_autotune", c_int, [c_int]),
+    ("imps_set_graph_capture", c_int, [c_int]),
     ("imps_gemm_dev", c_int, [c_void_p, c_char, c_char, c_int, c_int, c_int, c_double, c_void_p, c_int,
                               c_void_p, c_int, c_double, c_void_p, c_int, c_int]),
     ("imps_dev_copy", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),

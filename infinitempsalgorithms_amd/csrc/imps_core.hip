@@ -105,6 +105,13 @@ struct imps_ctx {
   hipEvent_t ev_ready = nullptr;  // parent: state and environments enqueued
   hipEvent_t ev_done = nullptr;   // worker: its solve enqueued
   std::set<long> pretuned;        // (D, w, d, prec) whose H_AC/H_C GEMM classes were tuned uncontended
+  // captured kernel sequences (hipGraph) keyed by the sequence tag and the buffers it touches;
+  // exec == nullptr: seen once, captured on the next run
+  struct Graph {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Graph> graphs;
 };
 
 static thread_local std::string g_create_error;
@@ -147,10 +154,17 @@ void dfree(double*& p) {
   p = nullptr;
 }
 
+void clear_graphs(imps_ctx* c) {
+  for (auto& g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
 double* ws(imps_ctx* c, const std::string& name, size_t n) {
   DevBuf& b = c->ws[name];
   if (b.n < n) {
     HIPCHK(hipStreamSynchronize(c->stream));
+    clear_graphs(c);  // captured sequences may hold the old buffer
     dfree(b.p);
     b.p = dalloc(n);
     b.n = n;
@@ -204,6 +218,50 @@ const double* d2h_small(imps_ctx* c, const void* src, size_t bytes) {
 void d2d(imps_ctx* c, double* dst, const double* src, size_t n) {
   if (dst == src || n == 0) return;
   HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+}
+
+// Graph capture of fixed kernel sequences (process-wide switch, on by default; 0 runs every
+// sequence eagerly, for tests and measurement).
+std::atomic<int> g_graphs{1};
+
+// Runs `enqueue` (kernel launches and device-to-device copies on c->stream only: no host
+// synchronisation, no allocation, no memset) through a hipGraph cached under `key`: the first run
+// of a key is eager (the GEMM tuner and the workspaces see it), the second captures the sequence
+// and launches the graph, later runs replay it.  The key must name every buffer the sequence
+// touches that can change between runs; workspaces are covered because a workspace that grows
+// drops every captured graph of the context.
+void run_graphed(imps_ctx* c, const std::vector<uintptr_t>& key, const std::function<void()>& enqueue) {
+  if (g_graphs.load(std::memory_order_relaxed) == 0) {
+    enqueue();
+    return;
+  }
+  for (auto& g : c->graphs)
+    if (g.key == key) {
+      if (g.exec) {
+        HIPCHK(hipGraphLaunch(g.exec, c->stream));
+        return;
+      }
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+      try {
+        enqueue();
+      } catch (...) {
+        (void)hipStreamEndCapture(c->stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(c->stream, &graph));
+      hipGraphExec_t exec = nullptr;
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(e);
+      g.exec = exec;
+      HIPCHK(hipGraphLaunch(g.exec, c->stream));
+      return;
+    }
+  if (c->graphs.size() >= 32) clear_graphs(c);
+  c->graphs.push_back(imps_ctx::Graph{key, nullptr});
+  enqueue();
 }
 
 // ------------------------------------------------------------------ BLAS-1 (FP64)
@@ -1527,12 +1585,17 @@ void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q, int
 // The end of one gauge sweep (ortho.jl:176-201 / 135-163): Cn /= ||Cn||, then eps = ||Cn - Cpre||
 // (Cpre is overwritten), read back together with the sweep's Cholesky breakdown flag in one
 // host round trip.  Returns eps; *broke receives the flag.
-double sweep_end(imps_ctx* c, size_t n, double* Cn, double* Cpre, const int* flag, bool* broke) {
+void sweep_end_enqueue(imps_ctx* c, size_t n, double* Cn, double* Cpre) {
   normalize_dev(c, n, Cn);
   HIPCHK(launch_axpby(n, 1.0, Cn, -1.0, Cpre, c->stream));
   double* e2 = ws(c, "sw_e2", 1);
   HIPCHK(launch_multidot(n, 1, Cpre, n, Cpre, c->dpart, c->stream));
   HIPCHK(launch_partials_reduce(1, dot_blocks(n), c->dpart, e2, c->stream));
+}
+
+// the readback half of sweep_end (after sweep_end_enqueue)
+double sweep_end_read(imps_ctx* c, const int* flag, bool* broke) {
+  double* e2 = ws(c, "sw_e2", 1);
   HIPCHK(hipMemcpyAsync(c->hscal, e2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(c->hscal + 1, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1540,6 +1603,11 @@ double sweep_end(imps_ctx* c, size_t n, double* Cn, double* Cpre, const int* fla
   std::memcpy(&f, c->hscal + 1, sizeof(int));
   *broke = f != 0;
   return std::sqrt(std::max(0.0, c->hscal[0]));
+}
+
+double sweep_end(imps_ctx* c, size_t n, double* Cn, double* Cpre, const int* flag, bool* broke) {
+  sweep_end_enqueue(c, n, Cn, Cpre);
+  return sweep_end_read(c, flag, broke);
 }
 
 constexpr double kTolMin = 1.0e-14, kTolMax = 1.0e-4;
@@ -1582,18 +1650,24 @@ int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* 
       double* Ldum = ws(c, "ro_Ldum", DD);
       lqpos(c, D, D, ev, Cn, Ldum);
     }
-    d2d(c, Cpre, Cn, DD);
-    d2d(c, Csave, Cn, DD);
     // the sweep runs with the Cholesky breakdown checks deferred to its end (one host round trip
-    // per sweep); on a breakdown it is redone from its input with per-site checks and the
-    // Householder fallback
-    HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
-    for (int i = N - 1; i >= 0; --i) {
-      gemm2(c, 'N', 'N', D * d, D, D, 1.0, A[i], (long)D * d, c->C[i], D, 0.0, M, (long)D * d);
-      lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i], sflag);
-    }
+    // per sweep), as one captured kernel sequence (run_graphed); on a breakdown it is redone from
+    // its input with per-site checks and the Householder fallback
+    std::vector<uintptr_t> key{1, (uintptr_t)N, (uintptr_t)D, (uintptr_t)d};
+    for (int i = 0; i < N; ++i)
+      for (const void* q : {(const void*)A[i], (const void*)c->C[i], (const void*)c->AR[i]}) key.push_back((uintptr_t)q);
+    run_graphed(c, key, [&] {
+      d2d(c, Cpre, Cn, DD);
+      d2d(c, Csave, Cn, DD);
+      HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
+      for (int i = N - 1; i >= 0; --i) {
+        gemm2(c, 'N', 'N', D * d, D, D, 1.0, A[i], (long)D * d, c->C[i], D, 0.0, M, (long)D * d);
+        lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i], sflag);
+      }
+      sweep_end_enqueue(c, DD, Cn, Cpre);
+    });
     bool broke = false;
-    eps = sweep_end(c, DD, Cn, Cpre, sflag, &broke);
+    eps = sweep_end_read(c, sflag, &broke);
     if (broke) {
       d2d(c, Cn, Csave, DD);
       d2d(c, Cpre, Csave, DD);
@@ -1973,6 +2047,7 @@ double galerkin(imps_ctx* c, int prec) {
 // ------------------------------------------------------------------ state management
 
 void free_state(imps_ctx* c) {
+  clear_graphs(c);
   for (auto* v : {&c->AL, &c->AR, &c->AC, &c->C, &c->pAC, &c->pC, &c->pAL, &c->GL, &c->GR})
     for (double*& p : *v) dfree(p);
   for (auto* v : {&c->AL, &c->AR, &c->AC, &c->C, &c->pAC, &c->pC, &c->pAL, &c->GL, &c->GR}) v->clear();
@@ -2149,6 +2224,7 @@ imps_ctx* worker(imps_ctx* c, int k) {
 // frees what a worker owns (never the borrowed model, state or environments)
 void destroy_worker(imps_ctx* w) {
   if (w->stream) (void)hipStreamSynchronize(w->stream);
+  clear_graphs(w);
   for (auto& kv : w->ws) dfree(kv.second.p);
   dfree(w->dpart);
   if (w->hpart) (void)hipHostFree(w->hpart);
@@ -3379,6 +3455,12 @@ int imps_gemm_dev(imps_ctx* c, char transa, char transb, int m, int n, int k, do
     require(ldb >= std::max(1, transb == 'N' ? k : n), IMPS_E_ARG, "imps_gemm_dev: ldb");
     gemm2(c, transa, transb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, prec);
   });
+}
+
+int imps_set_graph_capture(int on) {
+  if (on != 0 && on != 1) return IMPS_E_ARG;
+  g_graphs.store(on);
+  return IMPS_OK;
 }
 
 int imps_set_gemm_autotune(int on) {

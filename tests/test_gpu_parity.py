@@ -921,3 +921,34 @@ def test_vumps_finalize_and_solver_objects(imps, ctx, oracle):
                                                          finalize=lambda it, p, h, e: (seen.append(it), (p, e))[1]),
                                      envs=envs)
     assert seen == list(range(1, ii.niter + 1))
+
+
+def test_graph_captured_gauge_sweeps_match_eager(imps, ctx, oracle):
+    # the gauge sweeps run as captured hipGraphs replayed from the second sweep on (many replays per
+    # VUMPS iteration, with the eigensolves' eager work and host synchronisations in between): the
+    # whole trajectory and the final state equal the eager run's bit for bit, twice in a row
+    lib = imps._lib.load()
+    H = imps.models.fermi_hubbard(t=1.0, U=4.0, mu=2.0, nsites=2)
+    psi0 = oracle.randomimps(24, [4, 4], seed=3)
+    st = imps.CanonicalIMPS(psi0.AL, psi0.AR, psi0.C, psi0.AC)
+
+    def run():
+        envs = imps.GPUCache(st, H, ctx)
+        ctx.call("imps_envs_reset")
+        psi, _, info = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-14, maxiter=12), envs=envs)
+        return psi, info
+
+    try:
+        assert lib.imps_set_graph_capture(0) == 0
+        p0, i0 = run()
+        assert lib.imps_set_graph_capture(1) == 0
+        for _ in range(2):
+            p1, i1 = run()
+            assert i1.losses == i0.losses
+            assert i1.timings["gauge_sweeps"] == i0.timings["gauge_sweeps"] > 2 * i0.niter
+            for xs, ys in zip((p0.AL, p0.AR, p0.C, p0.AC), (p1.AL, p1.AR, p1.C, p1.AC)):
+                for x, y in zip(xs, ys):
+                    assert np.array_equal(x, y)
+    finally:
+        lib.imps_set_graph_capture(1)
+    assert lib.imps_set_graph_capture(2) == -1
