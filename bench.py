@@ -39,11 +39,11 @@ sys.path.insert(0, ROOT)
 METRIC = "VUMPS time-to-Galerkin<1e-8 and H_AC TFLOP/s at D=1024, mixed vs FP64 GPU"
 PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
-# HBM bytes per launch of the dominant H_AC GEMM measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (FETCH_SIZE scaled by about 2.0 from a copy kernel of known size; mean over 34
-# launches in two runs); keyed by (D, precision, kernel).  profiles/hac_d1024_hubbard_traffic.md
-# holds the runs.
-TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 771.0 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 684.0 * 2 ** 20}
+# HBM bytes per launch of the H_AC GEMMs measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (FETCH_SIZE scaled by 2.0, the factor a copy kernel of known size shows); keyed by (D, precision,
+# kernel).  profiles/hac_d1024_hubbard_mfma_util_and_traffic.md holds the runs.
+TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 835.4 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 675.7 * 2 ** 20,
+           (1024, "bf16x3", "gemm1_GLx"): 309.1 * 2 ** 20, (1024, "bf16x3", "gemm2_GR"): 304.1 * 2 ** 20}
 
 
 def log(msg):
