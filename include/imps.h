@@ -186,6 +186,44 @@ int imps_energy(imps_ctx* ctx, double* e);
  * has no automatic differentiation.  dW: nsites host arrays (w, d, w, d). */
 int imps_energy_gradient(imps_ctx* ctx, double* const* dW);
 
+/* ---- fixed-point response (adjoint of the VUMPS fixed point) -------------------------
+ * The reference has no derivative of its fixed point; these entry points differentiate it with
+ * the reference's own operators: H_AC (src/algorithms/groundstates/effective.jl:90-102), the
+ * slice pushes (envs.jl:104-114), regularize! (envs.jl:28-32), linsolve (envs.jl:14-19) and the
+ * level order of DMRGCache (envs.jl:321-518).  One-site unit cell, converged state, environments
+ * of the Hamiltonian current (imps_envs_recalculate).
+ *
+ * For an objective given as a Schur MPO Wobj (wo, d, wo, d) (its value per site is
+ * expectationvalue(ψ, Wobj), expval.jl:93-112; a one-site observable O is the wo = 2 MPO with O
+ * in block (0, 1)) the response of the fixed point to the Hamiltonian tensor is
+ *   dO/dW = -2 d/dW <Φ(L)| H[W] |ψ>,  (H_eff + K) L = g = P[H_AC^O(AC)],
+ * H_eff the zero-momentum tangent-space Hamiltonian (one-defect environments solved as device
+ * linsolves), K the curvature of the uniform-MPS manifold, the system solved by MINRES on the
+ * device, and the sandwich the reverse H_AC contraction plus the adjoint environments (bra defect,
+ * ket ψ).  The total derivative of <O> is this plus the explicit dependence of Wobj on W (none
+ * for a fixed observable).  For the energy itself (Wobj = W) g is the Galerkin residual and the
+ * response vanishes: imps_energy_gradient is then the whole derivative. */
+typedef struct imps_response_info {
+  int iterations;          /* MINRES iterations */
+  int converged;           /* the MINRES estimate reached tol */
+  double relres;           /* |(H_eff + K) L - g| / |g| of the returned L (one more application) */
+  double estimate;         /* MINRES residual estimate relative to |g| */
+  long hessian_applies;    /* applications of H_eff + K */
+  long level_solves;       /* GMRES solves of defect-environment levels */
+  long level_ops;          /* their operator applications */
+  long unconverged_solves; /* level solves that stopped at maxiter */
+} imps_response_info;
+
+/* value <- <Wobj> per site, dW (w, d, w, d) <- dO/dW as above.  tol: MINRES tolerance relative to
+ * |g|; rcond: relative cutoff of the pseudo-inverse of C (Schmidt values below rcond * s_max carry
+ * no curvature term).  IMPS_E_UNSUPPORTED for nsites != 1 or w*d > 32. */
+int imps_fixedpoint_gradient(imps_ctx* ctx, int wo, int d, const double* Wobj, double tol, int maxiter,
+                             double rcond, double* value, double* dW, imps_response_info* info);
+
+/* out <- P (H_eff + K) P B for a host tensor B (D, d, D) (the operator of the response system,
+ * P = 1 - AL AL^T); tol: relative tolerance of its level solves. */
+int imps_tangent_hessian_apply(imps_ctx* ctx, double rcond, double tol, const double* B, double* out);
+
 /* expectationvalue(ψ, (site,) => O) (expval.jl:33-37), O: (d, d). */
 int imps_local_expectation(imps_ctx* ctx, int site, const double* O, double* val);
 

@@ -8,7 +8,8 @@ host-side mirror of the reference's API (``find_groundstate``, ``VUMPS``,
 from . import _lib
 from .api import (IDMRG, TDVP, VUMPS, DynamicTol, SolverTol, alg_eigsolve, alg_environments, alg_gauge, correlator, load_mpo, load_state, save_mpo, save_state, dense_expectation, dominant_env, DenseIMPO, dot, fidelity, infidelity, norm, AC_hamiltonian, C_hamiltonian, CanonicalIMPS, Context, DMRGCache,
                   GPUCache, IterativeConvergenceInfo, Lanczos, SparseIMPO, TransferMatrix,
-                  calc_galerkin, canonicalimps, energy_gradient, expectationvalue, find_groundstate,
+                  calc_galerkin, canonicalimps, energy_gradient, fixedpoint_gradient, objective_mpo_1site,
+                  tangent_hessian_apply, expectationvalue, find_groundstate,
                   find_groundstates_batched, fixedpoint,
                   leftenv, random_tensors, randomimps, recalculate_, regauge_, regularize_,
                   rightenv, time_evolve, timestep)
@@ -17,6 +18,6 @@ from . import models
 __all__ = [
     "DynamicTol", "SolverTol", "alg_eigsolve", "alg_environments", "alg_gauge", "correlator", "load_mpo", "load_state", "save_mpo", "save_state", "dense_expectation", "dominant_env", "DenseIMPO", "dot", "fidelity", "infidelity", "norm", "IDMRG", "TDVP", "time_evolve", "timestep", "VUMPS", "AC_hamiltonian", "C_hamiltonian", "CanonicalIMPS", "Context", "DMRGCache", "GPUCache",
     "IterativeConvergenceInfo", "Lanczos", "SparseIMPO", "TransferMatrix", "calc_galerkin",
-    "canonicalimps", "energy_gradient", "expectationvalue", "find_groundstate", "find_groundstates_batched", "fixedpoint", "leftenv", "random_tensors",
+    "canonicalimps", "energy_gradient", "fixedpoint_gradient", "objective_mpo_1site", "tangent_hessian_apply", "expectationvalue", "find_groundstate", "find_groundstates_batched", "fixedpoint", "leftenv", "random_tensors",
     "randomimps", "recalculate_", "regauge_", "regularize_", "rightenv", "models",
 ]

@@ -39,6 +39,12 @@ class KrylovInfo(ctypes.Structure):
     _fields_ = [("converged", c_int), ("numiter", c_int), ("numops", c_int), ("normres", c_double)]
 
 
+class ResponseInfo(ctypes.Structure):
+    _fields_ = [("iterations", c_int), ("converged", c_int), ("relres", c_double), ("estimate", c_double),
+                ("hessian_applies", ctypes.c_long), ("level_solves", ctypes.c_long), ("level_ops", ctypes.c_long),
+                ("unconverged_solves", ctypes.c_long)]
+
+
 FINALIZE_CB = ctypes.CFUNCTYPE(c_int, c_int, c_void_p, c_void_p)
 
 
@@ -119,6 +125,9 @@ SIGNATURES = [
     ("imps_galerkin", c_int, [c_void_p, P_double]),
     ("imps_energy", c_int, [c_void_p, P_double]),
     ("imps_energy_gradient", c_int, [c_void_p, PP_double]),
+    ("imps_fixedpoint_gradient", c_int, [c_void_p, c_int, c_int, P_double, c_double, c_int, c_double, P_double,
+                                         P_double, ctypes.POINTER(ResponseInfo)]),
+    ("imps_tangent_hessian_apply", c_int, [c_void_p, c_double, c_double, P_double, P_double]),
     ("imps_local_expectation", c_int, [c_void_p, c_int, P_double, P_double]),
     ("imps_vumps_default_params", c_int, [ctypes.POINTER(VumpsParams)]),
     ("imps_vumps_run", c_int, [c_void_p, ctypes.POINTER(VumpsParams), ctypes.POINTER(VumpsInfo), P_double,

@@ -619,6 +619,76 @@ def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
     return out
 
 
+def objective_mpo_1site(O):
+    """Schur MPO (w = 2) whose value per site is the one-site expectation value <O>: the identity at
+    the unit corners and O in block (0, 1)."""
+    O = np.asarray(O, dtype=np.float64)
+    d = O.shape[0]
+    if O.shape != (d, d):
+        raise ValueError("objective_mpo_1site: O must be (d, d)")
+    Wo = np.zeros((2, d, 2, d), order="F")
+    Wo[0, :, 0, :] = np.eye(d)
+    Wo[1, :, 1, :] = np.eye(d)
+    Wo[0, :, 1, :] = O
+    return Wo
+
+
+@dataclass
+class ResponseInfo:
+    iterations: int
+    converged: bool
+    relres: float
+    estimate: float
+    hessian_applies: int
+    level_solves: int
+    level_ops: int
+    unconverged_solves: int
+
+
+def fixedpoint_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache, Wobj, tol=1e-10, maxiter=1000,
+                        rcond=1e-10):
+    """Response of a converged one-site VUMPS fixed point: (value, dO/dW, info) for the objective
+    <Wobj> per site (a Schur MPO (wo, d, wo, d); ``objective_mpo_1site(O)`` for a one-site
+    observable) with respect to the Hamiltonian tensor W (w, d, w, d).
+
+    The reference has no derivative of its fixed point; this is the tangent-space linear response
+    built from its operators (H_AC effective.jl:90-102, the slice pushes envs.jl:104-114,
+    regularize! envs.jl:28-32, linsolve envs.jl:14-19): (H_eff + K) L = P[H_AC^O(AC)] solved by
+    MINRES on the device, dO/dW = -2 d/dW <Phi(L)|H[W]|psi> by the reverse H_AC contraction and
+    the adjoint environments.  The derivative of any scalar f(W) = <Wobj> through the fixed point
+    is then sum(dW * dW/dp) for a parameter p."""
+    if len(H) != 1:
+        raise ValueError("fixedpoint_gradient: one-site unit cell only")
+    if psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ket = psi
+        envs.ctx.call("imps_envs_recalculate", 1e-14, 300, 30)
+    Wo = fortran(Wobj)
+    if Wo.ndim != 4 or Wo.shape[0] != Wo.shape[2] or Wo.shape[1] != H.d or Wo.shape[3] != H.d:
+        raise ValueError(f"fixedpoint_gradient: objective must be (wo, {H.d}, wo, {H.d}), got {Wo.shape}")
+    dW = np.zeros((H.w, H.d, H.w, H.d), order="F")
+    val = ctypes.c_double()
+    info = _lib.ResponseInfo()
+    envs.ctx.call("imps_fixedpoint_gradient", Wo.shape[0], H.d, dptr(Wo), float(tol), int(maxiter), float(rcond),
+                  ctypes.byref(val), dptr(dW), ctypes.byref(info))
+    return val.value, dW, ResponseInfo(info.iterations, bool(info.converged), info.relres, info.estimate,
+                                       info.hessian_applies, info.level_solves, info.level_ops,
+                                       info.unconverged_solves)
+
+
+def tangent_hessian_apply(psi: CanonicalIMPS, envs: GPUCache, B, rcond=1e-10, tol=1e-13):
+    """P (H_eff + K) P B: the operator of the fixed-point response system (one-site cell)."""
+    if psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ket = psi
+    Bf = fortran(B)
+    if Bf.shape != psi.AL[0].shape:
+        raise ValueError(f"tangent_hessian_apply: B must have the MPS shape {psi.AL[0].shape}, got {Bf.shape}")
+    out = np.zeros_like(Bf, order="F")
+    envs.ctx.call("imps_tangent_hessian_apply", float(rcond), float(tol), dptr(Bf), dptr(out))
+    return out
+
+
 # ---------------------------------------------------------------------------- solver objects
 
 

@@ -842,49 +842,62 @@ struct SliceTerm {
   std::vector<double> Wb;
 };
 
-void slice_sum_left(imps_ctx* c, const std::vector<SliceTerm>& terms, const double* A, double* out,
-                    long ldo) {
+// With distinct layers (the defect environments of the fixed-point response):
+// out = sum_terms bra^T mix(W_t, L_t ket) + beta out.
+void slice_sum_left_bk(imps_ctx* c, const std::vector<SliceTerm>& terms, const double* bra, const double* ket,
+                       double* out, long ldo, double beta) {
   const int D = c->D, d = c->d;
   if (terms.empty()) {
-    HIPCHK(launch_fill2d(D, D, 0.0, out, ldo, c->stream));
+    if (beta == 0.0) HIPCHK(launch_fill2d(D, D, 0.0, out, ldo, c->stream));
     return;
   }
   double* Z = ws(c, "sl_Z", (size_t)D * d * D);
   double* Y = ws(c, "sl_Y", (size_t)D * d * D);
   bool first = true;
   for (const SliceTerm& t : terms) {
-    const double* src = A;
+    const double* src = ket;
     if (t.L) {
-      gemm2(c, 'N', 'N', D, d * D, D, 1.0, t.L, t.ld, A, D, 0.0, Y, D);
+      gemm2(c, 'N', 'N', D, d * D, D, 1.0, t.L, t.ld, ket, D, 0.0, Y, D);
       src = Y;
     }
     HIPCHK(launch_mix_mid(D, d, D, t.Wb.data(), src, Z, first ? 0.0 : 1.0, c->stream));
     first = false;
   }
-  gemm2(c, 'T', 'N', D, D, d * D, 1.0, A, (long)d * D, Z, (long)d * D, 0.0, out, ldo);
+  gemm2(c, 'T', 'N', D, D, d * D, 1.0, bra, (long)d * D, Z, (long)d * D, beta, out, ldo);
+}
+
+void slice_sum_left(imps_ctx* c, const std::vector<SliceTerm>& terms, const double* A, double* out,
+                    long ldo) {
+  slice_sum_left_bk(c, terms, A, A, out, ldo, 0.0);
 }
 
 // Right slice sum (envs.jl:110-114, 162-177)
-void slice_sum_right(imps_ctx* c, const std::vector<SliceTerm>& terms, const double* A, double* out,
-                     long ldo) {
+// out = sum_terms mix(W_t, ket R_t) bra^T + beta out (distinct layers)
+void slice_sum_right_bk(imps_ctx* c, const std::vector<SliceTerm>& terms, const double* bra, const double* ket,
+                        double* out, long ldo, double beta) {
   const int D = c->D, d = c->d;
   if (terms.empty()) {
-    HIPCHK(launch_fill2d(D, D, 0.0, out, ldo, c->stream));
+    if (beta == 0.0) HIPCHK(launch_fill2d(D, D, 0.0, out, ldo, c->stream));
     return;
   }
   double* Z = ws(c, "sr_Z", (size_t)D * d * D);
   double* Y = ws(c, "sr_Y", (size_t)D * d * D);
   bool first = true;
   for (const SliceTerm& t : terms) {
-    const double* src = A;
+    const double* src = ket;
     if (t.L) {
-      gemm2(c, 'N', 'N', d * D, D, D, 1.0, A, (long)d * D, t.L, t.ld, 0.0, Y, (long)d * D);
+      gemm2(c, 'N', 'N', d * D, D, D, 1.0, ket, (long)d * D, t.L, t.ld, 0.0, Y, (long)d * D);
       src = Y;
     }
     HIPCHK(launch_mix_mid(D, d, D, t.Wb.data(), src, Z, first ? 0.0 : 1.0, c->stream));
     first = false;
   }
-  gemm2(c, 'N', 'T', D, D, d * D, 1.0, Z, D, A, D, 0.0, out, ldo);
+  gemm2(c, 'N', 'T', D, D, d * D, 1.0, Z, D, bra, D, beta, out, ldo);
+}
+
+void slice_sum_right(imps_ctx* c, const std::vector<SliceTerm>& terms, const double* A, double* out,
+                     long ldo) {
+  slice_sum_right_bk(c, terms, A, A, out, ldo, 0.0);
 }
 
 // MPO-channel pushes (transfermatrix.jl:11-13, 31-35) as sums of slice pushes over the nonzero
@@ -2044,6 +2057,407 @@ double galerkin(imps_ctx* c, int prec) {
   return eps;
 }
 
+// ------------------------------------------------------------------ fixed-point response
+//
+// Tangent-space linear response of a converged one-site state.  The reference has no derivative
+// of its fixed point; this one is built from its operators: H_AC (effective.jl:90-102), the slice
+// pushes (envs.jl:104-114), regularize! (envs.jl:28-32), the GMRES linsolve (envs.jl:14-19) and
+// the level order of DMRGCache (envs.jl:321-518).
+//
+// Tangent vectors are tensors B of the MPS shape with AL^T B = 0; P = 1 - AL AL^T.  The Hessian
+// of the energy density along AL + t B C^-1 is H_eff + K with
+//   H_eff B = P[(H_AC - lambda) B + H_AC[LK, GR](AR) + H_AC[GL, RK](AL)]   (lambda = <AC, H_AC AC>)
+//   K B     = P[H_AC[GL, RKb](AC) + C^-T H_AC[LKb, GR](AC)]                 (manifold curvature)
+// whose one-defect environments are sums over the defect's distance, solved level by level; an
+// identity diagonal level solves x - T x + tr(dual x) fix = rhs - tr(dual rhs) fix with the fixed
+// point `fix` of the mixed transfer matrix and its dual:
+//   LK   ket B on the left, then ket AR / bra AL          fix C,     dual C^T
+//   RK   ket B on the right, then ket AL / bra AR         fix C,     dual C^T
+//   LKb  bra B on the left, then bra AR / ket AL          fix C^T,   dual C
+//   RKb  bra C^-1 B on the right, then AR / AR            fix I,     dual C^T C
+// For an objective MPO O with tangent gradient g = P[H_AC^O(AC)] the response of <O> to the
+// Hamiltonian tensor is dO/dW = -2 d/dW <Phi(L)| H[W] |psi> with (H_eff + K) L = g: the sandwich
+// of the reverse H_AC contraction at the defect site plus the adjoint environments (bra defect,
+// ket psi) on either side of it.  C^-1 is the pseudo-inverse with relative cutoff rcond.
+
+struct Tangent {
+  const double *AL, *AR, *C, *AC, *GL, *GR;
+  bool gl_id, gr_id;                 // level 0 of GL / level w-1 of GR is the identity
+  double lam = 0.0;
+  double *Cinv, *Ct, *CtC, *CCt, *Id;  // D x D on the device
+  double tol = 1e-13;                // relative tolerance of the level solves
+  long solves = 0, ops = 0, hessian_applies = 0, unconverged = 0;
+};
+
+// v -= tr(dual v) fix for n x n matrices (v with leading dimension ldv; dual, fix contiguous),
+// the trace reduced and applied on the device
+void sub_trace_fix(imps_ctx* c, int n, double* v, long ldv, const double* dual, const double* fix) {
+  int np = 0;
+  const size_t maxp = (size_t)((n + 31) / 32) * ((n + 31) / 32);
+  double* part = ws(c, "stf_part", maxp);
+  HIPCHK(launch_trace_prod_parts(n, dual, n, v, ldv, part, &np, c->stream));
+  HIPCHK(launch_reduce_axpy_mat(np, part, -1.0, fix, n, v, ldv, n, c->stream));
+}
+
+// H_AC[GL, GR] x (effective.jl:90-102) with every level through the GEMMs: environments of any
+// level structure (the defect environments have no identity level), the model of site 0
+void hac_apply_env(imps_ctx* c, const double* GL, const double* GR, const double* x, double* y) {
+  const int D = c->D, w = c->w, d = c->d;
+  double* T1 = ws(c, "rsp_T1", (size_t)D * w * d * D);
+  double* T2 = ws(c, "rsp_T2", (size_t)D * w * d * D);
+  gemm2(c, 'N', 'N', D * w, d * D, D, 1.0, GL, (long)D * w, x, D, 0.0, T1, (long)D * w);
+  HIPCHK(launch_wmix(D, w, d, c->Wm[0], T1, T2, nullptr, nullptr, c->stream));
+  gemm2(c, 'N', 'N', D * d, D, D * w, 1.0, T2, (long)D * d, GR, (long)D * w, 0.0, y, (long)D * d);
+}
+
+// E <- P E = E - AL (AL^T E) (the projector of calc_galerkin, vumps.jl:111-124)
+void tangent_project(imps_ctx* c, const double* AL, double* E) {
+  const int D = c->D, d = c->d;
+  double* q = ws(c, "tg_q", (size_t)D * D);
+  gemm2(c, 'T', 'N', D, D, d * D, 1.0, AL, (long)d * D, E, (long)d * D, 0.0, q, D);
+  gemm2(c, 'N', 'N', d * D, D, D, -1.0, AL, (long)d * D, q, D, 1.0, E, (long)d * D);
+}
+
+// One level of a defect environment: x = rhs + push(x, M) with the diagonal block M; rhs (D x D,
+// contiguous) is overwritten; the solution goes to out (leading dimension ldo).
+void defect_level_solve(imps_ctx* c, Tangent& t, bool left, double* rhs, const std::vector<double>& M,
+                        const double* bra, const double* ket, const double* fix, const double* dual, double* out,
+                        long ldo) {
+  const int D = c->D;
+  const size_t DD = (size_t)D * D;
+  if (is_zero(M)) {
+    HIPCHK(launch_copy2d(D, D, rhs, D, out, ldo, c->stream));
+    return;
+  }
+  const bool eye = is_eye(M, c->d, 0.0);
+  if (eye) sub_trace_fix(c, D, rhs, D, dual, fix);
+  const double nb = nrm2(c, DD, rhs);
+  if (nb == 0.0) {
+    HIPCHK(launch_fill2d(D, D, 0.0, out, ldo, c->stream));
+    return;
+  }
+  Op op = [&](const double* X, double* o) {
+    std::vector<SliceTerm> tt{SliceTerm{X, D, M}};
+    if (left)
+      slice_sum_left_bk(c, tt, bra, ket, o, D, 0.0);
+    else
+      slice_sum_right_bk(c, tt, bra, ket, o, D, 0.0);
+    if (eye) sub_trace_fix(c, D, o, D, dual, fix);
+  };
+  double* x = ws(c, "df_x", DD);
+  fill(c, DD, 0.0, x);
+  KInfo info;
+  gmres(c, op, DD, rhs, x, t.tol * nb, (int)std::min<size_t>(DD, 30), 300, 1.0, -1.0, &info);
+  t.solves++;
+  t.ops += info.numops;
+  if (!info.converged) t.unconverged++;
+  HIPCHK(launch_copy2d(D, D, x, D, out, ldo, c->stream));
+}
+
+// Levels lo..hi of a defect environment E (D, w, D) of the Hamiltonian of site 0; levels outside
+// lo..hi are zero.  Left (level j ascending):
+//   E_j = sum_{i<=j} push(G_i, W[i,:,j,:]; sbra, sket) + sum_{lo<=i<j} push(E_i, W[i,:,j,:]; bra, ket)
+//         + push(E_j, W[j,:,j,:]; bra, ket)
+// right (level i descending) with W[i,:,j,:], j >= i, and the right pushes.  g_id: the level of
+// G that is the identity (-1: none), applied without a GEMM.
+void defect_env(imps_ctx* c, Tangent& t, bool left, const double* G, int g_id, const double* sbra, const double* sket,
+                const double* bra, const double* ket, const double* fix, const double* dual, int lo, int hi,
+                double* E) {
+  const int D = c->D, w = c->w, d = c->d;
+  const long ldg = (long)D * w;
+  const std::vector<double>& W = c->Wh[0];
+  fill(c, (size_t)D * w * D, 0.0, E);
+  double* rhs = ws(c, "df_rhs", (size_t)D * D);
+  for (int step = 0; step <= hi - lo; ++step) {
+    const int j = left ? lo + step : hi - step;  // the level being solved
+    std::vector<SliceTerm> ts, te;
+    if (left) {
+      for (int i = 0; i <= j; ++i) {
+        std::vector<double> M = wblock_of(W, w, d, i, j);
+        if (!is_zero(M)) ts.push_back(SliceTerm{i == g_id ? nullptr : G + (size_t)D * i, ldg, M});
+        if (i >= lo && i < j && !is_zero(M)) te.push_back(SliceTerm{E + (size_t)D * i, ldg, M});
+      }
+      slice_sum_left_bk(c, ts, sbra, sket, rhs, D, 0.0);
+      slice_sum_left_bk(c, te, bra, ket, rhs, D, 1.0);
+    } else {
+      for (int k = j; k < w; ++k) {
+        std::vector<double> M = wblock_of(W, w, d, j, k);
+        if (!is_zero(M)) ts.push_back(SliceTerm{k == g_id ? nullptr : G + (size_t)D * k, ldg, M});
+        if (k > j && k <= hi && !is_zero(M)) te.push_back(SliceTerm{E + (size_t)D * k, ldg, M});
+      }
+      slice_sum_right_bk(c, ts, sbra, sket, rhs, D, 0.0);
+      slice_sum_right_bk(c, te, bra, ket, rhs, D, 1.0);
+    }
+    defect_level_solve(c, t, left, rhs, wblock_of(W, w, d, j, j), bra, ket, fix, dual, E + (size_t)D * j, ldg);
+  }
+}
+
+// out = P (H_eff + K) P B (projecting the input keeps the operator symmetric on the whole space)
+void tangent_hessian(imps_ctx* c, Tangent& t, const double* Bin, double* out) {
+  const int D = c->D, w = c->w, d = c->d;
+  const size_t n = (size_t)D * d * D;
+  double* B = ws(c, "tg_B", n);
+  double* X = ws(c, "tg_X", n);
+  double* CiB = ws(c, "tg_CiB", n);
+  double* Ek = ws(c, "tg_env", (size_t)D * w * D);
+  const int gl0 = t.gl_id ? 0 : -1, grw = t.gr_id ? w - 1 : -1;
+  d2d(c, B, Bin, n);
+  tangent_project(c, t.AL, B);
+  // H_eff
+  hac_apply_env(c, t.GL, t.GR, B, out);
+  axpby(c, n, -t.lam, B, 1.0, out);
+  defect_env(c, t, true, t.GL, gl0, t.AL, B, t.AL, t.AR, t.C, t.Ct, 1, w - 1, Ek);
+  hac_apply_env(c, Ek, t.GR, t.AR, X);
+  axpby(c, n, 1.0, X, 1.0, out);
+  defect_env(c, t, false, t.GR, grw, t.AR, B, t.AR, t.AL, t.C, t.Ct, 1, w - 1, Ek);
+  hac_apply_env(c, t.GL, Ek, t.AL, X);
+  axpby(c, n, 1.0, X, 1.0, out);
+  // curvature
+  gemm2(c, 'N', 'N', D, d * D, D, 1.0, t.Cinv, D, B, D, 0.0, CiB, D);
+  defect_env(c, t, false, t.GR, grw, CiB, t.AR, t.AR, t.AR, t.Id, t.CtC, 1, w - 1, Ek);
+  hac_apply_env(c, t.GL, Ek, t.AC, X);
+  axpby(c, n, 1.0, X, 1.0, out);
+  defect_env(c, t, true, t.GL, gl0, B, t.AL, t.AR, t.AL, t.Ct, t.C, 1, w - 1, Ek);
+  hac_apply_env(c, Ek, t.GR, t.AC, X);
+  gemm2(c, 'T', 'N', D, d * D, D, 1.0, t.Cinv, D, X, D, 1.0, out, D);
+  tangent_project(c, t.AL, out);
+  t.hessian_applies++;
+}
+
+// acc[W layout (w, d, w, d)] += scale * sum L[b,i,a] bra[b,u,c] ket[a,s,x] R[x,j,c]: the reverse
+// H_AC contraction (the derivative of <bra| H_AC[L, R] |ket> with respect to W[i,u,j,s])
+void sandwich_acc(imps_ctx* c, const double* L, const double* bra, const double* ket, const double* R, double scale,
+                  std::vector<double>& acc) {
+  const int D = c->D, w = c->w, d = c->d, wd = w * d;
+  double* T1 = ws(c, "rsp_T1", (size_t)D * w * d * D);
+  double* T3 = ws(c, "rsp_T2", (size_t)D * w * d * D);
+  double* Gp = ws(c, "eg_part", (size_t)wd * wd * D);
+  double* Gs = ws(c, "eg_sum", (size_t)wd * wd);
+  gemm2(c, 'N', 'N', D * w, d * D, D, 1.0, L, (long)D * w, ket, D, 0.0, T1, (long)D * w);
+  gemm2(c, 'N', 'T', D * d, D * w, D, 1.0, bra, (long)D * d, R, (long)D * w, 0.0, T3, (long)D * d);
+  HIPCHK(launch_gsand(D, w, d, T1, T3, Gp, c->stream));
+  HIPCHK(launch_partials_reduce(wd * wd, D, Gp, Gs, c->stream));
+  const double* h = d2h_small(c, Gs, sizeof(double) * wd * wd);
+  for (int i = 0; i < w; ++i)
+    for (int u = 0; u < d; ++u)
+      for (int j = 0; j < w; ++j)
+        for (int s2 = 0; s2 < d; ++s2)
+          acc[i + (size_t)w * (u + (size_t)d * (j + (size_t)w * s2))] +=
+              scale * h[(i + (size_t)w * s2) + (size_t)wd * (u + (size_t)d * j)];
+}
+
+// Cinv = pseudo-inverse of the D x D matrix Cm, singular values below rcond * s_max dropped
+// (rocSOLVER gesvd, C = U S V^T, Cinv = V S^+ U^T)
+void pinv_cutoff(imps_ctx* c, const double* Cm, double rcond, double* Cinv) {
+  const int D = c->D;
+  const size_t DD = (size_t)D * D;
+  double* A = ws(c, "pv_A", DD);
+  double* U = ws(c, "pv_U", DD);
+  double* Vt = ws(c, "pv_Vt", DD);
+  double* S = ws(c, "pv_S", (size_t)2 * D + 2);
+  int* info = (int*)ws(c, "pv_info", 1);
+  d2d(c, A, Cm, DD);
+  RBCHK(rocsolver_dgesvd(c->rb, rocblas_svect_all, rocblas_svect_all, D, D, A, D, S, U, D, Vt, D, S + D,
+                         rocblas_outofplace, info));
+  std::vector<double> sv(D);
+  d2h(c, sv.data(), S, D);
+  int inf = 0;
+  d2h_bytes(c, &inf, info, sizeof(int));
+  require(inf == 0, IMPS_E_SOLVER, "pseudo-inverse of C: gesvd did not converge");
+  for (int k = 0; k < D; ++k) {
+    const double s = sv[k];
+    HIPCHK(launch_scale(D, (s > rcond * sv[0] && s > 0.0) ? 1.0 / s : 0.0, U + (size_t)D * k, c->stream));
+  }
+  gemm2(c, 'T', 'T', D, D, D, 1.0, Vt, D, U, D, 0.0, Cinv, D);
+}
+
+Tangent tangent_setup(imps_ctx* c, double rcond, double tol) {
+  const int D = c->D, d = c->d;
+  const size_t DD = (size_t)D * D, n = (size_t)D * d * D;
+  Tangent t;
+  t.AL = c->AL[0];
+  t.AR = c->AR[0];
+  t.C = c->C[0];
+  t.AC = c->AC[0];
+  t.GL = c->GL[0];
+  t.GR = c->GR[0];
+  t.gl_id = c->gl_id[0] != 0;
+  t.gr_id = c->gr_id[0] != 0;
+  t.tol = tol;
+  t.Cinv = ws(c, "tg_Cinv", DD);
+  t.Ct = ws(c, "tg_Ct", DD);
+  t.CtC = ws(c, "tg_CtC", DD);
+  t.CCt = ws(c, "tg_CCt", DD);
+  t.Id = ws(c, "tg_Id", DD);
+  pinv_cutoff(c, t.C, rcond, t.Cinv);
+  HIPCHK(launch_transpose(D, D, t.C, D, t.Ct, D, c->stream));
+  gemm2(c, 'T', 'N', D, D, D, 1.0, t.C, D, t.C, D, 0.0, t.CtC, D);
+  gemm2(c, 'N', 'T', D, D, D, 1.0, t.C, D, t.C, D, 0.0, t.CCt, D);
+  fill(c, DD, 0.0, t.Id);
+  HIPCHK(launch_set_identity(D, t.Id, D, c->stream));
+  double* y = ws(c, "tg_X", n);
+  hac_apply_env(c, t.GL, t.GR, t.AC, y);
+  t.lam = dot(c, n, t.AC, y);
+  return t;
+}
+
+// d/dW <Phi(B)| H[W] |psi> of site 0 in the (w, d, w, d) layout: the defect site, then the adjoint
+// environments (bra defect, ket psi) for the MPO tensors right of it (bra AR / ket AR between) and
+// left of it (bra AL / ket AL between)
+std::vector<double> tangent_w_sandwich(imps_ctx* c, Tangent& t, const double* B) {
+  const int D = c->D, w = c->w, d = c->d;
+  std::vector<double> S((size_t)w * d * w * d, 0.0);
+  double* Ek = ws(c, "tg_env", (size_t)D * w * D);
+  const int gl0 = t.gl_id ? 0 : -1, grw = t.gr_id ? w - 1 : -1;
+  sandwich_acc(c, t.GL, B, t.AC, t.GR, 1.0, S);
+  if (w >= 3) {
+    defect_env(c, t, true, t.GL, gl0, B, t.AC, t.AR, t.AR, t.CtC, t.Id, 1, w - 2, Ek);
+    sandwich_acc(c, Ek, t.AR, t.AR, t.GR, 1.0, S);
+  }
+  defect_env(c, t, false, t.GR, grw, B, t.AC, t.AL, t.AL, t.CCt, t.Id, 1, w - 1, Ek);
+  sandwich_acc(c, t.GL, t.AL, t.AL, Ek, 1.0, S);
+  return S;
+}
+
+// MINRES (Paige & Saunders 1975) for a symmetric operator from x = 0, on device vectors with the
+// recurrence's scalars on the host; stops when the residual estimate is below max(tol |b|, atol).
+// Returns the iteration count; *est receives the estimate relative to |b|.
+int minres_dev(imps_ctx* c, const Op& op, size_t n, const double* b, double* x, double tol, int maxiter, double atol,
+               double* est) {
+  const double beta1 = nrm2(c, n, b);
+  fill(c, n, 0.0, x);
+  *est = 0.0;
+  if (beta1 <= atol) return 0;
+  double* r1 = ws(c, "mr_r1", n);
+  double* r2 = ws(c, "mr_r2", n);
+  double* y = ws(c, "mr_y", n);
+  double* v = ws(c, "mr_v", n);
+  double* wv = ws(c, "mr_w", n);
+  double* w2 = ws(c, "mr_w2", n);
+  double* sp = ws(c, "mr_w1", n);
+  d2d(c, r1, b, n);
+  d2d(c, r2, b, n);
+  d2d(c, y, b, n);
+  fill(c, n, 0.0, wv);
+  fill(c, n, 0.0, w2);
+  double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
+  int it = 0;
+  while (it < maxiter) {
+    ++it;
+    axpby(c, n, 1.0 / beta, r2, 0.0, v);  // r2 holds the latest Lanczos residual
+    op(v, y);
+    if (it >= 2) axpby(c, n, -(beta / oldb), r1, 1.0, y);
+    const double alfa = dot(c, n, v, y);
+    axpby(c, n, -(alfa / beta), r2, 1.0, y);
+    std::swap(r1, r2);  // r1 <- r2
+    std::swap(r2, y);   // r2 <- y (the old r1 buffer becomes y's, free until the next op)
+    oldb = beta;
+    beta = nrm2(c, n, r2);
+    const double oldeps = epsln;
+    const double delta = cs * dbar + sn * alfa;
+    const double gbar = sn * dbar - cs * alfa;
+    epsln = sn * beta;
+    dbar = -cs * beta;
+    const double gamma = std::max(std::hypot(gbar, beta), std::numeric_limits<double>::epsilon());
+    cs = gbar / gamma;
+    sn = beta / gamma;
+    const double phi = cs * phibar;
+    phibar = sn * phibar;
+    // w1, w2 = w2, w; w = (v - oldeps w1 - delta w2) / gamma, built in the spare buffer
+    axpby(c, n, 1.0 / gamma, v, 0.0, sp);
+    axpby(c, n, -oldeps / gamma, w2, 1.0, sp);
+    axpby(c, n, -delta / gamma, wv, 1.0, sp);
+    double* old_w1 = w2;
+    w2 = wv;
+    wv = sp;
+    sp = old_w1;
+    axpby(c, n, phi, wv, 1.0, x);
+    if (!std::isfinite(phibar)) break;
+    if (phibar <= std::max(tol * beta1, atol) || beta == 0.0) break;
+  }
+  *est = phibar / beta1;
+  return it;
+}
+
+// The context's model and environments, set aside while those of another MPO (an objective) are
+// built on the same state; restored on destruction (also when an exception leaves the scope).
+struct ModelSwap {
+  imps_ctx* c;
+  int w, d;
+  std::vector<std::vector<double>> Wh;
+  std::vector<double*> Wm, GL, GR;
+  std::vector<char> gl_id, gr_id;
+  bool envs_valid;
+  int planes_site, hc_planes_site;
+  std::vector<double*> owned;
+  ModelSwap(imps_ctx* ctx, int wo, const double* Wobj) : c(ctx) {
+    const int N = c->N, D = c->D, dd = c->d;
+    w = c->w;
+    d = c->d;
+    Wh.swap(c->Wh);
+    Wm.swap(c->Wm);
+    GL.swap(c->GL);
+    GR.swap(c->GR);
+    gl_id.swap(c->gl_id);
+    gr_id.swap(c->gr_id);
+    envs_valid = c->envs_valid;
+    planes_site = c->planes_site;
+    hc_planes_site = c->hc_planes_site;
+    c->w = wo;
+    c->envs_valid = false;
+    c->planes_site = -1;
+    c->hc_planes_site = -1;
+    const int wd = wo * dd;
+    std::vector<double> Wl(Wobj, Wobj + (size_t)wo * dd * wo * dd);
+    std::vector<double> Wmh((size_t)wd * wd);
+    for (int i = 0; i < wo; ++i)
+      for (int u = 0; u < dd; ++u)
+        for (int j = 0; j < wo; ++j)
+          for (int s = 0; s < dd; ++s)
+            Wmh[(u + (size_t)dd * j) + (size_t)wd * (i + (size_t)wo * s)] =
+                Wl[i + (size_t)wo * (u + (size_t)dd * (j + (size_t)wo * s))];
+    try {
+      for (int l = 0; l < N; ++l) {
+        c->Wh.push_back(Wl);
+        double* dp = dalloc(Wmh.size());
+        owned.push_back(dp);
+        h2d(c, dp, Wmh.data(), Wmh.size());
+        c->Wm.push_back(dp);
+        double* gl = dalloc((size_t)D * wo * D);
+        owned.push_back(gl);
+        double* gr = dalloc((size_t)D * wo * D);
+        owned.push_back(gr);
+        c->GL.push_back(gl);
+        c->GR.push_back(gr);
+        c->gl_id.push_back(0);
+        c->gr_id.push_back(0);
+      }
+    } catch (...) {
+      restore();
+      throw;
+    }
+  }
+  void restore() {
+    (void)hipStreamSynchronize(c->stream);
+    for (double*& p : owned) dfree(p);
+    owned.clear();
+    c->w = w;
+    c->d = d;
+    c->Wh.swap(Wh);
+    c->Wm.swap(Wm);
+    c->GL.swap(GL);
+    c->GR.swap(GR);
+    c->gl_id.swap(gl_id);
+    c->gr_id.swap(gr_id);
+    c->envs_valid = envs_valid;
+    c->planes_site = planes_site;
+    c->hc_planes_site = hc_planes_site;
+  }
+  ~ModelSwap() {
+    if (c) restore();
+  }
+};
+
 // ------------------------------------------------------------------ state management
 
 void free_state(imps_ctx* c) {
@@ -2751,6 +3165,91 @@ int imps_energy_gradient(imps_ctx* c, double* const* dW) {
     need_envs(c);
     require(c->w * c->d <= 32, IMPS_E_UNSUPPORTED, "imps_energy_gradient: w*d > 32 is not supported");
     for (int site = 0; site < c->N; ++site) energy_gradient(c, site, dW[site]);
+  });
+}
+
+int imps_fixedpoint_gradient(imps_ctx* c, int wo, int d, const double* Wobj, double tol, int maxiter, double rcond,
+                             double* value, double* dW, imps_response_info* info) {
+  if (!c || !Wobj || !dW) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    require(c->N == 1, IMPS_E_UNSUPPORTED, "imps_fixedpoint_gradient: one-site unit cell only");
+    require(d == c->d, IMPS_E_ARG, "imps_fixedpoint_gradient: objective's physical dimension differs from the model's");
+    require(wo >= 2 && wo * d <= 64, IMPS_E_ARG, "imps_fixedpoint_gradient: objective level count out of range");
+    require(c->w * c->d <= 32, IMPS_E_UNSUPPORTED, "imps_fixedpoint_gradient: w*d > 32 is not supported");
+    require(tol > 0.0 && maxiter >= 1 && rcond >= 0.0 && rcond < 1.0, IMPS_E_ARG,
+            "imps_fixedpoint_gradient: tol > 0, maxiter >= 1 and 0 <= rcond < 1 required");
+    for (int u = 0; u < d; ++u)
+      for (int s = 0; s < d; ++s) {
+        const double e = u == s ? 1.0 : 0.0;
+        require(Wobj[(size_t)wo * (u + (size_t)d * ((size_t)wo * s))] == e &&
+                    Wobj[(wo - 1) + (size_t)wo * (u + (size_t)d * ((wo - 1) + (size_t)wo * s))] == e,
+                IMPS_E_ARG, "imps_fixedpoint_gradient: levels 0 and w-1 of the objective must be unit levels");
+      }
+    const int D = c->D;
+    const size_t n = (size_t)D * d * D;
+    double* g = ws(c, "fp_g", n);
+    double val = 0.0;
+    {
+      // the objective's environments on the same state, its value and tangent gradient
+      ModelSwap sw(c, wo, Wobj);
+      EnvStats est;
+      envs_build(c, 1e-14, 300, 30, &est);
+      val = energy(c);
+      hac_apply(c, 0, c->AC[0], g, PREC_FP64);
+      tangent_project(c, c->AL[0], g);
+    }
+    Tangent t = tangent_setup(c, rcond, std::min(1e-13, 1e-2 * tol));
+    double* Lm = ws(c, "fp_L", n);
+    // P (H_eff + K) P + (1 - P): the complement of the tangent space (the Hessian's null space,
+    // where g has no component) gets the eigenvalue 1, which keeps MINRES's recurrence estimate
+    // tied to the true residual (on the singular form rounding components along the null space
+    // detach the two)
+    double* q = ws(c, "fp_q", (size_t)D * D);
+    Op op = [&](const double* v, double* o) {
+      tangent_hessian(c, t, v, o);
+      gemm2(c, 'T', 'N', D, D, d * D, 1.0, t.AL, (long)d * D, v, (long)d * D, 0.0, q, D);
+      gemm2(c, 'N', 'N', d * D, D, D, 1.0, t.AL, (long)d * D, q, D, 1.0, o, (long)d * D);
+    };
+    double estimate = 0.0;
+    const int its = minres_dev(c, op, n, g, Lm, tol, maxiter, 1e-14, &estimate);
+    double* res = ws(c, "fp_res", n);
+    tangent_hessian(c, t, Lm, res);
+    axpby(c, n, -1.0, g, 1.0, res);
+    const double ng = nrm2(c, n, g);
+    const double relres = nrm2(c, n, res) / std::max(ng, 1e-300);
+    std::vector<double> S = tangent_w_sandwich(c, t, Lm);
+    for (size_t k = 0; k < S.size(); ++k) dW[k] = -2.0 * S[k];
+    if (value) *value = val;
+    if (info) {
+      info->iterations = its;
+      info->converged = (estimate <= tol || ng <= 1e-14) ? 1 : 0;
+      info->relres = relres;
+      info->estimate = estimate;
+      info->hessian_applies = t.hessian_applies;
+      info->level_solves = t.solves;
+      info->level_ops = t.ops;
+      info->unconverged_solves = t.unconverged;
+    }
+  });
+}
+
+int imps_tangent_hessian_apply(imps_ctx* c, double rcond, double tol, const double* B, double* out) {
+  if (!c || !B || !out) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    need_envs(c);
+    require(c->N == 1, IMPS_E_UNSUPPORTED, "imps_tangent_hessian_apply: one-site unit cell only");
+    require(tol > 0.0 && rcond >= 0.0 && rcond < 1.0, IMPS_E_ARG,
+            "imps_tangent_hessian_apply: tol > 0 and 0 <= rcond < 1 required");
+    const size_t n = (size_t)c->D * c->d * c->D;
+    Tangent t = tangent_setup(c, rcond, tol);
+    double* x = ws(c, "fp_L", n);
+    double* y = ws(c, "fp_res", n);
+    h2d(c, x, B, n);
+    tangent_hessian(c, t, x, y);
+    d2h(c, out, y, n);
   });
 }
 

@@ -140,6 +140,9 @@ hipError_t launch_trace_prod_parts(int n, const double* l, long ldl, const doubl
 // and *out = c when out != nullptr (one workgroup)
 hipError_t launch_reduce_add_diag(int np, const double* part, double factor, double* x, long ld, int n, double* out,
                                   hipStream_t s);
+// c = sum_{b < np} part[b] (fixed order), then v[i + ldv*j] += factor * c * fix[i + ldf*j] (n x n)
+hipError_t launch_reduce_axpy_mat(int np, const double* part, double factor, const double* fix, long ldf, double* v,
+                                  long ldv, int n, hipStream_t s);
 // column/row sign fix for QRpos: Q[:,i] *= s_i, R[i,:] *= s_i with s_i = sign(R[i,i]);
 // sbuf: n doubles of device scratch
 hipError_t launch_qr_signfix(int m, int n, double* Q, long ldq, double* R, long ldr, double* sbuf,

@@ -1299,6 +1299,34 @@ hipError_t launch_reduce_add_diag(int np, const double* part, double factor, dou
   return hipGetLastError();
 }
 
+// c = sum_b part[b] (fixed order; every workgroup sums the partials itself), then
+// v[i + ldv*j] += factor * c * fix[i + ldf*j] for i, j < n.  Grid: (ceil(n / 256), n).
+__global__ void __launch_bounds__(256) reduce_axpy_mat_kernel(int np, const double* __restrict__ part, double factor,
+                                                              const double* __restrict__ fix, long ldf,
+                                                              double* __restrict__ v, long ldv, int n) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < np; b += 256) s += part[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double f = factor * red[0];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < n) v[i + ldv * j] += f * fix[i + ldf * j];
+}
+
+hipError_t launch_reduce_axpy_mat(int np, const double* part, double factor, const double* fix, long ldf, double* v,
+                                  long ldv, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_axpy_mat_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, np, part, factor, fix, ldf,
+                     v, ldv, n);
+  return hipGetLastError();
+}
+
 __device__ inline double sgn_nonneg(double x) { return x < 0.0 ? -1.0 : 1.0; }
 
 __global__ void qr_signs_kernel(int n, const double* R, long ldr, double* sg) {

@@ -124,3 +124,30 @@ def test_energy_objective_has_no_response():
     val, dW, _ = R.fixedpoint_gradient(psi, W, envs, W)
     assert abs(val - O.expectation_energy(psi, [W], envs)) < 1e-12
     assert np.abs(dW).max() < 1e-9
+
+
+def test_response_of_sz_matches_tfim_closed_form():
+    """d<sigma_z>/dh through the fixed point of the TFIM (J = 1, h = 1.3, gapped) against the
+    closed form -e''(h) of the free-fermion energy e(h) = -(1/pi) int_0^pi sqrt(1 + h^2 - 2 h cos k)
+    (e'(h) = -<sigma_z>): -e'' = (1/pi) int_0^pi sin^2 k / (1 + h^2 - 2 h cos k)^(3/2) dk.  A
+    dropped curvature term or a wrong sign of an environment term misses it by far more than the
+    truncation error at D = 16."""
+    import math
+
+    from scipy import integrate
+
+    h, D = 1.3, 16
+    W = O.tfim_mpo(1.0, h)
+    psi, envs = _groundstate(W, D)
+    val, dW, info = R.fixedpoint_gradient(psi, W, envs, R.objective_mpo_1site(O.sigma_z()), tol=1e-11)
+    assert info["relres"] < 1e-7, info
+    pred = float(np.sum(dW * (O.tfim_mpo(1.0, h + 1.0) - W)))
+    exact = integrate.quad(lambda k: math.sin(k) ** 2 / (1 + h * h - 2 * h * math.cos(k)) ** 1.5, 0, math.pi,
+                           epsabs=1e-14, epsrel=1e-14)[0] / math.pi
+    assert abs(pred - exact) < 1e-9, (pred, exact)
+    # without the curvature term the prediction is visibly off
+    ts = R.TangentSpace(psi, W, envs)
+    _, g = R.objective_gradient(psi, R.objective_mpo_1site(O.sigma_z()))
+    Lh, _, _ = R.minres(lambda B: ts.project(ts.heff_apply(ts.project(B))) + B - ts.project(B), g, 1e-11, 500)
+    pred_h = float(np.sum(-2.0 * ts.w_sandwich(Lh) * (O.tfim_mpo(1.0, h + 1.0) - W)))
+    assert abs(pred_h - exact) > 1e3 * abs(pred - exact)
