@@ -446,7 +446,10 @@ def gradients(imps):
     energy (step 1e-5), and de/dDelta of XXZ (D=512, Delta drawn from U[0.5,1.5] with PCG64(5))
     against central differences; the gradient contracts imps_energy_gradient with dW/dparameter
     (the MPOs are linear in h and Delta)."""
+    import math
+
     import numpy as np
+    from scipy import integrate
     out = {}
 
     def solve(H, D, seed, tol, psi0=None, maxiter=1000):
@@ -481,6 +484,36 @@ def gradients(imps):
     fd = (ep - em) / (2 * dh)
     out["tfim_D128"] = {"h": h, "itererr": info.itererr, "de_dh": g, "minus_sz": -sz, "fd": fd,
                         "abs_err_vs_sz": abs(g + sz), "abs_err_vs_fd": abs(g - fd), "gradient_seconds": dt}
+    # the fixed-point response of <sz> (imps_fixedpoint_gradient) against the closed form -e''(h)
+    t0 = time.perf_counter()
+    _, dW, rinfo = imps.fixedpoint_gradient(psi, H, envs, imps.objective_mpo_1site(imps.models.sigma_z()), tol=1e-10)
+    dt = time.perf_counter() - t0
+    pred = float(np.sum(dW * (imps.models.tfim_hamiltonian(J=1.0, h=h + 1.0, nsites=1).Ws[0] - H.Ws[0])))
+    exact = integrate.quad(lambda k: math.sin(k) ** 2 / (1 + h * h - 2 * h * math.cos(k)) ** 1.5, 0, math.pi,
+                           epsabs=1e-14, epsrel=1e-14)[0] / math.pi
+    out["tfim_D128_response_sz"] = {"dsz_dh": pred, "closed_form": exact, "abs_err": abs(pred - exact),
+                                    "minres_iterations": rinfo.iterations, "relres": rinfo.relres,
+                                    "level_solves": rinfo.level_solves, "level_ops": rinfo.level_ops,
+                                    "seconds": dt}
+    ctx.close()
+    # response of the nearest-neighbour <Sz Sz> to Delta, XXZ one-site cell, D=256 (timing)
+    sz2 = np.diag([0.5, -0.5])
+    Wszsz = np.zeros((3, 2, 3, 2))
+    Wszsz[0, :, 0, :] = np.eye(2)
+    Wszsz[2, :, 2, :] = np.eye(2)
+    Wszsz[0, :, 1, :] = sz2
+    Wszsz[1, :, 2, :] = sz2
+    H = imps.models.heisenberg_hamiltonian(J=1.0, Delta=0.7, nsites=1)
+    psi, envs, info, ctx = solve(H, 256, 2, 1e-10, maxiter=1500)
+    t0 = time.perf_counter()
+    val, dW, rinfo = imps.fixedpoint_gradient(psi, H, envs, Wszsz, tol=1e-10)
+    dt = time.perf_counter() - t0
+    pred = float(np.sum(dW * (imps.models.heisenberg_hamiltonian(J=1.0, Delta=1.7, nsites=1).Ws[0] - H.Ws[0])))
+    out["xxz_D256_response_szsz"] = {"Delta": 0.7, "itererr": info.itererr, "szsz": val, "dszsz_dDelta": pred,
+                                     "minres_iterations": rinfo.iterations, "relres": rinfo.relres,
+                                     "level_solves": rinfo.level_solves, "level_ops": rinfo.level_ops,
+                                     "seconds": dt}
+    log(f"bench: response XXZ D=256: {rinfo.iterations} MINRES iterations, relres {rinfo.relres:.1e}, {dt:.1f}s")
     ctx.close()
     # XXZ, N=2, D=512
     delta = float(np.random.Generator(np.random.PCG64(5)).uniform(0.5, 1.5))
