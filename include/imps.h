@@ -419,6 +419,24 @@ typedef struct imps_batch_instance {
 int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imps_vumps_params* p,
                      int nworkers, double gauge_tol);
 
+/* The same sweep as one device batch: every instance runs find_groundstate (vumps.jl:51-86) from
+ * CanonicalIMPS(As) (canonicalmps.jl:151-170), and all instances advance in lockstep, one host
+ * thread and one HIP stream for the whole batch: each Krylov step (Lanczos eigsolve, GMRES
+ * linsolve, Arnoldi gauge fixed point) of all active instances is one grouped GEMM launch per
+ * contraction stage, one grouped Gram-Schmidt sequence and one readback; every QR/LQ sweep of all
+ * instances is one batched Householder launch.  No counterpart in the reference (its sweeps are
+ * loops over find_groundstate).  Scope: one-site unit cells with the same w and d (bond
+ * dimensions may differ: multiples of 16, d D <= 2048), FP64 or FP32 bulk arithmetic (the FP32
+ * phase switches to FP64 eigensolves below refine_below or at a stall), no finalize callback;
+ * IMPS_E_UNSUPPORTED otherwise.  The message of a failure is read with imps_last_error(NULL). */
+typedef struct imps_batch_stats {
+  long rounds;   /* lockstep rounds (each ends with one host synchronisation) */
+  long launches; /* kernel launches for the whole batch */
+  double seconds;
+} imps_batch_stats;
+int imps_vumps_batch_device(int device, int ninst, imps_batch_instance* inst, const imps_vumps_params* p,
+                            double gauge_tol, imps_batch_stats* stats);
+
 /* ---- measurement helpers ----------------------------------------------------------- */
 
 /* Runs `reps` H_AC applications on device-resident random data of the current

@@ -45,6 +45,10 @@ class ResponseInfo(ctypes.Structure):
                 ("unconverged_solves", ctypes.c_long)]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_long), ("launches", ctypes.c_long), ("seconds", c_double)]
+
+
 FINALIZE_CB = ctypes.CFUNCTYPE(c_int, c_int, c_void_p, c_void_p)
 
 
@@ -141,6 +145,8 @@ SIGNATURES = [
                                   P_double]),
     ("imps_dense_expectation", c_int, [c_void_p, c_int, c_int, c_int, PP_double, c_double, c_int, c_int, P_double,
                                        PP_double, PP_double]),
+    ("imps_vumps_batch_device", c_int, [c_int, c_int, ctypes.POINTER(BatchInstance), ctypes.POINTER(VumpsParams),
+                                        c_double, ctypes.POINTER(BatchStats)]),
     ("imps_vumps_batch", c_int, [c_int, c_int, ctypes.POINTER(BatchInstance), ctypes.POINTER(VumpsParams), c_int,
                                  c_double]),
     ("imps_bench_hac", c_int, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(HacTiming)]),

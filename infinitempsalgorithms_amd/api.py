@@ -980,12 +980,34 @@ def time_evolve(psi0: CanonicalIMPS, H: SparseIMPO, t_span, alg: TDVP | None = N
     return psi, envs, history
 
 
-def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, device=0, gauge_tol=1e-12):
-    """Batched parameter sweep: find_groundstate (vumps.jl:51-86) for every
-    (H::SparseIMPO, raw tensors As) in `problems`, the instances running concurrently on
-    the GPU (one context and stream per worker thread).  Returns one dict per problem
-    with niter, converged, itererr, energy (unit cell), seconds and hac_applies."""
+def device_batch_eligible(problems, alg: VUMPS | None = None):
+    """True when imps_vumps_batch_device takes the sweep: one-site cells with a common MPO shape,
+    bond dimensions that are multiples of 16 (d D <= 2048), FP64 or FP32 bulk arithmetic and no
+    finalize callback."""
     alg = alg or VUMPS()
+    if not problems or alg.prec not in (_lib.IMPS_PREC_FP64, _lib.IMPS_PREC_FP32) or alg.finalize is not None:
+        return False
+    H0 = problems[0][0]
+    for H, As in problems:
+        D = np.asarray(As[0]).shape[0]
+        if len(H) != 1 or H.w != H0.w or H.d != H0.d or D % 16 or D < 16 or H.d * D > 2048:
+            return False
+    return H0.w * H0.d <= 64 and H0.d <= 8
+
+
+def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, device=0, gauge_tol=1e-12,
+                              mode="auto", stats=None):
+    """Batched parameter sweep: find_groundstate (vumps.jl:51-86) for every
+    (H::SparseIMPO, raw tensors As) in `problems`.  mode "device": one lockstep device batch
+    (imps_vumps_batch_device: grouped kernels for all instances, one host thread); "threads": the
+    instances on `nworkers` host threads, one context and stream each (imps_vumps_batch, any unit
+    cell or precision); "auto" takes the device batch when `device_batch_eligible`.  Returns one
+    dict per problem with niter, converged, itererr, energy (unit cell), seconds and hac_applies;
+    a dict passed as `stats` receives the device batch's rounds, launches and seconds."""
+    alg = alg or VUMPS()
+    if mode not in ("auto", "device", "threads"):
+        raise ValueError("find_groundstates_batched: mode must be 'auto', 'device' or 'threads'")
+    use_device = mode == "device" or (mode == "auto" and device_batch_eligible(problems, alg))
     L = _lib.load()
     keep = []
     arr = (_lib.BatchInstance * len(problems))()
@@ -997,7 +1019,20 @@ def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, de
         keep.append((Ws, As, kw, ka))
         arr[k] = _lib.BatchInstance(len(Ws), H.w, H.d, As[0].shape[0], pw, pa, 0, 0, 0, 0.0, 0.0, 0.0, 0)
     p = alg.params()
-    st = L.imps_vumps_batch(int(device), len(problems), arr, ctypes.byref(p), int(nworkers), float(gauge_tol))
+    if use_device:
+        bs = _lib.BatchStats()
+        st = L.imps_vumps_batch_device(int(device), len(problems), arr, ctypes.byref(p), float(gauge_tol),
+                                       ctypes.byref(bs))
+        if st != 0:
+            msg = L.imps_last_error(None)
+            raise _lib.ImpsError(f"imps_vumps_batch_device failed with status {st}: "
+                                 f"{msg.decode() if msg else ''}")
+        if stats is not None:
+            stats.update({"rounds": bs.rounds, "launches": bs.launches, "seconds": bs.seconds, "mode": "device"})
+    else:
+        st = L.imps_vumps_batch(int(device), len(problems), arr, ctypes.byref(p), int(nworkers), float(gauge_tol))
+        if stats is not None:
+            stats.update({"mode": "threads"})
     out = [{"status": r.status, "niter": r.niter, "converged": bool(r.converged), "itererr": r.itererr,
             "energy": r.energy, "seconds": r.seconds, "hac_applies": r.hac_applies} for r in arr]
     if st != 0:

@@ -16,7 +16,7 @@ LIB = os.path.join(PKG, "libimps.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IMPS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["imps_kernels.hip", "imps_planes.hip", "imps_core.hip", "imps_small.cpp", "imps_io.cpp"]
+SOURCES = ["imps_kernels.hip", "imps_planes.hip", "imps_core.hip", "imps_batch.hip", "imps_small.cpp", "imps_io.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-I", INCLUDE, "-I", CSRC]

@@ -133,12 +133,14 @@ def _sweep_problems(imps):
                     for k, (h, D) in enumerate(zip(hs, Ds))]
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_tfim_sweep_64_instances_full_config(imps, prec):
+@pytest.mark.parametrize("prec,mode", [(0, "device"), (1, "device"), (0, "threads"), (1, "threads")])
+def test_tfim_sweep_64_instances_full_config(imps, prec, mode):
     hs, Ds, problems = _sweep_problems(imps)
     assert sorted(set(int(D) for D in Ds)) == [32, 64, 128, 256]
     alg = imps.VUMPS(tol=1e-8, maxiter=200, prec=prec, refine_below=1e-6 if prec else 0.0)
-    res = imps.find_groundstates_batched(problems, alg, nworkers=8)
+    stats = {}
+    res = imps.find_groundstates_batched(problems, alg, nworkers=8, mode=mode, stats=stats)
+    print("sweep", mode, prec, stats, flush=True)
     assert len(res) == 64
     for h, r in zip(hs, res):
         assert r["converged"] and r["itererr"] <= 1e-8
