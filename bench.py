@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="batched parameter sweep: 64 TFIM instances, h in U[0.5,1.5], D in {32,64,128,256}")
     ap.add_argument("--sweep-workers", type=int, default=8)
+    ap.add_argument("--sweep-mode", default="both", choices=["device", "threads", "both"],
+                    help="--sweep: the lockstep device batch, the threaded batch, or both")
     ap.add_argument("--gradient", action="store_true",
                     help="energy gradients: TFIM D=128 de/dh and XXZ D=512 de/dDelta against finite differences")
     ap.add_argument("--vumps-iters", type=int, default=3)
@@ -425,20 +427,35 @@ def parameter_sweep(args, imps, prec):
     problems = [(imps.models.tfim_hamiltonian(J=1.0, h=float(h), nsites=1), imps.random_tensors(int(D), [2], seed=k))
                 for k, (h, D) in enumerate(zip(hs, Ds))]
     alg = imps.VUMPS(tol=1e-8, maxiter=200, prec=prec, refine_below=args.refine_below)
-    log(f"bench: batched sweep of {len(problems)} TFIM instances, {args.sweep_workers} workers")
-    t0 = time.perf_counter()
-    res = imps.find_groundstates_batched(problems, alg, nworkers=args.sweep_workers)
-    wall = time.perf_counter() - t0
-    flop = sum(r["hac_applies"] * 4.0 * 3 * 2 * float(D) ** 3 for r, D in zip(res, Ds))
-    err = [abs(r["energy"] - tfim_exact_energy(float(h))) for r, h in zip(res, hs)]
-    return {"instances": len(res), "workers": args.sweep_workers, "prec": args.prec, "wall_seconds": wall,
-            "converged": sum(r["converged"] for r in res), "total_iterations": sum(r["niter"] for r in res),
-            "hac_applies": sum(r["hac_applies"] for r in res), "hac_tflops": flop / wall / 1e12,
-            "max_itererr": max(r["itererr"] for r in res),
-            "energy_error_vs_exact": {"max": max(err), "median": float(np.median(err))},
-            "per_D": {int(D): {"n": int(sum(Ds == D)), "mean_seconds": float(np.mean([r["seconds"] for r, x in zip(res, Ds) if x == D])),
-                               "mean_iters": float(np.mean([r["niter"] for r, x in zip(res, Ds) if x == D]))}
-                      for D in (32, 64, 128, 256)}}
+    out = {}
+    energies = {}
+    modes = ["device", "threads"] if args.sweep_mode == "both" else [args.sweep_mode]
+    for mode in modes:
+        log(f"bench: batched sweep of {len(problems)} TFIM instances, mode {mode}")
+        stats = {}
+        t0 = time.perf_counter()
+        res = imps.find_groundstates_batched(problems, alg, nworkers=args.sweep_workers, mode=mode, stats=stats)
+        wall = time.perf_counter() - t0
+        flop = sum(r["hac_applies"] * 4.0 * 3 * 2 * float(D) ** 3 for r, D in zip(res, Ds))
+        err = [abs(r["energy"] - tfim_exact_energy(float(h))) for r, h in zip(res, hs)]
+        energies[mode] = [r["energy"] for r in res]
+        out[mode] = {"instances": len(res), "workers": args.sweep_workers if mode == "threads" else 1,
+                     "prec": args.prec, "wall_seconds": wall, "stats": stats,
+                     "converged": sum(r["converged"] for r in res), "total_iterations": sum(r["niter"] for r in res),
+                     "max_iterations": max(r["niter"] for r in res),
+                     "hac_applies": sum(r["hac_applies"] for r in res), "hac_tflops": flop / wall / 1e12,
+                     "max_itererr": max(r["itererr"] for r in res),
+                     "energy_error_vs_exact": {"max": max(err), "median": float(np.median(err))},
+                     "per_D": {int(D): {"n": int(sum(Ds == D)),
+                                        "mean_seconds": float(np.mean([r["seconds"] for r, x in zip(res, Ds) if x == D])),
+                                        "mean_iters": float(np.mean([r["niter"] for r, x in zip(res, Ds) if x == D]))}
+                               for D in (32, 64, 128, 256)}}
+        log(f"bench: sweep {mode}: {wall:.2f}s, {out[mode]['converged']} converged, max |e - exact| {max(err):.1e}")
+    if len(modes) == 2:
+        out["device_speedup"] = out["threads"]["wall_seconds"] / out["device"]["wall_seconds"]
+        out["max_energy_diff_device_threads"] = max(abs(a - b) for a, b in zip(energies["device"],
+                                                                                energies["threads"]))
+    return out
 
 
 def gradients(imps):

@@ -430,9 +430,12 @@ int imps_vumps_batch(int device, int ninst, imps_batch_instance* inst, const imp
  * phase switches to FP64 eigensolves below refine_below or at a stall), no finalize callback;
  * IMPS_E_UNSUPPORTED otherwise.  The message of a failure is read with imps_last_error(NULL). */
 typedef struct imps_batch_stats {
-  long rounds;   /* lockstep rounds (each ends with one host synchronisation) */
+  long rounds;   /* lockstep rounds (each ends with one host synchronisation), all groups */
   long launches; /* kernel launches for the whole batch */
   double seconds;
+  int groups;    /* lockstep groups run concurrently (one host thread and stream each) */
+  double t_phase[6]; /* wall seconds of the slowest group in: canonicalisation, local update,
+                        gauge_step!, recalculate!, calc_galerkin, expectationvalue */
 } imps_batch_stats;
 int imps_vumps_batch_device(int device, int ninst, imps_batch_instance* inst, const imps_vumps_params* p,
                             double gauge_tol, imps_batch_stats* stats);

@@ -46,7 +46,8 @@ class ResponseInfo(ctypes.Structure):
 
 
 class BatchStats(ctypes.Structure):
-    _fields_ = [("rounds", ctypes.c_long), ("launches", ctypes.c_long), ("seconds", c_double)]
+    _fields_ = [("rounds", ctypes.c_long), ("launches", ctypes.c_long), ("seconds", c_double), ("groups", c_int),
+                ("t_phase", c_double * 6)]
 
 
 FINALIZE_CB = ctypes.CFUNCTYPE(c_int, c_int, c_void_p, c_void_p)

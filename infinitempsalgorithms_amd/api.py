@@ -1028,7 +1028,10 @@ def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, de
             raise _lib.ImpsError(f"imps_vumps_batch_device failed with status {st}: "
                                  f"{msg.decode() if msg else ''}")
         if stats is not None:
-            stats.update({"rounds": bs.rounds, "launches": bs.launches, "seconds": bs.seconds, "mode": "device"})
+            stats.update({"rounds": bs.rounds, "launches": bs.launches, "seconds": bs.seconds, "mode": "device",
+                          "groups": bs.groups,
+                          "phase_seconds": dict(zip(["canonicalize", "local_update", "gauge", "envs", "galerkin",
+                                                     "energy"], list(bs.t_phase)))})
     else:
         st = L.imps_vumps_batch(int(device), len(problems), arr, ctypes.byref(p), int(nworkers), float(gauge_tol))
         if stats is not None:

@@ -18,8 +18,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -51,6 +56,14 @@ void brequire(bool ok, int code, const std::string& msg) {
 }
 
 // ====================================================================== grouped kernels
+
+// Operand pointers of the grouped kernels are read from descriptor arrays in device memory, so the
+// compiler cannot tell that they address global memory; they are cast to the global address space
+// (address space 1) to get global rather than flat loads and stores (flat accesses would also count
+// against the LDS wait counter and serialise the LDS traffic of the tiled kernels).
+typedef __attribute__((address_space(1))) double gdbl;
+#define GP(p) ((gdbl*)(p))
+#define GPC(p) ((const gdbl*)(p))
 
 // Index of the item whose first block is the largest one <= b (items sorted by b0).
 template <typename T>
@@ -122,8 +135,8 @@ __global__ void __launch_bounds__(256) ggemm_kernel(const GG* __restrict__ items
   for (int kt = 0; kt < nkt; ++kt) {
     const int kg0 = kt * BK;
     const int kc = kg0 / g.Kc, kk0 = kg0 - kc * g.Kc;
-    const double* Ab = g.A + (long)kc * g.sAc + (long)kk0 * g.sAk;
-    const double* Bb = g.B + (long)kc * g.sBc + (long)kk0 * g.sBk;
+    const gdbl* Ab = GPC(g.A) + (long)kc * g.sAc + (long)kk0 * g.sAk;
+    const gdbl* Bb = GPC(g.B) + (long)kc * g.sBc + (long)kk0 * g.sBk;
     double ra[AE], rb[BE];
 #pragma unroll
     for (int e = 0; e < AE; ++e) {
@@ -174,7 +187,7 @@ __global__ void __launch_bounds__(256) ggemm_kernel(const GG* __restrict__ items
       for (int r = 0; r < 4; ++r) {
         const int n = n0 + wn + j * 16 + MT::row(lk, r);
         if (n < g.N) {
-          double* p = g.C + m + (long)n * g.ldc;
+          gdbl* p = GP(g.C) + m + (long)n * g.ldc;
           const double v = g.alpha * (double)acc[i][j][r];
           *p = (g.beta == 0.0) ? v : v + g.beta * *p;
         }
@@ -211,32 +224,36 @@ __global__ void __launch_bounds__(256) gvec_kernel(const GV* __restrict__ items,
   const GV it = items[find_item(items, nitems, (int)blockIdx.x)];
   const long stride = (long)it.nblk * 256;
   const long i0 = (long)((int)blockIdx.x - it.b0) * 256 + threadIdx.x;
+  const gdbl* x = GPC(it.x);
+  gdbl* y = GP(it.y);
+  const gdbl* X = GPC(it.X);
+  const gdbl* coef = GPC(it.coef);
   double sc = 0.0;
   if (it.op == GV_ISQRT) {
-    const double s = *it.x;
+    const double s = *x;
     if (!(s > 0.0)) return;
     sc = 1.0 / sqrt(s);
   }
   for (long i = i0; i < it.n; i += stride) {
     switch (it.op) {
-      case GV_COPY: it.y[i] = it.x[i]; break;
-      case GV_AXPBY: it.y[i] = it.a * it.x[i] + it.b * it.y[i]; break;
-      case GV_ISQRT: it.y[i] *= sc; break;
-      case GV_FILL: it.y[i] = it.a; break;
+      case GV_COPY: y[i] = x[i]; break;
+      case GV_AXPBY: y[i] = it.a * x[i] + it.b * y[i]; break;
+      case GV_ISQRT: y[i] *= sc; break;
+      case GV_FILL: y[i] = it.a; break;
       case GV_MAXPY: {
         double s = 0.0;
-        for (int v = 0; v < it.nv; ++v) s += it.coef[v] * it.X[(long)v * it.ldX + i];
-        it.y[i] = (it.b == 0.0 ? 0.0 : it.b * it.y[i]) + it.a * s;
+        for (int v = 0; v < it.nv; ++v) s += coef[v] * X[(long)v * it.ldX + i];
+        y[i] = (it.b == 0.0 ? 0.0 : it.b * y[i]) + it.a * s;
         break;
       }
       case GV_COPY2D: {
         const long r = i % it.rows, c = i / it.rows;
-        it.y[r + it.ldy * c] = it.x[r + it.ldx * c];
+        y[r + it.ldy * c] = x[r + it.ldx * c];
         break;
       }
       case GV_IDENT2D: {
         const long r = i % it.rows, c = i / it.rows;
-        it.y[r + it.ldy * c] = (r == c) ? 1.0 : 0.0;
+        y[r + it.ldy * c] = (r == c) ? 1.0 : 0.0;
         break;
       }
       default: break;
@@ -264,7 +281,7 @@ __device__ inline double bwave_sum(double v) {
 }
 
 template <int NV>
-__device__ inline void bblock_store(double (&acc)[NV], int nvalid, double* part, int stride, int blk) {
+__device__ inline void bblock_store(double (&acc)[NV], int nvalid, gdbl* part, int stride, int blk) {
   __shared__ double red[NV][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -285,17 +302,18 @@ __global__ void __launch_bounds__(256) gc_dots_kernel(const GC* __restrict__ ite
   const int blk = (int)blockIdx.x - it.b0;
   for (int vc = 0; vc < it.nv; vc += CH) {
     const int nc = min(CH, it.nv - vc);
-    const double* Xc = it.V + (size_t)vc * it.n;
+    const gdbl* Xc = GPC(it.V) + (size_t)vc * it.n;
+    const gdbl* w = GPC(it.w);
     double acc[CH];
 #pragma unroll
     for (int t = 0; t < CH; ++t) acc[t] = 0.0;
     for (long i = (long)blk * 256 + threadIdx.x; i < it.n; i += (long)it.nb * 256) {
-      const double yi = it.w[i];
+      const double yi = w[i];
 #pragma unroll
       for (int t = 0; t < CH; ++t)
         if (t < nc) acc[t] += Xc[(size_t)t * it.n + i] * yi;
     }
-    bblock_store<CH>(acc, nc, it.part + (size_t)vc * it.nb, it.nb, blk);
+    bblock_store<CH>(acc, nc, GP(it.part) + (size_t)vc * it.nb, it.nb, blk);
   }
 }
 
@@ -307,20 +325,22 @@ __global__ void __launch_bounds__(256) gc_axpy_kernel(const GC* __restrict__ ite
   const GC it = items[find_item(items, nitems, (int)blockIdx.x)];
   const int blk = (int)blockIdx.x - it.b0;
   __shared__ double c[MAXV];
-  if (threadIdx.x < MAXV) c[threadIdx.x] = ((int)threadIdx.x < it.nv) ? it.coef[coff + threadIdx.x] : 0.0;
+  if (threadIdx.x < MAXV) c[threadIdx.x] = ((int)threadIdx.x < it.nv) ? GPC(it.coef)[coff + threadIdx.x] : 0.0;
   __syncthreads();
+  const gdbl* V = GPC(it.V);
+  gdbl* w = GP(it.w);
   double acc[NA];
 #pragma unroll
   for (int t = 0; t < NA; ++t) acc[t] = 0.0;
   for (long i = (long)blk * 256 + threadIdx.x; i < it.n; i += (long)it.nb * 256) {
     double xv[MAXV];
-    double wi = it.w[i];
+    double wi = w[i];
 #pragma unroll
     for (int v = 0; v < MAXV; ++v) {
-      xv[v] = (v < it.nv) ? it.V[(size_t)v * it.n + i] : 0.0;
+      xv[v] = (v < it.nv) ? V[(size_t)v * it.n + i] : 0.0;
       wi -= c[v] * xv[v];
     }
-    it.w[i] = wi;
+    w[i] = wi;
     if (DOTS) {
 #pragma unroll
       for (int v = 0; v < NA; ++v) acc[v] += xv[v] * wi;
@@ -328,7 +348,7 @@ __global__ void __launch_bounds__(256) gc_axpy_kernel(const GC* __restrict__ ite
       acc[0] += wi * wi;
     }
   }
-  bblock_store<NA>(acc, DOTS ? it.nv : 1, it.part, it.nb, blk);
+  bblock_store<NA>(acc, DOTS ? it.nv : 1, GP(it.part), it.nb, blk);
 }
 
 // coef[coff + v] = sum_b part[v nb + b] (nv values: nvals < 0 takes the item's nv); grid (items, 64)
@@ -338,17 +358,18 @@ __global__ void __launch_bounds__(64) gc_reduce_kernel(const GC* __restrict__ it
   const int nv = nvals < 0 ? it.nv : nvals;
   if (v >= nv) return;
   double s = 0.0;
-  for (int b = threadIdx.x; b < it.nb; b += 64) s += it.part[(size_t)v * it.nb + b];
+  for (int b = threadIdx.x; b < it.nb; b += 64) s += GPC(it.part)[(size_t)v * it.nb + b];
   s = bwave_sum(s);
-  if (threadIdx.x == 0) it.coef[coff + v] = s;
+  if (threadIdx.x == 0) GP(it.coef)[coff + v] = s;
 }
 
 __global__ void __launch_bounds__(256) gc_scale_kernel(const GC* __restrict__ items, int nitems) {
   const GC it = items[find_item(items, nitems, (int)blockIdx.x)];
-  const double s = it.coef[128];
+  const double s = GPC(it.coef)[128];
   if (!(s > 0.0)) return;
   const double inv = 1.0 / sqrt(s);
-  for (long i = (long)((int)blockIdx.x - it.b0) * 256 + threadIdx.x; i < it.n; i += (long)it.nb * 256) it.w[i] *= inv;
+  gdbl* w = GP(it.w);
+  for (long i = (long)((int)blockIdx.x - it.b0) * 256 + threadIdx.x; i < it.n; i += (long)it.nb * 256) w[i] *= inv;
 }
 
 // ---- grouped H_AC W-mix (the launch_wmix contraction): T2[b,u,bb,j] = sum W[i,u,j,s] T1[b,i,s,bb],
@@ -371,12 +392,12 @@ __global__ void __launch_bounds__(256) gwmix_kernel(const GW* __restrict__ items
   const int bb = loc / nbx;
   const int b = (loc % nbx) * 256 + threadIdx.x;
   const int wd = w * d, D = it.D;
-  for (int t = threadIdx.x; t < wd * wd; t += blockDim.x) Ws[t] = it.Wm[t];
+  for (int t = threadIdx.x; t < wd * wd; t += blockDim.x) Ws[t] = GPC(it.Wm)[t];
   __syncthreads();
   if (b >= D) return;
   double x[MAXIN];
-  const double* src = it.T1 + b + (size_t)D * wd * bb;
-  const double* src0 = it.X0 ? it.X0 + b + (size_t)D * d * bb : nullptr;
+  const gdbl* src = GPC(it.T1) + b + (size_t)D * wd * bb;
+  const gdbl* src0 = it.X0 ? GPC(it.X0) + b + (size_t)D * d * bb : nullptr;
 #pragma unroll
   for (int q = 0; q < MAXIN; ++q)
     if (q < wd) {
@@ -384,7 +405,7 @@ __global__ void __launch_bounds__(256) gwmix_kernel(const GW* __restrict__ items
       x[q] = (src0 && i == 0) ? src0[(size_t)D * (q / w)] : src[(size_t)D * q];
     }
   for (int j = 0; j < w; ++j) {
-    double* dst = (it.Y && j == w - 1) ? it.Y + b + (size_t)D * d * bb : it.T2 + b + (size_t)D * d * (bb + (size_t)D * j);
+    gdbl* dst = (it.Y && j == w - 1) ? GP(it.Y) + b + (size_t)D * d * bb : GP(it.T2) + b + (size_t)D * d * (bb + (size_t)D * j);
     for (int u = 0; u < d; ++u) {
       const int o = u + d * j;
       double acc = 0.0;
@@ -414,23 +435,24 @@ __global__ void __launch_bounds__(256) gmix_kernel(const GM* __restrict__ items,
   const int a = (loc % nbx) * 256 + threadIdx.x;
   if (a >= it.D1) return;
   const size_t base = (size_t)a + (size_t)it.D1 * d * bcol;
+  const gdbl* in = GPC(it.in);
   double x[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s)
-    if (s < d) x[s] = it.in[base + (size_t)it.D1 * s];
+    if (s < d) x[s] = in[base + (size_t)it.D1 * s];
   const double beta = it.beta;
   for (int u = 0; u < d; ++u) {
     double acc = 0.0;
 #pragma unroll
     for (int s = 0; s < 8; ++s)
       if (s < d) acc += it.M[u + d * s] * x[s];
-    double* p = it.out + base + (size_t)it.D1 * u;
+    gdbl* p = GP(it.out) + base + (size_t)it.D1 * u;
     *p = (beta == 0.0) ? acc : acc + beta * *p;
   }
 }
 
 // ---- grouped regularize! with the identity: v -= tr(l v) I (n x n; v leading dimension ldv,
-// l contiguous); one workgroup per item
+// l contiguous and symmetric); one workgroup per item
 struct GR {
   int n;
   long ldv;
@@ -438,20 +460,24 @@ struct GR {
   const double* l;
 };
 
-__global__ void __launch_bounds__(256) greg_kernel(const GR* __restrict__ items) {
+__global__ void __launch_bounds__(1024) greg_kernel(const GR* __restrict__ items) {
   const GR it = items[blockIdx.x];
-  __shared__ double red[4];
+  __shared__ double red[16];
   double s = 0.0;
   const long nn = (long)it.n * it.n;
-  for (long e = threadIdx.x; e < nn; e += 256) {
-    const long a = e % it.n, b = e / it.n;
-    s += it.l[a + (long)it.n * b] * it.v[b + it.ldv * a];
+  // tr(l v) = sum_{a,b} l[a + n b] v[b + ldv a]; l is symmetric here (C C^T or C^T C), so both
+  // operands are read along their columns
+  for (long e = threadIdx.x; e < nn; e += 1024) {
+    const long b = e % it.n, a = e / it.n;
+    s += GPC(it.l)[b + (long)it.n * a] * GPC(it.v)[b + it.ldv * a];
   }
   s = bwave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  const double t = red[0] + red[1] + red[2] + red[3];
-  for (int i = threadIdx.x; i < it.n; i += 256) it.v[(long)i * it.ldv + i] -= t;
+  double t = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) t += red[k];
+  for (int i = threadIdx.x; i < it.n; i += 1024) GP(it.v)[(long)i * it.ldv + i] -= t;
 }
 
 // ---- per-instance normalisation and change of a bond matrix (the end of a gauge sweep):
@@ -468,7 +494,9 @@ __global__ void __launch_bounds__(256) gnormdiff_kernel(const GN* __restrict__ i
   __shared__ double red[4];
   __shared__ double bc;
   double s = 0.0;
-  for (long i = threadIdx.x; i < it.n; i += 256) s += it.C[i] * it.C[i];
+  gdbl* C = GP(it.C);
+  const gdbl* Cpre = GPC(it.Cpre);
+  for (long i = threadIdx.x; i < it.n; i += 256) s += C[i] * C[i];
   s = bwave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -480,16 +508,16 @@ __global__ void __launch_bounds__(256) gnormdiff_kernel(const GN* __restrict__ i
   const double inv = bc;
   double e = 0.0;
   for (long i = threadIdx.x; i < it.n; i += 256) {
-    const double c = it.C[i] * inv;
-    it.C[i] = c;
-    const double df = c - it.Cpre[i];
+    const double c = C[i] * inv;
+    C[i] = c;
+    const double df = c - Cpre[i];
     e += df * df;
   }
   e = bwave_sum(e);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
   __syncthreads();
-  if (threadIdx.x == 0) *it.out = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) *GP(it.out) = red[0] + red[1] + red[2] + red[3];
 }
 
 // ---- batched Householder QRpos / LQpos, one workgroup of 1024 threads per matrix.
@@ -498,11 +526,14 @@ __global__ void __launch_bounds__(256) gnormdiff_kernel(const GN* __restrict__ i
 // LQ (trans = 1): the input is M (nn x mm, ldi) = L Q with X = M^T: Q (nn x mm) to qout, L (nn x
 // nn, lower, diagonal >= 0) to rout.
 // Reflectors as LAPACK's dlarfg (beta = -sign(x0) |x|, tau = (beta - x0) / beta, v0 = 1; tau = 0
-// when the tail is zero); Q by backward accumulation; then the sign of each R row (L column) is
-// made non-negative together with the matching Q column (row).  Scratch: W (mm x nn) and
-// Q (mm x nn) per item.
+// when the tail is zero), in panels of nb columns: a panel is factored column by column, its
+// reflectors go to LDS, and every trailing column is then loaded once into the registers of one
+// wavefront, receives the panel's nb reflectors there and is stored once (one pass over the
+// trailing matrix per panel instead of per column).  Q is accumulated backwards panel by panel in
+// the same way; finally each R row (L column) is made non-negative with its Q column (row).
+// Scratch: W (mm x nn) and Q (mm x nn) per item.
 struct GQ {
-  int mm, nn, trans;
+  int mm, nn, trans, want_q;
   const double* in;
   long ldi;
   double* qout;
@@ -515,6 +546,7 @@ struct GQ {
 
 constexpr int QR_THREADS = 1024;
 constexpr int QR_MAXM = 2048;
+constexpr int QR_LDS = 8192;  // doubles of panel reflectors (64 KiB)
 
 __device__ inline double qr_block_sum(double v, double* red) {
   v = bwave_sum(v);
@@ -528,94 +560,202 @@ __device__ inline double qr_block_sum(double v, double* red) {
   return t;
 }
 
+// Applies reflectors t = t0, t0 + dt, ... (count nt) of the panel starting at row k0 to the
+// columns c0 and c1 (c1 may be null; rows k0..mm-1), held in registers of one wavefront (RPL rows
+// per lane, RPL * 64 >= L).  Vs[t L + (i - k0)] = v_i of reflector t (zero above its own row),
+// taus[t] its tau.  Two columns at a time keep two reduction chains in flight.
+template <int RPL>
+__device__ inline void qr_apply_panel(gdbl* c0, gdbl* c1, int k0, int L, const double* Vs, const double* taus,
+                                      int t0, int nt, int dt) {
+  const int lane = threadIdx.x & 63;
+  double x[RPL], y[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int i = r * 64 + lane;
+    x[r] = (i < L) ? c0[k0 + i] : 0.0;
+    y[r] = (c1 && i < L) ? c1[k0 + i] : 0.0;
+  }
+  for (int c = 0, t = t0; c < nt; ++c, t += dt) {
+    const double tk = taus[t];
+    if (tk == 0.0) continue;
+    const double* v = Vs + (size_t)t * L;
+    double p = 0.0, q = 0.0;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int i = r * 64 + lane;
+      const double vi = (i < L) ? v[i] : 0.0;
+      p += vi * x[r];
+      q += vi * y[r];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      p += __shfl_xor(p, off, 64);
+      q += __shfl_xor(q, off, 64);
+    }
+    const double f = tk * p, g = tk * q;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int i = r * 64 + lane;
+      const double vi = (i < L) ? v[i] : 0.0;
+      x[r] -= f * vi;
+      y[r] -= g * vi;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int i = r * 64 + lane;
+    if (i < L) {
+      c0[k0 + i] = x[r];
+      if (c1) c1[k0 + i] = y[r];
+    }
+  }
+}
+
+template <int RPL>
 __global__ void __launch_bounds__(QR_THREADS) gqr_kernel(const GQ* __restrict__ items) {
   const GQ it = items[blockIdx.x];
   const int mm = it.mm, nn = it.nn;
-  double* W = it.W;
-  double* Q = it.Q;
-  __shared__ double v[QR_MAXM];
+  gdbl* W = GP(it.W);
+  gdbl* Q = GP(it.Q);
+  const gdbl* in = GPC(it.in);
+  gdbl* qout = GP(it.qout);
+  gdbl* rout = GP(it.rout);
+  __shared__ double Vs[QR_LDS];
   __shared__ double tau[QR_MAXM / 2];
   __shared__ double red[QR_THREADS / 64];
+  __shared__ double look[2];
+  __shared__ double rdiag[16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr int NW = QR_THREADS / 64;
+  const int nb = max(1, min(16, QR_LDS / mm));
   const long tot = (long)mm * nn;
   for (long e = tid; e < tot; e += QR_THREADS) {
     const long i = e % mm, j = e / mm;
-    W[e] = it.trans ? it.in[j + it.ldi * i] : it.in[i + it.ldi * j];
+    W[e] = it.trans ? in[j + it.ldi * i] : in[i + it.ldi * j];
   }
   __syncthreads();
-  for (int k = 0; k < nn; ++k) {
-    // reflector of column k (rows k..mm-1)
-    double s = 0.0;
-    for (int i = k + 1 + tid; i < mm; i += QR_THREADS) s += W[i + (long)mm * k] * W[i + (long)mm * k];
-    const double tail = qr_block_sum(s, red);
-    const double x0 = W[k + (long)mm * k];
-    double tk = 0.0, beta = x0, scal = 0.0;
-    if (tail > 0.0) {
-      const double nrm = sqrt(x0 * x0 + tail);
-      beta = x0 >= 0.0 ? -nrm : nrm;
-      tk = (beta - x0) / beta;
-      scal = 1.0 / (x0 - beta);
+  for (int k0 = 0; k0 < nn; k0 += nb) {
+    const int kb = min(nb, nn - k0);
+    const int L = mm - k0;
+    // the panel (rows k0..mm-1 of columns k0..k0+kb-1) is factored in LDS, column t at Vs + t L;
+    // the wave that updates column t + 1 also leaves its new tail norm and diagonal entry in LDS
+    for (int e = tid; e < kb * L; e += QR_THREADS) {
+      const int t = e / L, i = e - t * L;
+      Vs[e] = W[k0 + i + (long)mm * (k0 + t)];
     }
-    for (int i = k + tid; i < mm; i += QR_THREADS) v[i] = (i == k) ? 1.0 : W[i + (long)mm * k] * scal;
-    if (tid == 0) tau[k] = tk;
     __syncthreads();
-    if (tid == 0) W[k + (long)mm * k] = beta;
-    for (int i = k + 1 + tid; i < mm; i += QR_THREADS) W[i + (long)mm * k] = v[i];
-    // trailing columns: one wave per column
-    if (tk != 0.0) {
-      for (int j = k + 1 + wave; j < nn; j += NW) {
-        double* col = W + (long)mm * j;
-        double p = 0.0;
-        for (int i = k + lane; i < mm; i += 64) p += v[i] * col[i];
-        p = __shfl(bwave_sum(p), 0, 64);
-        const double f = tk * p;
-        for (int i = k + lane; i < mm; i += 64) col[i] -= f * v[i];
+    for (int t = 0; t < kb; ++t) {
+      const int k = k0 + t;
+      double* pc = Vs + (size_t)t * L;
+      double tail, x0;
+      if (t == 0) {
+        double s = 0.0;
+        for (int i = t + 1 + tid; i < L; i += QR_THREADS) s += pc[i] * pc[i];
+        tail = qr_block_sum(s, red);
+        x0 = pc[t];
+      } else {
+        tail = look[0];
+        x0 = look[1];
       }
+      double tk = 0.0, beta = x0, scal = 0.0;
+      if (tail > 0.0) {
+        const double nrm = sqrt(x0 * x0 + tail);
+        beta = x0 >= 0.0 ? -nrm : nrm;
+        tk = (beta - x0) / beta;
+        scal = 1.0 / (x0 - beta);
+      }
+      __syncthreads();  // every thread has read x0 before row t becomes the reflector's 1
+      for (int i = t + 1 + tid; i < L; i += QR_THREADS) pc[i] *= scal;
+      if (tid == 0) {
+        tau[k] = tk;
+        rdiag[t] = beta;
+        pc[t] = 1.0;
+      }
+      __syncthreads();
+      for (int j = t + 1 + wave; j < kb; j += NW) {
+        double* pj = Vs + (size_t)j * L;
+        if (tk != 0.0) {
+          double p = 0.0;
+          for (int i = t + lane; i < L; i += 64) p += pc[i] * pj[i];
+          p = __shfl(bwave_sum(p), 0, 64);
+          const double f = tk * p;
+          for (int i = t + lane; i < L; i += 64) pj[i] -= f * pc[i];
+        }
+        if (j == t + 1) {
+          double q = 0.0;
+          for (int i = t + 2 + lane; i < L; i += 64) q += pj[i] * pj[i];
+          q = bwave_sum(q);
+          if (lane == 0) {
+            look[0] = q;
+            look[1] = pj[t + 1];
+          }
+        }
+      }
+      __syncthreads();
     }
+    // back to W (R above the diagonal, beta on it, reflectors below); then the R part in LDS is
+    // cleared so that column t holds exactly reflector t for the trailing update
+    for (int e = tid; e < kb * L; e += QR_THREADS) {
+      const int t = e / L, i = e - t * L;
+      W[k0 + i + (long)mm * (k0 + t)] = (i == t) ? rdiag[t] : Vs[e];
+    }
+    __syncthreads();
+    for (int e = tid; e < kb * L; e += QR_THREADS) {
+      const int t = e / L, i = e - t * L;
+      if (i < t) Vs[e] = 0.0;
+    }
+    __syncthreads();
+    // trailing columns: one pass each, the panel's reflectors applied in registers
+    for (int j = k0 + kb + 2 * wave; j < nn; j += 2 * NW)
+      qr_apply_panel<RPL>(W + (long)mm * j, j + 1 < nn ? W + (long)mm * (j + 1) : nullptr, k0, L, Vs, tau + k0, 0,
+                          kb, 1);
     __syncthreads();
   }
-  // Q = H_0 ... H_{nn-1} [I; 0] by backward accumulation
-  for (long e = tid; e < tot; e += QR_THREADS) {
+  // Q = H_0 ... H_{nn-1} [I; 0], accumulated backwards panel by panel (skipped when only R is
+  // wanted)
+  const int np = it.want_q ? (nn + nb - 1) / nb : 0;
+  for (long e = tid; it.want_q && e < tot; e += QR_THREADS) {
     const long i = e % mm, j = e / mm;
     Q[e] = (i == j) ? 1.0 : 0.0;
   }
-  __syncthreads();
-  for (int k = nn - 1; k >= 0; --k) {
-    const double tk = tau[k];
-    if (tk != 0.0) {
-      for (int i = k + tid; i < mm; i += QR_THREADS) v[i] = (i == k) ? 1.0 : W[i + (long)mm * k];
-      __syncthreads();
-      for (int j = k + wave; j < nn; j += NW) {
-        double* col = Q + (long)mm * j;
-        double p = 0.0;
-        for (int i = k + lane; i < mm; i += 64) p += v[i] * col[i];
-        p = __shfl(bwave_sum(p), 0, 64);
-        const double f = tk * p;
-        for (int i = k + lane; i < mm; i += 64) col[i] -= f * v[i];
+  for (int pi = np - 1; pi >= 0; --pi) {
+    const int k0 = pi * nb;
+    const int kb = min(nb, nn - k0);
+    const int L = mm - k0;
+    __syncthreads();
+    for (int t = 0; t < kb; ++t) {
+      const int k = k0 + t;
+      double* vt = Vs + (size_t)t * L;
+      for (int i = tid; i < L; i += QR_THREADS) {
+        const int row = k0 + i;
+        vt[i] = (row < k) ? 0.0 : (row == k ? 1.0 : W[row + (long)mm * k]);
       }
     }
     __syncthreads();
+    for (int j = k0 + 2 * wave; j < nn; j += 2 * NW)
+      qr_apply_panel<RPL>(Q + (long)mm * j, j + 1 < nn ? Q + (long)mm * (j + 1) : nullptr, k0, L, Vs, tau + k0, kb - 1,
+                          kb, -1);
   }
+  __syncthreads();
   // signs of the diagonal of R, then the outputs
   for (int i = tid; i < nn; i += QR_THREADS) tau[i] = (W[i + (long)mm * i] < 0.0) ? -1.0 : 1.0;
   __syncthreads();
-  for (long e = tid; e < tot; e += QR_THREADS) {
+  for (long e = tid; it.want_q && e < tot; e += QR_THREADS) {
     const long i = e % mm, j = e / mm;
     const double q = Q[e] * tau[j];
     if (it.trans)
-      it.qout[j + it.ldq * i] = q;
+      qout[j + it.ldq * i] = q;
     else
-      it.qout[i + it.ldq * j] = q;
+      qout[i + it.ldq * j] = q;
   }
   const long rr = (long)nn * nn;
   for (long e = tid; e < rr; e += QR_THREADS) {
     const long i = e % nn, j = e / nn;
     const double r = (i <= j) ? W[i + (long)mm * j] * tau[i] : 0.0;
     if (it.trans)
-      it.rout[j + it.ldr * i] = r;
+      rout[j + it.ldr * i] = r;
     else
-      it.rout[i + it.ldr * j] = r;
+      rout[i + it.ldr * j] = r;
   }
 }
 
@@ -821,18 +961,86 @@ struct Phase {
 int red_blocks(long n) { return (int)std::min<long>(64, std::max<long>(1, (n + 2047) / 2048)); }
 int vec_blocks(long n) { return (int)std::min<long>(512, std::max<long>(1, (n + 1023) / 1024)); }
 
+// Host threads for the per-instance bookkeeping of a round (projected eigenproblems, Givens
+// rotations): run(n, f) calls f(0..n-1) spread over the pool and the calling thread.
+struct Pool {
+  std::vector<std::thread> th;
+  std::mutex mu;
+  std::condition_variable cv, cv_done;
+  const std::function<void(int)>* job = nullptr;
+  int n = 0, busy = 0;
+  long gen = 0;
+  bool stop = false;
+  std::atomic<int> next{0};
+  void start(int nt) {
+    for (int i = 0; i < nt; ++i) th.emplace_back([this] { loop(); });
+  }
+  void work() {
+    int k;
+    while ((k = next.fetch_add(1)) < n) (*job)(k);
+  }
+  void loop() {
+    long seen = 0;
+    while (true) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || gen != seen; });
+        if (stop) return;
+        seen = gen;
+      }
+      work();
+      std::lock_guard<std::mutex> lk(mu);
+      if (--busy == 0) cv_done.notify_all();
+    }
+  }
+  void run(int count, const std::function<void(int)>& f) {
+    if (count < 4 || th.empty()) {
+      for (int k = 0; k < count; ++k) f(k);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      job = &f;
+      n = count;
+      next = 0;
+      busy = (int)th.size();
+      gen++;
+    }
+    cv.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu);
+    cv_done.wait(lk, [&] { return busy == 0; });
+  }
+  void shutdown() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto& t : th) t.join();
+    th.clear();
+  }
+};
+
 struct Batch {
   int device = 0;
   hipStream_t s = nullptr;
+  Pool pool;
+  // kernel launches of the current round, issued by submit() after one upload of the round's
+  // descriptors (no host reads happen between them)
+  std::vector<std::function<void()>> pending;
   Arena ar;
   Scratch sc;
   double* hres = nullptr;  // pinned readback
   size_t hres_cap = 0;
   int w = 0, d = 0;
   long launches = 0, rounds = 0;
+  // wall seconds per phase: canonicalisation, local update (eigensolves + regauge!), gauge_step!,
+  // recalculate!, calc_galerkin, expectationvalue
+  double t_phase[6] = {0, 0, 0, 0, 0, 0};
   std::vector<Inst*> insts;
 
-  void init(int dev) {
+  void init(int dev, int pool_threads) {
     device = dev;
     BCHK(hipSetDevice(dev));
     BCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -841,8 +1049,10 @@ struct Batch {
     BCHK(hipMalloc((void**)&sc.d, sc.cap * sizeof(double)));
     hres_cap = (size_t)1 << 20;
     BCHK(hipHostMalloc((void**)&hres, hres_cap * sizeof(double), hipHostMallocDefault));
+    pool.start(pool_threads);
   }
   void release() {
+    pool.shutdown();
     if (s) (void)hipStreamSynchronize(s);
     for (Inst* in : insts) {
       if (in->mem) (void)hipFree(in->mem);
@@ -855,7 +1065,19 @@ struct Batch {
     if (s) (void)hipStreamDestroy(s);
     s = nullptr;
   }
+  void enqueue(std::function<void()> f) { pending.push_back(std::move(f)); }
+  // one descriptor upload, then the round's launches in order
+  void submit() {
+    ar.flush(s);
+    for (auto& f : pending) {
+      f();
+      BCHK(hipGetLastError());
+    }
+    launches += (long)pending.size();
+    pending.clear();
+  }
   void sync() {
+    submit();
     BCHK(hipStreamSynchronize(s));
     ar.reset();
     sc.off = 0;
@@ -871,14 +1093,13 @@ struct Batch {
       t += g.mtiles * ((g.N + 63) / 64);
     }
     const GG* dg = ar.put(gs.data(), gs.size());
-    ar.flush(s);
     if (t == 0) return;
+    const int ng = (int)gs.size();
+    hipStream_t st = s;
     if (prec == PREC_FP32)
-      hipLaunchKernelGGL(ggemm_kernel<float>, dim3(t), dim3(256), 0, s, dg, (int)gs.size());
+      enqueue([=] { hipLaunchKernelGGL(ggemm_kernel<float>, dim3(t), dim3(256), 0, st, dg, ng); });
     else
-      hipLaunchKernelGGL(ggemm_kernel<double>, dim3(t), dim3(256), 0, s, dg, (int)gs.size());
-    BCHK(hipGetLastError());
-    launches++;
+      enqueue([=] { hipLaunchKernelGGL(ggemm_kernel<double>, dim3(t), dim3(256), 0, st, dg, ng); });
   }
   void launch_vecs(std::vector<GV>& vs) {
     if (vs.empty()) return;
@@ -889,10 +1110,9 @@ struct Batch {
       t += v.nblk;
     }
     const GV* dv = ar.put(vs.data(), vs.size());
-    ar.flush(s);
-    hipLaunchKernelGGL(gvec_kernel, dim3(t), dim3(256), 0, s, dv, (int)vs.size());
-    BCHK(hipGetLastError());
-    launches++;
+    const int nv = (int)vs.size();
+    hipStream_t st = s;
+    enqueue([=] { hipLaunchKernelGGL(gvec_kernel, dim3(t), dim3(256), 0, st, dv, nv); });
   }
   // items at one position of several problems: GEMMs grouped per precision, vector ops together
   void run_items(std::vector<Item*>& items) {
@@ -1048,16 +1268,14 @@ void Batch::apply_ops(const std::vector<Phase*>& ph) {
       t += ((g.D + 255) / 256) * g.D;
     }
     const GW* dg = ar.put(wm.data(), wm.size());
-    ar.flush(s);
-    const int wd = w * d;
+    const int wd = w * d, ng = (int)wm.size(), ww = w, dd = d;
+    hipStream_t st = s;
     if (wd <= 16)
-      hipLaunchKernelGGL(gwmix_kernel<16>, dim3(t), dim3(256), 0, s, dg, (int)wm.size(), w, d);
+      enqueue([=] { hipLaunchKernelGGL(gwmix_kernel<16>, dim3(t), dim3(256), 0, st, dg, ng, ww, dd); });
     else if (wd <= 32)
-      hipLaunchKernelGGL(gwmix_kernel<32>, dim3(t), dim3(256), 0, s, dg, (int)wm.size(), w, d);
+      enqueue([=] { hipLaunchKernelGGL(gwmix_kernel<32>, dim3(t), dim3(256), 0, st, dg, ng, ww, dd); });
     else
-      hipLaunchKernelGGL(gwmix_kernel<64>, dim3(t), dim3(256), 0, s, dg, (int)wm.size(), w, d);
-    BCHK(hipGetLastError());
-    launches++;
+      enqueue([=] { hipLaunchKernelGGL(gwmix_kernel<64>, dim3(t), dim3(256), 0, st, dg, ng, ww, dd); });
   }
   if (!mx.empty()) {
     int t = 0;
@@ -1066,10 +1284,9 @@ void Batch::apply_ops(const std::vector<Phase*>& ph) {
       t += ((m.D1 + 255) / 256) * m.D2;
     }
     const GM* dm = ar.put(mx.data(), mx.size());
-    ar.flush(s);
-    hipLaunchKernelGGL(gmix_kernel, dim3(t), dim3(256), 0, s, dm, (int)mx.size(), d);
-    BCHK(hipGetLastError());
-    launches++;
+    const int ng = (int)mx.size(), dd = d;
+    hipStream_t st = s;
+    enqueue([=] { hipLaunchKernelGGL(gmix_kernel, dim3(t), dim3(256), 0, st, dm, ng, dd); });
   }
   launch_vecs(cp);
   {
@@ -1079,10 +1296,9 @@ void Batch::apply_ops(const std::vector<Phase*>& ph) {
   }
   if (!rg.empty()) {
     const GR* dr = ar.put(rg.data(), rg.size());
-    ar.flush(s);
-    hipLaunchKernelGGL(greg_kernel, dim3((unsigned)rg.size()), dim3(256), 0, s, dr);
-    BCHK(hipGetLastError());
-    launches++;
+    const unsigned ng = (unsigned)rg.size();
+    hipStream_t st = s;
+    enqueue([=] { hipLaunchKernelGGL(greg_kernel, dim3(ng), dim3(1024), 0, st, dr); });
   }
 }
 
@@ -1156,27 +1372,25 @@ void Batch::execute(const std::vector<Phase*>& ph) {
     int maxnv = 1;
     for (const GC& g : gcs) maxnv = std::max(maxnv, g.nv);
     const GC* dg = ar.put(gcs.data(), gcs.size());
-    ar.flush(s);
     const int ni = (int)gcs.size();
-    hipLaunchKernelGGL(gc_dots_kernel, dim3(t), dim3(256), 0, s, dg, ni);
-    hipLaunchKernelGGL(gc_reduce_kernel, dim3(ni, maxnv), dim3(64), 0, s, dg, 0, -1);
-    hipLaunchKernelGGL(gc_axpy_kernel<true>, dim3(t), dim3(256), 0, s, dg, ni, 0);
-    hipLaunchKernelGGL(gc_reduce_kernel, dim3(ni, maxnv), dim3(64), 0, s, dg, 64, -1);
-    hipLaunchKernelGGL(gc_axpy_kernel<false>, dim3(t), dim3(256), 0, s, dg, ni, 64);
-    hipLaunchKernelGGL(gc_reduce_kernel, dim3(ni, 1), dim3(64), 0, s, dg, 128, 1);
-    hipLaunchKernelGGL(gc_scale_kernel, dim3(t), dim3(256), 0, s, dg, ni);
-    BCHK(hipGetLastError());
-    launches += 7;
+    hipStream_t st = s;
+    enqueue([=] { hipLaunchKernelGGL(gc_dots_kernel, dim3(t), dim3(256), 0, st, dg, ni); });
+    enqueue([=] { hipLaunchKernelGGL(gc_reduce_kernel, dim3(ni, maxnv), dim3(64), 0, st, dg, 0, -1); });
+    enqueue([=] { hipLaunchKernelGGL(gc_axpy_kernel<true>, dim3(t), dim3(256), 0, st, dg, ni, 0); });
+    enqueue([=] { hipLaunchKernelGGL(gc_reduce_kernel, dim3(ni, maxnv), dim3(64), 0, st, dg, 64, -1); });
+    enqueue([=] { hipLaunchKernelGGL(gc_axpy_kernel<false>, dim3(t), dim3(256), 0, st, dg, ni, 64); });
+    enqueue([=] { hipLaunchKernelGGL(gc_reduce_kernel, dim3(ni, 1), dim3(64), 0, st, dg, 128, 1); });
+    enqueue([=] { hipLaunchKernelGGL(gc_scale_kernel, dim3(t), dim3(256), 0, st, dg, ni); });
   }
   if (!gds.empty()) {
     const int t = prefix(gds);
     const GC* dg = ar.put(gds.data(), gds.size());
-    ar.flush(s);
-    hipLaunchKernelGGL(gc_dots_kernel, dim3(t), dim3(256), 0, s, dg, (int)gds.size());
-    hipLaunchKernelGGL(gc_reduce_kernel, dim3((unsigned)gds.size(), 1), dim3(64), 0, s, dg, 0, 1);
-    BCHK(hipGetLastError());
-    launches += 2;
+    const int ni = (int)gds.size();
+    hipStream_t st = s;
+    enqueue([=] { hipLaunchKernelGGL(gc_dots_kernel, dim3(t), dim3(256), 0, st, dg, ni); });
+    enqueue([=] { hipLaunchKernelGGL(gc_reduce_kernel, dim3(ni, 1), dim3(64), 0, st, dg, 0, 1); });
   }
+  submit();
   if (nres) {
     brequire(nres <= hres_cap, IMPS_E_ARG, "batch: readback too large");
     BCHK(hipMemcpyAsync(hres, res, nres * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1224,7 +1438,7 @@ void run_problems(Batch& B, const std::vector<Prob*>& ps) {
       }
     if (act.empty()) break;
     B.execute(act);
-    for (size_t k = 0; k < pa.size(); ++k) pa[k]->consume(*act[k]);
+    B.pool.run((int)pa.size(), [&](int k) { pa[k]->consume(*act[k]); });
   }
 }
 
@@ -1609,6 +1823,7 @@ double b_updatetol(double factor, int it, double eps, double tmin = kBTolMin, do
 // ---- batched QRpos / LQpos
 struct QRJob {
   int mm, nn, trans;
+  bool want_q;
   const double* in;
   long ldi;
   double* q;
@@ -1629,6 +1844,7 @@ void run_qr(Batch& B, const std::vector<QRJob>& jobs) {
     g.mm = j.mm;
     g.nn = j.nn;
     g.trans = j.trans;
+    g.want_q = j.want_q ? 1 : 0;
     g.in = j.in;
     g.ldi = j.ldi;
     g.qout = j.q;
@@ -1640,10 +1856,16 @@ void run_qr(Batch& B, const std::vector<QRJob>& jobs) {
     items.push_back(g);
   }
   const GQ* dq = B.ar.put(items.data(), items.size());
-  B.ar.flush(B.s);
-  hipLaunchKernelGGL(gqr_kernel, dim3((unsigned)items.size()), dim3(QR_THREADS), 0, B.s, dq);
-  BCHK(hipGetLastError());
-  B.launches++;
+  const unsigned ng = (unsigned)items.size();
+  int mmax = 0;
+  for (const GQ& g : items) mmax = std::max(mmax, g.mm);
+  hipStream_t st = B.s;
+  if (mmax <= 512)
+    B.enqueue([=] { hipLaunchKernelGGL(gqr_kernel<8>, dim3(ng), dim3(QR_THREADS), 0, st, dq); });
+  else if (mmax <= 1024)
+    B.enqueue([=] { hipLaunchKernelGGL(gqr_kernel<16>, dim3(ng), dim3(QR_THREADS), 0, st, dq); });
+  else
+    B.enqueue([=] { hipLaunchKernelGGL(gqr_kernel<32>, dim3(ng), dim3(QR_THREADS), 0, st, dq); });
 }
 
 // ---- uniform_leftorth! / uniform_rightorth! (ortho.jl:109-202) for one-site cells, lockstep:
@@ -1715,7 +1937,7 @@ std::vector<int> gauge_sweeps(Batch& B, const std::vector<Inst*>& in, bool left,
         Inst& I = *in[ak[j]];
         brequire(ap[j].status == IMPS_OK, ap[j].status, "batch: gauge fixed point failed");
         // left: C = R of QRpos(ev); right: C = L of LQpos(ev)
-        qj.push_back(QRJob{I.D, I.D, left ? 0 : 1, I.ev, I.D, I.Qdump, I.D, I.C, I.D, &I, 0});
+        qj.push_back(QRJob{I.D, I.D, left ? 0 : 1, false, I.ev, I.D, I.Qdump, I.D, I.C, I.D, &I, 0});
       }
       run_qr(B, qj);
     }
@@ -1734,11 +1956,11 @@ std::vector<int> gauge_sweeps(Batch& B, const std::vector<Inst*>& in, bool left,
       if (left) {
         // M = C A (D x dD) into AL, QR in place: AL = Q, C = R
         g0[j] = gemm_item(D, d * D, D, 1.0, I.C, 1, D, A[k], 1, D, 0.0, I.Mg, D);
-        qj.push_back(QRJob{d * D, D, 0, I.Mg, (long)d * D, I.AL, (long)d * D, I.C, D, &I, 0});
+        qj.push_back(QRJob{d * D, D, 0, true, I.Mg, (long)d * D, I.AL, (long)d * D, I.C, D, &I, 0});
       } else {
         // M = A C (dD x D as D x dD), LQ: C = L, AR = Q
         g0[j] = gemm_item(d * D, D, D, 1.0, A[k], 1, (long)d * D, I.C, 1, D, 0.0, I.Mg, (long)d * D);
-        qj.push_back(QRJob{d * D, D, 1, I.Mg, D, I.AR, D, I.C, D, &I, 0});
+        qj.push_back(QRJob{d * D, D, 1, true, I.Mg, D, I.AR, D, I.C, D, &I, 0});
       }
       s1.push_back(&g0[j]);
       GN g{};
@@ -1752,11 +1974,11 @@ std::vector<int> gauge_sweeps(Batch& B, const std::vector<Inst*>& in, bool left,
     B.run_items(s1);
     run_qr(B, qj);
     const GN* dn = B.ar.put(items.data(), items.size());
-    B.ar.flush(B.s);
-    hipLaunchKernelGGL(gnormdiff_kernel, dim3((unsigned)items.size()), dim3(256), 0, B.s, dn);
-    BCHK(hipGetLastError());
+    const unsigned ng = (unsigned)items.size();
+    hipStream_t st = B.s;
+    B.enqueue([=] { hipLaunchKernelGGL(gnormdiff_kernel, dim3(ng), dim3(256), 0, st, dn); });
+    B.submit();
     BCHK(hipMemcpyAsync(B.hres, e2, P * sizeof(double), hipMemcpyDeviceToHost, B.s));
-    B.launches += 2;
     B.rounds++;
     B.sync();
     for (size_t k : live) {
@@ -1839,10 +2061,9 @@ void slice_sums(Batch& B, const std::vector<Inst*>& in, bool left, const std::ve
         tb += ((m.D1 + 255) / 256) * m.D2;
       }
       const GM* dm = B.ar.put(mx.data(), mx.size());
-      B.ar.flush(B.s);
-      hipLaunchKernelGGL(gmix_kernel, dim3(tb), dim3(256), 0, B.s, dm, (int)mx.size(), B.d);
-      BCHK(hipGetLastError());
-      B.launches++;
+      const int ng = (int)mx.size(), dd = B.d;
+      hipStream_t st = B.s;
+      B.enqueue([=] { hipLaunchKernelGGL(gmix_kernel, dim3(tb), dim3(256), 0, st, dm, ng, dd); });
     }
   }
   std::vector<Item> gs;
@@ -1975,10 +2196,9 @@ void envs_batch(Batch& B, const std::vector<Inst*>& in, const std::vector<double
     B.run_items(bp);
     if (!rg.empty()) {
       const GR* dr = B.ar.put(rg.data(), rg.size());
-      B.ar.flush(B.s);
-      hipLaunchKernelGGL(greg_kernel, dim3((unsigned)rg.size()), dim3(256), 0, B.s, dr);
-      BCHK(hipGetLastError());
-      B.launches++;
+      const unsigned ng = (unsigned)rg.size();
+      hipStream_t st = B.s;
+      B.enqueue([=] { hipLaunchKernelGGL(greg_kernel, dim3(ng), dim3(1024), 0, st, dr); });
     }
   };
   auto level_empty = [&](const Inst& I, int i) {
@@ -2150,8 +2370,8 @@ void regauge_batch(Batch& B, const std::vector<Inst*>& in) {
   std::vector<QRJob> qj;
   for (Inst* I : in) {
     const int D = I->D, d = I->d;
-    qj.push_back(QRJob{d * D, D, 0, I->pAC, (long)d * D, I->Qa, (long)d * D, I->Rq, D, I, 0});
-    qj.push_back(QRJob{D, D, 0, I->pC, D, I->Qc, D, I->Rq2, D, I, 1});
+    qj.push_back(QRJob{d * D, D, 0, true, I->pAC, (long)d * D, I->Qa, (long)d * D, I->Rq, D, I, 0});
+    qj.push_back(QRJob{D, D, 0, true, I->pC, D, I->Qc, D, I->Rq2, D, I, 1});
   }
   run_qr(B, qj);
   std::vector<Item> g;
@@ -2213,6 +2433,13 @@ void vumps_batch(Batch& B, std::vector<Inst*>& all, const imps_vumps_params* p, 
   const BTols st{p};
   auto t0 = std::chrono::steady_clock::now();
   auto secs = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  double tmark = 0.0;
+  auto lap = [&](int ph) {
+    B.sync();
+    const double t = secs();
+    B.t_phase[ph] += t - tmark;
+    tmark = t;
+  };
   // CanonicalIMPS(As) (canonicalmps.jl:151-170): gaugefix!(:LR) of the raw tensors (staged in pAC)
   {
     std::vector<Item> id;
@@ -2235,6 +2462,7 @@ void vumps_batch(Batch& B, std::vector<Inst*>& all, const imps_vumps_params* p, 
     }
     gauge_sweeps(B, all, false, A, C0, tl, p->gauge_maxiter);
     mul_al_c(B, all);
+    lap(0);
   }
   const long hac0 = 0;
   (void)hac0;
@@ -2246,6 +2474,7 @@ void vumps_batch(Batch& B, std::vector<Inst*>& all, const imps_vumps_params* p, 
     std::vector<double> te;
     for (Inst* I : all) te.push_back(st.envs(0, I->eps));
     envs_batch(B, all, te, p->envs_maxiter, p->envs_krylovdim);
+    lap(3);
   }
   for (Inst* I : all) {
     I->prec = p->prec;
@@ -2308,15 +2537,20 @@ void vumps_batch(Batch& B, std::vector<Inst*>& all, const imps_vumps_params* p, 
       act[k]->lam_c = lc[k].lambda;
     }
     regauge_batch(B, act);
+    lap(1);
     std::vector<double> tg, te;
     for (Inst* I : act) {
       tg.push_back(st.gauge(siter, I->eps));
       te.push_back(st.envs(siter, I->eps));
     }
     gauge_step_batch(B, act, tg, p->gauge_maxiter);
+    lap(2);
     envs_batch(B, act, te, p->envs_maxiter, p->envs_krylovdim);
+    lap(3);
     galerkin_batch(B, act);
+    lap(4);
     energy_batch(B, act);
+    lap(5);
     for (Inst* I : act) {
       I->niter = it;
       I->stalled = I->eps > 0.9 * I->best;
@@ -2342,7 +2576,6 @@ extern "C" {
 int imps_vumps_batch_device(int device, int ninst, imps_batch_instance* inst, const imps_vumps_params* p,
                             double gauge_tol, imps_batch_stats* stats) {
   if (ninst < 0 || (ninst > 0 && !inst) || !p) return IMPS_E_ARG;
-  Batch B;
   int rc = IMPS_OK;
   std::string msg;
   auto t0 = std::chrono::steady_clock::now();
@@ -2363,52 +2596,101 @@ int imps_vumps_batch_device(int device, int ninst, imps_batch_instance* inst, co
                "imps_vumps_batch_device: every instance needs the same MPO level count and physical dimension");
       brequire(in.D >= 16 && in.D % 16 == 0 && in.D <= QR_MAXM / 2 && (long)d * in.D <= QR_MAXM, IMPS_E_UNSUPPORTED,
                "imps_vumps_batch_device: bond dimensions must be multiples of 16 with d D <= 2048");
-    }
-    brequire(w >= 2 && w * d <= 64 && d <= 8, IMPS_E_UNSUPPORTED, "imps_vumps_batch_device: w d <= 64, d <= 8");
-    B.init(device);
-    B.w = w;
-    B.d = d;
-    for (int k = 0; k < ninst; ++k) {
-      Inst* I = new Inst();
-      B.insts.push_back(I);
-      I->idx = k;
-      I->D = inst[k].D;
-      I->d = d;
-      I->w = w;
-      I->Wh.assign(inst[k].W[0], inst[k].W[0] + (size_t)w * d * w * d);
+      const std::vector<double> Wh(in.W[0], in.W[0] + (size_t)w * d * w * d);
       for (int u = 0; u < d; ++u)
-        for (int s = 0; s < d; ++s) {
-          const double e = u == s ? 1.0 : 0.0;
-          brequire(I->Wh[(size_t)w * (u + (size_t)d * ((size_t)w * s))] == e &&
-                       I->Wh[(w - 1) + (size_t)w * (u + (size_t)d * ((w - 1) + (size_t)w * s))] == e,
+        for (int s2 = 0; s2 < d; ++s2) {
+          const double e = u == s2 ? 1.0 : 0.0;
+          brequire(Wh[(size_t)w * (u + (size_t)d * ((size_t)w * s2))] == e &&
+                       Wh[(w - 1) + (size_t)w * (u + (size_t)d * ((w - 1) + (size_t)w * s2))] == e,
                    IMPS_E_ARG, "imps_vumps_batch_device: levels 0 and w-1 must be unit levels");
         }
-      alloc_inst(*I, p->krylovdim, p->envs_krylovdim);
-      const int wd = w * d;
-      std::vector<double> Wm((size_t)wd * wd);
-      for (int i = 0; i < w; ++i)
-        for (int u = 0; u < d; ++u)
-          for (int j = 0; j < w; ++j)
-            for (int s = 0; s < d; ++s)
-              Wm[(u + (size_t)d * j) + (size_t)wd * (i + (size_t)w * s)] =
-                  I->Wh[i + (size_t)w * (u + (size_t)d * (j + (size_t)w * s))];
-      BCHK(hipMemcpy(I->Wm, Wm.data(), Wm.size() * sizeof(double), hipMemcpyHostToDevice));
-      BCHK(hipMemcpy(I->pAC, inst[k].A[0], I->nA * sizeof(double), hipMemcpyHostToDevice));
     }
-    vumps_batch(B, B.insts, p, gauge_tol);
-    for (int k = 0; k < ninst; ++k) {
-      const Inst& I = *B.insts[k];
-      inst[k].status = IMPS_OK;
-      inst[k].niter = I.niter;
-      inst[k].converged = I.converged ? 1 : 0;
-      inst[k].itererr = I.eps;
-      inst[k].energy = I.f;
-      inst[k].seconds = I.seconds;
-      inst[k].hac_applies = (int)I.hac_applies;
-    }
+    brequire(w >= 2 && w * d <= 64 && d <= 8, IMPS_E_UNSUPPORTED, "imps_vumps_batch_device: w d <= 64, d <= 8");
+    // The instances are dealt, largest D^3 first, to a few lockstep groups that run concurrently
+    // (one host thread and stream each): a group's host-side work and its slowest instances overlap
+    // with the other groups' kernels.
+    std::vector<int> order(ninst);
+    for (int k = 0; k < ninst; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return (double)inst[a].D * inst[a].D * inst[a].D > (double)inst[b].D * inst[b].D * inst[b].D;
+    });
+    const int G = ninst >= 16 ? 4 : (ninst >= 4 ? 2 : 1);
+    std::vector<std::vector<int>> members(G);
+    for (int k = 0; k < ninst; ++k) members[k % G].push_back(order[k]);
+    std::vector<int> grc(G, IMPS_OK);
+    std::vector<std::string> gmsg(G);
+    std::vector<long> grounds(G, 0), glaunch(G, 0);
+    std::vector<std::vector<double>> gphase(G, std::vector<double>(6, 0.0));
+    auto run_group = [&](int g) {
+      Batch B;
+      try {
+        B.init(device, std::max(1, 8 / G - 1));
+        B.w = w;
+        B.d = d;
+        for (int k : members[g]) {
+          Inst* I = new Inst();
+          B.insts.push_back(I);
+          I->idx = k;
+          I->D = inst[k].D;
+          I->d = d;
+          I->w = w;
+          I->Wh.assign(inst[k].W[0], inst[k].W[0] + (size_t)w * d * w * d);
+          alloc_inst(*I, p->krylovdim, p->envs_krylovdim);
+          const int wd = w * d;
+          std::vector<double> Wm((size_t)wd * wd);
+          for (int i = 0; i < w; ++i)
+            for (int u = 0; u < d; ++u)
+              for (int j = 0; j < w; ++j)
+                for (int s2 = 0; s2 < d; ++s2)
+                  Wm[(u + (size_t)d * j) + (size_t)wd * (i + (size_t)w * s2)] =
+                      I->Wh[i + (size_t)w * (u + (size_t)d * (j + (size_t)w * s2))];
+          BCHK(hipMemcpyAsync(I->Wm, Wm.data(), Wm.size() * sizeof(double), hipMemcpyHostToDevice, B.s));
+          BCHK(hipMemcpyAsync(I->pAC, inst[k].A[0], I->nA * sizeof(double), hipMemcpyHostToDevice, B.s));
+          BCHK(hipStreamSynchronize(B.s));
+        }
+        vumps_batch(B, B.insts, p, gauge_tol);
+        for (const Inst* I : B.insts) {
+          imps_batch_instance& o = inst[I->idx];
+          o.status = IMPS_OK;
+          o.niter = I->niter;
+          o.converged = I->converged ? 1 : 0;
+          o.itererr = I->eps;
+          o.energy = I->f;
+          o.seconds = I->seconds;
+          o.hac_applies = (int)I->hac_applies;
+        }
+      } catch (const BatchError& e) {
+        grc[g] = e.code;
+        gmsg[g] = e.what();
+      } catch (const std::exception& e) {
+        grc[g] = IMPS_E_ARG;
+        gmsg[g] = e.what();
+      }
+      grounds[g] = B.rounds;
+      glaunch[g] = B.launches;
+      for (int k = 0; k < 6; ++k) gphase[g][k] = B.t_phase[k];
+      B.release();
+    };
+    std::vector<std::thread> th;
+    for (int g = 1; g < G; ++g) th.emplace_back(run_group, g);
+    run_group(0);
+    for (auto& t : th) t.join();
+    for (int g = 0; g < G; ++g)
+      if (grc[g] != IMPS_OK) {
+        rc = grc[g];
+        msg = gmsg[g];
+        for (int k : members[g]) inst[k].status = rc;
+      }
     if (stats) {
-      stats->rounds = B.rounds;
-      stats->launches = B.launches;
+      stats->rounds = 0;
+      stats->launches = 0;
+      for (int k = 0; k < 6; ++k) stats->t_phase[k] = 0.0;
+      for (int g = 0; g < G; ++g) {
+        stats->rounds += grounds[g];
+        stats->launches += glaunch[g];
+        for (int k = 0; k < 6; ++k) stats->t_phase[k] = std::max(stats->t_phase[k], gphase[g][k]);
+      }
+      stats->groups = G;
       stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
   } catch (const BatchError& e) {
@@ -2418,10 +2700,10 @@ int imps_vumps_batch_device(int device, int ninst, imps_batch_instance* inst, co
     rc = IMPS_E_ARG;
     msg = e.what();
   }
-  B.release();
   if (rc != IMPS_OK) {
     set_global_error(msg);
-    for (int k = 0; k < ninst; ++k) inst[k].status = rc;
+    for (int k = 0; k < ninst; ++k)
+      if (inst[k].status == IMPS_OK) inst[k].status = rc;
   }
   return rc;
 }
