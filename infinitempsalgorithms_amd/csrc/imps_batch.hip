@@ -759,6 +759,335 @@ __global__ void __launch_bounds__(QR_THREADS) gqr_kernel(const GQ* __restrict__ 
   }
 }
 
+// ---- batched Householder QRpos / LQpos with compact-WY trailing updates on FP64 MFMA
+// (mm <= 512).  The matrix is kept row-major in the W scratch (ld nn), Q row-major in the Q
+// scratch.  Per panel of 16 columns: the panel is factored in LDS exactly as in gqr_kernel, T of
+// the compact form H_k0 ... H_k0+15 = I - V T V^T follows from G = V^T V (one MFMA product) and
+// dlarft's recurrence, and every block of 16 trailing columns X gets X -= V (T^T (V^T X)) as two
+// MFMA products (Q accumulation: Q -= V (T (V^T Q))).  Same reflectors and same results as
+// gqr_kernel up to rounding.
+constexpr int WY_NB = 16;
+constexpr int WY_MAXM = 512;
+constexpr int WY_LDV = WY_MAXM + 1;
+
+typedef double wv4 __attribute__((ext_vector_type(4)));
+
+__device__ inline wv4 wy_mma(double a, double b, wv4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// T (16 x 16, Ts[s * 16 + t]) of the panel's kb reflectors Vs (rows 0..L-1, reflector t at
+// Vs[t * WY_LDV + r], columns kb..15 zero) and taus; called by every thread.  G = V^T V is summed
+// from 16 per-wave MFMA partials (rows split over the waves, partials in Pw), then 16 lanes run
+// dlarft's recurrence T[s][t] = -tau_t sum_{s <= q < t} T[s][q] G[q][t] with G's column t loaded
+// ahead of each step.
+__device__ inline void wy_make_t(const double* Vs, int L, int kb, const double* taus, double* Gs, double* Ts,
+                                 double (*Pw)[256]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NW = QR_THREADS / 64;
+  {
+    wv4 acc = {0, 0, 0, 0};
+    const int per = ((L + 4 * NW - 1) / (4 * NW)) * 4;  // rows per wave, a multiple of 4
+    const int r0 = wave * per, r1 = min(L, r0 + per);
+    for (int k4 = r0; k4 < r1; k4 += 4) {
+      const int r = k4 + (lane >> 4);
+      const double v = (r < r1) ? Vs[(lane & 15) * WY_LDV + r] : 0.0;
+      acc = wy_mma(v, v, acc);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Pw[wave][((lane >> 4) + 4 * q) * 16 + (lane & 15)] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    double g = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) g += Pw[w][threadIdx.x];
+    Gs[threadIdx.x] = g;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int s = threadIdx.x;
+    double tr[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      double gc[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) gc[q] = Gs[q * 16 + t];
+      double sum = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q < t) sum += (q >= s ? tr[q] : 0.0) * gc[q];
+      const double tt = (t < kb) ? taus[t] : 0.0;
+      tr[t] = (s < t) ? -tt * sum : (s == t ? tt : 0.0);
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) Ts[s * 16 + t] = tr[t];
+  }
+  __syncthreads();
+}
+
+// X[rows k0 + 0..L-1][columns c0..c0+15] -= V (T' (V^T X)) with T' = T^T (transT) or T, X row-major
+// (ld nn, nn columns); one wavefront, Xw/Uw its 16 x 16 exchange buffers.  Rows go in chunks of 64:
+// lane (lc, lg) holds rows base + 4 m + lg (m < 16) of column c0 + lc, which is both the MFMA B
+// operand layout of V^T X and the accumulator layout of V U, so a chunk is loaded once per phase
+// with its 16 loads in flight (the next chunk's loads are issued before this chunk's products).
+__device__ inline void wy_load_chunk(const gdbl* X, int nn, int k0, int L, int base, int col, bool cok, int lg,
+                                     double (&xb)[16]) {
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int r = base + 4 * m + lg;
+    xb[m] = (cok && r < L) ? X[(long)(k0 + r) * nn + col] : 0.0;
+  }
+}
+
+__device__ inline void wy_apply_block(gdbl* X, int nn, int mm, int k0, int L, int c0, const double* Vs,
+                                      const double* Ts, bool transT, double* Xw, double* Uw) {
+  const int lane = threadIdx.x & 63;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int col = c0 + lc;
+  const bool cok = col < nn;
+  (void)mm;
+  // Wt = V^T X (16 x 16)
+  wv4 acc = {0, 0, 0, 0};
+  double xb[16], xn[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) xn[m] = 0.0;
+  wy_load_chunk(X, nn, k0, L, 0, col, cok, lg, xb);
+  for (int base = 0; base < L; base += 64) {
+    if (base + 64 < L) wy_load_chunk(X, nn, k0, L, base + 64, col, cok, lg, xn);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int r = base + 4 * m + lg;
+      const double v = (r < L) ? Vs[lc * WY_LDV + r] : 0.0;
+      acc = wy_mma(v, xb[m], acc);
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) xb[m] = xn[m];
+  }
+  // lane holds Wt[t = lg + 4 q][c = lc]
+#pragma unroll
+  for (int q = 0; q < 4; ++q) Xw[(lg + 4 * q) * 16 + lc] = acc[q];
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the exchange is written
+  __builtin_amdgcn_wave_barrier();
+  // U = T' Wt: U[t][c] = sum_s T'[t][s] Wt[s][c]
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int t = lg + 4 * q;
+    double u = 0.0;
+#pragma unroll
+    for (int s2 = 0; s2 < 16; ++s2) u += (transT ? Ts[s2 * 16 + t] : Ts[t * 16 + s2]) * Xw[s2 * 16 + lc];
+    Uw[t * 16 + lc] = u;
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+  // B operand of V U for K step kk: U[4 kk + lg][lc]
+  double bu[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) bu[kk] = Uw[(4 * kk + lg) * 16 + lc];
+  // X -= V U, chunks of 64 rows (four row tiles of 16)
+  wy_load_chunk(X, nn, k0, L, 0, col, cok, lg, xb);
+  for (int base = 0; base < L; base += 64) {
+    if (base + 64 < L) wy_load_chunk(X, nn, k0, L, base + 64, col, cok, lg, xn);
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) {
+      const int i0 = base + 16 * jt;
+      wv4 d2 = {0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int r = i0 + lc;
+        const double a = (r < L) ? Vs[(4 * kk + lg) * WY_LDV + r] : 0.0;
+        d2 = wy_mma(a, bu[kk], d2);
+      }
+      // d2 reg q: row i0 + lg + 4 q = base + 4 (4 jt + q) + lg, column lc -> xb[4 jt + q]
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xb[4 * jt + q] -= d2[q];
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int r = base + 4 * m + lg;
+      if (cok && r < L) X[(long)(k0 + r) * nn + col] = xb[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) xb[m] = xn[m];
+  }
+}
+
+__global__ void __launch_bounds__(QR_THREADS) gqr_wy_kernel(const GQ* __restrict__ items) {
+  const GQ it = items[blockIdx.x];
+  const int mm = it.mm, nn = it.nn;
+  gdbl* W = GP(it.W);
+  gdbl* Q = GP(it.Q);
+  const gdbl* in = GPC(it.in);
+  gdbl* qout = GP(it.qout);
+  gdbl* rout = GP(it.rout);
+  __shared__ double Vs[WY_NB * WY_LDV];
+  __shared__ double Xs[QR_THREADS / 64][256];
+  __shared__ double Us[QR_THREADS / 64][256];
+  __shared__ double Ts[256];
+  __shared__ double Gs[256];
+  __shared__ double tau[WY_MAXM];
+  __shared__ double red[QR_THREADS / 64];
+  __shared__ double look[2];
+  __shared__ double rdiag[WY_NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = QR_THREADS / 64;
+  const long tot = (long)mm * nn;
+  // W (row-major) = X: the LQ input M = X^T is read along its columns; the QR input is staged
+  // through LDS tiles of 64 x 64 so that both sides stay coalesced
+  if (it.trans) {
+    for (long e = tid; e < tot; e += QR_THREADS) {
+      const long i = e / nn, j = e - i * nn;
+      W[e] = in[j + it.ldi * i];
+    }
+  } else {
+    double* tile = Vs;  // 64 x 65
+    for (int i0 = 0; i0 < mm; i0 += 64)
+      for (int j0 = 0; j0 < nn; j0 += 64) {
+        for (int e = tid; e < 4096; e += QR_THREADS) {
+          const int i = e & 63, j = e >> 6;
+          tile[i * 65 + j] = (i0 + i < mm && j0 + j < nn) ? in[(i0 + i) + it.ldi * (long)(j0 + j)] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < 4096; e += QR_THREADS) {
+          const int j = e & 63, i = e >> 6;
+          if (i0 + i < mm && j0 + j < nn) W[(long)(i0 + i) * nn + j0 + j] = tile[i * 65 + j];
+        }
+        __syncthreads();
+      }
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < nn; k0 += WY_NB) {
+    const int kb = min(WY_NB, nn - k0);
+    const int L = mm - k0;
+    for (int e = tid; e < WY_NB * L; e += QR_THREADS) {
+      const int r = e / WY_NB, t = e - r * WY_NB;
+      Vs[t * WY_LDV + r] = (t < kb) ? W[(long)(k0 + r) * nn + k0 + t] : 0.0;
+    }
+    __syncthreads();
+    for (int t = 0; t < kb; ++t) {
+      const int k = k0 + t;
+      double* pc = Vs + t * WY_LDV;
+      double tail, x0;
+      if (t == 0) {
+        double s = 0.0;
+        for (int i = t + 1 + tid; i < L; i += QR_THREADS) s += pc[i] * pc[i];
+        tail = qr_block_sum(s, red);
+        x0 = pc[t];
+      } else {
+        tail = look[0];
+        x0 = look[1];
+      }
+      double tk = 0.0, beta = x0, scal = 0.0;
+      if (tail > 0.0) {
+        const double nrm = sqrt(x0 * x0 + tail);
+        beta = x0 >= 0.0 ? -nrm : nrm;
+        tk = (beta - x0) / beta;
+        scal = 1.0 / (x0 - beta);
+      }
+      __syncthreads();
+      for (int i = t + 1 + tid; i < L; i += QR_THREADS) pc[i] *= scal;
+      if (tid == 0) {
+        tau[k] = tk;
+        rdiag[t] = beta;
+        pc[t] = 1.0;
+      }
+      __syncthreads();
+      for (int j = t + 1 + wave; j < kb; j += NW) {
+        double* pj = Vs + j * WY_LDV;
+        if (tk != 0.0) {
+          double p = 0.0;
+          for (int i = t + lane; i < L; i += 64) p += pc[i] * pj[i];
+          p = __shfl(bwave_sum(p), 0, 64);
+          const double f = tk * p;
+          for (int i = t + lane; i < L; i += 64) pj[i] -= f * pc[i];
+        }
+        if (j == t + 1) {
+          double q = 0.0;
+          for (int i = t + 2 + lane; i < L; i += 64) q += pj[i] * pj[i];
+          q = bwave_sum(q);
+          if (lane == 0) {
+            look[0] = q;
+            look[1] = pj[t + 1];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // back to W; the R part of the panel in LDS is cleared (reflector t is zero above row t)
+    for (int e = tid; e < kb * L; e += QR_THREADS) {
+      const int r = e / kb, t = e - r * kb;
+      W[(long)(k0 + r) * nn + k0 + t] = (r == t) ? rdiag[t] : Vs[t * WY_LDV + r];
+    }
+    __syncthreads();
+    for (int e = tid; e < kb * L; e += QR_THREADS) {
+      const int r = e / kb, t = e - r * kb;
+      if (r < t) Vs[t * WY_LDV + r] = 0.0;
+    }
+    __syncthreads();
+    if (k0 + kb < nn) {
+      wy_make_t(Vs, L, kb, tau + k0, Gs, Ts, Xs);
+      for (int c0 = k0 + kb + WY_NB * wave; c0 < nn; c0 += WY_NB * NW)
+        wy_apply_block(W, nn, mm, k0, L, c0, Vs, Ts, true, Xs[wave], Us[wave]);
+    }
+    __syncthreads();
+  }
+  // Q = H_0 ... H_{nn-1} [I; 0], panels backwards: Q -= V (T (V^T Q)) on columns >= k0
+  if (it.want_q) {
+    for (long e = tid; e < tot; e += QR_THREADS) {
+      const long i = e / nn, j = e - i * nn;
+      Q[e] = (i == j) ? 1.0 : 0.0;
+    }
+    const int np = (nn + WY_NB - 1) / WY_NB;
+    for (int pi = np - 1; pi >= 0; --pi) {
+      const int k0 = pi * WY_NB;
+      const int kb = min(WY_NB, nn - k0);
+      const int L = mm - k0;
+      __syncthreads();
+      for (int e = tid; e < WY_NB * L; e += QR_THREADS) {
+        const int r = e / WY_NB, t = e - r * WY_NB;
+        Vs[t * WY_LDV + r] = (t >= kb || r < t) ? 0.0 : (r == t ? 1.0 : W[(long)(k0 + r) * nn + k0 + t]);
+      }
+      __syncthreads();
+      wy_make_t(Vs, L, kb, tau + k0, Gs, Ts, Xs);
+      for (int c0 = k0 + WY_NB * wave; c0 < nn; c0 += WY_NB * NW)
+        wy_apply_block(Q, nn, mm, k0, L, c0, Vs, Ts, false, Xs[wave], Us[wave]);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nn; i += QR_THREADS) tau[i] = (W[(long)i * nn + i] < 0.0) ? -1.0 : 1.0;
+  __syncthreads();
+  if (it.want_q) {
+    if (it.trans) {
+      for (long e = tid; e < tot; e += QR_THREADS) {
+        const long i = e / nn, j = e - i * nn;
+        qout[j + it.ldq * i] = Q[e] * tau[j];
+      }
+    } else {
+      double* tile = Vs;
+      for (int i0 = 0; i0 < mm; i0 += 64)
+        for (int j0 = 0; j0 < nn; j0 += 64) {
+          for (int e = tid; e < 4096; e += QR_THREADS) {
+            const int j = e & 63, i = e >> 6;
+            tile[i * 65 + j] = (i0 + i < mm && j0 + j < nn) ? Q[(long)(i0 + i) * nn + j0 + j] * tau[j0 + j] : 0.0;
+          }
+          __syncthreads();
+          for (int e = tid; e < 4096; e += QR_THREADS) {
+            const int i = e & 63, j = e >> 6;
+            if (i0 + i < mm && j0 + j < nn) qout[(i0 + i) + it.ldq * (long)(j0 + j)] = tile[i * 65 + j];
+          }
+          __syncthreads();
+        }
+    }
+  }
+  const long rr = (long)nn * nn;
+  for (long e = tid; e < rr; e += QR_THREADS) {
+    const long i = e / nn, j = e - i * nn;
+    const double r = (i <= j) ? W[i * nn + j] * tau[i] : 0.0;
+    if (it.trans)
+      rout[j + it.ldr * i] = r;
+    else
+      rout[i + it.ldr * j] = r;
+  }
+}
+
 }  // namespace
 
 // ====================================================================== host side
@@ -1860,8 +2189,8 @@ void run_qr(Batch& B, const std::vector<QRJob>& jobs) {
   int mmax = 0;
   for (const GQ& g : items) mmax = std::max(mmax, g.mm);
   hipStream_t st = B.s;
-  if (mmax <= 512)
-    B.enqueue([=] { hipLaunchKernelGGL(gqr_kernel<8>, dim3(ng), dim3(QR_THREADS), 0, st, dq); });
+  if (mmax <= WY_MAXM)
+    B.enqueue([=] { hipLaunchKernelGGL(gqr_wy_kernel, dim3(ng), dim3(QR_THREADS), 0, st, dq); });
   else if (mmax <= 1024)
     B.enqueue([=] { hipLaunchKernelGGL(gqr_kernel<16>, dim3(ng), dim3(QR_THREADS), 0, st, dq); });
   else
@@ -1879,6 +2208,10 @@ std::vector<int> gauge_sweeps(Batch& B, const std::vector<Inst*>& in, bool left,
   std::vector<int> it(P, 0);
   std::vector<double> eps(P, std::numeric_limits<double>::infinity());
   std::vector<char> act(P, 1);
+  // Q (AL or AR) of a sweep is only read by the fixed-point step before the next sweep (from the
+  // eig_miniter-th sweep on) and at the end: earlier sweeps compute R (L) alone, and an instance
+  // whose last sweep skipped Q gets it from the same M once it has converged
+  std::vector<char> have_q(P, 0);
   // C = C0 / |C0|
   {
     std::vector<Item> items;
@@ -1953,14 +2286,16 @@ std::vector<int> gauge_sweeps(Batch& B, const std::vector<Inst*>& in, bool left,
       const int D = I.D, d = I.d;
       c0[j] = vec_item(GV_COPY, I.nC, I.C, I.Cpre);
       s0.push_back(&c0[j]);
+      const bool wq = it[k] + 1 >= eig_miniter;
+      have_q[k] = wq ? 1 : 0;
       if (left) {
-        // M = C A (D x dD) into AL, QR in place: AL = Q, C = R
+        // M = C A (D x dD), QR: AL = Q, C = R
         g0[j] = gemm_item(D, d * D, D, 1.0, I.C, 1, D, A[k], 1, D, 0.0, I.Mg, D);
-        qj.push_back(QRJob{d * D, D, 0, true, I.Mg, (long)d * D, I.AL, (long)d * D, I.C, D, &I, 0});
+        qj.push_back(QRJob{d * D, D, 0, wq, I.Mg, (long)d * D, I.AL, (long)d * D, I.C, D, &I, 0});
       } else {
         // M = A C (dD x D as D x dD), LQ: C = L, AR = Q
         g0[j] = gemm_item(d * D, D, D, 1.0, A[k], 1, (long)d * D, I.C, 1, D, 0.0, I.Mg, (long)d * D);
-        qj.push_back(QRJob{d * D, D, 1, true, I.Mg, D, I.AR, D, I.C, D, &I, 0});
+        qj.push_back(QRJob{d * D, D, 1, wq, I.Mg, D, I.AR, D, I.C, D, &I, 0});
       }
       s1.push_back(&g0[j]);
       GN g{};
@@ -1987,6 +2322,18 @@ std::vector<int> gauge_sweeps(Batch& B, const std::vector<Inst*>& in, bool left,
       if (eps[k] < tol[k] || it[k] > maxiter) act[k] = 0;
     }
   }
+  // Q of the last sweep where it was skipped (R/L recomputed into scratch; C keeps the normalised one)
+  std::vector<QRJob> fin;
+  for (size_t k = 0; k < P; ++k) {
+    if (have_q[k]) continue;
+    Inst& I = *in[k];
+    const int D = I.D, d = I.d;
+    if (left)
+      fin.push_back(QRJob{d * D, D, 0, true, I.Mg, (long)d * D, I.AL, (long)d * D, I.Qdump, D, &I, 0});
+    else
+      fin.push_back(QRJob{d * D, D, 1, true, I.Mg, D, I.AR, D, I.Qdump, D, &I, 0});
+  }
+  run_qr(B, fin);
   return it;
 }
 
