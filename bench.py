@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--sweep-workers", type=int, default=8)
     ap.add_argument("--sweep-mode", default="both", choices=["device", "threads", "both"],
                     help="--sweep: the lockstep device batch, the threaded batch, or both")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the parameter sweep of the default run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the fixed-point response of the default run")
     ap.add_argument("--gradient", action="store_true",
                     help="energy gradients: TFIM D=128 de/dh and XXZ D=512 de/dDelta against finite differences")
     ap.add_argument("--vumps-iters", type=int, default=3)
@@ -196,6 +198,12 @@ def main():
 
     if not args.no_ttg and not args.vumps_only and not args.ttg:
         out["time_to_galerkin"] = time_to_galerkin(imps, H, N)
+
+    if not args.no_sweep and not args.vumps_only:
+        # the 64-instance TFIM parameter sweep: lockstep device batch against the threaded batch
+        out["sweep"] = parameter_sweep(args, imps, PREC["fp64"])
+    if not args.no_adjoint and not args.vumps_only:
+        out["adjoint"] = adjoint_check(imps)
 
     if args.idmrg > 0:
         out["idmrg"] = idmrg_sweeps(args, imps, H, D, N)
@@ -456,6 +464,32 @@ def parameter_sweep(args, imps, prec):
         out["max_energy_diff_device_threads"] = max(abs(a - b) for a, b in zip(energies["device"],
                                                                                 energies["threads"]))
     return out
+
+
+def adjoint_check(imps):
+    """The fixed-point response at TFIM J=1, h=1.3, D=128: d<sigma_z>/dh through the converged state
+    (imps_fixedpoint_gradient) against the closed form -e''(h) of the free-fermion energy."""
+    import math
+
+    import numpy as np
+    from scipy import integrate
+    h = 1.3
+    H = imps.models.tfim_hamiltonian(J=1.0, h=h, nsites=1)
+    ctx = imps.Context(0)
+    psi0, ctx = imps.randomimps(128, H, seed=0, ctx=ctx, tol=1e-12, maxiter=100)
+    envs = imps.GPUCache(psi0, H, ctx)
+    psi, envs, info = imps.find_groundstate(psi0, H, imps.VUMPS(tol=1e-12, maxiter=1000), envs=envs)
+    t0 = time.perf_counter()
+    _, dW, rinfo = imps.fixedpoint_gradient(psi, H, envs, imps.objective_mpo_1site(imps.models.sigma_z()), tol=1e-10)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    pred = float(np.sum(dW * (imps.models.tfim_hamiltonian(J=1.0, h=h + 1.0, nsites=1).Ws[0] - H.Ws[0])))
+    exact = integrate.quad(lambda k: math.sin(k) ** 2 / (1 + h * h - 2 * h * math.cos(k)) ** 1.5, 0, math.pi,
+                           epsabs=1e-14, epsrel=1e-14)[0] / math.pi
+    log(f"bench: adjoint TFIM D=128: d<sz>/dh {pred:.12f} (closed form {exact:.12f}), {dt:.2f}s")
+    return {"workload": "d<sigma_z>/dh through the VUMPS fixed point, TFIM J=1 h=1.3, D=128",
+            "itererr": info.itererr, "dsz_dh": pred, "closed_form": exact, "abs_err": abs(pred - exact),
+            "minres_iterations": rinfo.iterations, "relres": rinfo.relres, "seconds": dt}
 
 
 def gradients(imps):
