@@ -482,7 +482,7 @@ int imps_gemm_dev(imps_ctx* ctx, char transa, char transb, int m, int n, int k, 
 /* GEMM tile autotuning, process-wide, on by default (no counterpart in the reference, whose
  * contractions go to BLAS): the first FP64/FP32 GEMM of each shape times the library's
  * bit-identical tile variants once and keeps the fastest; 0 switches back to the fixed
- * shape heuristic; 2 + v (v = 0..4) runs tile variant v wherever it is valid and autotunes
+ * shape heuristic; 2 + v (v = 0..5) runs tile variant v wherever it is valid and autotunes
  * elsewhere (for tests and measurement).  Results do not depend on the setting.  Returns
  * IMPS_E_ARG for other values. */
 int imps_set_gemm_autotune(int on);

@@ -3963,7 +3963,7 @@ int imps_set_graph_capture(int on) {
 }
 
 int imps_set_gemm_autotune(int on) {
-  if (on < 0 || on > 2 + 4) return IMPS_E_ARG;
+  if (on < 0 || on > 2 + 5) return IMPS_E_ARG;
   g_gemm_autotune.store(on);
   return IMPS_OK;
 }

@@ -72,10 +72,12 @@ __device__ inline void gemm_tile_of(int bx, int by, int MT, int NT, int& mt, int
   nt = r / gsz;
 }
 
+// Epilogue of every tile variant: C = fma(beta, C_old, alpha * acc), one rounding order for all of
+// them, so that no variant depends on how the compiler contracts alpha acc + beta C_old.
 // VEC: every tile is full (M % BM, N % BN, K % BK, Kc % BK all zero) and each thread moves
 // 16-byte pairs along the contiguous direction of A and B (even strides, 16-byte aligned bases),
 // so the tile loads need no bounds checks.  GM: tile order (gemm_tile_of).
-template <typename TC, int BM, int BN, int BK, bool AMC, bool BKC, int OCC, bool VEC, int GM = 0>
+template <typename TC, int BM, int BN, int BK, bool AMC, bool BKC, int OCC, bool VEC, int GM = 0, bool PRIO = false>
 __global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
   typedef MfmaTraits<TC> MT;
   typedef typename MT::v4 v4;
@@ -229,6 +231,7 @@ __global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
       advance();
       load_tiles((kt + 1) * BK);
     }
+    if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int k4 = 0; k4 < BK; k4 += 4) {
       TC af[TM], bf[TN];
@@ -241,6 +244,7 @@ __global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = MT::mma(bf[j], af[i], acc[i][j]);
     }
+    if (PRIO) __builtin_amdgcn_s_setprio(0);
   }
 
   // split-K partial results go to the workspace slice of this z; the reduction kernel
@@ -266,7 +270,7 @@ __global__ void __launch_bounds__(256, OCC) gemm_kernel(GemmDesc g) {
         if (n < N) {
           double* p = Cout + m + (long)n * ldc;
           const double v = alpha * (double)acc[i][j][r];
-          *p = (beta == 0.0) ? v : v + beta * *p;
+          *p = (beta == 0.0) ? v : __builtin_fma(beta, *p, v);
         }
       }
     }
@@ -428,7 +432,7 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_kernel(GemmDesc g) {
         if (n < N) {
           double* p = Cout + m + (long)n * ldc;
           const double v = alpha * (double)acc[i][j][r];
-          *p = (beta == 0.0) ? v : v + beta * *p;
+          *p = (beta == 0.0) ? v : __builtin_fma(beta, *p, v);
         }
       }
     }
@@ -444,21 +448,21 @@ __global__ void splitk_reduce_kernel(int M, int N, int nz, const double* __restr
   double s = 0.0;
   for (int z = 0; z < nz; ++z) s += W[(size_t)z * M * N + m + (size_t)M * n];
   double* p = C + m + ldc * n;
-  *p = (beta == 0.0) ? alpha * s : alpha * s + beta * *p;
+  *p = (beta == 0.0) ? alpha * s : __builtin_fma(beta, *p, alpha * s);
 }
 
-template <typename TC, int BM, int BN, int BK, int OCC, bool VEC, int GM = 0>
+template <typename TC, int BM, int BN, int BK, int OCC, bool VEC, int GM = 0, bool PRIO = false>
 static hipError_t gemm_dispatch_tile(const GemmDesc& g, bool amc, bool bkc, int splitk, hipStream_t s) {
   dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, splitk);
   dim3 block(256);
   if (amc && bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, true, OCC, VEC, GM>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, true, OCC, VEC, GM, PRIO>), grid, block, 0, s, g);
   else if (amc && !bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, false, OCC, VEC, GM>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, true, false, OCC, VEC, GM, PRIO>), grid, block, 0, s, g);
   else if (!amc && bkc)
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, true, OCC, VEC, GM>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, true, OCC, VEC, GM, PRIO>), grid, block, 0, s, g);
   else
-    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, false, OCC, VEC, GM>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<TC, BM, BN, BK, false, false, OCC, VEC, GM, PRIO>), grid, block, 0, s, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || splitk == 1) return e;
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3((g.M + 255) / 256, g.N), dim3(256), 0, s, g.M, g.N, splitk,
@@ -511,9 +515,9 @@ static bool gemm_vec_ok(const GemmDesc& g, bool amc, bool bkc, int BM, int BN) {
 template <typename TC>
 static hipError_t gemm_select(const GemmDesc& g, bool amc, bool bkc, hipStream_t s) {
   const long t64x128 = (long)((g.M + 63) / 64) * ((g.N + 127) / 128);
-  // large shapes: 64 x 128 tiles at four workgroups per CU with paired loads
+  // large shapes: 64 x 128 tiles at four workgroups per CU with paired loads (variant 5)
   if (t64x128 >= 384 && gemm_vec_ok(g, amc, bkc, 64, 128))
-    return gemm_dispatch_tile<TC, 64, 128, 16, 4, true, 8>(g, amc, bkc, 1, s);
+    return gemm_dispatch_tile<TC, 64, 128, 16, 4, true, 8, true>(g, amc, bkc, 1, s);
   const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
   if (t128 >= 192) return gemm_dispatch_tile<TC, 128, 128, 16, 2, false>(g, amc, bkc, 1, s);
   const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
@@ -559,9 +563,12 @@ static bool gemm_normalise(GemmDesc& g, bool& amc, bool& bkc) {
 //   0: 64 x 128, occupancy 4, paired loads, XCD-grouped tile order (groups of 8 tile rows)
 //   1: 64 x 64, occupancy 4, paired loads   4: variant 0 in the identity tile order
 //   2: 128 x 128, occupancy 2                3: 64 x 64, occupancy 2
+//   5: variant 0 with the wave priority raised while it issues MFMAs (s_setprio), so that a wave
+//      with MFMAs ready wins the issue arbitration over one storing its next LDS stage
 template <typename TC>
 static hipError_t gemm_variant(const GemmDesc& g, bool amc, bool bkc, int v, hipStream_t s) {
   switch (v) {
+    case 5: return gemm_dispatch_tile<TC, 64, 128, 16, 4, true, 8, true>(g, amc, bkc, 1, s);
     case 0: return gemm_dispatch_tile<TC, 64, 128, 16, 4, true, 8>(g, amc, bkc, 1, s);
     case 1: return gemm_dispatch_tile<TC, 64, 64, 16, 4, true>(g, amc, bkc, 1, s);
     case 2: return gemm_dispatch_tile<TC, 128, 128, 16, 2, false>(g, amc, bkc, 1, s);
@@ -584,6 +591,7 @@ int gemm_tile_variants(const GemmDesc& g0, int prec, int* ids) {
   ids[n++] = 2;
   ids[n++] = 3;
   if (v128) ids[n++] = 4;
+  if (v128) ids[n++] = 5;
   return n;
 }
 
@@ -592,7 +600,8 @@ hipError_t launch_gemm_variant(const GemmDesc& g0, int prec, int v, hipStream_t 
   bool amc, bkc;
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (!gemm_normalise(g, amc, bkc)) return hipErrorInvalidValue;
-  if (((v == 0 || v == 4) && !gemm_vec_ok(g, amc, bkc, 64, 128)) || (v == 1 && !gemm_vec_ok(g, amc, bkc, 64, 64)))
+  if (((v == 0 || v == 4 || v == 5) && !gemm_vec_ok(g, amc, bkc, 64, 128)) ||
+      (v == 1 && !gemm_vec_ok(g, amc, bkc, 64, 64)))
     return hipErrorInvalidValue;
   if (prec == PREC_FP32) return gemm_variant<float>(g, amc, bkc, v, s);
   if (prec == PREC_FP64) return gemm_variant<double>(g, amc, bkc, v, s);

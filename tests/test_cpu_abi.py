@@ -140,10 +140,10 @@ def test_random_tensors_match_oracle(oracle):
 
 
 def test_gemm_autotune_mode_range(lib):
-    # 0 heuristic, 1 autotune, 2 + v a forced tile variant (v = 0..4); anything else is refused
-    for mode in (-1, 7, 100):
+    # 0 heuristic, 1 autotune, 2 + v a forced tile variant (v = 0..5); anything else is refused
+    for mode in (-1, 8, 100):
         assert lib.imps_set_gemm_autotune(mode) != 0
-    for mode in (0, 6, 1):
+    for mode in (0, 7, 1):
         assert lib.imps_set_gemm_autotune(mode) == 0
 
 

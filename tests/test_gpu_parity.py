@@ -83,7 +83,7 @@ def test_gemm_autotune_is_bit_identical(imps, ctx, prec, ta, tb, m, n, k):
     C0 = rng.standard_normal((m, n))
     outs = []
     # tune, cached choice, heuristic, then each tile variant forced where valid
-    for on in (1, 1, 0) + tuple(range(2, 7)):
+    for on in (1, 1, 0) + tuple(range(2, 8)):
         assert lib.imps_set_gemm_autotune(on) == 0
         dA, dB, dC = DevArray(ctx, A), DevArray(ctx, B), DevArray(ctx, C0)
         ctx.call("imps_gemm_dev", ta.encode(), tb.encode(), m, n, k, 0.7, dA.p, A.shape[0], dB.p, B.shape[0],
