@@ -71,6 +71,18 @@ const char* imps_last_error(const imps_ctx* ctx);
  * tompotensors(H) (mpohamiltonian.jl:150): W[site] has shape (w, d, w, d).
  * Levels 0 and w-1 must be the unit levels (W[0,:,0,:] = W[w-1,:,w-1,:] = I). */
 int imps_set_mpo(imps_ctx* ctx, int nsites, int w, int d, const double* const* W);
+/* SparseIMPO with rectangular Schur tensors and per-site physical dimensions
+ * (src/operators/mpohamiltonian.jl:26-31 and 57-66, the level-chain closure; phydims at :88):
+ * W[l] has shape (levels[l-1], dims[l], levels[l], dims[l]) (levels[l] = bonddim(H, l), the
+ * levels of the bond right of site l; levels[-1] = levels[nsites-1]), with its unit levels at
+ * (0, 0) and (levels[l-1]-1, levels[l]-1).  The chain is embedded exactly into the square form
+ * (w = max levels, d = max dims) on which every other call runs: bond level i < levels - 1 keeps
+ * its index, the closing level goes to w - 1, and physical indices s >= dims[l] form an empty
+ * sector (identity on the unit levels, zero elsewhere) that no operation of the library
+ * populates.  States, environments and gradients are then exchanged in that square form (states
+ * with zeros in the empty sectors: imps_set_state and imps_set_state_raw refuse others with
+ * IMPS_E_ARG).  Closure (levels agree bond by bond) holds by construction of the arguments. */
+int imps_set_mpo_chain(imps_ctx* ctx, int nsites, const int* dims, const int* levels, const double* const* W);
 
 /* CanonicalIMPS (src/states/canonicalmps.jl:17-40): AL, AR, AC of shape (D, d, D)
  * and C of shape (D, D) per site; C[l] is the bond right of site l. */

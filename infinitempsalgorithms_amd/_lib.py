@@ -95,6 +95,7 @@ SIGNATURES = [
     ("imps_destroy", c_int, [c_void_p]),
     ("imps_last_error", ctypes.c_char_p, [c_void_p]),
     ("imps_set_mpo", c_int, [c_void_p, c_int, c_int, c_int, PP_double]),
+    ("imps_set_mpo_chain", c_int, [c_void_p, c_int, P_int, P_int, PP_double]),
     ("imps_set_state", c_int, [c_void_p, c_int, c_int, PP_double, PP_double, PP_double, PP_double]),
     ("imps_get_state", c_int, [c_void_p, PP_double, PP_double, PP_double, PP_double]),
     ("imps_state_dims", c_int, [c_void_p, P_int, P_int, P_int]),

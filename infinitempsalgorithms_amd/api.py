@@ -51,6 +51,7 @@ class Context:
         self.h = h
         self.L = L
         self.N = self.w = self.d = self.D = 0
+        self.dims, self.levels = [], []
         _LIVE.add(self)
 
     def __enter__(self):
@@ -78,16 +79,61 @@ class Context:
     def set_mpo(self, H: "SparseIMPO"):
         Ws = [fortran(W) for W in H.Ws]
         pp, keep = ptr_array(Ws)
-        self.call("imps_set_mpo", len(Ws), H.w, H.d, pp)
+        if H.square:
+            self.call("imps_set_mpo", len(Ws), H.w, H.d, pp)
+        else:
+            dims = (ctypes.c_int * len(Ws))(*H.dims)
+            levels = (ctypes.c_int * len(Ws))(*H.levels)
+            self.call("imps_set_mpo_chain", len(Ws), dims, levels, pp)
         self.N, self.w, self.d = len(Ws), H.w, H.d
+        self.dims, self.levels = list(H.dims), list(H.levels)
+
+    # The library runs on the square embedding of the model (imps_set_mpo_chain): states carry
+    # d = max dims physical indices with zeros in the empty sectors, environments w = max levels.
+    # These map the reference's per-site shapes to it and back.
+    def pad_site(self, l, x):
+        """(D, dims[l], D) -> (D, d, D) with zeros in the empty sector."""
+        x = fortran(x)
+        if not self.d or x.shape[1] == self.d:
+            return x
+        out = np.zeros((x.shape[0], self.d, x.shape[2]), order="F")
+        out[:, :x.shape[1], :] = x
+        return out
+
+    def unpad_site(self, l, x):
+        dl = self.dims[l % self.N] if self.d else x.shape[1]
+        return x if dl == x.shape[1] else np.asfortranarray(x[:, :dl, :])
+
+    def pad_operator(self, O):
+        """A one-site operator (dl x dl) as its (d x d) form, zero on the empty sector."""
+        O = fortran(O)
+        if O.shape[0] == self.d:
+            return O
+        out = np.zeros((self.d, self.d), order="F")
+        out[:O.shape[0], :O.shape[1]] = O
+        return out
+
+    def bond_levels(self, bond):
+        """Embedded level indices of the bond right of site `bond` (0 .. levels-2, then w-1)."""
+        L = self.levels[bond % self.N]
+        return list(range(L - 1)) + [self.w - 1]
+
+    def _check_state_shapes(self, what, D, tensors):
+        if not self.d:
+            return  # without a model the library itself refuses the call
+        for l, x in enumerate(tensors):
+            if x.shape != (D, self.dims[l % self.N], D):
+                raise ValueError(f"{what}: site {l} tensor must be ({D}, {self.dims[l % self.N]}, {D}), got {x.shape}")
 
     def set_state(self, psi: "CanonicalIMPS"):
         AL, AR, C, AC = ([fortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
         D = psi.D
-        # (without a model the library itself refuses the call)
-        if self.d and (not (len(AL) == len(AR) == len(C) == len(AC)) or
-                       any(x.shape != (D, self.d, D) for x in AL + AR + AC) or any(x.shape != (D, D) for x in C)):
-            raise ValueError(f"set_state: every AL, AR, AC must be ({D}, {self.d}, {D}) and every C ({D}, {D})")
+        if self.d:
+            if not (len(AL) == len(AR) == len(C) == len(AC)) or any(x.shape != (D, D) for x in C):
+                raise ValueError(f"set_state: AL, AR, C, AC per site with every C ({D}, {D})")
+            for xs in (AL, AR, AC):
+                self._check_state_shapes("set_state", D, xs)
+            AL, AR, AC = ([self.pad_site(l, x) for l, x in enumerate(xs)] for xs in (AL, AR, AC))
         keep = [ptr_array(x) for x in (AL, AR, C, AC)]
         self.call("imps_set_state", len(AL), psi.D, *(k[0] for k in keep))
         self.D = psi.D
@@ -95,8 +141,8 @@ class Context:
     def set_state_raw(self, As, tol, maxiter):
         As = [fortran(a) for a in As]
         D = As[0].shape[0]
-        if self.d and any(a.shape != (D, self.d, D) for a in As):
-            raise ValueError(f"set_state_raw: every tensor must be ({D}, {self.d}, {D})")
+        self._check_state_shapes("set_state_raw", D, As)
+        As = [self.pad_site(l, a) for l, a in enumerate(As)]
         pp, keep = ptr_array(As)
         self.call("imps_set_state_raw", len(As), As[0].shape[0], pp, float(tol), int(maxiter))
         self.D = As[0].shape[0]
@@ -120,15 +166,28 @@ class Context:
         AC = [np.zeros((D, d, D), order="F") for _ in range(N)]
         keep = [ptr_array(x) for x in (AL, AR, C, AC)]
         self.call("imps_get_state", *(k[0] for k in keep))
+        AL, AR, AC = ([self.unpad_site(l, x) for l, x in enumerate(xs)] for xs in (AL, AR, AC))
         return CanonicalIMPS(AL, AR, C, AC)
+
+    def _env_bond(self, site, side):
+        # leftenv(site) lives on the bond left of the site, rightenv(site) on the bond right of it
+        return site - 1 if side == _lib.IMPS_SIDE_LEFT else site
 
     def get_env(self, site, side):
         out = np.zeros((self.D, self.w, self.D), order="F")
         self.call("imps_get_env", int(site), int(side), dptr(out))
-        return out
+        lv = self.bond_levels(self._env_bond(int(site), int(side)))
+        return out if len(lv) == self.w else np.asfortranarray(out[:, lv, :])
 
     def set_env(self, site, side, G):
         G = fortran(G)
+        lv = self.bond_levels(self._env_bond(int(site), int(side)))
+        if G.shape[1] != self.w:
+            if G.shape[1] != len(lv):
+                raise ValueError(f"set_env: the environment has {len(lv)} levels, got {G.shape[1]}")
+            Gp = np.zeros((G.shape[0], self.w, G.shape[2]), order="F")
+            Gp[:, lv, :] = G
+            G = Gp
         self.call("imps_set_env", int(site), int(side), dptr(G))
 
 
@@ -136,22 +195,35 @@ class Context:
 
 
 class SparseIMPO:
-    """SparseIMPO (src/operators/mpohamiltonian.jl:33-56): per-site Schur tensors, stored
-    densely as ``tompotensor`` arrays (w, d, w, d).  Square tensors with the same w and d
-    on every site are supported."""
+    """SparseIMPO (src/operators/mpohamiltonian.jl:33-66): per-site Schur tensors, stored
+    densely as ``tompotensor`` arrays (w_left, d, w_right, d).  The level counts may differ from
+    bond to bond (rectangular Schur tensors) and the physical dimension from site to site, as
+    long as the level chain closes (Ws[l] right levels == Ws[l+1] left levels, periodically;
+    mpohamiltonian.jl:57-66).  The first and the last level of every tensor are unit levels.
+    The device runs on the chain's exact square embedding (imps_set_mpo_chain)."""
 
     def __init__(self, Ws):
         Ws = [fortran(W) for W in Ws]
         if not Ws:
             raise ValueError("Ws must not be empty")
-        w, d = Ws[0].shape[0], Ws[0].shape[1]
+        N = len(Ws)
+        for l, W in enumerate(Ws):
+            if W.ndim != 4 or W.shape[1] != W.shape[3] or W.shape[0] < 2 or W.shape[2] < 2:
+                raise ValueError(f"Ws[{l}] must have shape (w_left, d, w_right, d) with at least two levels, "
+                                 f"got {W.shape}")
+        for l in range(N):
+            if Ws[l].shape[2] != Ws[(l + 1) % N].shape[0]:
+                raise ValueError(f"level chain mismatch: Ws[{l}] right levels {Ws[l].shape[2]} vs "
+                                 f"Ws[{(l + 1) % N}] left levels {Ws[(l + 1) % N].shape[0]}")
         for W in Ws:
-            if W.shape != (w, d, w, d):
-                raise ValueError(f"all Schur tensors must have shape {(w, d, w, d)} (level chain mismatch)")
-            if not (np.array_equal(W[0, :, 0, :], np.eye(d)) and np.array_equal(W[w - 1, :, w - 1, :], np.eye(d))):
-                raise ValueError("levels 1 and w must be unit levels")
+            wl, d, wr, _ = W.shape
+            if not (np.array_equal(W[0, :, 0, :], np.eye(d)) and np.array_equal(W[wl - 1, :, wr - 1, :], np.eye(d))):
+                raise ValueError("the first and the last level must be unit levels")
         self.Ws = Ws
-        self.w, self.d = w, d
+        self.dims = [W.shape[1] for W in Ws]
+        self.levels = [W.shape[2] for W in Ws]  # the bond right of each site
+        self.w, self.d = max(self.levels), max(self.dims)
+        self.square = all(W.shape == (self.w, self.d, self.w, self.d) for W in Ws)
 
     def __len__(self):
         return len(self.Ws)
@@ -160,14 +232,24 @@ class SparseIMPO:
         return self.Ws[i % len(self.Ws)]
 
     def phydims(self):
-        return [self.d] * len(self.Ws)
+        """phydims(H) (mpohamiltonian.jl:88)."""
+        return list(self.dims)
 
     def tompotensors(self):
         """tompotensors(H) (mpohamiltonian.jl:150)."""
         return [W.copy(order="F") for W in self.Ws]
 
     def bonddim(self, l=None):
-        """Number of Schur levels (mpohamiltonian.jl:72-84)."""
+        """bonddim(H, ℓ): levels of the bond right of site ℓ (0-based); bonddim(H): the uniform
+        level count, a ValueError for rectangular chains (mpohamiltonian.jl:68-83)."""
+        if l is not None:
+            return self.levels[l % len(self.Ws)]
+        if not all(W.shape[0] == W.shape[2] == self.levels[0] for W in self.Ws):
+            raise ValueError("non-uniform level structure (rectangular Schur chain); use bonddim(l) or max_bonddim()")
+        return self.levels[0]
+
+    def max_bonddim(self):
+        """max_bonddim(H) (mpohamiltonian.jl:84)."""
         return self.w
 
 
@@ -278,10 +360,11 @@ class MPO_AC_Hamiltonian:
         self.site, self.envs, self.prec = site, envs, prec
 
     def __call__(self, x):
-        x = fortran(x)
+        ctx = self.envs.ctx
+        x = ctx.pad_site(self.site, x)
         y = np.zeros_like(x, order="F")
-        self.envs.ctx.call("imps_hac_apply", int(self.site), dptr(x), dptr(y), int(self.prec))
-        return y
+        ctx.call("imps_hac_apply", int(self.site), dptr(x), dptr(y), int(self.prec))
+        return ctx.unpad_site(self.site, y)
 
 
 class MPO_C_Hamiltonian:
@@ -329,6 +412,7 @@ def fixedpoint(op, x0, which="SR", alg: Lanczos | None = None):
     if isinstance(op, MPO_AC_Hamiltonian):
         v = np.zeros((ctx.D, ctx.d, ctx.D), order="F")
         ctx.call("imps_eigsolve_ac", int(op.site), ctypes.byref(p), ctypes.byref(lam), dptr(v), ctypes.byref(info))
+        v = ctx.unpad_site(op.site, v)
     elif isinstance(op, MPO_C_Hamiltonian):
         v = np.zeros((ctx.D, ctx.D), order="F")
         ctx.call("imps_eigsolve_c", int(op.site), ctypes.byref(p), ctypes.byref(lam), dptr(v), ctypes.byref(info))
@@ -380,7 +464,7 @@ def regauge_(envs: GPUCache, site):
     ctx = envs.ctx
     al = np.zeros((ctx.D, ctx.d, ctx.D), order="F")
     ctx.call("imps_regauge", int(site), dptr(al))
-    return al
+    return ctx.unpad_site(site, al)
 
 
 def calc_galerkin(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
@@ -405,8 +489,10 @@ def expectationvalue(psi: CanonicalIMPS, op, envs: GPUCache):
     else:
         site, O = op
         O = fortran(O)
-        if O.shape != (psi.d, psi.d):
-            raise ValueError(f"expectationvalue: the operator must be {psi.d} x {psi.d}")
+        dl = psi.AL[int(site) % len(psi)].shape[1]
+        if O.shape != (dl, dl):
+            raise ValueError(f"expectationvalue: the operator at site {site} must be {dl} x {dl}")
+        O = envs.ctx.pad_operator(O)
         envs.ctx.call("imps_local_expectation", int(site), dptr(O), ctypes.byref(val))
     return val.value
 
@@ -421,8 +507,10 @@ def correlator(psi: CanonicalIMPS, O1, O2, i: int, js, envs: GPUCache):
     scalar = isinstance(js, (int, np.integer))
     jl = [int(js)] if scalar else [int(j) for j in js]
     O1, O2 = fortran(O1), fortran(O2)
-    if O1.shape != (psi.d, psi.d) or O2.shape != (psi.d, psi.d):
-        raise ValueError(f"correlator: operators must be {psi.d} x {psi.d}")
+    di = psi.AL[int(i) % len(psi)].shape[1]
+    if O1.shape != (di, di) or any(O2.shape != (psi.AL[j % len(psi)].shape[1],) * 2 for j in jl):
+        raise ValueError("correlator: O1 must match the physical dimension of site i and O2 that of every j")
+    O1, O2 = envs.ctx.pad_operator(O1), envs.ctx.pad_operator(O2)
     jarr = np.array(jl, dtype=np.int32)
     out = np.zeros(len(jl))
     envs.ctx.call("imps_correlator", dptr(O1), dptr(O2), int(i), len(jl),
@@ -440,7 +528,7 @@ def dot(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
     if psi1 is not envs.ket:
         envs.ctx.set_state(psi1)
         envs.ket = psi1
-    AL2 = [fortran(a) for a in psi2.AL]
+    AL2 = [envs.ctx.pad_site(l, a) for l, a in enumerate(psi2.AL)]
     pp, keep = ptr_array(AL2)
     val = np.zeros(2)
     envs.ctx.call("imps_dot_state", len(AL2), pp, dptr(val))
@@ -464,8 +552,9 @@ def infidelity(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
 
 class DenseIMPO:
     """DenseIMPO (src/operators/infinitempo.jl:12-30): per-site dense MPO tensors (wl, d, wr, d),
-    periodic over the unit cell; each right bond must match the next site's left bond.  The
-    library's device kernels take square tensors with one w on every site."""
+    periodic over the unit cell; each right bond must match the next site's left bond.  Bond and
+    physical dimensions may differ from site to site: the device runs on the tensors zero-padded
+    to one square shape (_dense_square)."""
 
     def __init__(self, Ws):
         Ws = [fortran(W) for W in Ws]
@@ -483,6 +572,29 @@ class DenseIMPO:
 
     def __getitem__(self, i):
         return self.Ws[i % len(self.Ws)]
+
+
+def _dense_square(Ws, psi, ctx, what):
+    """DenseIMPO tensors with per-bond and per-site physical dimensions zero-padded to one square
+    shape (w = the largest bond, d = the context's physical dimension).  The padded bond indices
+    and physical sectors carry no weight: each transfer map becomes T (+) 0, with T's nonzero
+    spectrum and eigenvectors (zeros on the padding), so every value is unchanged."""
+    Ws = [fortran(x) for x in Ws]
+    for l in range(len(psi)):
+        x = Ws[l % len(Ws)]
+        if x.ndim != 4 or x.shape[1] != x.shape[3] or x.shape[1] != psi.AL[l].shape[1]:
+            raise ValueError(f"{what}: physical dimensions of W and ψ differ at site {l}")
+    w = max(max(x.shape[0], x.shape[2]) for x in Ws)
+    d = ctx.d or Ws[0].shape[1]
+    out = []
+    for x in Ws:
+        if x.shape == (w, d, w, d):
+            out.append(x)
+            continue
+        P = np.zeros((w, d, w, d), order="F")
+        P[:x.shape[0], :x.shape[1], :x.shape[2], :x.shape[3]] = x
+        out.append(P)
+    return out, w, d
 
 
 def dominant_env(psi: CanonicalIMPS, envs, W=None, side: str = "left", which: str = "LM",
@@ -509,18 +621,15 @@ def dominant_env(psi: CanonicalIMPS, envs, W=None, side: str = "left", which: st
         ctx.call("imps_dominant_env", 0 if side == "left" else 1, 0, 1, psi.d, None, float(tol), int(krylovdim),
                  int(maxiter), ctypes.byref(lam), dptr(out))
         return lam.value, out
-    Ws = W.Ws if isinstance(W, DenseIMPO) else [fortran(x) for x in W]
-    w, d = Ws[0].shape[0], Ws[0].shape[1]
-    for x in Ws:
-        if x.shape != (w, d, w, d):
-            raise ValueError(f"dominant_env: every tensor must have shape {(w, d, w, d)}")
-    if d != psi.d:
-        raise ValueError("dominant_env: physical dimensions of W and ψ differ")
+    W0 = W.Ws if isinstance(W, DenseIMPO) else list(W)
+    Ws, w, d = _dense_square(W0, psi, ctx, "dominant_env")
     pp, keep = ptr_array(Ws)
     out = np.zeros((D, w, D), order="F")
     ctx.call("imps_dominant_env", 0 if side == "left" else 1, len(Ws), w, d, pp, float(tol), int(krylovdim),
              int(maxiter), ctypes.byref(lam), dptr(out))
-    return lam.value, out
+    # both environments live on the bond left of site 0
+    wb = np.shape(W0[0])[0]
+    return lam.value, (out if wb == w else np.asfortranarray(out[:, :wb, :]))
 
 
 def dense_expectation(psi: CanonicalIMPS, W, envs, tol: float = 1e-10, krylovdim: int = 30, maxiter: int = 300,
@@ -535,13 +644,8 @@ def dense_expectation(psi: CanonicalIMPS, W, envs, tol: float = 1e-10, krylovdim
             envs.ket = psi
     else:
         ctx.set_state(psi)
-    Ws = W.Ws if isinstance(W, DenseIMPO) else [fortran(x) for x in W]
-    w, d = Ws[0].shape[0], Ws[0].shape[1]
-    for x in Ws:
-        if x.shape != (w, d, w, d):
-            raise ValueError(f"dense_expectation: every tensor must have shape {(w, d, w, d)}")
-    if d != psi.d:
-        raise ValueError("dense_expectation: physical dimensions of W and ψ differ")
+    W0 = W.Ws if isinstance(W, DenseIMPO) else list(W)
+    Ws, w, d = _dense_square(W0, psi, ctx, "dense_expectation")
     pp, keep = ptr_array(Ws)
     N, D = len(psi), psi.D
     lefts = [np.zeros((D, w, D), order="F") for _ in range(N)]
@@ -551,7 +655,12 @@ def dense_expectation(psi: CanonicalIMPS, W, envs, tol: float = 1e-10, krylovdim
     val = ctypes.c_double()
     ctx.call("imps_dense_expectation", len(Ws), w, d, pp, float(tol), int(krylovdim), int(maxiter),
              ctypes.byref(val), pl if return_envs else None, pr if return_envs else None)
-    return (val.value, lefts, rights) if return_envs else val.value
+    if not return_envs:
+        return val.value
+    # lefts[l] on the bond left of site l, rights[l] on the bond right of it
+    lefts = [np.asfortranarray(G[:, :np.shape(W0[l % len(W0)])[0], :]) for l, G in enumerate(lefts)]
+    rights = [np.asfortranarray(G[:, :np.shape(W0[l % len(W0)])[2], :]) for l, G in enumerate(rights)]
+    return val.value, lefts, rights
 
 
 def _file_call(name, *args):
@@ -566,6 +675,8 @@ def save_state(psi: CanonicalIMPS, path):
     """Writes ψ (AL, AR, C, AC of every site) to a state file (imps_write_state_file); the
     library's own format, for checkpoint and resume (the reference has no file format)."""
     AL, AR, C, AC = ([fortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
+    if any(x.shape != AL[0].shape for x in AL + AR + AC):
+        raise ValueError("save_state: the state file holds one physical dimension for every site")
     keep = [ptr_array(x) for x in (AL, AR, C, AC)]
     _file_call("imps_write_state_file", os.fsencode(path), len(AL), psi.D, psi.d, *(k[0] for k in keep))
 
@@ -616,7 +727,16 @@ def energy_gradient(psi: CanonicalIMPS, H: SparseIMPO, envs: GPUCache):
     out = [np.zeros((H.w, H.d, H.w, H.d), order="F") for _ in range(len(H))]
     pp, keep = ptr_array(out)
     envs.ctx.call("imps_energy_gradient", pp)
-    return out
+    if H.square:
+        return out
+    N = len(H)
+    res = []
+    for l, G in enumerate(out):
+        li = envs.ctx.bond_levels(l - 1)
+        lj = envs.ctx.bond_levels(l)
+        dl = H.dims[l]
+        res.append(np.asfortranarray(G[np.ix_(li, range(dl), lj, range(dl))]))
+    return res
 
 
 def objective_mpo_1site(O):
@@ -990,7 +1110,7 @@ def device_batch_eligible(problems, alg: VUMPS | None = None):
     H0 = problems[0][0]
     for H, As in problems:
         D = np.asarray(As[0]).shape[0]
-        if len(H) != 1 or H.w != H0.w or H.d != H0.d or D % 16 or D < 16 or H.d * D > 2048:
+        if len(H) != 1 or not H.square or H.w != H0.w or H.d != H0.d or D % 16 or D < 16 or H.d * D > 2048:
             return False
     return H0.w * H0.d <= 64 and H0.d <= 8
 
@@ -1012,6 +1132,9 @@ def find_groundstates_batched(problems, alg: VUMPS | None = None, nworkers=8, de
     keep = []
     arr = (_lib.BatchInstance * len(problems))()
     for k, (H, As) in enumerate(problems):
+        if not H.square:
+            raise ValueError("find_groundstates_batched: the batch takes square Schur chains with one physical "
+                             "dimension (use find_groundstate for rectangular or heterogeneous chains)")
         Ws = [fortran(W) for W in H.Ws]
         As = [fortran(a) for a in As]
         pw, kw = ptr_array(Ws)

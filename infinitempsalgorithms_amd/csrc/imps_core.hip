@@ -70,6 +70,9 @@ struct imps_ctx {
   std::string err;
   // model: square Schur tensors (w, d, w, d) per site
   int N = 0, w = 0, d = 0;
+  // physical dimension of each site; sites with pdim[l] < d carry empty sectors s >= pdim[l]
+  // (imps_set_mpo_chain), which no state may populate
+  std::vector<int> pdim;
   std::vector<std::vector<double>> Wh;
   std::vector<double*> Wm;  // per-site H_AC mixing matrix (w d) x (w d)
   // state
@@ -2848,8 +2851,69 @@ int imps_set_mpo(imps_ctx* c, int nsites, int w, int d, const double* const* W) 
     c->GR.clear();
     c->gl_id.clear();
     c->gr_id.clear();
+    c->pdim.assign(nsites, d);
     HIPCHK(hipStreamSynchronize(c->stream));
   });
+}
+
+// Schur chains with per-site shapes: rectangular Schur tensors (levels that differ from bond to
+// bond) and per-site physical dimensions, embedded exactly into the square form that every
+// algorithm of the library runs on.  Bond level i < w_b - 1 keeps its index, the closing level
+// w_b - 1 goes to w - 1 (w = max w_b), and physical indices s >= dims[l] form an empty sector: the
+// unit levels act there as the identity, every other block is zero, so each effective operator
+// maps states without weight in the empty sectors to states without weight there, and the QR/LQ
+// gauge steps keep their zero rows and columns zero.
+int imps_set_mpo_chain(imps_ctx* c, int nsites, const int* dims, const int* levels, const double* const* W) {
+  if (!c) return IMPS_E_ARG;
+  std::vector<std::vector<double>> sq;
+  int wm = 0, dm = 0;
+  const int code = run_guarded(c, [&] {
+    require(nsites > 0 && dims && levels && W, IMPS_E_ARG, "imps_set_mpo_chain: bad arguments");
+    for (int l = 0; l < nsites; ++l) {
+      require(dims[l] >= 1 && levels[l] >= 2 && W[l] != nullptr, IMPS_E_ARG,
+              "imps_set_mpo_chain: each site needs d >= 1, >= 2 levels on its bond and a tensor");
+      wm = std::max(wm, levels[l]);
+      dm = std::max(dm, dims[l]);
+    }
+    require(wm * dm <= 64 && dm <= 8, IMPS_E_UNSUPPORTED, "imps_set_mpo_chain: w*d > 64 or d > 8 is not supported");
+    sq.assign(nsites, std::vector<double>((size_t)wm * dm * wm * dm, 0.0));
+    for (int l = 0; l < nsites; ++l) {
+      const int wl = levels[(l + nsites - 1) % nsites], wr = levels[l], d = dims[l];
+      auto lev = [&](int i, int wb) { return i == wb - 1 ? wm - 1 : i; };
+      double* P = sq[l].data();
+      for (int i = 0; i < wl; ++i)
+        for (int u = 0; u < d; ++u)
+          for (int j = 0; j < wr; ++j)
+            for (int t = 0; t < d; ++t)
+              P[lev(i, wl) + (size_t)wm * (u + (size_t)dm * (lev(j, wr) + (size_t)wm * t))] =
+                  W[l][i + (size_t)wl * (u + (size_t)d * (j + (size_t)wr * t))];
+      for (int u = d; u < dm; ++u) {
+        P[0 + (size_t)wm * (u + (size_t)dm * (0 + (size_t)wm * u))] = 1.0;
+        P[(wm - 1) + (size_t)wm * (u + (size_t)dm * ((wm - 1) + (size_t)wm * u))] = 1.0;
+      }
+    }
+  });
+  if (code != IMPS_OK) return code;
+  std::vector<const double*> pp(nsites);
+  for (int l = 0; l < nsites; ++l) pp[l] = sq[l].data();
+  // the square chain goes through imps_set_mpo's checks (the unit corners included)
+  const int rc = imps_set_mpo(c, nsites, wm, dm, pp.data());
+  if (rc == IMPS_OK) c->pdim.assign(dims, dims + nsites);
+  return rc;
+}
+
+// every state tensor (D, d, D) must vanish on the empty physical sectors of its site
+static void require_empty_sectors(imps_ctx* c, int D, const double* const* A, const char* what) {
+  if (!A) return;
+  for (int l = 0; l < c->N; ++l) {
+    if (!A[l] || c->pdim.size() != (size_t)c->N || c->pdim[l] >= c->d) continue;
+    for (int s = c->pdim[l]; s < c->d; ++s)
+      for (int b = 0; b < D; ++b)
+        for (int a = 0; a < D; ++a)
+          require(A[l][a + (size_t)D * (s + (size_t)c->d * b)] == 0.0, IMPS_E_ARG,
+                  std::string(what) + ": a tensor has weight in an empty physical sector of site " +
+                      std::to_string(l));
+  }
 }
 
 int imps_set_state(imps_ctx* c, int nsites, int D, const double* const* AL, const double* const* AR,
@@ -2859,6 +2923,9 @@ int imps_set_state(imps_ctx* c, int nsites, int D, const double* const* AL, cons
     need_mpo(c);
     require(nsites == c->N && D >= 1, IMPS_E_ARG, "imps_set_state: unit cell or D mismatch");
     require(D <= c->d * D, IMPS_E_ARG, "imps_set_state: D exceeds d*D");
+    require_empty_sectors(c, D, AL, "imps_set_state");
+    require_empty_sectors(c, D, AR, "imps_set_state");
+    require_empty_sectors(c, D, AC, "imps_set_state");
     alloc_state(c, nsites, D, c->d);
     const size_t nA = (size_t)D * c->d * D, nC = (size_t)D * D;
     for (int l = 0; l < nsites; ++l) {
@@ -2905,6 +2972,7 @@ int imps_set_state_raw(imps_ctx* c, int nsites, int D, const double* const* A, d
   return run_guarded(c, [&] {
     need_mpo(c);
     require(nsites == c->N && D >= 1 && A, IMPS_E_ARG, "imps_set_state_raw: unit cell mismatch");
+    require_empty_sectors(c, D, A, "imps_set_state_raw");
     alloc_state(c, nsites, D, c->d);
     const size_t nA = (size_t)D * c->d * D, nC = (size_t)D * D;
     // raw tensors are staged in the pending AC buffers (the gauge sweeps read copies)
