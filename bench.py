@@ -42,7 +42,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 # HBM bytes per launch of the H_AC GEMMs measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (FETCH_SIZE scaled by 2.0, the factor a copy kernel of known size shows); keyed by (D, precision,
 # kernel).  profiles/hac_d1024_hubbard_mfma_util_and_traffic.md holds the runs.
-TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 835.4 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 675.7 * 2 ** 20,
+TRAFFIC = {(1024, "fp64", "gemm1_GLx"): 737.2 * 2 ** 20, (1024, "fp64", "gemm2_GR"): 539.5 * 2 ** 20,
            (1024, "bf16x3", "gemm1_GLx"): 309.1 * 2 ** 20, (1024, "bf16x3", "gemm2_GR"): 304.1 * 2 ** 20}
 
 
