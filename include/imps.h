@@ -342,15 +342,51 @@ int imps_idmrg_run(imps_ctx* ctx, const imps_vumps_params* p, imps_vumps_info* i
                    double* losses, int nloss, imps_iter_cb cb, void* user);
 
 /* timestep(ψ, H, dt, alg::TDVP, envs) (src/algorithms/timeevo/tdvp.jl:33-52) for real dt
- * (imaginary time, dt = -τ; the library is real-valued, so real-time steps are not available):
- * every site's AC and C become exp(dt H_AC) AC and exp(dt H_C) C (Lanczos exponential with
- * tolerance tol, krylovdim, maxiter: KrylovKit.exponentiate as used by tdvp.jl:14-19), AL follows
- * from regauge! (ortho.jl:214-219), the state is rebuilt as CanonicalIMPS(ALs, C[end]) with
- * (gauge_tol, gauge_maxiter) and the environments are recalculated.  Builds DMRGCache first when
- * no environments exist.  A non-converged exponential is reported on stderr, as the reference
- * warns. */
+ * (imaginary time, dt = -τ): every site's AC and C become exp(dt H_AC) AC and exp(dt H_C) C
+ * (Lanczos exponential with tolerance tol, krylovdim, maxiter: KrylovKit.exponentiate as used by
+ * tdvp.jl:14-19), AL follows from regauge! (ortho.jl:214-219), the state is rebuilt as
+ * CanonicalIMPS(ALs, C[end]) with (gauge_tol, gauge_maxiter) and the environments are
+ * recalculated.  Builds DMRGCache first when no environments exist.  A non-converged exponential
+ * is reported on stderr, as the reference warns.  Works on real and complex states. */
 int imps_tdvp_timestep(imps_ctx* ctx, double dt, double tol, int krylovdim, int maxiter, double gauge_tol,
                        int gauge_maxiter);
+
+/* The same step for a complex dt = dt_re + i dt_im (tdvp.jl:33-52 with dt = -im t: real-time
+ * evolution exp(-i t H)); IMPS_E_STATE when dt_im != 0 and the state is real (time_evolve,
+ * tdvp.jl:91-112, promotes a real state to complex first: imps_set_state_complex). */
+int imps_tdvp_timestep_complex(imps_ctx* ctx, double dt_re, double dt_im, double tol, int krylovdim, int maxiter,
+                               double gauge_tol, int gauge_maxiter);
+
+/* ---- complex states (CanonicalIMPS{ComplexF64}, canonicalmps.jl:17-40).  Tensors are
+ * ComplexF64 arrays as Julia stores them: column-major (D, d, D) or (D, D), real and imaginary
+ * parts interleaved (2 doubles per entry).  The device holds their exact real embedding (each
+ * entry z as the 2 x 2 block [[Re z, -Im z], [Im z, Re z]], bond dimension 2 D), on which the
+ * real kernels run unchanged: products, adjoints (as transposes) and the positive-diagonal
+ * QR/LQ/Cholesky factors commute with the embedding.  On a context holding a complex state:
+ * find_groundstate (imps_vumps_run), the environments (imps_envs_recalculate), imps_energy,
+ * imps_galerkin, imps_gaugefix_right, both TDVP steps and imps_local_expectation /
+ * imps_correlator with symmetric real operators work as for real states; imps_state_dims reports
+ * the complex bond dimension; the entry points that pass real state-shaped host arrays or need
+ * real arithmetic (imps_get_state, the H_AC / H_C / transfer applies, get/set_env, the
+ * eigensolve and regauge primitives, IDMRG, dot, dominant_env, dense_expectation, the gradients,
+ * checkpointing) return IMPS_E_UNSUPPORTED. */
+
+/* CanonicalIMPS{ComplexF64} from canonical tensors (the complex twin of imps_set_state; AC may
+ * be NULL: AL C). */
+int imps_set_state_complex(imps_ctx* ctx, int nsites, int D, const double* const* AL, const double* const* AR,
+                           const double* const* C, const double* const* AC);
+
+/* CanonicalIMPS(As) for complex raw tensors (canonicalmps.jl:151-170; randomimps(ComplexF64, ...),
+ * constructors.jl:13-22): gaugefix!(:LR) from C0 = I as imps_set_state_raw. */
+int imps_set_state_raw_complex(imps_ctx* ctx, int nsites, int D, const double* const* A, double tol, int maxiter);
+
+/* The complex state's AL, AR, C, AC (any pointer array or entry may be NULL); IMPS_E_STATE for a
+ * real state. */
+int imps_get_state_complex(imps_ctx* ctx, double* const* AL, double* const* AR, double* const* C,
+                           double* const* AC);
+
+/* *flag = 1 when the context holds a complex state, 0 for a real one. */
+int imps_state_is_complex(imps_ctx* ctx, int* flag);
 
 /* correlator(ψ, O1, O2, i, js) (src/observables/correlators.jl:13-42) on the context's state:
  * out[k] = ⟨ψ|O1[i] O2[js[k]]|ψ⟩ / dot(ψ, ψ) for i < js[0] < js[1] < ..., O1 and O2 d x d

@@ -141,6 +141,11 @@ SIGNATURES = [
     ("imps_idmrg_run", c_int, [c_void_p, ctypes.POINTER(VumpsParams), ctypes.POINTER(VumpsInfo), P_double,
                                c_int, ITER_CB, c_void_p]),
     ("imps_tdvp_timestep", c_int, [c_void_p, c_double, c_double, c_int, c_int, c_double, c_int]),
+    ("imps_tdvp_timestep_complex", c_int, [c_void_p, c_double, c_double, c_double, c_int, c_int, c_double, c_int]),
+    ("imps_set_state_complex", c_int, [c_void_p, c_int, c_int, PP_double, PP_double, PP_double, PP_double]),
+    ("imps_set_state_raw_complex", c_int, [c_void_p, c_int, c_int, PP_double, c_double, c_int]),
+    ("imps_get_state_complex", c_int, [c_void_p, PP_double, PP_double, PP_double, PP_double]),
+    ("imps_state_is_complex", c_int, [c_void_p, P_int]),
     ("imps_correlator", c_int, [c_void_p, P_double, P_double, c_int, c_int, c_void_p, P_double]),
     ("imps_dot_state", c_int, [c_void_p, c_int, PP_double, P_double]),
     ("imps_dominant_env", c_int, [c_void_p, c_int, c_int, c_int, c_int, PP_double, c_double, c_int, c_int, P_double,
@@ -210,3 +215,22 @@ def ptr_array(arrays):
 
 def fortran(a):
     return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def zfortran(a):
+    """complex128 Fortran-ordered copy (ComplexF64 layout: re, im interleaved)."""
+    return np.asfortranarray(np.asarray(a, dtype=np.complex128))
+
+
+def zptr(a: np.ndarray):
+    """double* to a complex128 Fortran-ordered array (2 doubles per entry)."""
+    assert a.dtype == np.complex128 and a.flags.f_contiguous
+    return a.ctypes.data_as(P_double)
+
+
+def zptr_array(arrays):
+    """double*[] built from a list of Fortran-ordered complex128 arrays."""
+    arr = (P_double * len(arrays))()
+    for i, a in enumerate(arrays):
+        arr[i] = zptr(a)
+    return ctypes.cast(arr, PP_double), arr

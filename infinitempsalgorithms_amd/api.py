@@ -19,7 +19,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import check, dptr, fortran, ptr_array
+from ._lib import check, dptr, fortran, ptr_array, zfortran, zptr_array
 
 
 # Contexts still open when the interpreter exits are closed by an atexit hook, which runs before
@@ -52,6 +52,7 @@ class Context:
         self.L = L
         self.N = self.w = self.d = self.D = 0
         self.dims, self.levels = [], []
+        self.cplx = False  # the context holds a complex state
         _LIVE.add(self)
 
     def __enter__(self):
@@ -92,11 +93,12 @@ class Context:
     # d = max dims physical indices with zeros in the empty sectors, environments w = max levels.
     # These map the reference's per-site shapes to it and back.
     def pad_site(self, l, x):
-        """(D, dims[l], D) -> (D, d, D) with zeros in the empty sector."""
-        x = fortran(x)
+        """(D, dims[l], D) -> (D, d, D) with zeros in the empty sector (complex arrays stay
+        complex)."""
+        x = zfortran(x) if np.iscomplexobj(x) else fortran(x)
         if not self.d or x.shape[1] == self.d:
             return x
-        out = np.zeros((x.shape[0], self.d, x.shape[2]), order="F")
+        out = np.zeros((x.shape[0], self.d, x.shape[2]), dtype=x.dtype, order="F")
         out[:, :x.shape[1], :] = x
         return out
 
@@ -126,6 +128,8 @@ class Context:
                 raise ValueError(f"{what}: site {l} tensor must be ({D}, {self.dims[l % self.N]}, {D}), got {x.shape}")
 
     def set_state(self, psi: "CanonicalIMPS"):
+        if psi.is_complex():
+            return self._set_state_complex(psi)
         AL, AR, C, AC = ([fortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
         D = psi.D
         if self.d:
@@ -137,15 +141,34 @@ class Context:
         keep = [ptr_array(x) for x in (AL, AR, C, AC)]
         self.call("imps_set_state", len(AL), psi.D, *(k[0] for k in keep))
         self.D = psi.D
+        self.cplx = False
+
+    def _set_state_complex(self, psi: "CanonicalIMPS"):
+        # CanonicalIMPS{ComplexF64}: the library holds the exact real embedding (imps_set_state_complex)
+        AL, AR, C, AC = ([zfortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
+        D = psi.D
+        if self.d:
+            if not (len(AL) == len(AR) == len(C) == len(AC)) or any(x.shape != (D, D) for x in C):
+                raise ValueError(f"set_state: AL, AR, C, AC per site with every C ({D}, {D})")
+            for xs in (AL, AR, AC):
+                self._check_state_shapes("set_state", D, xs)
+            AL, AR, AC = ([self.pad_site(l, x) for l, x in enumerate(xs)] for xs in (AL, AR, AC))
+        keep = [zptr_array(x) for x in (AL, AR, C, AC)]
+        self.call("imps_set_state_complex", len(AL), D, *(k[0] for k in keep))
+        self.D = D
+        self.cplx = True
 
     def set_state_raw(self, As, tol, maxiter):
-        As = [fortran(a) for a in As]
+        cplx = any(np.iscomplexobj(a) for a in As)
+        As = [zfortran(a) if cplx else fortran(a) for a in As]
         D = As[0].shape[0]
         self._check_state_shapes("set_state_raw", D, As)
         As = [self.pad_site(l, a) for l, a in enumerate(As)]
-        pp, keep = ptr_array(As)
-        self.call("imps_set_state_raw", len(As), As[0].shape[0], pp, float(tol), int(maxiter))
+        pp, keep = (zptr_array if cplx else ptr_array)(As)
+        self.call("imps_set_state_raw_complex" if cplx else "imps_set_state_raw", len(As), As[0].shape[0], pp,
+                  float(tol), int(maxiter))
         self.D = As[0].shape[0]
+        self.cplx = cplx
 
     def checkpoint(self, path):
         """Writes the context's state to a state file (imps_checkpoint_state)."""
@@ -157,15 +180,17 @@ class Context:
         n, D, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self.call("imps_state_dims", ctypes.byref(n), ctypes.byref(D), ctypes.byref(d))
         self.D = D.value
+        self.cplx = False
 
     def get_state(self):
         N, D, d = self.N, self.D, self.d
-        AL = [np.zeros((D, d, D), order="F") for _ in range(N)]
-        AR = [np.zeros((D, d, D), order="F") for _ in range(N)]
-        C = [np.zeros((D, D), order="F") for _ in range(N)]
-        AC = [np.zeros((D, d, D), order="F") for _ in range(N)]
-        keep = [ptr_array(x) for x in (AL, AR, C, AC)]
-        self.call("imps_get_state", *(k[0] for k in keep))
+        dt = np.complex128 if self.cplx else np.float64
+        AL = [np.zeros((D, d, D), dtype=dt, order="F") for _ in range(N)]
+        AR = [np.zeros((D, d, D), dtype=dt, order="F") for _ in range(N)]
+        C = [np.zeros((D, D), dtype=dt, order="F") for _ in range(N)]
+        AC = [np.zeros((D, d, D), dtype=dt, order="F") for _ in range(N)]
+        keep = [(zptr_array if self.cplx else ptr_array)(x) for x in (AL, AR, C, AC)]
+        self.call("imps_get_state_complex" if self.cplx else "imps_get_state", *(k[0] for k in keep))
         AL, AR, AC = ([self.unpad_site(l, x) for l, x in enumerate(xs)] for xs in (AL, AR, AC))
         return CanonicalIMPS(AL, AR, C, AC)
 
@@ -277,15 +302,32 @@ class CanonicalIMPS:
     def copy(self):
         return CanonicalIMPS(*[[x.copy(order="F") for x in xs] for xs in (self.AL, self.AR, self.C, self.AC)])
 
+    def is_complex(self):
+        """scalartype(ψ) <: Complex."""
+        return any(np.iscomplexobj(x) for xs in (self.AL, self.AR, self.C, self.AC) for x in xs)
+
+    def complex(self):
+        """complex(ψ) (tdvp.jl:97-100 promotes a real chain before real-time steps)."""
+        return CanonicalIMPS(*[[np.asfortranarray(x, dtype=np.complex128) for x in xs]
+                               for xs in (self.AL, self.AR, self.C, self.AC)])
+
     def norm(self):
         """norm(ψ) = ‖AC[1]‖ (canonicalmps.jl:238)."""
         return float(np.linalg.norm(self.AC[0]))
 
 
-def random_tensors(D, phydims, seed):
+def random_tensors(D, phydims, seed, dtype=np.float64):
     """Raw ψ₀ tensors: N(0,1) entries from numpy PCG64(seed), site by site, column-major
-    (the role of randn in randomimps, src/states/constructors.jl:13-22)."""
+    (the role of randn in randomimps, src/states/constructors.jl:13-22); for a complex dtype each
+    site draws the real parts, then the imaginary parts."""
     rng = np.random.Generator(np.random.PCG64(seed))
+    if np.issubdtype(np.dtype(dtype), np.complexfloating):
+        out = []
+        for d in phydims:
+            re = rng.standard_normal(D * d * D)
+            im = rng.standard_normal(D * d * D)
+            out.append(np.reshape(re + 1j * im, (D, d, D), order="F"))
+        return out
     return [np.reshape(rng.standard_normal(D * d * D), (D, d, D), order="F") for d in phydims]
 
 
@@ -298,9 +340,10 @@ def canonicalimps(As, H: SparseIMPO, ctx: Context | None = None, tol=1e-13, maxi
     return ctx.get_state(), ctx
 
 
-def randomimps(D, H: SparseIMPO, seed=0, ctx: Context | None = None, **kw):
-    """randomimps(Float64, phydims(H); D) (constructors.jl:13-22) with the seeded generator."""
-    return canonicalimps(random_tensors(D, H.phydims(), seed), H, ctx, **kw)
+def randomimps(D, H: SparseIMPO, seed=0, ctx: Context | None = None, dtype=np.float64, **kw):
+    """randomimps(T, phydims(H); D) (constructors.jl:13-22) with the seeded generator; T is
+    Float64 or ComplexF64 (dtype=complex)."""
+    return canonicalimps(random_tensors(D, H.phydims(), seed, dtype), H, ctx, **kw)
 
 
 # ---------------------------------------------------------------------------- environments
@@ -488,7 +531,9 @@ def expectationvalue(psi: CanonicalIMPS, op, envs: GPUCache):
         envs.ctx.call("imps_energy", ctypes.byref(val))
     else:
         site, O = op
-        O = fortran(O)
+        if np.iscomplexobj(O) and np.any(np.imag(O) != 0):
+            raise ValueError("expectationvalue: the library takes real operators")
+        O = fortran(np.real(O))
         dl = psi.AL[int(site) % len(psi)].shape[1]
         if O.shape != (dl, dl):
             raise ValueError(f"expectationvalue: the operator at site {site} must be {dl} x {dl}")
@@ -1063,18 +1108,26 @@ class TDVP:
 
 
 def timestep(psi: CanonicalIMPS, H: SparseIMPO, dt, alg: TDVP | None = None, envs: GPUCache | None = None):
-    """timestep(ψ, H, dt, alg::TDVP, envs) (tdvp.jl:33-52) for real dt (imaginary time,
-    dt = -τ): returns (ψ′, envs)."""
+    """timestep(ψ, H, dt, alg::TDVP, envs) (tdvp.jl:33-52): dt is the coefficient of the
+    exponent, exp(dt H) — real dt = -τ for imaginary time, dt = -im t for real time.  A real ψ
+    with a non-real dt is promoted to complex (the Julia step promotes through exponentiate).
+    Returns (ψ′, envs)."""
     alg = alg or TDVP()
-    if isinstance(dt, complex) or np.iscomplexobj(dt):
-        raise ValueError("timestep: the library is real-valued; real-time steps (complex dt) are not supported")
+    dt = complex(dt)
+    if dt.imag != 0.0 and not psi.is_complex():
+        psi = psi.complex()
     if envs is None:
         envs = DMRGCache(psi, H)
     elif psi is not envs.ket:
         envs.ctx.set_state(psi)
+        envs.ket = psi
         envs.ctx.call("imps_envs_reset")
-    envs.ctx.call("imps_tdvp_timestep", float(dt), float(alg.tol), int(alg.krylovdim), int(alg.maxiter),
-                  float(alg.gauge_tol), int(alg.gauge_maxiter))
+    if dt.imag == 0.0:
+        envs.ctx.call("imps_tdvp_timestep", float(dt.real), float(alg.tol), int(alg.krylovdim), int(alg.maxiter),
+                      float(alg.gauge_tol), int(alg.gauge_maxiter))
+    else:
+        envs.ctx.call("imps_tdvp_timestep_complex", float(dt.real), float(dt.imag), float(alg.tol),
+                      int(alg.krylovdim), int(alg.maxiter), float(alg.gauge_tol), int(alg.gauge_maxiter))
     psi2 = envs.ctx.get_state()
     envs.ket = psi2
     return psi2, envs
@@ -1082,13 +1135,21 @@ def timestep(psi: CanonicalIMPS, H: SparseIMPO, dt, alg: TDVP | None = None, env
 
 def time_evolve(psi0: CanonicalIMPS, H: SparseIMPO, t_span, alg: TDVP | None = None, envs: GPUCache | None = None,
                 observer=None):
-    """time_evolve(ψ₀, H, t_span, alg::TDVP, envs; observer) (tdvp.jl:91-112) for real steps
-    dt = t_span[i+1] - t_span[i]: returns (ψ, envs, history) with observer(ψ, iter, t) (default:
-    the cell energy) at every point of t_span."""
+    """time_evolve(ψ₀, H, t_span, alg::TDVP, envs; observer) (tdvp.jl:91-112): steps
+    dt = t_span[i+1] - t_span[i] (real: imaginary time; -im t: real time, the state promoted to
+    complex first); returns (ψ, envs, history) with observer(ψ, iter, t) (default: the cell
+    energy) at every point of t_span."""
     alg = alg or TDVP()
-    if envs is None:
-        envs = DMRGCache(psi0, H)
     psi = psi0
+    if len(t_span) > 1 and complex(t_span[1] - t_span[0]).imag != 0.0 and not psi.is_complex():
+        psi = psi.complex()
+    if envs is None:
+        envs = DMRGCache(psi, H)
+    elif psi is not envs.ket:
+        envs.ctx.set_state(psi)
+        envs.ket = psi
+        envs.ctx.call("imps_envs_reset")
+        envs.ctx.call("imps_envs_recalculate", 1e-10, 300, 30)
 
     def obs(p, it, t):
         return expectationvalue(p, H, envs) if observer is None else observer(p, it, t)

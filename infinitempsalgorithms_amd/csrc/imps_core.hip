@@ -12,6 +12,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -78,6 +79,11 @@ struct imps_ctx {
   // state
   int D = 0;
   std::vector<double*> AL, AR, AC, C, pAC, pC, pAL;
+  // the state is complex: AL, AR, AC, C (and the environments) hold the real embedding of the
+  // complex tensors of bond dimension D / 2 (launch_cplx_embed), C and AC scaled by 1/sqrt(2) so
+  // that every trace-normalised quantity (energy, Galerkin error, expectation values) equals the
+  // complex one
+  bool cplx = false;
   // environments
   bool envs_valid = false;
   std::vector<double*> GL, GR;
@@ -531,6 +537,21 @@ void gemm2(imps_ctx* c, char ta, char tb, int M, int N, int K, double alpha, con
 
 void need_mpo(imps_ctx* c) { require(c->N > 0 && c->w > 0, IMPS_E_STATE, "MPO not set"); }
 void need_state(imps_ctx* c) { require(c->D > 0 && !c->AL.empty(), IMPS_E_STATE, "state not set"); }
+// entry points whose host arrays or arithmetic are real-only refuse a complex state
+void require_real_state(imps_ctx* c, const char* what) {
+  require(!c->cplx, IMPS_E_UNSUPPORTED,
+          std::string(what) + ": not available for a complex state (real host arrays or real arithmetic only)");
+}
+// on a complex state the real kernels return the real part of an expectation value, which is the
+// whole value for a symmetric (Hermitian) real operator; other operators are refused there
+void require_symmetric_if_complex(imps_ctx* c, const double* O, const char* what) {
+  if (!c->cplx) return;
+  const int d = c->d;
+  for (int j = 0; j < d; ++j)
+    for (int i = 0; i < j; ++i)
+      require(O[i + (size_t)d * j] == O[j + (size_t)d * i], IMPS_E_UNSUPPORTED,
+              std::string(what) + ": a complex state takes symmetric (Hermitian) real operators only");
+}
 void need_envs(imps_ctx* c) { require(c->envs_valid, IMPS_E_STATE, "environments not computed"); }
 void need_match(imps_ctx* c) {
   need_mpo(c);
@@ -1226,6 +1247,105 @@ KInfo lanczos_expm(imps_ctx* c, const Op& op, size_t n, const double* x0, double
     remaining -= tau;
     if (remaining == 0.0 || !converged || steps >= maxiter) {
       converged = converged && remaining == 0.0;
+      break;
+    }
+  }
+  axpby(c, n, scale, Vc(0), 0.0, xout);
+  info.converged = converged ? 1 : 0;
+  info.numiter = steps;
+  return info;
+}
+
+// exp(t op) x0 for complex t on the real embedding of a complex vector (rows in re/im pairs, see
+// launch_cplx_embed): op is the embedding of a Hermitian operator, so the Lanczos basis and the
+// tridiagonal T are real and exp(t T) e1 has complex coefficients y; the result is
+// V Re(y) + i V Im(y), with i applied by launch_cplx_axpy_i.  Same acceptance and time-splitting
+// rules as lanczos_expm (the integrate of tdvp.jl:14-19 with a complex dt).
+KInfo lanczos_expm_c(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xout, std::complex<double> t,
+                     double tol, int kdim, int maxiter) {
+  using cd = std::complex<double>;
+  const int kmax = (int)std::min<size_t>(kdim, n);
+  require(kmax >= 1 && kmax < kMaxVec, IMPS_E_ARG, "exponentiate: krylovdim out of range");
+  require(n % 2 == 0, IMPS_E_ARG, "exponentiate: complex step on an odd-length vector");
+  double* V = ws(c, "lx_V", (size_t)(kmax + 1) * n);
+  double* tmp = ws(c, "lx_tmp", n);
+  double* tmpi = ws(c, "lx_tmpi", n);
+  auto Vc = [&](int k) { return V + (size_t)k * n; };
+  const int ldt = kmax + 1;
+  std::vector<double> T((size_t)ldt * ldt), h(kmax + 1);
+  double scale = nrm2(c, n, x0);
+  require(scale > 0 && std::isfinite(scale), IMPS_E_ARG, "exponentiate: zero or non-finite start vector");
+  axpby(c, n, 1.0 / scale, x0, 0.0, Vc(0));
+  cd remaining = t;
+  KInfo info;
+  bool converged = true;
+  int steps = 0;
+  while (true) {
+    std::fill(T.begin(), T.end(), 0.0);
+    int K = 0;
+    double beta = 0.0;
+    cd tau = remaining;
+    std::vector<cd> y;
+    bool accepted = false;
+    while (true) {
+      op(Vc(K), Vc(K + 1));
+      info.numops++;
+      cgs2_enqueue(c, n, K + 1, V, Vc(K + 1));
+      cgs2_wait(c, K + 1, h.data(), &beta);
+      for (int i = 0; i <= K; ++i) {
+        T[i + (size_t)ldt * K] = h[i];
+        T[K + (size_t)ldt * i] = h[i];
+      }
+      K += 1;
+      std::vector<double> A((size_t)K * K), theta(K), U((size_t)K * K);
+      for (int j = 0; j < K; ++j)
+        for (int i = 0; i < K; ++i) A[i + (size_t)K * j] = 0.5 * (T[i + (size_t)ldt * j] + T[j + (size_t)ldt * i]);
+      sym_eig(K, A.data(), theta.data(), U.data());
+      auto y_of = [&](cd tt) {
+        std::vector<cd> r(K, cd(0.0, 0.0));
+        for (int j = 0; j < K; ++j) {
+          const cd f = std::exp(tt * theta[j]) * U[(size_t)K * j];
+          for (int i = 0; i < K; ++i) r[i] += U[i + (size_t)K * j] * f;
+        }
+        return r;
+      };
+      y = y_of(remaining);
+      if (std::abs(beta * y[K - 1]) <= tol || beta == 0.0) {
+        accepted = true;
+        tau = remaining;
+        break;
+      }
+      if (K == kmax) {
+        cd tt = remaining;
+        for (int j = 0; j < 60; ++j) {
+          tt *= 0.5;
+          std::vector<cd> yt = y_of(tt);
+          if (std::abs(beta * yt[K - 1]) <= tol) {
+            accepted = true;
+            tau = tt;
+            y = yt;
+            break;
+          }
+        }
+        break;
+      }
+    }
+    steps++;
+    if (!accepted) converged = false;
+    std::vector<double> yr(K), yi(K);
+    for (int i = 0; i < K; ++i) {
+      yr[i] = y[i].real();
+      yi[i] = y[i].imag();
+    }
+    multiaxpy(c, n, K, V, n, yr.data(), 0.0, tmp);
+    multiaxpy(c, n, K, V, n, yi.data(), 0.0, tmpi);
+    HIPCHK(launch_cplx_axpy_i(n, 1.0, tmpi, tmp, c->stream));
+    const double ny = nrm2(c, n, tmp);
+    scale *= ny;
+    axpby(c, n, 1.0 / ny, tmp, 0.0, Vc(0));
+    remaining -= tau;
+    if (remaining == cd(0.0, 0.0) || !converged || steps >= maxiter) {
+      converged = converged && remaining == cd(0.0, 0.0);
       break;
     }
   }
@@ -2470,6 +2590,7 @@ void free_state(imps_ctx* c) {
   for (auto* v : {&c->AL, &c->AR, &c->AC, &c->C, &c->pAC, &c->pC, &c->pAL, &c->GL, &c->GR}) v->clear();
   c->envs_valid = false;
   c->D = 0;
+  c->cplx = false;
 }
 
 void alloc_state(imps_ctx* c, int N, int D, int d) {
@@ -2946,7 +3067,7 @@ int imps_state_dims(imps_ctx* c, int* nsites, int* D, int* d) {
   return run_guarded(c, [&] {
     need_state(c);
     *nsites = (int)c->AL.size();
-    *D = c->D;
+    *D = c->cplx ? c->D / 2 : c->D;
     *d = c->d;
   });
 }
@@ -2955,6 +3076,7 @@ int imps_get_state(imps_ctx* c, double* const* AL, double* const* AR, double* co
                    double* const* AC) {
   if (!c) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_get_state");
     need_state(c);
     const int N = (int)c->AL.size();
     const size_t nA = (size_t)c->D * c->d * c->D, nC = (size_t)c->D * c->D;
@@ -2994,9 +3116,134 @@ int imps_set_state_raw(imps_ctx* c, int nsites, int D, const double* const* A, d
   });
 }
 
+// ---- complex states: the real embedding (2 D, d, 2 D) of the caller's ComplexF64 tensors
+static void require_empty_sectors_c(imps_ctx* c, int D, const double* const* A, const char* what) {
+  if (!A) return;
+  for (int l = 0; l < c->N; ++l) {
+    if (!A[l] || c->pdim.size() != (size_t)c->N || c->pdim[l] >= c->d) continue;
+    for (int s = c->pdim[l]; s < c->d; ++s)
+      for (int b = 0; b < D; ++b)
+        for (int a = 0; a < D; ++a) {
+          const size_t k = 2 * (a + (size_t)D * (s + (size_t)c->d * b));
+          require(A[l][k] == 0.0 && A[l][k + 1] == 0.0, IMPS_E_ARG,
+                  std::string(what) + ": a tensor has weight in an empty physical sector of site " +
+                      std::to_string(l));
+        }
+  }
+}
+
+// host complex (D1, m, D2) -> embedded device tensor
+static void embed_from_host(imps_ctx* c, const double* Z, int D1, int m, int D2, double scale, double* R) {
+  const size_t n2 = (size_t)2 * D1 * m * D2;
+  double* stagez = ws(c, "cx_stage", n2);
+  h2d(c, stagez, Z, n2);
+  HIPCHK(launch_cplx_embed(D1, m, D2, stagez, scale, R, c->stream));
+}
+
+// embedded device tensor -> host complex (D1, m, D2)
+static void extract_to_host(imps_ctx* c, double* R, int D1, int m, int D2, double scale, double* Z) {
+  const size_t n2 = (size_t)2 * D1 * m * D2;
+  double* stagez = ws(c, "cx_stage", n2);
+  HIPCHK(launch_cplx_extract(D1, m, D2, R, scale, stagez, false, c->stream));
+  d2h(c, Z, stagez, n2);
+}
+
+// projects every state tensor back onto the embedded form (removes rounding drift out of it)
+static void project_state_c(imps_ctx* c) {
+  const int Dc = c->D / 2, d = c->d;
+  for (size_t l = 0; l < c->AL.size(); ++l) {
+    HIPCHK(launch_cplx_extract(Dc, d, Dc, c->AL[l], 1.0, nullptr, true, c->stream));
+    HIPCHK(launch_cplx_extract(Dc, d, Dc, c->AR[l], 1.0, nullptr, true, c->stream));
+    HIPCHK(launch_cplx_extract(Dc, d, Dc, c->AC[l], 1.0, nullptr, true, c->stream));
+    HIPCHK(launch_cplx_extract(Dc, 1, Dc, c->C[l], 1.0, nullptr, true, c->stream));
+  }
+}
+
+int imps_set_state_complex(imps_ctx* c, int nsites, int D, const double* const* AL, const double* const* AR,
+                           const double* const* Cm, const double* const* AC) {
+  if (!c || !AL || !AR || !Cm) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_mpo(c);
+    require(nsites == c->N && D >= 1, IMPS_E_ARG, "imps_set_state_complex: unit cell or D mismatch");
+    for (int l = 0; l < nsites; ++l)
+      require(AL[l] && AR[l] && Cm[l] && (!AC || AC[l]), IMPS_E_ARG, "imps_set_state_complex: null tensor");
+    require_empty_sectors_c(c, D, AL, "imps_set_state_complex");
+    require_empty_sectors_c(c, D, AR, "imps_set_state_complex");
+    require_empty_sectors_c(c, D, AC, "imps_set_state_complex");
+    alloc_state(c, nsites, 2 * D, c->d);
+    c->cplx = true;
+    const double h = 1.0 / std::sqrt(2.0);
+    for (int l = 0; l < nsites; ++l) {
+      embed_from_host(c, AL[l], D, c->d, D, 1.0, c->AL[l]);
+      embed_from_host(c, AR[l], D, c->d, D, 1.0, c->AR[l]);
+      embed_from_host(c, Cm[l], D, 1, D, h, c->C[l]);
+      if (AC)
+        embed_from_host(c, AC[l], D, c->d, D, h, c->AC[l]);
+      else
+        mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_set_state_raw_complex(imps_ctx* c, int nsites, int D, const double* const* A, double tol, int maxiter) {
+  if (!c || !A) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_mpo(c);
+    require(nsites == c->N && D >= 1, IMPS_E_ARG, "imps_set_state_raw_complex: unit cell mismatch");
+    for (int l = 0; l < nsites; ++l) require(A[l] != nullptr, IMPS_E_ARG, "imps_set_state_raw_complex: null tensor");
+    require_empty_sectors_c(c, D, A, "imps_set_state_raw_complex");
+    const int D2 = 2 * D;
+    alloc_state(c, nsites, D2, c->d);
+    c->cplx = true;
+    const size_t nC = (size_t)D2 * D2;
+    std::vector<double*> raw(nsites);
+    for (int l = 0; l < nsites; ++l) {
+      raw[l] = c->pAC[l];
+      embed_from_host(c, A[l], D, c->d, D, 1.0, raw[l]);
+    }
+    double* I = ws(c, "sr_I", nC);
+    HIPCHK(launch_set_identity(D2, I, D2, c->stream));
+    uniform_leftorth(c, raw, I, tol, maxiter);
+    double* C0 = ws(c, "sr_C0", nC);
+    d2d(c, C0, c->C[nsites - 1], nC);
+    uniform_rightorth(c, c->AL, C0, tol, maxiter);
+    for (int l = 0; l < nsites; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+    project_state_c(c);
+    c->envs_valid = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int imps_get_state_complex(imps_ctx* c, double* const* AL, double* const* AR, double* const* Cm,
+                           double* const* AC) {
+  if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    require(c->cplx, IMPS_E_STATE, "imps_get_state_complex: the state is real (imps_get_state)");
+    const int N = (int)c->AL.size(), Dc = c->D / 2, d = c->d;
+    const double s2 = std::sqrt(2.0);
+    for (int l = 0; l < N; ++l) {
+      if (AL && AL[l]) extract_to_host(c, c->AL[l], Dc, d, Dc, 1.0, AL[l]);
+      if (AR && AR[l]) extract_to_host(c, c->AR[l], Dc, d, Dc, 1.0, AR[l]);
+      if (Cm && Cm[l]) extract_to_host(c, c->C[l], Dc, 1, Dc, s2, Cm[l]);
+      if (AC && AC[l]) extract_to_host(c, c->AC[l], Dc, d, Dc, s2, AC[l]);
+    }
+  });
+}
+
+int imps_state_is_complex(imps_ctx* c, int* flag) {
+  if (!c || !flag) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    *flag = c->cplx ? 1 : 0;
+  });
+}
+
 int imps_hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
   if (!c || !x || !y) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_hac_apply");
     need_match(c);
     need_envs(c);
     const size_t n = (size_t)c->D * c->d * c->D;
@@ -3011,6 +3258,7 @@ int imps_hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec) 
 int imps_hac_apply_dev(imps_ctx* c, int site, const double* d_x, double* d_y, int prec) {
   if (!c || !d_x || !d_y) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_hac_apply_dev");
     need_match(c);
     need_envs(c);
     hac_apply(c, site, d_x, d_y, prec);
@@ -3020,6 +3268,7 @@ int imps_hac_apply_dev(imps_ctx* c, int site, const double* d_x, double* d_y, in
 int imps_hac_apply_noalloc(imps_ctx* c, int site, const double* d_x, double* d_y, int prec) {
   if (!c || !d_x || !d_y) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_hac_apply_noalloc");
     need_match(c);
     need_envs(c);
     require(prec == PREC_FP64 || prec == PREC_FP32, IMPS_E_UNSUPPORTED,
@@ -3042,6 +3291,7 @@ int imps_hac_apply_noalloc(imps_ctx* c, int site, const double* d_x, double* d_y
 int imps_hc_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
   if (!c || !x || !y) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_hc_apply");
     need_match(c);
     need_envs(c);
     const size_t n = (size_t)c->D * c->D;
@@ -3056,6 +3306,7 @@ int imps_hc_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
 int imps_hc_apply_dev(imps_ctx* c, int site, const double* d_x, double* d_y, int prec) {
   if (!c || !d_x || !d_y) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_hc_apply_dev");
     need_match(c);
     need_envs(c);
     hc_apply(c, site, d_x, d_y, prec);
@@ -3065,6 +3316,7 @@ int imps_hc_apply_dev(imps_ctx* c, int site, const double* d_x, double* d_y, int
 int imps_transfer_apply(imps_ctx* c, int side, const double* v, double* out) {
   if (!c || !v || !out) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_transfer_apply");
     need_state(c);
     require(side == IMPS_SIDE_LEFT || side == IMPS_SIDE_RIGHT, IMPS_E_ARG, "imps_transfer_apply: bad side");
     const int N = (int)c->AL.size(), D = c->D;
@@ -3121,6 +3373,7 @@ int imps_envs_reset(imps_ctx* c) {
 int imps_get_env(imps_ctx* c, int site, int side, double* out) {
   if (!c || !out) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_get_env");
     need_envs(c);
     site = wrap(c, site);
     const size_t n = (size_t)c->D * c->w * c->D;
@@ -3131,6 +3384,7 @@ int imps_get_env(imps_ctx* c, int site, int side, double* out) {
 int imps_set_env(imps_ctx* c, int site, int side, const double* in) {
   if (!c || !in) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_set_env");
     need_match(c);
     alloc_envs(c);
     site = wrap(c, site);
@@ -3164,6 +3418,7 @@ int imps_eigsolve_ac(imps_ctx* c, int site, const imps_krylov_params* p, double*
                      imps_krylov_info* info) {
   if (!c || !p) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_eigsolve_ac");
     need_match(c);
     need_envs(c);
     site = wrap(c, site);
@@ -3179,6 +3434,7 @@ int imps_eigsolve_c(imps_ctx* c, int site, const imps_krylov_params* p, double* 
                     imps_krylov_info* info) {
   if (!c || !p) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_eigsolve_c");
     need_match(c);
     need_envs(c);
     site = wrap(c, site);
@@ -3193,6 +3449,7 @@ int imps_eigsolve_c(imps_ctx* c, int site, const imps_krylov_params* p, double* 
 int imps_regauge(imps_ctx* c, int site, double* al) {
   if (!c) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_regauge");
     need_state(c);
     site = wrap(c, site);
     regauge_site(c, site);
@@ -3229,6 +3486,7 @@ int imps_energy(imps_ctx* c, double* e) {
 int imps_energy_gradient(imps_ctx* c, double* const* dW) {
   if (!c || !dW) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_energy_gradient");
     need_match(c);
     need_envs(c);
     require(c->w * c->d <= 32, IMPS_E_UNSUPPORTED, "imps_energy_gradient: w*d > 32 is not supported");
@@ -3240,6 +3498,7 @@ int imps_fixedpoint_gradient(imps_ctx* c, int wo, int d, const double* Wobj, dou
                              double* value, double* dW, imps_response_info* info) {
   if (!c || !Wobj || !dW) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_fixedpoint_gradient");
     need_match(c);
     need_envs(c);
     require(c->N == 1, IMPS_E_UNSUPPORTED, "imps_fixedpoint_gradient: one-site unit cell only");
@@ -3306,6 +3565,7 @@ int imps_fixedpoint_gradient(imps_ctx* c, int wo, int d, const double* Wobj, dou
 int imps_tangent_hessian_apply(imps_ctx* c, double rcond, double tol, const double* B, double* out) {
   if (!c || !B || !out) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_tangent_hessian_apply");
     need_match(c);
     need_envs(c);
     require(c->N == 1, IMPS_E_UNSUPPORTED, "imps_tangent_hessian_apply: one-site unit cell only");
@@ -3325,6 +3585,7 @@ int imps_local_expectation(imps_ctx* c, int site, const double* O, double* val) 
   if (!c || !O || !val) return IMPS_E_ARG;
   return run_guarded(c, [&] {
     need_match(c);
+    require_symmetric_if_complex(c, O, "imps_local_expectation");
     site = wrap(c, site);
     const size_t n = (size_t)c->D * c->d * c->D;
     double* t = ws(c, "lx_t", n);
@@ -3531,6 +3792,7 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
                    imps_iter_cb cb, void* user) {
   if (!c || !p) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_idmrg_run");
     need_match(c);
     const int N = c->N, D = c->D, d = c->d, w = c->w;
     const long hac0 = c->hac_applies;
@@ -3624,30 +3886,51 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
   });
 }
 
+// One TDVP step (tdvp.jl:33-52); a complex dt needs a complex state (the embedded form), whose
+// tensors are projected back onto the embedded form at the end.
+static void tdvp_step(imps_ctx* c, std::complex<double> dt, double tol, int krylovdim, int maxiter,
+                      double gauge_tol, int gauge_maxiter) {
+  need_match(c);
+  require(std::isfinite(dt.real()) && std::isfinite(dt.imag()), IMPS_E_ARG, "imps_tdvp_timestep: dt must be finite");
+  require(dt.imag() == 0.0 || c->cplx, IMPS_E_STATE,
+          "imps_tdvp_timestep: a complex dt (real-time step) needs a complex state (imps_set_state_complex)");
+  const int N = c->N, D = c->D, d = c->d;
+  // envs = DMRGCache(psi, H) when none exist (tdvp.jl:34)
+  if (!c->envs_valid) envs_recalculate(c, 1.0e-10, 300, 30, nullptr);
+  const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
+  bool ok = true;
+  for (int loc = 0; loc < N; ++loc) {
+    Op hac = [&](const double* x, double* y) { hac_apply(c, loc, x, y, PREC_FP64); };
+    Op hc = [&](const double* x, double* y) { hc_apply(c, loc, x, y, PREC_FP64); };
+    if (dt.imag() == 0.0) {
+      ok = lanczos_expm(c, hac, nA, c->AC[loc], c->pAC[loc], dt.real(), tol, krylovdim, maxiter).converged && ok;
+      ok = lanczos_expm(c, hc, nC, c->C[loc], c->pC[loc], dt.real(), tol, krylovdim, maxiter).converged && ok;
+    } else {
+      ok = lanczos_expm_c(c, hac, nA, c->AC[loc], c->pAC[loc], dt, tol, krylovdim, maxiter).converged && ok;
+      ok = lanczos_expm_c(c, hc, nC, c->C[loc], c->pC[loc], dt, tol, krylovdim, maxiter).converged && ok;
+    }
+  }
+  if (!ok) fprintf(stderr, "imps: TDVP integrate: exponentiate not converged (dt = %g%+gi)\n", dt.real(), dt.imag());
+  // ALs = regauge!(ACs, Cs) (ortho.jl:214-219); CanonicalIMPS(ALs, C[end]) = gaugefix!(:R)
+  // (canonicalmps.jl:180-195); recalculate!(envs, psi', H)
+  for (int loc = 0; loc < N; ++loc) regauge_site(c, loc);
+  gauge_step(c, gauge_tol, gauge_maxiter, nullptr);
+  if (c->cplx) project_state_c(c);
+  envs_recalculate(c, 1.0e-10, 300, 30, nullptr);
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 int imps_tdvp_timestep(imps_ctx* c, double dt, double tol, int krylovdim, int maxiter, double gauge_tol,
                        int gauge_maxiter) {
   if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] { tdvp_step(c, std::complex<double>(dt, 0.0), tol, krylovdim, maxiter, gauge_tol, gauge_maxiter); });
+}
+
+int imps_tdvp_timestep_complex(imps_ctx* c, double dt_re, double dt_im, double tol, int krylovdim, int maxiter,
+                               double gauge_tol, int gauge_maxiter) {
+  if (!c) return IMPS_E_ARG;
   return run_guarded(c, [&] {
-    need_match(c);
-    require(std::isfinite(dt), IMPS_E_ARG, "imps_tdvp_timestep: dt must be finite");
-    const int N = c->N, D = c->D, d = c->d;
-    // envs = DMRGCache(psi, H) when none exist (tdvp.jl:34)
-    if (!c->envs_valid) envs_recalculate(c, 1.0e-10, 300, 30, nullptr);
-    const size_t nA = (size_t)D * d * D, nC = (size_t)D * D;
-    bool ok = true;
-    for (int loc = 0; loc < N; ++loc) {
-      Op hac = [&](const double* x, double* y) { hac_apply(c, loc, x, y, PREC_FP64); };
-      Op hc = [&](const double* x, double* y) { hc_apply(c, loc, x, y, PREC_FP64); };
-      ok = lanczos_expm(c, hac, nA, c->AC[loc], c->pAC[loc], dt, tol, krylovdim, maxiter).converged && ok;
-      ok = lanczos_expm(c, hc, nC, c->C[loc], c->pC[loc], dt, tol, krylovdim, maxiter).converged && ok;
-    }
-    if (!ok) fprintf(stderr, "imps: TDVP integrate: exponentiate not converged (dt = %g)\n", dt);
-    // ALs = regauge!(ACs, Cs) (ortho.jl:214-219); CanonicalIMPS(ALs, C[end]) = gaugefix!(:R)
-    // (canonicalmps.jl:180-195); recalculate!(envs, psi', H)
-    for (int loc = 0; loc < N; ++loc) regauge_site(c, loc);
-    gauge_step(c, gauge_tol, gauge_maxiter, nullptr);
-    envs_recalculate(c, 1.0e-10, 300, 30, nullptr);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    tdvp_step(c, std::complex<double>(dt_re, dt_im), tol, krylovdim, maxiter, gauge_tol, gauge_maxiter);
   });
 }
 
@@ -3655,6 +3938,8 @@ int imps_correlator(imps_ctx* c, const double* O1, const double* O2, int i, int 
   if (!c || !O1 || !O2 || nj < 0 || (nj > 0 && (!js || !out))) return IMPS_E_ARG;
   return run_guarded(c, [&] {
     need_state(c);
+    require_symmetric_if_complex(c, O1, "imps_correlator");
+    require_symmetric_if_complex(c, O2, "imps_correlator");
     const int N = (int)c->AL.size(), D = c->D, d = c->d;
     for (int k = 0; k < nj; ++k)
       require(js[k] > (k == 0 ? i : js[k - 1]), IMPS_E_ARG, "imps_correlator: need i < j1 < j2 < ...");
@@ -3694,6 +3979,7 @@ int imps_correlator(imps_ctx* c, const double* O1, const double* O2, int i, int 
 int imps_dot_state(imps_ctx* c, int nsites, const double* const* AL2, double* out) {
   if (!c || !AL2 || !out) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_dot_state");
     need_state(c);
     const int N = (int)c->AL.size(), D = c->D, d = c->d;
     require(nsites == N, IMPS_E_ARG, "imps_dot_state: unit cells differ");
@@ -3787,6 +4073,7 @@ int imps_dominant_env(imps_ctx* c, int side, int nw, int w, int d, const double*
                       int krylovdim, int maxiter, double* lambda, double* env) {
   if (!c || !lambda || !env || (side != 0 && side != 1)) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_dominant_env");
     need_state(c);
     require(tol > 0 && krylovdim >= 1 && maxiter >= 1, IMPS_E_ARG, "imps_dominant_env: bad solver parameters");
     std::vector<std::vector<double>> Ws;
@@ -3805,6 +4092,7 @@ int imps_dense_expectation(imps_ctx* c, int nw, int w, int d, const double* cons
                            int maxiter, double* value, double* const* GL_out, double* const* GR_out) {
   if (!c || !value) return IMPS_E_ARG;
   return run_guarded(c, [&] {
+    require_real_state(c, "imps_dense_expectation");
     need_state(c);
     require(tol > 0 && krylovdim >= 1 && maxiter >= 1, IMPS_E_ARG,
             "imps_dense_expectation: bad solver parameters");

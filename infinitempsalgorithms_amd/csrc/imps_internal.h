@@ -125,6 +125,14 @@ hipError_t launch_scale_inv_sqrt(size_t n, double* x, const double* sq, hipStrea
 // y = a*x + b*y
 hipError_t launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t s);
 hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s);
+// Complex tensors (D1, m, D2), re/im interleaved, and their real embedding (2 D1, m, 2 D2) with
+// each entry z as the block [[Re z, -Im z], [Im z, Re z]]: R = scale * embed(Z); Z = scale *
+// (nearest complex tensor of R), R projected onto the embedded form when project (Z may be null);
+// y += alpha * i w for embedded tensors of n doubles with an even leading dimension.
+hipError_t launch_cplx_embed(int D1, int m, int D2, const double* Z, double scale, double* R, hipStream_t s);
+hipError_t launch_cplx_extract(int D1, int m, int D2, double* R, double scale, double* Z, bool project,
+                               hipStream_t s);
+hipError_t launch_cplx_axpy_i(size_t n, double alpha, const double* w, double* y, hipStream_t s);
 // diagonal of an n x n matrix (leading dimension ld) += v
 hipError_t launch_add_diag(int n, double v, double* x, long ld, hipStream_t s);
 // x[i + ld*j] = v for i < rows, j < cols

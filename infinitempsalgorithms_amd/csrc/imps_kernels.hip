@@ -1564,4 +1564,77 @@ hipError_t launch_fill_random(size_t n, uint64_t seed, double* x, hipStream_t s)
   return hipGetLastError();
 }
 
+// ---- complex states on the real kernels: each complex entry z of a (D1, m, D2) tensor becomes
+// the real 2x2 block [[Re z, -Im z], [Im z, Re z]] at rows 2a, 2a+1 and columns 2b, 2b+1 of a
+// (2 D1, m, 2 D2) tensor.  The map is an algebra homomorphism (products, adjoints as transposes,
+// and the positive-diagonal QR/LQ/Cholesky factors all commute with it), so the real algorithms
+// run unchanged on the embedded tensors.
+
+// R = scale * embed(Z); Z complex (re, im interleaved), column-major (D1, m, D2)
+__global__ void cplx_embed_kernel(int D1, int m, int D2, const double* __restrict__ Z, double scale,
+                                  double* __restrict__ R) {
+  const size_t n = (size_t)D1 * m * D2;
+  const size_t ldr = (size_t)2 * D1;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t a = idx % D1, kb = idx / D1, k = kb % m, b = kb / m;
+    const double re = scale * Z[2 * idx], im = scale * Z[2 * idx + 1];
+    const size_t c0 = ldr * (k + (size_t)m * (2 * b)), c1 = ldr * (k + (size_t)m * (2 * b + 1));
+    R[2 * a + c0] = re;
+    R[2 * a + 1 + c0] = im;
+    R[2 * a + c1] = -im;
+    R[2 * a + 1 + c1] = re;
+  }
+}
+
+// Z = scale * (the complex tensor nearest to R): each 2x2 block [[p, q], [r, t]] gives
+// ((p + t) / 2, (r - q) / 2); with project, R is also overwritten by the embedding of that value
+__global__ void cplx_extract_kernel(int D1, int m, int D2, double* __restrict__ R, double scale,
+                                    double* __restrict__ Z, bool project) {
+  const size_t n = (size_t)D1 * m * D2;
+  const size_t ldr = (size_t)2 * D1;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t a = idx % D1, kb = idx / D1, k = kb % m, b = kb / m;
+    const size_t c0 = ldr * (k + (size_t)m * (2 * b)), c1 = ldr * (k + (size_t)m * (2 * b + 1));
+    const double p = R[2 * a + c0], r = R[2 * a + 1 + c0], q = R[2 * a + c1], t = R[2 * a + 1 + c1];
+    const double re = 0.5 * (p + t), im = 0.5 * (r - q);
+    if (Z) {
+      Z[2 * idx] = scale * re;
+      Z[2 * idx + 1] = scale * im;
+    }
+    if (project) {
+      R[2 * a + c0] = re;
+      R[2 * a + 1 + c0] = im;
+      R[2 * a + c1] = -im;
+      R[2 * a + 1 + c1] = re;
+    }
+  }
+}
+
+// y += alpha * i w on embedded tensors whose leading (row) dimension is even: multiplying by i is
+// the left product with [[0, -1], [1, 0]] on each row pair
+__global__ void cplx_axpy_i_kernel(size_t npairs, double alpha, const double* __restrict__ w, double* __restrict__ y) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += (size_t)gridDim.x * blockDim.x) {
+    const double w0 = w[2 * k], w1 = w[2 * k + 1];
+    y[2 * k] -= alpha * w1;
+    y[2 * k + 1] += alpha * w0;
+  }
+}
+
+hipError_t launch_cplx_embed(int D1, int m, int D2, const double* Z, double scale, double* R, hipStream_t s) {
+  hipLaunchKernelGGL(cplx_embed_kernel, ew_grid((size_t)D1 * m * D2), dim3(256), 0, s, D1, m, D2, Z, scale, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_cplx_extract(int D1, int m, int D2, double* R, double scale, double* Z, bool project,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(cplx_extract_kernel, ew_grid((size_t)D1 * m * D2), dim3(256), 0, s, D1, m, D2, R, scale, Z,
+                     project);
+  return hipGetLastError();
+}
+
+hipError_t launch_cplx_axpy_i(size_t n, double alpha, const double* w, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(cplx_axpy_i_kernel, ew_grid(n / 2), dim3(256), 0, s, n / 2, alpha, w, y);
+  return hipGetLastError();
+}
+
 }  // namespace imps

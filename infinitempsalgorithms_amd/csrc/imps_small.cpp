@@ -102,7 +102,20 @@ int hess_eigvals(int n, const double* H, int ldh, double* wr, double* wi) {
     while (lo > 0) {
       const double sub = std::abs(A(lo, lo - 1));
       const double dia = std::abs(A(lo, lo)) + std::abs(A(lo - 1, lo - 1));
-      if (sub <= eps * (dia > 0.0 ? dia : hnorm)) {
+      bool negligible = sub <= eps * (dia > 0.0 ? dia : hnorm);
+      if (!negligible && iter >= 30) {
+        // after 30 steps on one window also the criterion of Ahues and Tisseur (as in LAPACK's
+        // xLAHQR), which deflates a nearly defective pair of close eigenvalues: the Hessenberg
+        // matrices of Krylov spaces of operators with paired spectra (complex states in their
+        // real embedding) hold such pairs, where the plain test stalls
+        const double up = std::abs(A(lo - 1, lo));
+        const double ab = std::max(sub, up), ba = std::min(sub, up);
+        const double d1 = std::abs(A(lo, lo)), d2 = std::abs(A(lo - 1, lo - 1) - A(lo, lo));
+        const double aa = std::max(d1, d2), bb = std::min(d1, d2);
+        const double s = aa + ab;
+        negligible = s > 0.0 && ba * (ab / s) <= eps * (bb * (aa / s));
+      }
+      if (negligible) {
         A(lo, lo - 1) = 0.0;
         break;
       }
@@ -112,6 +125,21 @@ int hess_eigvals(int n, const double* H, int ldh, double* wr, double* wi) {
       wr[hi] = A(hi, hi).real();
       wi[hi] = A(hi, hi).imag();
       --hi;
+      iter = 0;
+      continue;
+    }
+    if (lo == hi - 1 && iter >= 30) {
+      // an isolated 2 x 2 block that the shifted steps do not split (a nearly defective pair, whose
+      // subdiagonal stalls near sqrt(eps)): its two eigenvalues in closed form
+      const cplx a11 = A(hi - 1, hi - 1), a12 = A(hi - 1, hi), a21 = A(hi, hi - 1), a22 = A(hi, hi);
+      const cplx m = 0.5 * (a11 + a22), q = 0.5 * (a11 - a22);
+      const cplx r = std::sqrt(q * q + a12 * a21);
+      const cplx l1 = m + r, l2 = m - r;
+      wr[hi] = l2.real();
+      wi[hi] = l2.imag();
+      wr[hi - 1] = l1.real();
+      wi[hi - 1] = l1.imag();
+      hi -= 2;
       iter = 0;
       continue;
     }

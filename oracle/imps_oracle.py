@@ -17,8 +17,9 @@ reshapes):
   (src/algorithms/groundstates/effective.jl:3-5).
 
 Scope: real Float64 data, square Schur tensors with the same level count on every site
-and a uniform bond dimension (the five benchmark configurations).  Complex data,
-rectangular Schur chains and non-uniform bond profiles are outside this restatement.
+and a uniform bond dimension (the five benchmark configurations).  Rectangular Schur chains
+and non-uniform bond profiles are outside this restatement; complex states (real-time TDVP)
+are restated densely in imps_complex.py.
 
 KrylovKit (un-vendored dependency of the reference, unpinned in Project.toml:8) is
 restated from its published algorithms, parameterised like KrylovKit:

@@ -1,0 +1,184 @@
+"""Complex states on the device (CanonicalIMPS{ComplexF64}): state round trips, canonical form,
+energies and TDVP steps with complex dt (real time), checked against the complex restatement
+(oracle/imps_complex.py), the closed-form precession and the reference's real-time TDVP test
+(test/algorithms/tdvp.jl:15-24)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def imps():
+    import infinitempsalgorithms_amd as m
+    return m
+
+
+@pytest.fixture(scope="module")
+def cx():
+    from oracle import imps_complex
+    return imps_complex
+
+
+@pytest.fixture(scope="module")
+def ctx(imps):
+    c = imps.Context(0)
+    yield c
+    c.close()
+
+
+def _complex_raw(oracle, D, ds, seed):
+    re = oracle.random_tensors(D, ds, seed)
+    im = oracle.random_tensors(D, ds, seed + 1000)
+    return [a + 0.4j * b for a, b in zip(re, im)]
+
+
+def _as_lib(imps, p):
+    return imps.CanonicalIMPS([x.copy() for x in p.AL], [x.copy() for x in p.AR], [x.copy() for x in p.C],
+                              [x.copy() for x in p.AC])
+
+
+def test_complex_state_round_trip_and_canonical_form(imps, ctx, oracle, cx):
+    W = oracle.xxz_mpo(1.0, 0.8)
+    H = imps.SparseIMPO([W, W])
+    raw = _complex_raw(oracle, 4, [2, 2], 21)
+    ref = cx.canonical_from_tensors(raw)
+    print("complex: set_state_raw_complex N=2 D=4", flush=True)
+    ctx.set_mpo(H)
+    ctx.set_state_raw(raw, 1e-14, 2000)
+    got = ctx.get_state()
+    assert got.is_complex() and got.D == 4
+    for a, b in zip(got.AL + got.AR + got.C + got.AC, ref.AL + ref.AR + ref.C + ref.AC):
+        assert rel(a, b) < 1e-9
+    # set_state_complex / get_state_complex round trip is exact up to the embedding's averaging
+    ctx.set_state(_as_lib(imps, ref))
+    back = ctx.get_state()
+    for a, b in zip(back.AL + back.AR + back.C + back.AC, ref.AL + ref.AR + ref.C + ref.AC):
+        assert rel(a, b) < 1e-15
+    flag = __import__("ctypes").c_int()
+    ctx.call("imps_state_is_complex", __import__("ctypes").byref(flag))
+    assert flag.value == 1
+
+
+@pytest.mark.parametrize("model,N,D,dt", [("xxz", 2, 4, -0.05j), ("tfim", 1, 6, -0.03 - 0.02j),
+                                          ("xxz", 2, 4, -0.05)])
+def test_complex_tdvp_step_matches_oracle(imps, oracle, cx, model, N, D, dt):
+    W = oracle.xxz_mpo(1.0, 0.8) if model == "xxz" else oracle.tfim_mpo(1.0, 0.9)
+    Ws = [W] * N
+    psi = cx.canonical_from_tensors(_complex_raw(oracle, D, [2] * N, 31 + N))
+    env = cx.dmrg_cache(psi, Ws)
+    e0 = cx.expectation_energy(psi, Ws, env)
+    p2, env2 = cx.tdvp_timestep(psi, Ws, env, dt)
+    e2 = cx.expectation_energy(p2, Ws, env2)
+    print(f"complex: TDVP step {model} N={N} D={D} dt={dt}", flush=True)
+    H = imps.SparseIMPO(Ws)
+    with imps.Context(0) as c:
+        st = _as_lib(imps, psi)
+        envs = imps.DMRGCache(st, H, ctx=c)
+        assert abs(imps.expectationvalue(st, H, envs) - e0.real) < 1e-10
+        g2, envs = imps.timestep(st, H, dt, imps.TDVP(tol=1e-12, gauge_tol=1e-14, gauge_maxiter=2000), envs)
+        assert g2.is_complex()
+        for a, b in zip(g2.AL + g2.AR + g2.C + g2.AC, p2.AL + p2.AR + p2.C + p2.AC):
+            assert rel(a, b) < 1e-8
+        assert abs(imps.expectationvalue(g2, H, envs) - e2.real) < 1e-10
+        assert abs(imps.calc_galerkin(g2, H, envs)) >= 0.0
+
+
+def test_product_state_precession_on_device(imps, oracle, cx):
+    # H = -h sum sz from |+x> (D = 1): TDVP is exact, <sx>(t) = cos 2ht; the states against the
+    # complex restatement fix the sign of i in the exponent (<sy>(t) = -sin 2ht there)
+    h = 0.7
+    W = oracle.tfim_mpo(0.0, h)
+    H = imps.SparseIMPO([W])
+    a = np.zeros((1, 2, 1), dtype=complex)
+    a[0, :, 0] = [1 / math.sqrt(2), 1 / math.sqrt(2)]
+    ref = cx.canonical_from_tensors([a])
+    renv = cx.dmrg_cache(ref, [W])
+    print("complex: precession D=1, 10 real-time steps", flush=True)
+    with imps.Context(0) as c:
+        st = _as_lib(imps, ref)
+        envs = imps.DMRGCache(st, H, ctx=c)
+        sx = oracle.sigma_x()
+        for k in range(1, 11):
+            st, envs = imps.timestep(st, H, -0.1j, imps.TDVP(tol=1e-13), envs)
+            ref, renv = cx.tdvp_timestep(ref, [W], renv, -0.1j)
+            ex, ey = cx.larmor(h, 0.1 * k)
+            assert abs(imps.expectationvalue(st, (0, sx), envs) - ex) < 1e-10
+            assert abs(cx.local_expectation1(st, 0, np.array([[0, -1j], [1j, 0]])) - ey) < 1e-10
+            assert rel(st.AC[0], ref.AC[0]) < 1e-10
+
+
+def test_real_time_from_ground_state_reference_protocol(imps, oracle, cx):
+    # test/algorithms/tdvp.jl:15-24 on the device: XXX, two-site cell, VUMPS(D = 8, tol = 1e-9)
+    # ground state, t_span = -im (0:0.01:0.1): 11 history points, energy kept to 1e-6 and the
+    # norm to 1e-8; the TDVP with explicit gauge parameters gives the same history to 1e-10;
+    # every point equals the complex restatement's energy to 1e-9
+    from infinitempsalgorithms_amd import models
+    m = models.heisenberg_xxz(J=1.0, Delta=1.0)
+    H = imps.SparseIMPO([m.bulk, m.bulk])
+    print("complex: XXX D=8 ground state, then 10 real-time steps", flush=True)
+    with imps.Context(0) as c:
+        g, envs, info = imps.find_groundstate(None, H, imps.VUMPS(D=8, maxiter=200, tol=1e-9), ctx=c, seed=3)
+        e0 = imps.expectationvalue(g, H, envs) / 2
+        tspan = [-1j * 0.01 * k for k in range(11)]
+        obs = lambda p, k, t: imps.expectationvalue(p, H, envs) / 2
+        pt, envs, hist = imps.time_evolve(g, H, tspan, imps.TDVP(), envs, observer=obs)
+        assert len(hist) == 11
+        assert abs(hist[-1] - e0) < 1e-6
+        assert abs(imps.norm(pt) - 1) < 1e-8
+        assert pt.is_complex() and float(np.abs(pt.AC[0].imag).max()) > 1e-4
+        with imps.Context(0) as c2:
+            envs2 = imps.DMRGCache(g, H, ctx=c2)
+            obs2 = lambda p, k, t: imps.expectationvalue(p, H, envs2) / 2
+            _, _, hist2 = imps.time_evolve(g, H, tspan, imps.TDVP(gauge_tol=1e-13, gauge_maxiter=300), envs2,
+                                           observer=obs2)
+        assert np.allclose(hist2, hist, atol=1e-10, rtol=0)
+    ref = cx.from_real(g)
+    renv = cx.dmrg_cache(ref, [m.bulk, m.bulk])
+    hist_o = [cx.expectation_energy(ref, [m.bulk, m.bulk], renv).real / 2]
+    for _ in range(10):
+        ref, renv = cx.tdvp_timestep(ref, [m.bulk, m.bulk], renv, -0.01j)
+        hist_o.append(cx.expectation_energy(ref, [m.bulk, m.bulk], renv).real / 2)
+    assert np.allclose(hist, hist_o, atol=1e-9, rtol=0)
+
+
+def test_complex_vumps_matches_real_ground_state(imps, oracle):
+    # find_groundstate from randomimps(ComplexF64, ...): the complex run reaches the real run's
+    # ground-state energy (XXZ, two-site cell, D = 8)
+    W = oracle.xxz_mpo(1.0, 0.5)
+    H = imps.SparseIMPO([W, W])
+    print("complex: VUMPS XXZ D=8 from a complex random state", flush=True)
+    with imps.Context(0) as c:
+        st, _ = imps.randomimps(8, H, seed=4, ctx=c, dtype=complex)
+        assert st.is_complex()
+        gc, envs, info_c = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-10, maxiter=500), ctx=c)
+        assert gc.is_complex() and info_c.converged
+    with imps.Context(0) as c:
+        sr, _ = imps.randomimps(8, H, seed=4, ctx=c)
+        _, _, info_r = imps.find_groundstate(sr, H, imps.VUMPS(tol=1e-10, maxiter=500), ctx=c)
+    assert abs(info_c.energy - info_r.energy) < 1e-9
+
+
+def test_complex_state_refuses_real_only_entry_points(imps, oracle, cx):
+    from infinitempsalgorithms_amd import _lib
+    W = oracle.tfim_mpo(1.0, 0.9)
+    H = imps.SparseIMPO([W])
+    psi = cx.canonical_from_tensors(_complex_raw(oracle, 4, [2], 41))
+    with imps.Context(0) as c:
+        st = _as_lib(imps, psi)
+        envs = imps.DMRGCache(st, H, ctx=c)
+        with pytest.raises(_lib.ImpsError, match="status -5"):
+            imps.leftenv(envs, 0)
+        with pytest.raises(_lib.ImpsError, match="status -5"):
+            imps.expectationvalue(st, (0, np.array([[0.0, 1.0], [0.0, 0.0]])), envs)
+        with pytest.raises(_lib.ImpsError, match="status -5"):
+            imps.AC_hamiltonian(0, st, H, st, envs)(psi.AC[0].real.copy())
+        # symmetric operators are whole on complex states
+        sz = oracle.sigma_z()
+        assert abs(imps.expectationvalue(st, (0, sz), envs) - cx.local_expectation1(psi, 0, sz).real) < 1e-12
