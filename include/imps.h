@@ -88,8 +88,29 @@ int imps_set_mpo_chain(imps_ctx* ctx, int nsites, const int* dims, const int* le
  * and C of shape (D, D) per site; C[l] is the bond right of site l. */
 int imps_set_state(imps_ctx* ctx, int nsites, int D, const double* const* AL,
                    const double* const* AR, const double* const* C, const double* const* AC);
-/* Unit cell, bond and physical dimension of the context's state (IMPS_E_STATE without one). */
+/* Unit cell, bond and physical dimension of the context's state (IMPS_E_STATE without one); D is
+ * the largest bond dimension when they differ from bond to bond. */
 int imps_state_dims(imps_ctx* ctx, int* nsites, int* D, int* d);
+
+/* CanonicalIMPS with bond dimensions that differ from bond to bond (canonicalmps.jl:17-40; the
+ * per-site Dl/Dr of DMRGCache, envs.jl:327-343): chi[l] is the dimension of the bond right of
+ * site l, D = max chi, and every tensor is passed at (D, d, D) (C at (D, D)) with zeros outside
+ * its active block (rows chi[l-1], columns chi[l]).  Each site must admit a full-rank tensor
+ * (chi[l] <= d chi[l-1], chi[l-1] <= d chi[l]).  The library keeps the zeros: the effective
+ * operators, transfer maps and Krylov solvers map such tensors to such tensors, and the QRpos /
+ * LQpos factorisations of the gauge steps act on the active blocks only, so VUMPS, IDMRG, TDVP and
+ * the observables preserve the profile.  The environments are returned at (D, w, D); their
+ * active block is leftenv(l): chi[l-1], rightenv(l): chi[l]. */
+int imps_set_state_nonuniform(imps_ctx* ctx, int nsites, int D, const int* chi, const double* const* AL,
+                              const double* const* AR, const double* const* C, const double* const* AC);
+
+/* CanonicalIMPS(As) for raw tensors with bond dimensions chi (canonicalmps.jl:151-170):
+ * gaugefix!(:LR) from C0 = I (the active block) as imps_set_state_raw. */
+int imps_set_state_raw_nonuniform(imps_ctx* ctx, int nsites, int D, const int* chi, const double* const* A,
+                                  double tol, int maxiter);
+
+/* bonddim(ψ, l) for every bond (chi[l]: the bond right of site l; all equal for a uniform state). */
+int imps_bond_dims(imps_ctx* ctx, int* chi);
 int imps_get_state(imps_ctx* ctx, double* const* AL, double* const* AR, double* const* C,
                    double* const* AC);
 

@@ -42,6 +42,9 @@ class Context:
     """One libimps context (a HIP stream plus device-resident state) on `device`.  Close it with
     close() (or use it as a context manager); contexts left open are closed at interpreter exit."""
 
+    cplx = False  # the context holds a complex state
+    chi = None    # bond dimensions (bond l right of site l) when they differ from bond to bond
+
     def __init__(self, device=0):
         L = _lib.load()
         h = ctypes.c_void_p()
@@ -53,6 +56,7 @@ class Context:
         self.N = self.w = self.d = self.D = 0
         self.dims, self.levels = [], []
         self.cplx = False  # the context holds a complex state
+        self.chi = None    # bond dimensions (bond l right of site l) when they differ from bond to bond
         _LIVE.add(self)
 
     def __enter__(self):
@@ -93,18 +97,38 @@ class Context:
     # d = max dims physical indices with zeros in the empty sectors, environments w = max levels.
     # These map the reference's per-site shapes to it and back.
     def pad_site(self, l, x):
-        """(D, dims[l], D) -> (D, d, D) with zeros in the empty sector (complex arrays stay
-        complex)."""
+        """(Dl, dims[l], Dr) -> (D, d, D) with zeros in the empty physical sector and outside the
+        site's active bond block (complex arrays stay complex)."""
         x = zfortran(x) if np.iscomplexobj(x) else fortran(x)
-        if not self.d or x.shape[1] == self.d:
+        D1, D2 = (self.D, self.D) if self.chi else (x.shape[0], x.shape[2])
+        if (not self.d or x.shape[1] == self.d) and x.shape[0] == D1 and x.shape[2] == D2:
             return x
-        out = np.zeros((x.shape[0], self.d, x.shape[2]), dtype=x.dtype, order="F")
-        out[:, :x.shape[1], :] = x
+        out = np.zeros((D1, self.d or x.shape[1], D2), dtype=x.dtype, order="F")
+        out[:x.shape[0], :x.shape[1], :x.shape[2]] = x
         return out
 
     def unpad_site(self, l, x):
         dl = self.dims[l % self.N] if self.d else x.shape[1]
+        if self.chi:
+            N = len(self.chi)
+            return np.asfortranarray(x[:self.chi[(l - 1) % N], :dl, :self.chi[l % N]])
         return x if dl == x.shape[1] else np.asfortranarray(x[:, :dl, :])
+
+    def pad_bond(self, l, x):
+        """A bond matrix (chi[l], chi[l]) of the bond right of site l as (D, D)."""
+        x = zfortran(x) if np.iscomplexobj(x) else fortran(x)
+        D = self.D if self.chi else x.shape[0]
+        if x.shape == (D, D):
+            return x
+        out = np.zeros((D, D), dtype=x.dtype, order="F")
+        out[:x.shape[0], :x.shape[1]] = x
+        return out
+
+    def unpad_bond(self, l, x):
+        if not self.chi:
+            return x
+        c = self.chi[l % len(self.chi)]
+        return np.asfortranarray(x[:c, :c])
 
     def pad_operator(self, O):
         """A one-site operator (dl x dl) as its (d x d) form, zero on the empty sector."""
@@ -130,6 +154,9 @@ class Context:
     def set_state(self, psi: "CanonicalIMPS"):
         if psi.is_complex():
             return self._set_state_complex(psi)
+        if not psi.uniform_bonds():
+            return self._set_state_nonuniform(psi)
+        self.chi = None
         AL, AR, C, AC = ([fortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
         D = psi.D
         if self.d:
@@ -142,9 +169,36 @@ class Context:
         self.call("imps_set_state", len(AL), psi.D, *(k[0] for k in keep))
         self.D = psi.D
         self.cplx = False
+        self.chi = None
+
+    def _set_state_nonuniform(self, psi: "CanonicalIMPS"):
+        # bond dimensions that differ from bond to bond: every tensor at the largest one with
+        # zeros outside its active block (imps_set_state_nonuniform)
+        chi = psi.bonddims()
+        N = len(psi)
+        for l in range(N):
+            want = (chi[(l - 1) % N], self.dims[l % self.N] if self.d else psi.AL[l].shape[1], chi[l])
+            for xs in (psi.AL, psi.AR, psi.AC):
+                if xs[l].shape != want:
+                    raise ValueError(f"set_state: site {l} tensor must be {want}, got {xs[l].shape}")
+            if psi.C[l].shape != (chi[l], chi[l]):
+                raise ValueError(f"set_state: C[{l}] must be ({chi[l]}, {chi[l]})")
+        D = max(chi)
+        self.D, self.chi = D, list(chi)
+        AL, AR, AC = ([self.pad_site(l, x) for l, x in enumerate(xs)] for xs in (psi.AL, psi.AR, psi.AC))
+        C = [self.pad_bond(l, x) for l, x in enumerate(psi.C)]
+        keep = [ptr_array(x) for x in (AL, AR, C, AC)]
+        carr = (ctypes.c_int * N)(*chi)
+        try:
+            self.call("imps_set_state_nonuniform", N, D, carr, *(k[0] for k in keep))
+        except Exception:
+            self.chi = None
+            raise
+        self.cplx = False
 
     def _set_state_complex(self, psi: "CanonicalIMPS"):
         # CanonicalIMPS{ComplexF64}: the library holds the exact real embedding (imps_set_state_complex)
+        self.chi = None
         AL, AR, C, AC = ([zfortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
         D = psi.D
         if self.d:
@@ -159,6 +213,26 @@ class Context:
         self.cplx = True
 
     def set_state_raw(self, As, tol, maxiter):
+        N = len(As)
+        chi = [a.shape[2] for a in As]
+        if any(As[l].shape[0] != chi[(l - 1) % N] for l in range(N)):
+            raise ValueError("set_state_raw: bond dimensions of neighbouring tensors differ")
+        if len(set(chi)) > 1:
+            if any(np.iscomplexobj(a) for a in As):
+                raise ValueError("set_state_raw: complex states take one bond dimension")
+            D = max(chi)
+            self.D, self.chi = D, list(chi)
+            Ap = [self.pad_site(l, fortran(a)) for l, a in enumerate(As)]
+            pp, keep = ptr_array(Ap)
+            carr = (ctypes.c_int * N)(*chi)
+            try:
+                self.call("imps_set_state_raw_nonuniform", N, D, carr, pp, float(tol), int(maxiter))
+            except Exception:
+                self.chi = None
+                raise
+            self.cplx = False
+            return
+        self.chi = None
         cplx = any(np.iscomplexobj(a) for a in As)
         As = [zfortran(a) if cplx else fortran(a) for a in As]
         D = As[0].shape[0]
@@ -181,6 +255,7 @@ class Context:
         self.call("imps_state_dims", ctypes.byref(n), ctypes.byref(D), ctypes.byref(d))
         self.D = D.value
         self.cplx = False
+        self.chi = None
 
     def get_state(self):
         N, D, d = self.N, self.D, self.d
@@ -192,6 +267,7 @@ class Context:
         keep = [(zptr_array if self.cplx else ptr_array)(x) for x in (AL, AR, C, AC)]
         self.call("imps_get_state_complex" if self.cplx else "imps_get_state", *(k[0] for k in keep))
         AL, AR, AC = ([self.unpad_site(l, x) for l, x in enumerate(xs)] for xs in (AL, AR, AC))
+        C = [self.unpad_bond(l, x) for l, x in enumerate(C)]
         return CanonicalIMPS(AL, AR, C, AC)
 
     def _env_bond(self, site, side):
@@ -201,7 +277,11 @@ class Context:
     def get_env(self, site, side):
         out = np.zeros((self.D, self.w, self.D), order="F")
         self.call("imps_get_env", int(site), int(side), dptr(out))
-        lv = self.bond_levels(self._env_bond(int(site), int(side)))
+        bond = self._env_bond(int(site), int(side))
+        lv = self.bond_levels(bond)
+        if self.chi:
+            c = self.chi[bond % len(self.chi)]
+            out = out[:c, :, :c]
         return out if len(lv) == self.w else np.asfortranarray(out[:, lv, :])
 
     def set_env(self, site, side, G):
@@ -302,6 +382,13 @@ class CanonicalIMPS:
     def copy(self):
         return CanonicalIMPS(*[[x.copy(order="F") for x in xs] for xs in (self.AL, self.AR, self.C, self.AC)])
 
+    def bonddims(self):
+        """bonddim(ψ, l) for every bond l (the bond right of site l)."""
+        return [a.shape[2] for a in self.AL]
+
+    def uniform_bonds(self):
+        return len(set(self.bonddims())) == 1 and self.AL[0].shape[0] == self.AL[0].shape[2]
+
     def is_complex(self):
         """scalartype(ψ) <: Complex."""
         return any(np.iscomplexobj(x) for xs in (self.AL, self.AR, self.C, self.AC) for x in xs)
@@ -321,14 +408,18 @@ def random_tensors(D, phydims, seed, dtype=np.float64):
     (the role of randn in randomimps, src/states/constructors.jl:13-22); for a complex dtype each
     site draws the real parts, then the imaginary parts."""
     rng = np.random.Generator(np.random.PCG64(seed))
+    N = len(phydims)
+    # D: one bond dimension, or one per bond (bond l right of site l)
+    chi = [int(D)] * N if np.ndim(D) == 0 else [int(x) for x in D]
+    shapes = [(chi[(l - 1) % N], phydims[l], chi[l]) for l in range(N)]
     if np.issubdtype(np.dtype(dtype), np.complexfloating):
         out = []
-        for d in phydims:
-            re = rng.standard_normal(D * d * D)
-            im = rng.standard_normal(D * d * D)
-            out.append(np.reshape(re + 1j * im, (D, d, D), order="F"))
+        for sh in shapes:
+            re = rng.standard_normal(int(np.prod(sh)))
+            im = rng.standard_normal(int(np.prod(sh)))
+            out.append(np.reshape(re + 1j * im, sh, order="F"))
         return out
-    return [np.reshape(rng.standard_normal(D * d * D), (D, d, D), order="F") for d in phydims]
+    return [np.reshape(rng.standard_normal(int(np.prod(sh))), sh, order="F") for sh in shapes]
 
 
 def canonicalimps(As, H: SparseIMPO, ctx: Context | None = None, tol=1e-13, maxiter=300):
@@ -342,7 +433,7 @@ def canonicalimps(As, H: SparseIMPO, ctx: Context | None = None, tol=1e-13, maxi
 
 def randomimps(D, H: SparseIMPO, seed=0, ctx: Context | None = None, dtype=np.float64, **kw):
     """randomimps(T, phydims(H); D) (constructors.jl:13-22) with the seeded generator; T is
-    Float64 or ComplexF64 (dtype=complex)."""
+    Float64 or ComplexF64 (dtype=complex); D is one bond dimension or one per bond."""
     return canonicalimps(random_tensors(D, H.phydims(), seed, dtype), H, ctx, **kw)
 
 
@@ -417,10 +508,11 @@ class MPO_C_Hamiltonian:
         self.site, self.envs, self.prec = site, envs, prec
 
     def __call__(self, x):
-        x = fortran(x)
+        ctx = self.envs.ctx
+        x = ctx.pad_bond(self.site, x)
         y = np.zeros_like(x, order="F")
-        self.envs.ctx.call("imps_hc_apply", int(self.site), dptr(x), dptr(y), int(self.prec))
-        return y
+        ctx.call("imps_hc_apply", int(self.site), dptr(x), dptr(y), int(self.prec))
+        return ctx.unpad_bond(self.site, y)
 
 
 def AC_hamiltonian(site, below, operator, above, envs: GPUCache, prec=_lib.IMPS_PREC_FP64):
@@ -459,6 +551,7 @@ def fixedpoint(op, x0, which="SR", alg: Lanczos | None = None):
     elif isinstance(op, MPO_C_Hamiltonian):
         v = np.zeros((ctx.D, ctx.D), order="F")
         ctx.call("imps_eigsolve_c", int(op.site), ctypes.byref(p), ctypes.byref(lam), dptr(v), ctypes.byref(info))
+        v = ctx.unpad_bond(op.site, v)
     else:
         raise TypeError("fixedpoint: operator must come from AC_hamiltonian or C_hamiltonian")
     return lam.value, v, info

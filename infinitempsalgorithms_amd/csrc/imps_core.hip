@@ -84,6 +84,10 @@ struct imps_ctx {
   // that every trace-normalised quantity (energy, Galerkin error, expectation values) equals the
   // complex one
   bool cplx = false;
+  // active bond dimensions (bond l right of site l) when they differ from bond to bond: the
+  // tensors are held at the largest one, D, with zeros outside the active blocks; empty for a
+  // uniform state
+  std::vector<int> chi;
   // environments
   bool envs_valid = false;
   std::vector<double*> GL, GR;
@@ -369,12 +373,13 @@ double trace_prod(imps_ctx* c, int n, const double* l, const double* v) {
 
 // regularize!(v, l, I) (envs.jl:28-32) with the identity as rvec: v -= tr(l v) I, entirely on the
 // device (two launches, no host round trip), v (n x n, leading dimension ldv), l contiguous
-void regularize_dev(imps_ctx* c, int n, double* v, long ldv, const double* l) {
+// (nact < n: the identity of the bond's active block, for bonds held below the largest dimension)
+void regularize_dev(imps_ctx* c, int n, double* v, long ldv, const double* l, int nact = -1) {
   int np = 0;
   const size_t maxp = (size_t)((n + 31) / 32) * ((n + 31) / 32);
   double* part = ws(c, "rgd_part", maxp);
   HIPCHK(launch_trace_prod_parts(n, l, n, v, ldv, part, &np, c->stream));
-  HIPCHK(launch_reduce_add_diag(np, part, -1.0, v, ldv, n, nullptr, c->stream));
+  HIPCHK(launch_reduce_add_diag(np, part, -1.0, v, ldv, (nact >= 0 && nact < n) ? nact : n, nullptr, c->stream));
 }
 
 // ------------------------------------------------------------------ GEMM wrappers
@@ -1591,7 +1596,8 @@ int read_flag(imps_ctx* c, const int* flag) {
 // down.
 // With `defer` the breakdown flag is the caller's (cleared by the caller, read later) and A is not
 // backed up: the call returns true at once and the caller redoes its work on a breakdown.
-bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullptr) {
+bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullptr, int nact = -1) {
+  if (nact < 0 || nact > n) nact = n;
   const size_t nn = (size_t)n * n, mn = (size_t)m * n;
   double* G = ws(c, "cq_G", nn);
   double* R2 = ws(c, "cq_R2", nn);
@@ -1612,6 +1618,7 @@ bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullp
     if (pass == 0)
       HIPCHK(launch_add_diag_trace(n, 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16, G, n,
                                    c->stream));
+    HIPCHK(launch_pad_unit_diag(n, nact, G, n, c->stream));
     chol_async(c, n, G, true, fail, Ut);
     trsm_right_upper(c, m, cur, nxt, G, n, Ut, 0, n);  // nxt <- cur R_pass^{-1}
     std::swap(cur, nxt);
@@ -1626,6 +1633,7 @@ bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullp
   d2d(c, A, cur, mn);  // no-op when the result is already in A
   if (defer) {
     HIPCHK(launch_tri_keep(n, R, n, true, c->stream));
+    HIPCHK(launch_mask_block(n, n, nact, nact, R, n, c->stream));
     return true;
   }
   // a breakdown in any pass: restore A (the passes after it ran on substituted pivots)
@@ -1634,12 +1642,14 @@ bool cholqr3(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullp
     return false;
   }
   HIPCHK(launch_tri_keep(n, R, n, true, c->stream));
+  HIPCHK(launch_mask_block(n, n, nact, nact, R, n, c->stream));
   return true;
 }
 
 // LQ counterpart: M (m x n, ld m, m <= n) = L Q computed as Q (in place of M) and L (m x m,
 // lower, positive diagonal).
-bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L, int* defer = nullptr) {
+bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L, int* defer = nullptr, int nact = -1) {
+  if (nact < 0 || nact > m) nact = m;
   const size_t mm = (size_t)m * m, mn = (size_t)m * n;
   double* G = ws(c, "cq_G", mm);
   double* L2 = ws(c, "cq_R2", mm);
@@ -1659,6 +1669,7 @@ bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L, int* defer = nu
     if (pass == 0)
       HIPCHK(launch_add_diag_trace(m, 11.0 * ((double)m * n + (double)m * (m + 1)) * 1.1102230246251565e-16, G, m,
                                    c->stream));
+    HIPCHK(launch_pad_unit_diag(m, nact, G, m, c->stream));
     chol_async(c, m, G, false, fail, Ut);
     trsm_left_lower(c, m, n, cur, nxt, G, m, Ut, 0, m);  // nxt <- L_pass^{-1} cur
     std::swap(cur, nxt);
@@ -1673,6 +1684,7 @@ bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L, int* defer = nu
   d2d(c, M, cur, mn);
   if (defer) {
     HIPCHK(launch_tri_keep(m, L, m, false, c->stream));
+    HIPCHK(launch_mask_block(m, m, nact, nact, L, m, c->stream));
     return true;
   }
   if (read_flag(c, fail) != 0) {
@@ -1680,32 +1692,39 @@ bool cholqr3_lq(imps_ctx* c, int m, int n, double* M, double* L, int* defer = nu
     return false;
   }
   HIPCHK(launch_tri_keep(m, L, m, false, c->stream));
+  HIPCHK(launch_mask_block(m, m, nact, nact, L, m, c->stream));
   return true;
 }
 
 std::atomic<long> g_qr_fallbacks(0);
 
 // A (m x n, lda = m, m >= n) is overwritten by Q; R (n x n) receives the factor with a
-// non-negative diagonal (FiniteMPSAlgorithms' QRpos).
-void qrpos(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullptr) {
+// non-negative diagonal (FiniteMPSAlgorithms' QRpos).  nact < n: only the first nact columns
+// are active (the rest are zero, a bond held below the largest bond dimension); Q keeps zero
+// columns there and R zero rows and columns.
+void qrpos(imps_ctx* c, int m, int n, double* A, double* R, int* defer = nullptr, int nact = -1) {
   if (defer) {
-    cholqr3(c, m, n, A, R, defer);
+    cholqr3(c, m, n, A, R, defer, nact);
     return;
   }
-  if (cholqr3(c, m, n, A, R)) return;
+  if (cholqr3(c, m, n, A, R, nullptr, nact)) return;
   g_qr_fallbacks++;
   qrpos_householder(c, m, n, A, R);
+  if (nact >= 0 && nact < n) {
+    HIPCHK(launch_mask_block(m, n, m, nact, A, m, c->stream));
+    HIPCHK(launch_mask_block(n, n, nact, nact, R, n, c->stream));
+  }
 }
 
 // M (m x n, ld m, m <= n) = L Q with L (m x m) lower, non-negative diagonal and Q (m x n)
-// row-orthonormal (LQpos).  Q may alias M.
-void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q, int* defer = nullptr) {
+// row-orthonormal (LQpos).  Q may alias M.  nact < m: only the first nact rows are active.
+void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q, int* defer = nullptr, int nact = -1) {
   d2d(c, Q, M, (size_t)m * n);
   if (defer) {
-    cholqr3_lq(c, m, n, Q, L, defer);
+    cholqr3_lq(c, m, n, Q, L, defer, nact);
     return;
   }
-  if (cholqr3_lq(c, m, n, Q, L)) return;
+  if (cholqr3_lq(c, m, n, Q, L, nullptr, nact)) return;
   g_qr_fallbacks++;
   double* Mt = ws(c, "lq_Mt", (size_t)m * n);
   double* R = ws(c, "lq_R", (size_t)m * m);
@@ -1713,7 +1732,19 @@ void lqpos(imps_ctx* c, int m, int n, const double* M, double* L, double* Q, int
   qrpos_householder(c, n, m, Mt, R);
   HIPCHK(launch_transpose(m, m, R, m, L, m, c->stream));
   HIPCHK(launch_transpose(n, m, Mt, n, Q, m, c->stream));
+  if (nact >= 0 && nact < m) {
+    HIPCHK(launch_mask_block(m, n, nact, n, Q, m, c->stream));
+    HIPCHK(launch_mask_block(m, m, nact, nact, L, m, c->stream));
+  }
 }
+
+// active dimension of the bond right of `site` / left of `site` (-1: the whole D)
+int act_right(const imps_ctx* c, int site) {
+  if (c->chi.empty()) return -1;
+  const int N = (int)c->chi.size();
+  return c->chi[((site % N) + N) % N];
+}
+int act_left(const imps_ctx* c, int site) { return act_right(c, site - 1); }
 
 // ------------------------------------------------------------------ gauge fixing
 
@@ -1784,7 +1815,7 @@ int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* 
       };
       arnoldi_dominant(c, tm, DD, Cn, ev, tol_e, 30, 300, true, nullptr);
       double* Ldum = ws(c, "ro_Ldum", DD);
-      lqpos(c, D, D, ev, Cn, Ldum);
+      lqpos(c, D, D, ev, Cn, Ldum, nullptr, act_left(c, 0));
     }
     // the sweep runs with the Cholesky breakdown checks deferred to its end (one host round trip
     // per sweep), as one captured kernel sequence (run_graphed); on a breakdown it is redone from
@@ -1798,7 +1829,7 @@ int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* 
       HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
       for (int i = N - 1; i >= 0; --i) {
         gemm2(c, 'N', 'N', D * d, D, D, 1.0, A[i], (long)D * d, c->C[i], D, 0.0, M, (long)D * d);
-        lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i], sflag);
+        lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i], sflag, act_left(c, i));
       }
       sweep_end_enqueue(c, DD, Cn, Cpre);
     });
@@ -1809,7 +1840,7 @@ int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* 
       d2d(c, Cpre, Csave, DD);
       for (int i = N - 1; i >= 0; --i) {
         gemm2(c, 'N', 'N', D * d, D, D, 1.0, A[i], (long)D * d, c->C[i], D, 0.0, M, (long)D * d);
-        lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i]);
+        lqpos(c, D, d * D, M, c->C[(i - 1 + N) % N], c->AR[i], nullptr, act_left(c, i));
       }
       HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
       eps = sweep_end(c, DD, Cn, Cpre, sflag, &broke);
@@ -1853,7 +1884,7 @@ int uniform_leftorth(imps_ctx* c, const std::vector<double*>& A, const double* C
       };
       arnoldi_dominant(c, tm, DD, Cn, ev, tol_e, 30, 300, true, nullptr);
       double* Rf = ws(c, "lo_R", DD);
-      qrpos(c, D, D, ev, Rf);
+      qrpos(c, D, D, ev, Rf, nullptr, act_left(c, 0));
       d2d(c, Cn, Rf, DD);
     }
     d2d(c, Cpre, Cn, DD);
@@ -1863,7 +1894,7 @@ int uniform_leftorth(imps_ctx* c, const std::vector<double*>& A, const double* C
       // M = C[i-1] A[i] written straight into AL[i], then QR in place (checks deferred as in
       // uniform_rightorth)
       gemm2(c, 'N', 'N', D, d * D, D, 1.0, c->C[(i - 1 + N) % N], D, A[i], D, 0.0, c->AL[i], D);
-      qrpos(c, D * d, D, c->AL[i], c->C[i], sflag);
+      qrpos(c, D * d, D, c->AL[i], c->C[i], sflag, act_right(c, i));
     }
     bool broke = false;
     eps = sweep_end(c, DD, Cn, Cpre, sflag, &broke);
@@ -1872,7 +1903,7 @@ int uniform_leftorth(imps_ctx* c, const std::vector<double*>& A, const double* C
       d2d(c, Cpre, Csave, DD);
       for (int i = 0; i < N; ++i) {
         gemm2(c, 'N', 'N', D, d * D, D, 1.0, c->C[(i - 1 + N) % N], D, A[i], D, 0.0, c->AL[i], D);
-        qrpos(c, D * d, D, c->AL[i], c->C[i]);
+        qrpos(c, D * d, D, c->AL[i], c->C[i], nullptr, act_right(c, i));
       }
       HIPCHK(launch_fill_int(1, 0, sflag, c->stream));
       eps = sweep_end(c, DD, Cn, Cpre, sflag, &broke);
@@ -1938,6 +1969,11 @@ void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* s
     fill(c, (size_t)D * w * D, 0.0, c->GR[l]);
     HIPCHK(launch_set_identity(D, c->GL[l], ldg, c->stream));
     HIPCHK(launch_set_identity(D, c->GR[l] + (size_t)D * (w - 1), ldg, c->stream));
+    if (!c->chi.empty()) {  // the identity of each bond's active block
+      HIPCHK(launch_mask_block(D, D, act_left(c, l), act_left(c, l), c->GL[l], ldg, c->stream));
+      HIPCHK(launch_mask_block(D, D, act_right(c, l), act_right(c, l), c->GR[l] + (size_t)D * (w - 1), ldg,
+                               c->stream));
+    }
     c->gl_id[l] = 1;
     c->gr_id[l] = 1;
   }
@@ -1994,7 +2030,9 @@ void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* s
     }
     HIPCHK(launch_copy2d(D, D, x, D, slice(G, i), ldg, c->stream));
   };
-  auto regularize_slice = [&](double* v, long ld, const double* l) { regularize_dev(c, D, v, ld, l); };
+  auto regularize_slice = [&](double* v, long ld, const double* l, int nact) {
+    regularize_dev(c, D, v, ld, l, nact);
+  };
   auto ring_left = [&](const double* v, double* out, double* rh) {
     double* t0 = ws(c, "env_t0", DD);
     double* t1 = ws(c, "env_t1", DD);
@@ -2004,7 +2042,7 @@ void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* s
       push_left2(c, cur, D, c->AL[l], c->AL[l], dst, D);
       cur = dst;
     }
-    if (rh) regularize_slice(out, D, rh);
+    if (rh) regularize_slice(out, D, rh, act_left(c, 0));
   };
   auto ring_right = [&](const double* v, double* out, double* rh) {
     double* t0 = ws(c, "env_t0", DD);
@@ -2015,7 +2053,7 @@ void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* s
       push_right2(c, cur, D, c->AR[l], c->AR[l], dst, D);
       cur = dst;
     }
-    if (rh) regularize_slice(out, D, rh);
+    if (rh) regularize_slice(out, D, rh, act_right(c, N - 1));
   };
 
   // ---- left interior levels (envs.jl:364-421)
@@ -2028,7 +2066,7 @@ void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* s
       for (int l = 0; l < N; ++l) {
         double* rl = ws(c, "env_rhol", DD);
         gemm2(c, 'N', 'T', D, D, D, 1.0, c->C[(l - 1 + N) % N], D, c->C[(l - 1 + N) % N], D, 0.0, rl, D);
-        regularize_slice(slice(c->GL[l], i), ldg, rl);
+        regularize_slice(slice(c->GL[l], i), ldg, rl, act_left(c, l));
       }
     } else {
       if (!level_empty(c, i)) {
@@ -2063,7 +2101,7 @@ void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* s
     for (int l = 0; l < N; ++l) {
       double* rl = ws(c, "env_rhol", DD);
       gemm2(c, 'N', 'T', D, D, D, 1.0, c->C[(l - 1 + N) % N], D, c->C[(l - 1 + N) % N], D, 0.0, rl, D);
-      regularize_slice(slice(c->GL[l], i), ldg, rl);
+      regularize_slice(slice(c->GL[l], i), ldg, rl, act_left(c, l));
     }
   }
   // ---- right levels (envs.jl:461-515)
@@ -2076,7 +2114,7 @@ void envs_build(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvStats* s
       for (int l = 0; l < N; ++l) {
         double* rl = ws(c, "env_rhol", DD);
         gemm2(c, 'T', 'N', D, D, D, 1.0, c->C[l], D, c->C[l], D, 0.0, rl, D);
-        regularize_slice(slice(c->GR[l], i), ldg, rl);
+        regularize_slice(slice(c->GR[l], i), ldg, rl, act_right(c, l));
       }
     } else {
       if (!level_empty(c, i)) {
@@ -2112,6 +2150,7 @@ double dot_psi(imps_ctx* c) {
   double* v0 = ws(c, "dp_v0", DD);
   double* ev = ws(c, "dp_ev", DD);
   HIPCHK(launch_set_identity(D, v0, D, c->stream));
+  if (!c->chi.empty()) HIPCHK(launch_mask_block(D, D, act_left(c, 0), act_left(c, 0), v0, D, c->stream));
   Op tm = [&](const double* v, double* out) {
     double* t0 = ws(c, "dp_t0", DD);
     double* t1 = ws(c, "dp_t1", DD);
@@ -2591,6 +2630,7 @@ void free_state(imps_ctx* c) {
   c->envs_valid = false;
   c->D = 0;
   c->cplx = false;
+  c->chi.clear();
 }
 
 void alloc_state(imps_ctx* c, int N, int D, int d) {
@@ -2644,8 +2684,8 @@ void regauge_site(imps_ctx* c, int site) {
   double* R = ws(c, "rg_R", (size_t)D * D);
   d2d(c, Qac, c->pAC[site], (size_t)D * d * D);
   d2d(c, Qc, c->pC[site], (size_t)D * D);
-  qrpos(c, D * d, D, Qac, R);
-  qrpos(c, D, D, Qc, R);
+  qrpos(c, D * d, D, Qac, R, nullptr, act_right(c, site));
+  qrpos(c, D, D, Qc, R, nullptr, act_right(c, site));
   gemm2(c, 'N', 'T', D * d, D, D, 1.0, Qac, (long)D * d, Qc, D, 0.0, c->pAL[site], (long)D * d);
 }
 
@@ -3089,30 +3129,109 @@ int imps_get_state(imps_ctx* c, double* const* AL, double* const* AR, double* co
   });
 }
 
+// Active bond dimensions of a state held at the largest one (D): 1 <= chi[l] <= D with some
+// chi[l] == D, and each site's tensor can be full rank (chi[l] <= d_l chi[l-1] and
+// chi[l-1] <= d_l chi[l]); tensors must vanish outside their active blocks.
+static void check_bond_profile(imps_ctx* c, int nsites, int D, const int* chi, const char* what) {
+  require(chi != nullptr && nsites == c->N, IMPS_E_ARG, std::string(what) + ": unit cell mismatch");
+  int mx = 0;
+  for (int l = 0; l < nsites; ++l) {
+    require(chi[l] >= 1 && chi[l] <= D, IMPS_E_ARG, std::string(what) + ": bond dimension out of range");
+    mx = std::max(mx, chi[l]);
+  }
+  require(mx == D, IMPS_E_ARG, std::string(what) + ": D must be the largest bond dimension");
+  for (int l = 0; l < nsites; ++l) {
+    const int dl = c->pdim.size() == (size_t)c->N ? c->pdim[l] : c->d;
+    const int cl = chi[(l - 1 + nsites) % nsites], cr = chi[l];
+    require(cr <= dl * cl && cl <= dl * cr, IMPS_E_ARG,
+            std::string(what) + ": site " + std::to_string(l) + " cannot be full rank (bond dimensions " +
+                std::to_string(cl) + ", " + std::to_string(cr) + ")");
+  }
+}
+
+static void require_outside_zero(imps_ctx* c, int D, const int* chi, const double* const* A, bool mps,
+                                 const char* what) {
+  if (!A) return;
+  const int N = c->N, m = mps ? c->d : 1;
+  for (int l = 0; l < N; ++l) {
+    if (!A[l]) continue;
+    const int ra = mps ? chi[(l - 1 + N) % N] : chi[l], ca = chi[l];
+    for (int b = 0; b < D; ++b)
+      for (int k = 0; k < m; ++k)
+        for (int a = 0; a < D; ++a)
+          if (a >= ra || b >= ca)
+            require(A[l][a + (size_t)D * (k + (size_t)m * b)] == 0.0, IMPS_E_ARG,
+                    std::string(what) + ": site " + std::to_string(l) +
+                        " has weight outside its bond dimensions");
+  }
+}
+
+static void set_state_raw_impl(imps_ctx* c, int nsites, int D, const double* const* A, double tol, int maxiter,
+                               const int* chi) {
+  need_mpo(c);
+  require(nsites == c->N && D >= 1 && A, IMPS_E_ARG, "imps_set_state_raw: unit cell mismatch");
+  require_empty_sectors(c, D, A, "imps_set_state_raw");
+  alloc_state(c, nsites, D, c->d);
+  if (chi) c->chi.assign(chi, chi + nsites);
+  const size_t nA = (size_t)D * c->d * D, nC = (size_t)D * D;
+  // raw tensors are staged in the pending AC buffers (the gauge sweeps read copies)
+  std::vector<double*> raw(nsites);
+  for (int l = 0; l < nsites; ++l) {
+    raw[l] = c->pAC[l];
+    h2d(c, raw[l], A[l], nA);
+  }
+  double* I = ws(c, "sr_I", nC);
+  HIPCHK(launch_set_identity(D, I, D, c->stream));
+  if (chi) HIPCHK(launch_mask_block(D, D, chi[nsites - 1], chi[nsites - 1], I, D, c->stream));
+  // gaugefix!(:LR): left sweeps from C0 = I, then right sweeps from C[end]
+  uniform_leftorth(c, raw, I, tol, maxiter);
+  double* C0 = ws(c, "sr_C0", nC);
+  d2d(c, C0, c->C[nsites - 1], nC);
+  uniform_rightorth(c, c->AL, C0, tol, maxiter);
+  for (int l = 0; l < nsites; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+  c->envs_valid = false;
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 int imps_set_state_raw(imps_ctx* c, int nsites, int D, const double* const* A, double tol, int maxiter) {
   if (!c) return IMPS_E_ARG;
+  return run_guarded(c, [&] { set_state_raw_impl(c, nsites, D, A, tol, maxiter, nullptr); });
+}
+
+int imps_set_state_nonuniform(imps_ctx* c, int nsites, int D, const int* chi, const double* const* AL,
+                              const double* const* AR, const double* const* Cm, const double* const* AC) {
+  if (!c || !chi || !AL || !AR || !Cm) return IMPS_E_ARG;
+  int st = run_guarded(c, [&] {
+    need_mpo(c);
+    check_bond_profile(c, nsites, D, chi, "imps_set_state_nonuniform");
+    require_outside_zero(c, D, chi, AL, true, "imps_set_state_nonuniform");
+    require_outside_zero(c, D, chi, AR, true, "imps_set_state_nonuniform");
+    require_outside_zero(c, D, chi, AC, true, "imps_set_state_nonuniform");
+    require_outside_zero(c, D, chi, Cm, false, "imps_set_state_nonuniform");
+  });
+  if (st != IMPS_OK) return st;
+  st = imps_set_state(c, nsites, D, AL, AR, Cm, AC);
+  if (st != IMPS_OK) return st;
+  return run_guarded(c, [&] { c->chi.assign(chi, chi + nsites); });
+}
+
+int imps_set_state_raw_nonuniform(imps_ctx* c, int nsites, int D, const int* chi, const double* const* A, double tol,
+                                  int maxiter) {
+  if (!c || !chi || !A) return IMPS_E_ARG;
   return run_guarded(c, [&] {
     need_mpo(c);
-    require(nsites == c->N && D >= 1 && A, IMPS_E_ARG, "imps_set_state_raw: unit cell mismatch");
-    require_empty_sectors(c, D, A, "imps_set_state_raw");
-    alloc_state(c, nsites, D, c->d);
-    const size_t nA = (size_t)D * c->d * D, nC = (size_t)D * D;
-    // raw tensors are staged in the pending AC buffers (the gauge sweeps read copies)
-    std::vector<double*> raw(nsites);
-    for (int l = 0; l < nsites; ++l) {
-      raw[l] = c->pAC[l];
-      h2d(c, raw[l], A[l], nA);
-    }
-    double* I = ws(c, "sr_I", nC);
-    HIPCHK(launch_set_identity(D, I, D, c->stream));
-    // gaugefix!(:LR): left sweeps from C0 = I, then right sweeps from C[end]
-    uniform_leftorth(c, raw, I, tol, maxiter);
-    double* C0 = ws(c, "sr_C0", nC);
-    d2d(c, C0, c->C[nsites - 1], nC);
-    uniform_rightorth(c, c->AL, C0, tol, maxiter);
-    for (int l = 0; l < nsites; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
-    c->envs_valid = false;
-    HIPCHK(hipStreamSynchronize(c->stream));
+    check_bond_profile(c, nsites, D, chi, "imps_set_state_raw_nonuniform");
+    require_outside_zero(c, D, chi, A, true, "imps_set_state_raw_nonuniform");
+    set_state_raw_impl(c, nsites, D, A, tol, maxiter, chi);
+  });
+}
+
+int imps_bond_dims(imps_ctx* c, int* chi) {
+  if (!c || !chi) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    const int N = (int)c->AL.size();
+    for (int l = 0; l < N; ++l) chi[l] = c->chi.empty() ? (c->cplx ? c->D / 2 : c->D) : c->chi[l];
   });
 }
 
@@ -3820,7 +3939,7 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       for (int pos = 0; pos < N; ++pos) {
         eigsolve_ac(c, pos, tol_eig, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, PREC_FP64);
         d2d(c, c->AL[pos], c->pAC[pos], nA);
-        qrpos(c, D * d, D, c->AL[pos], c->C[pos]);
+        qrpos(c, D * d, D, c->AL[pos], c->C[pos], nullptr, act_right(c, pos));
         d2d(c, c->AC[pos], pos < N - 1 ? c->AL[pos] : c->pAC[pos], nA);
         // transfer_leftenv! (envs.jl:591-596): a pushed environment is no longer pinned
         const int nxt = (pos + 1) % N;
@@ -3831,7 +3950,7 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       for (int pos = N - 1; pos >= 0; --pos) {
         E = eigsolve_ac(c, pos, tol_eig, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, PREC_FP64).lambda;
         const int prv = (pos - 1 + N) % N;
-        lqpos(c, D, d * D, c->pAC[pos], c->C[prv], c->AR[pos]);
+        lqpos(c, D, d * D, c->pAC[pos], c->C[prv], c->AR[pos], nullptr, act_left(c, pos));
         d2d(c, c->AC[pos], pos > 0 ? c->AR[pos] : c->pAC[pos], nA);
         // transfer_rightenv! (envs.jl:599-604)
         push_mpo_right(c, pos, c->GR[pos], c->AR[pos], Env);

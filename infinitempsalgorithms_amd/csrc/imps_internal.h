@@ -133,6 +133,10 @@ hipError_t launch_cplx_embed(int D1, int m, int D2, const double* Z, double scal
 hipError_t launch_cplx_extract(int D1, int m, int D2, double* R, double scale, double* Z, bool project,
                                hipStream_t s);
 hipError_t launch_cplx_axpy_i(size_t n, double alpha, const double* w, double* y, hipStream_t s);
+// Bond-dependent bond dimensions (active blocks inside the largest): G[i][i] = 1 for
+// nact <= i < n; A[i + ld*j] = 0 for i >= ract or j >= cact.
+hipError_t launch_pad_unit_diag(int n, int nact, double* G, long ld, hipStream_t s);
+hipError_t launch_mask_block(int rows, int cols, int ract, int cact, double* A, long ld, hipStream_t s);
 // diagonal of an n x n matrix (leading dimension ld) += v
 hipError_t launch_add_diag(int n, double v, double* x, long ld, hipStream_t s);
 // x[i + ld*j] = v for i < rows, j < cols

@@ -1637,4 +1637,34 @@ hipError_t launch_cplx_axpy_i(size_t n, double alpha, const double* w, double* y
   return hipGetLastError();
 }
 
+// ---- bond dimensions that differ from bond to bond: the state is held at the largest bond
+// dimension with zeros outside each bond's active block
+
+// G[i + ld*i] = 1 for nact <= i < n (the Gram matrix of a matrix whose columns from nact on are
+// zero: the Cholesky factor then carries an identity there and the solves leave those columns zero)
+__global__ void pad_unit_diag_kernel(int n, int nact, double* G, long ld) {
+  const int i = nact + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) G[i + ld * i] = 1.0;
+}
+
+hipError_t launch_pad_unit_diag(int n, int nact, double* G, long ld, hipStream_t s) {
+  if (nact >= n) return hipSuccess;
+  hipLaunchKernelGGL(pad_unit_diag_kernel, dim3((n - nact + 255) / 256), dim3(256), 0, s, n, nact, G, ld);
+  return hipGetLastError();
+}
+
+// A[i + ld*j] = 0 for i >= ract or j >= cact (rows x cols)
+__global__ void mask_block_kernel(int rows, int cols, int ract, int cact, double* A, long ld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < rows && j < cols && (i >= ract || j >= cact)) A[i + ld * j] = 0.0;
+}
+
+hipError_t launch_mask_block(int rows, int cols, int ract, int cact, double* A, long ld, hipStream_t s) {
+  if (ract >= rows && cact >= cols) return hipSuccess;
+  hipLaunchKernelGGL(mask_block_kernel, dim3((rows + 255) / 256, cols), dim3(256), 0, s, rows, cols, ract, cact, A,
+                     ld);
+  return hipGetLastError();
+}
+
 }  // namespace imps

@@ -812,15 +812,20 @@ def dmrg_cache(psi, Ws, tol=Defaults.tol, maxiter=Defaults.maxiter,
     """DMRGCache(psi, H::SparseIMPO; tol, maxiter, krylovdim, init_lefts, init_rights):
     envs.jl:321-518 for square Schur tensors and a uniform bond dimension."""
     N = len(psi)
-    D = psi.AL[0].shape[0]
     w = Ws[0].shape[0]
-    lefts = [np.zeros((D, w, D)) for _ in range(N)]
-    rights = [np.zeros((D, w, D)) for _ in range(N)]
-    I = np.eye(D)
+    # bond dimensions may differ from bond to bond (envs.jl:327-343): lefts[l] lives on the bond
+    # left of site l, rights[l] on the bond right of it
+    Dl = [psi.AL[l].shape[0] for l in range(N)]
+    Dr = [psi.AL[l].shape[2] for l in range(N)]
+    lefts = [np.zeros((Dl[l], w, Dl[l])) for l in range(N)]
+    rights = [np.zeros((Dr[l], w, Dr[l])) for l in range(N)]
     for l in range(N):
-        lefts[l][:, 0, :] = I
-        rights[l][:, w - 1, :] = I
-    kdim = min(D * D, krylovdim)
+        lefts[l][:, 0, :] = np.eye(Dl[l])
+        rights[l][:, w - 1, :] = np.eye(Dr[l])
+    I = np.eye(Dl[0])      # bond N-1 = bond left of site 0: the cell's fixed-point bond
+    IL = [np.eye(Dl[l]) for l in range(N)]
+    IR = [np.eye(Dr[l]) for l in range(N)]
+    kdim = min(Dl[0] * Dl[0], krylovdim)
     stats = {"gmres_ops": 0, "solves": 0}
 
     def solve(op, rhs, x0):
@@ -850,7 +855,7 @@ def dmrg_cache(psi, Ws, tol=Defaults.tol, maxiter=Defaults.maxiter,
                 _left_cyclethrough(lefts, Ws, AL, i)
             for l in range(N):
                 rho_l = C[l - 1] @ C[l - 1].T
-                regularize(lefts[l][:, i, :], rho_l, I)
+                regularize(lefts[l][:, i, :], rho_l, IL[l])
         else:
             if not isemptylevel(Ws, i):
                 def op(X, i=i):
@@ -879,7 +884,7 @@ def dmrg_cache(psi, Ws, tol=Defaults.tol, maxiter=Defaults.maxiter,
     if N > 1:
         _left_closing_sweep(lefts, Ws, AL)
     for l in range(N):
-        regularize(lefts[l][:, w - 1, :], C[l - 1] @ C[l - 1].T, I)
+        regularize(lefts[l][:, w - 1, :], C[l - 1] @ C[l - 1].T, IL[l])
 
     # ---- right environments (envs.jl:461-515) ----
     for i in range(w - 2, -1, -1):
@@ -899,7 +904,7 @@ def dmrg_cache(psi, Ws, tol=Defaults.tol, maxiter=Defaults.maxiter,
             if N > 1:
                 _right_cyclethrough(rights, Ws, AR, i)
             for l in range(N):
-                regularize(rights[l][:, i, :], C[l].T @ C[l], I)
+                regularize(rights[l][:, i, :], C[l].T @ C[l], IR[l])
         else:
             if not isemptylevel(Ws, i):
                 def op(X, i=i):

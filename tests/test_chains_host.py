@@ -67,5 +67,16 @@ def test_context_padding_round_trip():
     assert c.unpad_site(1, np.ones((4, 3, 4))).shape == (4, 3, 4)
     O = c.pad_operator(SX)
     assert O.shape == (3, 3) and np.array_equal(O[:2, :2], SX) and not O[2, :].any() and not O[:, 2].any()
+    # a bond profile: tensors padded to the largest bond dimension and back
+    c.chi = [2, 4, 3]
+    y = np.asfortranarray(rng.standard_normal((3, 2, 2)))  # site 0: bonds chi[2] = 3, chi[0] = 2
+    q = c.pad_site(0, y)
+    assert q.shape == (4, 3, 4) and np.array_equal(q[:3, :2, :2], y) and np.count_nonzero(q) == y.size
+    assert np.array_equal(c.unpad_site(0, q), y)
+    Cb = np.asfortranarray(rng.standard_normal((4, 4)))
+    assert np.array_equal(c.unpad_bond(1, c.pad_bond(1, Cb)), Cb)
+    Cs = np.asfortranarray(rng.standard_normal((2, 2)))
+    assert c.pad_bond(0, Cs).shape == (4, 4) and np.array_equal(c.unpad_bond(0, c.pad_bond(0, Cs)), Cs)
+    c.chi = None
     # bond levels: the closing level goes to w - 1
     assert c.bond_levels(0) == [0, 1, 2] and c.bond_levels(1) == [0, 2] and c.bond_levels(-1) == [0, 2]

@@ -583,8 +583,11 @@ def test_tdvp_imaginary_time_matches_oracle(imps, ctx, oracle):
     assert abs(hist[-1] / 2 - (0.25 - math.log(2))) < 2e-3
     assert all(hist[i + 1] <= hist[i] + 1e-10 for i in range(len(hist) - 1))
     assert max(oracle.mixedcanonical_errors(oracle.CanonicalIMPS(psi.AL, psi.AR, psi.C, psi.AC))) < 1e-8
-    with pytest.raises(ValueError):
-        imps.timestep(psi, H, -0.01j, imps.TDVP(), envs)
+    # a complex dt (real time) promotes the state to complex (tdvp.jl:97-100) and keeps its norm
+    e_before = imps.expectationvalue(psi, H, envs)
+    pc, envs = imps.timestep(psi, H, -0.01j, imps.TDVP(), envs)
+    assert pc.is_complex() and abs(imps.norm(pc) - 1) < 1e-8
+    assert abs(imps.expectationvalue(pc, H, envs) - e_before) < 1e-6
 
 
 def test_correlator_and_fidelity_match_oracle(imps, ctx, oracle):
