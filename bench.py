@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--refine-below", type=float, default=1e-6, help="mixed precision: FP64 below this Galerkin error")
     ap.add_argument("--idmrg", type=int, default=0, help="also time this many single-site IDMRG sweeps")
     ap.add_argument("--tdvp", type=int, default=0, help="also time this many imaginary-time TDVP steps (dt = -0.05)")
+    ap.add_argument("--gauge-eig-miniter", type=int, default=10,
+                    help="gauge sweeps before the transfer-map eigenvector refresh (RightCanonical eig_miniter)")
     ap.add_argument("--eig-floor", type=float, default=0.0,
                     help="mixed precision: eigensolver tolerance floor (0: library default)")
     ap.add_argument("--calib", action="store_true", help="also run a 256 MiB copy kernel (traffic calibration)")
@@ -434,7 +436,8 @@ def parameter_sweep(args, imps, prec):
     Ds = rng.choice([32, 64, 128, 256], 64)
     problems = [(imps.models.tfim_hamiltonian(J=1.0, h=float(h), nsites=1), imps.random_tensors(int(D), [2], seed=k))
                 for k, (h, D) in enumerate(zip(hs, Ds))]
-    alg = imps.VUMPS(tol=1e-8, maxiter=200, prec=prec, refine_below=args.refine_below)
+    alg = imps.VUMPS(tol=1e-8, maxiter=200, prec=prec, refine_below=args.refine_below,
+                     gauge_eig_miniter=args.gauge_eig_miniter)
     out = {}
     energies = {}
     modes = ["device", "threads"] if args.sweep_mode == "both" else [args.sweep_mode]
@@ -605,7 +608,8 @@ def vumps_iterations(args, imps, H, prec, out, D, N):
         log(f"bench: VUMPS iter {it} eps {eps:.3e} e/site {f / N:.12f} t {el:.1f}s")
         return (args.ttg and el > args.ttg_seconds) or eps <= args.stop_below
 
-    alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec, refine_below=args.refine_below, eig_floor=args.eig_floor)
+    alg = imps.VUMPS(tol=1e-8, maxiter=maxit, prec=prec, refine_below=args.refine_below, eig_floor=args.eig_floor,
+                     gauge_eig_miniter=args.gauge_eig_miniter)
     _, _, info = imps.find_groundstate(psi, H, alg, envs=envs, callback=cb)
     rec = {"setup_seconds": t_setup, "iterations": info.niter, "loop_seconds": info.seconds,
            "seconds_per_iteration": info.seconds / max(info.niter, 1), "itererr": info.itererr,

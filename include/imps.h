@@ -311,6 +311,8 @@ typedef struct imps_vumps_params {
                              eigensolves keep the reduced-precision operator for Jacobi-Davidson
                              corrections (MINRES) of FP64 residuals, until |H x - theta x| <= tol;
                              0: FP64 Lanczos */
+  int gauge_eig_miniter;  /* RightCanonical eig_miniter = 10 (ortho.jl:22-31, 166-202): gauge sweeps
+                             before the dominant-eigenvector refresh of C from the mixed transfer map */
 } imps_vumps_params;
 
 /* IterativeConvergenceInfo (algdefs.jl:180-194) plus timings. */

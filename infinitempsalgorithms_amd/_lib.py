@@ -63,7 +63,8 @@ class VumpsParams(ctypes.Structure):
                 ("gauge_dynamic", c_int), ("gauge_tol", c_double), ("gauge_tol_min", c_double),
                 ("gauge_tol_max", c_double), ("envs_dynamic", c_int), ("envs_tol", c_double),
                 ("envs_tol_min", c_double), ("envs_tol_max", c_double), ("envs_krylovdim", c_int),
-                ("finalize", FINALIZE_CB), ("finalize_user", c_void_p), ("refine_eig", c_int)]
+                ("finalize", FINALIZE_CB), ("finalize_user", c_void_p), ("refine_eig", c_int),
+                ("gauge_eig_miniter", c_int)]
 
 
 class VumpsInfo(ctypes.Structure):

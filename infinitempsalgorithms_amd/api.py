@@ -1005,6 +1005,8 @@ def _set_solver(p, which, obj):
             p.tol_min, p.tol_max, p.eigs_tolfactor = obj.tol_min, obj.tol_max, obj.tol_factor
     elif which == "gauge":
         p.gauge_dynamic, p.gauge_tol, p.gauge_maxiter = int(dyn), float(inner.tol), int(inner.maxiter)
+        if getattr(inner, "eig_miniter", None) is not None:
+            p.gauge_eig_miniter = int(inner.eig_miniter)
         if dyn:
             p.gauge_tol_min, p.gauge_tol_max, p.gauge_tolfactor = obj.tol_min, obj.tol_max, obj.tol_factor
     else:
@@ -1024,6 +1026,7 @@ def _params(alg, prec, refine_below, eig_floor):
                                                                    alg.envs_maxiter)
     p.prec, p.refine_below, p.eig_floor = prec, refine_below, eig_floor
     p.refine_eig = int(bool(getattr(alg, "refine_eig", True)))
+    p.gauge_eig_miniter = int(getattr(alg, "gauge_eig_miniter", 10))
     for which, obj in (("eig", alg.alg_eigsolve), ("gauge", alg.alg_gauge), ("envs", alg.alg_environments)):
         if obj is not None:
             _set_solver(p, which, obj)
@@ -1064,6 +1067,7 @@ class VUMPS:
     refine_below: float = 0.0
     eig_floor: float = 0.0
     refine_eig: bool = True
+    gauge_eig_miniter: int = 10
     alg_eigsolve: object = None
     alg_gauge: object = None
     alg_environments: object = None

@@ -2743,13 +2743,14 @@ void mul_al_c(Batch& B, const std::vector<Inst*>& in) {
 }
 
 // gauge_step! (vumps.jl:23-32): gaugefix!(psi, AL, C; order = :R) from the regauged AL, AC = AL C
-void gauge_step_batch(Batch& B, const std::vector<Inst*>& in, const std::vector<double>& tol, int maxiter) {
+void gauge_step_batch(Batch& B, const std::vector<Inst*>& in, const std::vector<double>& tol, int maxiter,
+                      int eig_miniter) {
   std::vector<const double*> A, C0;
   for (Inst* I : in) {
     A.push_back(I->pAL);
     C0.push_back(I->C);
   }
-  std::vector<int> sw = gauge_sweeps(B, in, false, A, C0, tol, maxiter);
+  std::vector<int> sw = gauge_sweeps(B, in, false, A, C0, tol, maxiter, std::max(0, eig_miniter));
   std::vector<Item> cp;
   for (size_t k = 0; k < in.size(); ++k) {
     in[k]->sweeps += sw[k];
@@ -2890,7 +2891,7 @@ void vumps_batch(Batch& B, std::vector<Inst*>& all, const imps_vumps_params* p, 
       tg.push_back(st.gauge(siter, I->eps));
       te.push_back(st.envs(siter, I->eps));
     }
-    gauge_step_batch(B, act, tg, p->gauge_maxiter);
+    gauge_step_batch(B, act, tg, p->gauge_maxiter, p->gauge_eig_miniter);
     lap(2);
     envs_batch(B, act, te, p->envs_maxiter, p->envs_krylovdim);
     lap(3);

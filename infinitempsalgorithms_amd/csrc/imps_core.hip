@@ -2665,11 +2665,11 @@ void envs_recalculate(imps_ctx* c, double tol, int maxiter, int krylovdim, EnvSt
   envs_build(c, tol, maxiter, krylovdim, st);
 }
 
-void gauge_step(imps_ctx* c, double tol, int maxiter, int* sweeps) {
+void gauge_step(imps_ctx* c, double tol, int maxiter, int* sweeps, int eig_miniter = 10) {
   const int N = c->N, D = c->D, d = c->d;
   double* C0 = ws(c, "gs_C0", (size_t)D * D);
   d2d(c, C0, c->C[N - 1], (size_t)D * D);
-  const int sw = uniform_rightorth(c, c->pAL, C0, tol, maxiter);
+  const int sw = uniform_rightorth(c, c->pAL, C0, tol, maxiter, std::max(0, eig_miniter));
   for (int l = 0; l < N; ++l) {
     d2d(c, c->AL[l], c->pAL[l], (size_t)D * d * D);
     mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
@@ -3745,6 +3745,7 @@ int imps_vumps_default_params(imps_vumps_params* p) {
   p->finalize = nullptr;
   p->finalize_user = nullptr;
   p->refine_eig = 1;
+  p->gauge_eig_miniter = 10;
   return IMPS_OK;
 }
 
@@ -3860,7 +3861,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
         local_update(c, ta, tc, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, prec, lam_ac, lam_c, refining);
       });
       int sw = 0;
-      lap(tg, [&] { gauge_step(c, stol.gauge(siter, eps), p->gauge_maxiter, &sw); });
+      lap(tg, [&] { gauge_step(c, stol.gauge(siter, eps), p->gauge_maxiter, &sw, p->gauge_eig_miniter); });
       sweeps_total += sw;
       lap(te, [&] { envs_recalculate(c, stol.envs(siter, eps), p->envs_maxiter, p->envs_krylovdim, &est); });
       // finalize (vumps.jl:75), between recalculate! and calc_galerkin
