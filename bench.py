@@ -107,6 +107,7 @@ def main():
                     help="--sweep: the lockstep device batch, the threaded batch, or both")
     ap.add_argument("--no-sweep", action="store_true", help="skip the parameter sweep of the default run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the fixed-point response of the default run")
+    ap.add_argument("--no-realtime", action="store_true", help="skip the real-time TDVP record of the default run")
     ap.add_argument("--gradient", action="store_true",
                     help="energy gradients: TFIM D=128 de/dh and XXZ D=512 de/dDelta against finite differences")
     ap.add_argument("--vumps-iters", type=int, default=3)
@@ -206,6 +207,8 @@ def main():
         out["sweep"] = parameter_sweep(args, imps, PREC["fp64"])
     if not args.no_adjoint and not args.vumps_only:
         out["adjoint"] = adjoint_check(imps)
+    if not args.no_realtime and not args.vumps_only:
+        out["tdvp_real_time"] = realtime_tdvp(imps)
 
     if args.idmrg > 0:
         out["idmrg"] = idmrg_sweeps(args, imps, H, D, N)
@@ -416,6 +419,32 @@ def tdvp_steps(args, imps, H, D, N, dt=-0.05):
     ctx.close()
     return {"steps": args.tdvp, "dt": dt, "seconds_per_step": sum(times) / len(times),
             "energy_per_site_before": e0, "energy_per_site_after": e1}
+
+
+def realtime_tdvp(imps, D=64, steps=10, dt=-0.01j):
+    """Real-time TDVP (tdvp.jl:33-52 with dt = -im t, the state promoted to ComplexF64 and run on the
+    device as its real embedding of bond dimension 2 D): the XXX chain, two-site cell, from the
+    VUMPS(D) ground state, ten steps; seconds per step, energy drift and norm (the reference's
+    real-time test asserts drift < 1e-6 and |norm - 1| < 1e-8 at D = 8)."""
+    m = imps.models.heisenberg_xxz(J=1.0, Delta=1.0)
+    H = imps.SparseIMPO([m.bulk, m.bulk])
+    ctx = imps.Context(0)
+    g, envs, info = imps.find_groundstate(None, H, imps.VUMPS(D=D, tol=1e-9, maxiter=400), ctx=ctx, seed=3)
+    e0 = imps.expectationvalue(g, H, envs) / 2
+    psi = g
+    times = []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        psi, envs = imps.timestep(psi, H, dt, imps.TDVP(), envs)
+        times.append(time.perf_counter() - t0)
+    e1 = imps.expectationvalue(psi, H, envs) / 2
+    nrm = imps.norm(psi)
+    ctx.close()
+    log(f"bench: real-time TDVP XXX D={D}: {sum(times) / steps:.3f} s per step, energy drift {abs(e1 - e0):.2e}")
+    return {"workload": f"XXX (Delta = 1), two-site cell, VUMPS D={D} ground state, {steps} steps dt = -0.01i",
+            "complex_embedding_bond_dimension": 2 * D, "seconds_per_step": sum(times) / steps,
+            "first_step_seconds": times[0], "energy_per_site": e0, "energy_drift": abs(e1 - e0),
+            "norm_error": abs(nrm - 1.0), "vumps_iterations": info.niter}
 
 
 def tfim_exact_energy(h, J=1.0, n=200000):

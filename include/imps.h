@@ -201,7 +201,8 @@ int imps_regauge(imps_ctx* ctx, int site, double* al);
 
 /* gauge_step! (src/algorithms/groundstates/vumps.jl:23-32): AL <- pending ALs,
  * gaugefix!(ψ, AL, C[end]; order = :R, tol, maxiter) (ortho.jl:95-99, 166-202),
- * then AC = AL*C.  `sweeps` receives the number of LQ sweeps (may be NULL). */
+ * then AC = AL*C.  The pending ALs are those of the last imps_regauge, or the state's own ALs
+ * after the state was set.  `sweeps` receives the number of LQ sweeps (may be NULL). */
 int imps_gaugefix_right(imps_ctx* ctx, double tol, int maxiter, int* sweeps);
 
 /* calc_galerkin(ψ, H, envs) (vumps.jl:111-124). */

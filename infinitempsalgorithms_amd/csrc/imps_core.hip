@@ -641,15 +641,25 @@ struct HacBufs {
 void hac_apply_bufs(imps_ctx* c, int site, const double* x, double* y, int prec, const HacBufs& b,
                     hipEvent_t* ev);
 
+// For a complex state (the real embedding, D even) the outputs of the effective operators and of
+// the transfer pushes are projected back onto the embedded form: the operators preserve it
+// exactly, and the projection keeps rounding from seeding the conjugate sector, whose transfer
+// spectrum reaches the same modulus when the state is close to its own conjugate (a real state
+// after a phase), which would otherwise stall the Krylov solvers.
+void project_embedded(imps_ctx* c, int m, double* p) {
+  if (c->cplx) HIPCHK(launch_cplx_extract(c->D / 2, m, c->D / 2, p, 1.0, nullptr, true, c->stream));
+}
+
 void hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec, hipEvent_t* ev = nullptr) {
   const int D = c->D, w = c->w, d = c->d;
   site = wrap(c, site);
   if (prec == PREC_BF16X3 && hac_planes_ok(c, site)) {
     hac_apply_planes(c, site, x, y, ev);
-    return;
+  } else {
+    HacBufs b{ws(c, "hac_T1", (size_t)D * w * d * D), ws(c, "hac_T2", (size_t)D * w * d * D), nullptr, false};
+    hac_apply_bufs(c, site, x, y, prec, b, ev);
   }
-  HacBufs b{ws(c, "hac_T1", (size_t)D * w * d * D), ws(c, "hac_T2", (size_t)D * w * d * D), nullptr, false};
-  hac_apply_bufs(c, site, x, y, prec, b, ev);
+  project_embedded(c, d, y);
 }
 
 // the GEMM -> W mix -> GEMM chain of hac_apply on the buffers b (site already wrapped)
@@ -774,7 +784,14 @@ void hc_apply_planes(imps_ctx* c, int site, const double* x, double* y) {
 // H_C (effective.jl:72-76): y = sum_k GL_k x GR_k with GL = leftenv(site+1), GR = rightenv(site).
 // With identity structural levels, GL_0 x = x (copied into level 0 of T1) and the level-(w-1)
 // term is T1_{w-1} itself (copied into y, onto which the GEMM over the other levels adds).
+void hc_apply_impl(imps_ctx* c, int site, const double* x, double* y, int prec);
+
 void hc_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
+  hc_apply_impl(c, site, x, y, prec);
+  project_embedded(c, 1, y);
+}
+
+void hc_apply_impl(imps_ctx* c, int site, const double* x, double* y, int prec) {
   const int D = c->D, w = c->w;
   site = wrap(c, site);
   const int sl = wrap(c, site + 1);
@@ -829,6 +846,7 @@ void push_left2(imps_ctx* c, const double* X, long ldx, const double* above, con
   double* Y = ws(c, "pl_Y", (size_t)D * d * D);
   gemm2(c, 'N', 'N', D, d * D, D, 1.0, X, ldx, above, D, 0.0, Y, D, prec);
   gemm2(c, 'T', 'N', D, D, d * D, 1.0, below, (long)d * D, Y, (long)d * D, 0.0, out, ldo, prec);
+  if (ldo == D) project_embedded(c, 1, out);
 }
 
 // right: out[a',b'] = sum above[a',s,a] X[a,b] below[b',s,b]
@@ -838,6 +856,7 @@ void push_right2(imps_ctx* c, const double* X, long ldx, const double* above, co
   double* Y = ws(c, "pr_Y", (size_t)D * d * D);
   gemm2(c, 'N', 'N', d * D, D, D, 1.0, above, (long)d * D, X, ldx, 0.0, Y, (long)d * D, prec);
   gemm2(c, 'N', 'T', D, D, d * D, 1.0, Y, D, below, D, 0.0, out, ldo, prec);
+  if (ldo == D) project_embedded(c, 1, out);
 }
 
 // block W[l,:,i,:] of a (w, d, w, d) tensor as a (d x d) column-major matrix M[u + d*s]
@@ -1366,10 +1385,87 @@ KInfo lanczos_expm_c(imps_ctx* c, const Op& op, size_t n, const double* x0, doub
 // (complex, for a complex pair) eigenvector of the Hessenberg matrix; for a complex dominant pair
 // xout is the real part of the phase-fixed Ritz vector (a vector of the pair's real invariant
 // plane, from which the next restart spans it), the return value Re(lambda) and *lam_im Im(lambda).
+// Arnoldi in complex arithmetic for a complex state (real embedding, pairs of rows): the basis
+// holds v_k and i v_k side by side (slots 2k, 2k+1), so that Gram-Schmidt against every slot is
+// complex Gram-Schmidt and the projection is the complex Hessenberg matrix with entries
+// <v_j, w> + i <i v_j, w>.  On the embedding a complex eigenvalue lambda of the operator is the
+// pair (lambda, conj lambda) of a real operator, which a real Krylov space cannot separate when
+// Im lambda is tiny (the gauge fixed points of a state close to a real one); in complex
+// arithmetic it is one simple eigenvalue.  Same restart and convergence rules as the real case.
+double arnoldi_dominant_cplx(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xout, double tol, int kdim,
+                             int maxiter, bool eager, KInfo* info, bool* complex_dominant, double* lam_im) {
+  using cd = std::complex<double>;
+  // the embedded vectors of n doubles hold n / 4 complex numbers
+  const int kmax = (int)std::min<size_t>(kdim, std::max<size_t>(1, n / 4));
+  require(kmax >= 1 && 2 * (kmax + 1) <= kMaxVec, IMPS_E_ARG, "arnoldi: krylovdim out of range");
+  double* V = ws(c, "arz_V", (size_t)2 * (kmax + 1) * n);
+  auto Vc = [&](int slot) { return V + (size_t)slot * n; };
+  const int ldh = kmax + 1;
+  std::vector<cd> H((size_t)ldh * kmax), x(kmax);
+  std::vector<double> h(2 * (kmax + 1)), coef(2 * kmax);
+  int numiter = 1, numops = 0, K = 0;
+  bool conv = false;
+  cd lam(0.0, 0.0);
+  double res = 0.0;
+  const double nx = nrm2(c, n, x0);
+  require(nx > 0 && std::isfinite(nx), IMPS_E_ARG, "arnoldi: zero or non-finite start vector");
+  axpby(c, n, 1.0 / nx, x0, 0.0, Vc(0));
+  auto with_i = [&](int k) {  // slot 2k + 1 = i * slot 2k
+    fill(c, n, 0.0, Vc(2 * k + 1));
+    HIPCHK(launch_cplx_axpy_i(n, 1.0, Vc(2 * k), Vc(2 * k + 1), c->stream));
+  };
+  while (true) {
+    std::fill(H.begin(), H.end(), cd(0.0, 0.0));
+    K = 0;
+    with_i(0);
+    while (true) {
+      op(Vc(2 * K), Vc(2 * K + 2));
+      numops++;
+      double beta = 0.0;
+      cgs2_normalize(c, n, 2 * K + 2, V, Vc(2 * K + 2), h.data(), &beta);
+      for (int j = 0; j <= K; ++j) H[j + (size_t)ldh * K] = cd(h[2 * j], h[2 * j + 1]);
+      H[(K + 1) + (size_t)ldh * K] = cd(beta, 0.0);
+      K++;
+      with_i(K);
+      if (K == kmax || beta <= tol || eager) {
+        double last = 0.0;
+        require(hess_dominant_z(K, H.data(), ldh, &lam, x.data(), &last) == 0, IMPS_E_SOLVER,
+                "arnoldi: Hessenberg eigenvalue iteration failed");
+        res = std::fabs(beta) * last;
+        if (res <= tol) {
+          conv = true;
+          break;
+        }
+      }
+      if (K == kmax) break;
+    }
+    for (int k = 0; k < K; ++k) {
+      coef[2 * k] = x[k].real();
+      coef[2 * k + 1] = x[k].imag();
+    }
+    multiaxpy(c, n, 2 * K, V, n, coef.data(), 0.0, xout);
+    if (conv || numiter == maxiter) break;
+    const double nr = nrm2(c, n, xout);
+    axpby(c, n, 1.0 / nr, xout, 0.0, Vc(0));
+    numiter++;
+  }
+  if (info) {
+    info->converged = conv ? 1 : 0;
+    info->numiter = numiter;
+    info->numops = numops;
+    info->normres = res;
+  }
+  if (complex_dominant) *complex_dominant = std::fabs(lam.imag()) > 1e-12 * std::max(1.0, std::abs(lam));
+  if (lam_im) *lam_im = lam.imag();
+  return lam.real();
+}
+
 double arnoldi_dominant(imps_ctx* c, const Op& op, size_t n, const double* x0, double* xout,
                         double tol, int kdim, int maxiter, bool eager, KInfo* info, bool* complex_dominant = nullptr,
                         double* lam_im = nullptr) {
   const int kmax = (int)std::min<size_t>(kdim, n);
+  if (c->cplx && n % 2 == 0)
+    return arnoldi_dominant_cplx(c, op, n, x0, xout, tol, kdim, maxiter, eager, info, complex_dominant, lam_im);
   require(kmax >= 1 && kmax < kMaxVec, IMPS_E_ARG, "arnoldi: krylovdim out of range");
   double* V = ws(c, "ar_V", (size_t)(kmax + 1) * n);
   auto Vc = [&](int k) { return V + (size_t)k * n; };
@@ -2648,6 +2744,13 @@ void alloc_state(imps_ctx* c, int N, int D, int d) {
   }
 }
 
+// the pending ALs of a freshly set state are its ALs, so that gauge_step (imps_gaugefix_right)
+// before any regauge re-fixes the gauge of the state itself instead of reading stale buffers
+void init_pending(imps_ctx* c) {
+  const size_t nA = (size_t)c->D * c->d * c->D;
+  for (size_t l = 0; l < c->AL.size(); ++l) d2d(c, c->pAL[l], c->AL[l], nA);
+}
+
 void alloc_envs(imps_ctx* c) {
   if (!c->GL.empty()) return;
   const size_t n = (size_t)c->D * c->w * c->D;
@@ -2796,6 +2899,9 @@ imps_ctx* worker(imps_ctx* c, int k) {
   w->gl_id = c->gl_id;
   w->gr_id = c->gr_id;
   w->envs_valid = c->envs_valid;
+  w->cplx = c->cplx;
+  w->chi = c->chi;
+  w->pdim = c->pdim;
   return w;
 }
 
@@ -3098,6 +3204,7 @@ int imps_set_state(imps_ctx* c, int nsites, int D, const double* const* AL, cons
       else
         mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
     }
+    init_pending(c);
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
@@ -3189,6 +3296,7 @@ static void set_state_raw_impl(imps_ctx* c, int nsites, int D, const double* con
   d2d(c, C0, c->C[nsites - 1], nC);
   uniform_rightorth(c, c->AL, C0, tol, maxiter);
   for (int l = 0; l < nsites; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+  init_pending(c);
   c->envs_valid = false;
   HIPCHK(hipStreamSynchronize(c->stream));
 }
@@ -3301,6 +3409,7 @@ int imps_set_state_complex(imps_ctx* c, int nsites, int D, const double* const* 
       else
         mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
     }
+    init_pending(c);
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
@@ -3329,6 +3438,7 @@ int imps_set_state_raw_complex(imps_ctx* c, int nsites, int D, const double* con
     uniform_rightorth(c, c->AL, C0, tol, maxiter);
     for (int l = 0; l < nsites; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
     project_state_c(c);
+    init_pending(c);
     c->envs_valid = false;
     HIPCHK(hipStreamSynchronize(c->stream));
   });

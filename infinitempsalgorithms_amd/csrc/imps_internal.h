@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <complex>
+
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -200,5 +202,10 @@ int hess_dominant(int n, const double* H, int ldh, double* lam, double* u);
 // The same with the eigenvalue's imaginary part (lam_im, may be NULL) and |x_{n-1}| of the unit
 // complex eigenvector x (last, may be NULL; the Arnoldi residual is |beta| times it).
 int hess_dominant_c(int n, const double* H, int ldh, double* lam_re, double* lam_im, double* u, double* last);
+// The same for a complex upper Hessenberg matrix (column-major, ld ldh): eigenvalues (wr, wi),
+// and the dominant eigenpair with the unit eigenvector x and |x_{n-1}|; 0 on success.
+int hess_eigvals_z(int n, const std::complex<double>* H, int ldh, double* wr, double* wi);
+int hess_dominant_z(int n, const std::complex<double>* H, int ldh, std::complex<double>* lam,
+                    std::complex<double>* x, double* last);
 
 }  // namespace imps

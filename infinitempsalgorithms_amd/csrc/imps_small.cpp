@@ -78,15 +78,10 @@ void sym_eig(int n, double* A, double* w, double* V) {
 
 typedef std::complex<double> cplx;
 
-int hess_eigvals(int n, const double* H, int ldh, double* wr, double* wi) {
-  std::vector<cplx> a((size_t)n * n);
+// Eigenvalues of the complex n x n matrix `a` (upper Hessenberg, column-major, overwritten) by
+// the shifted complex QR iteration; hnorm is the largest entry modulus.
+static int hqr_complex(int n, std::vector<cplx>& a, double hnorm, double* wr, double* wi) {
   auto A = [&](int i, int j) -> cplx& { return a[i + (size_t)n * j]; };
-  double hnorm = 0.0;
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) {
-      A(i, j) = (i <= j + 1) ? cplx(H[i + (size_t)ldh * j], 0.0) : cplx(0.0, 0.0);
-      hnorm = std::max(hnorm, std::abs(A(i, j)));
-    }
   if (hnorm == 0.0) hnorm = 1.0;
   const double eps = 2.220446049250313e-16;
   int hi = n - 1;
@@ -187,13 +182,36 @@ int hess_eigvals(int n, const double* H, int ldh, double* wr, double* wi) {
   return 0;
 }
 
-// Solve (H - lam I) x = b by Gaussian elimination with partial pivoting (complex).
-static void shifted_solve(int n, const double* H, int ldh, cplx lam, std::vector<cplx>& x,
-                          double hnorm) {
+int hess_eigvals(int n, const double* H, int ldh, double* wr, double* wi) {
+  std::vector<cplx> a((size_t)n * n);
+  double hnorm = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      a[i + (size_t)n * j] = (i <= j + 1) ? cplx(H[i + (size_t)ldh * j], 0.0) : cplx(0.0, 0.0);
+      hnorm = std::max(hnorm, std::abs(a[i + (size_t)n * j]));
+    }
+  return hqr_complex(n, a, hnorm, wr, wi);
+}
+
+int hess_eigvals_z(int n, const cplx* H, int ldh, double* wr, double* wi) {
+  std::vector<cplx> a((size_t)n * n);
+  double hnorm = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      a[i + (size_t)n * j] = (i <= j + 1) ? H[i + (size_t)ldh * j] : cplx(0.0, 0.0);
+      hnorm = std::max(hnorm, std::abs(a[i + (size_t)n * j]));
+    }
+  return hqr_complex(n, a, hnorm, wr, wi);
+}
+
+// Solve (H - lam I) x = b by Gaussian elimination with partial pivoting (complex); H is the
+// n x n upper Hessenberg matrix given entrywise by h(i, j).
+template <typename HF>
+static void shifted_solve_f(int n, const HF& h, cplx lam, std::vector<cplx>& x, double hnorm) {
   std::vector<cplx> M((size_t)n * n);
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i)
-      M[i + (size_t)n * j] = cplx((i <= j + 1) ? H[i + (size_t)ldh * j] : 0.0, 0.0) - (i == j ? lam : 0.0);
+      M[i + (size_t)n * j] = ((i <= j + 1) ? h(i, j) : cplx(0.0, 0.0)) - (i == j ? lam : cplx(0.0, 0.0));
   std::vector<cplx> b = x;
   const double tiny = 1e-14 * (hnorm > 0.0 ? hnorm : 1.0);
   for (int k = 0; k < n; ++k) {
@@ -217,6 +235,41 @@ static void shifted_solve(int n, const double* H, int ldh, cplx lam, std::vector
     for (int j = i + 1; j < n; ++j) s -= M[i + (size_t)n * j] * x[j];
     x[i] = s / M[i + (size_t)n * i];
   }
+}
+
+static void shifted_solve(int n, const double* H, int ldh, cplx lam, std::vector<cplx>& x, double hnorm) {
+  shifted_solve_f(n, [&](int i, int j) { return cplx(H[i + (size_t)ldh * j], 0.0); }, lam, x, hnorm);
+}
+
+// Dominant (largest modulus) eigenpair of a complex upper Hessenberg matrix (the projection of
+// an Arnoldi process in complex arithmetic): returns 0, the eigenvalue in *lam, the unit
+// eigenvector in x (phase: largest entry real positive) and |x_{n-1}| in *last; 2 when the
+// eigenvalue iteration fails.
+int hess_dominant_z(int n, const cplx* H, int ldh, cplx* lam, cplx* x, double* last) {
+  std::vector<double> wr(n), wi(n);
+  if (hess_eigvals_z(n, H, ldh, wr.data(), wi.data()) != 0) return 2;
+  int best = 0;
+  for (int i = 1; i < n; ++i)
+    if (std::hypot(wr[i], wi[i]) > std::hypot(wr[best], wi[best])) best = i;
+  *lam = cplx(wr[best], wi[best]);
+  double hnorm = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i <= std::min(j + 1, n - 1); ++i) hnorm = std::max(hnorm, std::abs(H[i + (size_t)ldh * j]));
+  std::vector<cplx> v(n, cplx(1.0, 0.0));
+  for (int it = 0; it < 3; ++it) {
+    shifted_solve_f(n, [&](int i, int j) { return H[i + (size_t)ldh * j]; }, *lam, v, hnorm);
+    double nr = 0.0;
+    for (int i = 0; i < n; ++i) nr += std::norm(v[i]);
+    nr = std::sqrt(nr);
+    for (int i = 0; i < n; ++i) v[i] /= nr;
+  }
+  int k = 0;
+  for (int i = 1; i < n; ++i)
+    if (std::abs(v[i]) > std::abs(v[k])) k = i;
+  const cplx ph = std::abs(v[k]) / v[k];
+  for (int i = 0; i < n; ++i) x[i] = v[i] * ph;
+  if (last) *last = std::abs(x[n - 1]);
+  return 0;
 }
 
 int hess_dominant_c(int n, const double* H, int ldh, double* lam_re, double* lam_im, double* u, double* last) {
@@ -275,5 +328,14 @@ int imps_host_hess_dominant(int n, const double* H, int ldh, double* lam, double
 }
 int imps_host_hess_eigvals(int n, const double* H, int ldh, double* wr, double* wi) {
   return imps::hess_eigvals(n, H, ldh, wr, wi);
+}
+// complex Hessenberg (re, im interleaved, column-major): dominant eigenvalue and unit eigenvector
+int imps_host_hess_dominant_z(int n, const double* H, int ldh, double* lam, double* x, double* last) {
+  std::complex<double> l;
+  const int rc = imps::hess_dominant_z(n, reinterpret_cast<const std::complex<double>*>(H), ldh, &l,
+                                       reinterpret_cast<std::complex<double>*>(x), last);
+  lam[0] = l.real();
+  lam[1] = l.imag();
+  return rc;
 }
 }

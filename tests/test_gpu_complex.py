@@ -182,3 +182,28 @@ def test_complex_state_refuses_real_only_entry_points(imps, oracle, cx):
         # symmetric operators are whole on complex states
         sz = oracle.sigma_z()
         assert abs(imps.expectationvalue(st, (0, sz), envs) - cx.local_expectation1(psi, 0, sz).real) < 1e-12
+
+
+def test_gaugefix_right_of_a_freshly_set_state_keeps_it(imps, oracle, cx):
+    # imps_gaugefix_right before any regauge re-fixes the gauge of the state itself (its pending
+    # ALs are its ALs): a canonical state comes back unchanged, real and complex, and a TDVP step
+    # after it runs
+    import ctypes
+    W = oracle.xxz_mpo(1.0, 0.8)
+    H = imps.SparseIMPO([W, W])
+    real = oracle.randomimps(6, [2, 2], 17)
+    cplx = cx.canonical_from_tensors(_complex_raw(oracle, 6, [2, 2], 19))
+    print("complex: gaugefix_right after set_state", flush=True)
+    for psi in (real, cplx):
+        with imps.Context(0) as c:
+            st = _as_lib(imps, psi)
+            envs = imps.DMRGCache(st, H, ctx=c)
+            sw = ctypes.c_int()
+            c.call("imps_gaugefix_right", 1e-14, 300, ctypes.byref(sw))
+            got = c.get_state()
+            for a, b in zip(got.AL + got.AR + got.C + got.AC, psi.AL + psi.AR + psi.C + psi.AC):
+                assert rel(a, b) < 1e-10
+            envs.ket = got
+            c.call("imps_envs_recalculate", 1e-10, 300, 30)
+            p2, envs = imps.timestep(got, H, -0.01j, imps.TDVP(), envs)
+            assert abs(imps.norm(p2) - 1) < 1e-8

@@ -184,3 +184,27 @@ def test_hessenberg_eigenvalues_with_a_nearly_defective_pair():
     assert L.imps_host_hess_dominant(n, H.ctypes.data_as(_lib.P_double), n, ctypes.byref(lam),
                                      u.ctypes.data_as(_lib.P_double)) in (0, 1)
     assert abs(lam.value - want.max()) < 1e-7 * want.max()
+
+
+def test_complex_hessenberg_dominant_eigenpair():
+    # the complex Arnoldi projection's dominant eigenpair (host routine) against LAPACK, on a
+    # random complex Hessenberg matrix and on one whose dominant eigenvalue is nearly real
+    import ctypes
+    from infinitempsalgorithms_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(5)
+    for shift in (0.0, 3.0):
+        n = 12
+        Hm = np.triu(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), -1)
+        Hm[0, 0] += shift
+        Hf = np.asfortranarray(Hm)
+        lam = np.zeros(2)
+        x = np.zeros(n, dtype=complex)
+        last = ctypes.c_double()
+        assert L.imps_host_hess_dominant_z(n, Hf.ctypes.data_as(_lib.P_double), n, lam.ctypes.data_as(_lib.P_double),
+                                           x.ctypes.data_as(_lib.P_double), ctypes.byref(last)) == 0
+        w, V = np.linalg.eig(Hm)
+        k = int(np.argmax(np.abs(w)))
+        assert abs(complex(lam[0], lam[1]) - w[k]) < 1e-12 * abs(w[k])
+        assert np.linalg.norm(Hm @ x - w[k] * x) < 1e-10 and abs(np.linalg.norm(x) - 1) < 1e-12
+        assert abs(last.value - abs(x[-1])) < 1e-15
