@@ -123,3 +123,23 @@ def test_zero_padded_bond_leaves_the_energy_unchanged_on_device(imps, oracle):
     with imps.Context(0) as c:
         E1 = imps.expectationvalue(pp, H, imps.DMRGCache(pp, H, ctx=c))
     assert abs(E1 - E0) < 1e-10
+
+
+def test_larger_profile_vumps_matches_oracle(imps, oracle):
+    # bond dimensions [8, 16, 12] on a three-site XXZ cell: VUMPS energy and Galerkin trajectory
+    # against the oracle, the profile kept
+    W = oracle.xxz_mpo(1.0, 0.6)
+    Ws = [W] * 3
+    H = imps.SparseIMPO(Ws)
+    chi = [8, 16, 12]
+    psi = oracle.canonical_from_tensors(imps.random_tensors(chi, [2, 2, 2], 23))
+    g_o, env_o, info_o = oracle.find_groundstate(psi, Ws, oracle.VUMPS(tol=1e-9, maxiter=300))
+    print("bonds: VUMPS on a [8, 16, 12] XXZ cell", flush=True)
+    with imps.Context(0) as c:
+        st = _lib_state(imps, psi)
+        g, envs, info = imps.find_groundstate(st, H, imps.VUMPS(tol=1e-9, maxiter=300), envs=imps.GPUCache(st, H, c))
+        assert g.bonddims() == chi
+        assert abs(imps.expectationvalue(g, H, envs) - oracle.expectation_energy(g_o, Ws, env_o)) < 1e-8
+        k = min(4, len(info.losses), len(info_o.losses))
+        for a, b in zip(info.losses[:k], info_o.losses[:k]):
+            assert abs(a - b) <= 1e-6 * abs(b)

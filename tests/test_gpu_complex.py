@@ -207,3 +207,27 @@ def test_gaugefix_right_of_a_freshly_set_state_keeps_it(imps, oracle, cx):
             c.call("imps_envs_recalculate", 1e-10, 300, 30)
             p2, envs = imps.timestep(got, H, -0.01j, imps.TDVP(), envs)
             assert abs(imps.norm(p2) - 1) < 1e-8
+
+
+def test_complex_tdvp_step_matches_oracle_at_d24(imps, oracle, cx):
+    # a larger complex state (XXZ, two-site cell, D = 24, embedded 48): two real-time steps against
+    # the dense complex restatement
+    W = oracle.xxz_mpo(1.0, 0.7)
+    Ws = [W, W]
+    psi = cx.canonical_from_tensors(_complex_raw(oracle, 24, [2, 2], 51))
+    env = cx.dmrg_cache(psi, Ws)
+    refs = []
+    p = psi
+    for _ in range(2):
+        p, env = cx.tdvp_timestep(p, Ws, env, -0.02j)
+        refs.append((p, cx.expectation_energy(p, Ws, env).real))
+    print("complex: two real-time steps at D=24", flush=True)
+    H = imps.SparseIMPO(Ws)
+    with imps.Context(0) as c:
+        st = _as_lib(imps, psi)
+        envs = imps.DMRGCache(st, H, tol=1e-12, ctx=c)
+        for ref, e_ref in refs:
+            st, envs = imps.timestep(st, H, -0.02j, imps.TDVP(tol=1e-12, gauge_tol=1e-14, gauge_maxiter=2000), envs)
+            for a, b in zip(st.AL + st.C, ref.AL + ref.C):
+                assert rel(a, b) < 1e-7
+            assert abs(imps.expectationvalue(st, H, envs) - e_ref) < 1e-9
