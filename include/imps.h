@@ -389,11 +389,13 @@ int imps_tdvp_timestep_complex(imps_ctx* ctx, double dt_re, double dt_im, double
  * QR/LQ/Cholesky factors commute with the embedding.  On a context holding a complex state:
  * find_groundstate (imps_vumps_run), the environments (imps_envs_recalculate), imps_energy,
  * imps_galerkin, imps_gaugefix_right, both TDVP steps and imps_local_expectation /
- * imps_correlator with symmetric real operators work as for real states; imps_state_dims reports
- * the complex bond dimension; the entry points that pass real state-shaped host arrays or need
- * real arithmetic (imps_get_state, the H_AC / H_C / transfer applies, get/set_env, the
- * eigensolve and regauge primitives, IDMRG, dot, dominant_env, dense_expectation, the gradients,
- * checkpointing) return IMPS_E_UNSUPPORTED. */
+ * imps_correlator with symmetric real operators work as for real states; imps_hac_apply,
+ * imps_hc_apply, imps_transfer_apply, imps_get_env and the vectors of imps_eigsolve_ac/c take and
+ * return ComplexF64 arrays of the complex bond dimension (eigenvectors of unit norm);
+ * imps_state_dims reports the complex bond dimension; the entry points that pass real
+ * state-shaped host arrays or need real arithmetic (imps_get_state, set_env, regauge, the device-
+ * pointer applies, IDMRG, dot, dominant_env, dense_expectation, the gradients, checkpointing)
+ * return IMPS_E_UNSUPPORTED. */
 
 /* CanonicalIMPS{ComplexF64} from canonical tensors (the complex twin of imps_set_state; AC may
  * be NULL: AL C). */

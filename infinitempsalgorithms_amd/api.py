@@ -275,8 +275,8 @@ class Context:
         return site - 1 if side == _lib.IMPS_SIDE_LEFT else site
 
     def get_env(self, site, side):
-        out = np.zeros((self.D, self.w, self.D), order="F")
-        self.call("imps_get_env", int(site), int(side), dptr(out))
+        out = np.zeros((self.D, self.w, self.D), dtype=np.complex128 if self.cplx else np.float64, order="F")
+        self.call("imps_get_env", int(site), int(side), out.ctypes.data_as(_lib.P_double))
         bond = self._env_bond(int(site), int(side))
         lv = self.bond_levels(bond)
         if self.chi:
@@ -495,9 +495,10 @@ class MPO_AC_Hamiltonian:
 
     def __call__(self, x):
         ctx = self.envs.ctx
-        x = ctx.pad_site(self.site, x)
+        x = ctx.pad_site(self.site, zfortran(x) if ctx.cplx else x)
         y = np.zeros_like(x, order="F")
-        ctx.call("imps_hac_apply", int(self.site), dptr(x), dptr(y), int(self.prec))
+        ctx.call("imps_hac_apply", int(self.site), x.ctypes.data_as(_lib.P_double), y.ctypes.data_as(_lib.P_double),
+                 int(self.prec))
         return ctx.unpad_site(self.site, y)
 
 
@@ -509,9 +510,10 @@ class MPO_C_Hamiltonian:
 
     def __call__(self, x):
         ctx = self.envs.ctx
-        x = ctx.pad_bond(self.site, x)
+        x = ctx.pad_bond(self.site, zfortran(x) if ctx.cplx else x)
         y = np.zeros_like(x, order="F")
-        ctx.call("imps_hc_apply", int(self.site), dptr(x), dptr(y), int(self.prec))
+        ctx.call("imps_hc_apply", int(self.site), x.ctypes.data_as(_lib.P_double), y.ctypes.data_as(_lib.P_double),
+                 int(self.prec))
         return ctx.unpad_bond(self.site, y)
 
 
@@ -544,13 +546,16 @@ def fixedpoint(op, x0, which="SR", alg: Lanczos | None = None):
     info = _lib.KrylovInfo()
     lam = ctypes.c_double()
     ctx = op.envs.ctx
+    dt = np.complex128 if ctx.cplx else np.float64
     if isinstance(op, MPO_AC_Hamiltonian):
-        v = np.zeros((ctx.D, ctx.d, ctx.D), order="F")
-        ctx.call("imps_eigsolve_ac", int(op.site), ctypes.byref(p), ctypes.byref(lam), dptr(v), ctypes.byref(info))
+        v = np.zeros((ctx.D, ctx.d, ctx.D), dtype=dt, order="F")
+        ctx.call("imps_eigsolve_ac", int(op.site), ctypes.byref(p), ctypes.byref(lam), v.ctypes.data_as(_lib.P_double),
+                 ctypes.byref(info))
         v = ctx.unpad_site(op.site, v)
     elif isinstance(op, MPO_C_Hamiltonian):
-        v = np.zeros((ctx.D, ctx.D), order="F")
-        ctx.call("imps_eigsolve_c", int(op.site), ctypes.byref(p), ctypes.byref(lam), dptr(v), ctypes.byref(info))
+        v = np.zeros((ctx.D, ctx.D), dtype=dt, order="F")
+        ctx.call("imps_eigsolve_c", int(op.site), ctypes.byref(p), ctypes.byref(lam), v.ctypes.data_as(_lib.P_double),
+                 ctypes.byref(info))
         v = ctx.unpad_bond(op.site, v)
     else:
         raise TypeError("fixedpoint: operator must come from AC_hamiltonian or C_hamiltonian")
@@ -572,9 +577,11 @@ class TransferMatrix:
 
     def __call__(self, v):
         D = self.ctx.D
-        X = fortran(np.reshape(v, (D, D), order="F"))
-        out = np.zeros((D, D), order="F")
-        self.ctx.call("imps_transfer_apply", self.side, dptr(X), dptr(out))
+        conv = zfortran if self.ctx.cplx else fortran
+        X = conv(np.reshape(v, (D, D), order="F"))
+        out = np.zeros((D, D), dtype=X.dtype, order="F")
+        self.ctx.call("imps_transfer_apply", self.side, X.ctypes.data_as(_lib.P_double),
+                      out.ctypes.data_as(_lib.P_double))
         return out.reshape(-1, order="F") if np.ndim(v) == 1 else out
 
 
@@ -660,6 +667,10 @@ def dot(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
     """dot(ψ₁, ψ₂) (canonicalmps.jl:262-269): dominant eigenvalue of the mixed AL transfer
     matrix, on the device (ψ₁ becomes the context's state).  A float, or a complex number when the
     dominant eigenvalue is complex (as the reference returns it)."""
+    if psi1.is_complex() or psi2.is_complex():
+        # on the real embedding the mixed transfer map of psi1 with conj(psi2) is as dominant as
+        # the one asked for, so the overlap of complex states is not computed there
+        raise ValueError("dot: complex states are not supported")
     if len(psi2) != len(psi1) or any(a.shape != b.shape for a, b in zip(psi1.AL, psi2.AL)):
         raise ValueError("dot: the library takes states with the same unit cell, bond and physical "
                          "dimensions")

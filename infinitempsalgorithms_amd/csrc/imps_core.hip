@@ -3472,12 +3472,18 @@ int imps_state_is_complex(imps_ctx* c, int* flag) {
 int imps_hac_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
   if (!c || !x || !y) return IMPS_E_ARG;
   return run_guarded(c, [&] {
-    require_real_state(c, "imps_hac_apply");
     need_match(c);
     need_envs(c);
     const size_t n = (size_t)c->D * c->d * c->D;
     double* dx = ws(c, "abi_x", n);
     double* dy = ws(c, "abi_y", n);
+    if (c->cplx) {  // ComplexF64 x and y of the complex bond dimension
+      const int Dc = c->D / 2;
+      embed_from_host(c, x, Dc, c->d, Dc, 1.0, dx);
+      hac_apply(c, site, dx, dy, prec);
+      extract_to_host(c, dy, Dc, c->d, Dc, 1.0, y);
+      return;
+    }
     h2d(c, dx, x, n);
     hac_apply(c, site, dx, dy, prec);
     d2h(c, y, dy, n);
@@ -3520,12 +3526,18 @@ int imps_hac_apply_noalloc(imps_ctx* c, int site, const double* d_x, double* d_y
 int imps_hc_apply(imps_ctx* c, int site, const double* x, double* y, int prec) {
   if (!c || !x || !y) return IMPS_E_ARG;
   return run_guarded(c, [&] {
-    require_real_state(c, "imps_hc_apply");
     need_match(c);
     need_envs(c);
     const size_t n = (size_t)c->D * c->D;
     double* dx = ws(c, "abi_x", n);
     double* dy = ws(c, "abi_y", n);
+    if (c->cplx) {
+      const int Dc = c->D / 2;
+      embed_from_host(c, x, Dc, 1, Dc, 1.0, dx);
+      hc_apply(c, site, dx, dy, prec);
+      extract_to_host(c, dy, Dc, 1, Dc, 1.0, y);
+      return;
+    }
     h2d(c, dx, x, n);
     hc_apply(c, site, dx, dy, prec);
     d2h(c, y, dy, n);
@@ -3545,14 +3557,16 @@ int imps_hc_apply_dev(imps_ctx* c, int site, const double* d_x, double* d_y, int
 int imps_transfer_apply(imps_ctx* c, int side, const double* v, double* out) {
   if (!c || !v || !out) return IMPS_E_ARG;
   return run_guarded(c, [&] {
-    require_real_state(c, "imps_transfer_apply");
     need_state(c);
     require(side == IMPS_SIDE_LEFT || side == IMPS_SIDE_RIGHT, IMPS_E_ARG, "imps_transfer_apply: bad side");
     const int N = (int)c->AL.size(), D = c->D;
     const size_t DD = (size_t)D * D;
     double* a = ws(c, "tr_a", DD);
     double* b = ws(c, "tr_b", DD);
-    h2d(c, a, v, DD);
+    if (c->cplx)
+      embed_from_host(c, v, D / 2, 1, D / 2, 1.0, a);
+    else
+      h2d(c, a, v, DD);
     if (side == IMPS_SIDE_LEFT) {
       for (int l = 0; l < N; ++l) {
         push_left2(c, a, D, c->AL[l], c->AL[l], b, D);
@@ -3564,7 +3578,10 @@ int imps_transfer_apply(imps_ctx* c, int side, const double* v, double* out) {
         std::swap(a, b);
       }
     }
-    d2h(c, out, a, DD);
+    if (c->cplx)
+      extract_to_host(c, a, D / 2, 1, D / 2, 1.0, out);
+    else
+      d2h(c, out, a, DD);
   });
 }
 
@@ -3602,11 +3619,15 @@ int imps_envs_reset(imps_ctx* c) {
 int imps_get_env(imps_ctx* c, int site, int side, double* out) {
   if (!c || !out) return IMPS_E_ARG;
   return run_guarded(c, [&] {
-    require_real_state(c, "imps_get_env");
     need_envs(c);
     site = wrap(c, site);
     const size_t n = (size_t)c->D * c->w * c->D;
-    d2h(c, out, side == IMPS_SIDE_LEFT ? c->GL[site] : c->GR[site], n);
+    double* G = side == IMPS_SIDE_LEFT ? c->GL[site] : c->GR[site];
+    if (c->cplx) {  // ComplexF64 (D/2, w, D/2): the nearest complex tensor of each 2 x 2 block
+      extract_to_host(c, G, c->D / 2, c->w, c->D / 2, 1.0, out);
+      return;
+    }
+    d2h(c, out, G, n);
   });
 }
 
@@ -3647,14 +3668,18 @@ int imps_eigsolve_ac(imps_ctx* c, int site, const imps_krylov_params* p, double*
                      imps_krylov_info* info) {
   if (!c || !p) return IMPS_E_ARG;
   return run_guarded(c, [&] {
-    require_real_state(c, "imps_eigsolve_ac");
+
     need_match(c);
     need_envs(c);
     site = wrap(c, site);
     EigOut o = eigsolve_ac(c, site, p->tol, p->krylovdim, p->maxiter, p->eager != 0, p->prec);
     if (lambda) *lambda = o.lambda;
     fill_info(info, o.info);
-    if (vec) d2h(c, vec, c->pAC[site], (size_t)c->D * c->d * c->D);
+    if (vec && c->cplx) {  // unit complex norm (the embedding has twice the squared norm)
+      extract_to_host(c, c->pAC[site], c->D / 2, c->d, c->D / 2, std::sqrt(2.0), vec);
+    } else if (vec) {
+      d2h(c, vec, c->pAC[site], (size_t)c->D * c->d * c->D);
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
@@ -3663,14 +3688,18 @@ int imps_eigsolve_c(imps_ctx* c, int site, const imps_krylov_params* p, double* 
                     imps_krylov_info* info) {
   if (!c || !p) return IMPS_E_ARG;
   return run_guarded(c, [&] {
-    require_real_state(c, "imps_eigsolve_c");
+
     need_match(c);
     need_envs(c);
     site = wrap(c, site);
     EigOut o = eigsolve_c(c, site, p->tol, p->krylovdim, p->maxiter, p->eager != 0, p->prec);
     if (lambda) *lambda = o.lambda;
     fill_info(info, o.info);
-    if (vec) d2h(c, vec, c->pC[site], (size_t)c->D * c->D);
+    if (vec && c->cplx) {
+      extract_to_host(c, c->pC[site], c->D / 2, 1, c->D / 2, std::sqrt(2.0), vec);
+    } else if (vec) {
+      d2h(c, vec, c->pC[site], (size_t)c->D * c->D);
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }

@@ -174,11 +174,11 @@ def test_complex_state_refuses_real_only_entry_points(imps, oracle, cx):
         st = _as_lib(imps, psi)
         envs = imps.DMRGCache(st, H, ctx=c)
         with pytest.raises(_lib.ImpsError, match="status -5"):
-            imps.leftenv(envs, 0)
+            imps.regauge_(envs, 0)
         with pytest.raises(_lib.ImpsError, match="status -5"):
             imps.expectationvalue(st, (0, np.array([[0.0, 1.0], [0.0, 0.0]])), envs)
-        with pytest.raises(_lib.ImpsError, match="status -5"):
-            imps.AC_hamiltonian(0, st, H, st, envs)(psi.AC[0].real.copy())
+        with pytest.raises(ValueError, match="complex"):
+            imps.dot(st, st, envs)
         # symmetric operators are whole on complex states
         sz = oracle.sigma_z()
         assert abs(imps.expectationvalue(st, (0, sz), envs) - cx.local_expectation1(psi, 0, sz).real) < 1e-12
@@ -231,3 +231,39 @@ def test_complex_tdvp_step_matches_oracle_at_d24(imps, oracle, cx):
             for a, b in zip(st.AL + st.C, ref.AL + ref.C):
                 assert rel(a, b) < 1e-7
             assert abs(imps.expectationvalue(st, H, envs) - e_ref) < 1e-9
+
+
+def test_complex_primitives_match_oracle(imps, oracle, cx):
+    # the boundary primitives on a complex state (ComplexF64 arrays in and out): environments,
+    # H_AC, H_C, the transfer matrix and the lowest H_AC eigenpair against the dense complex
+    # restatement
+    W = oracle.xxz_mpo(1.0, 0.8)
+    Ws = [W, W]
+    psi = cx.canonical_from_tensors(_complex_raw(oracle, 6, [2, 2], 61))
+    env = cx.dmrg_cache(psi, Ws)
+    rng = np.random.default_rng(3)
+    print("complex: primitives at D=6", flush=True)
+    H = imps.SparseIMPO(Ws)
+    with imps.Context(0) as c:
+        st = _as_lib(imps, psi)
+        envs = imps.DMRGCache(st, H, tol=1e-13, ctx=c)
+        for l in range(2):
+            assert rel(imps.leftenv(envs, l), env.leftenv(l)) < 1e-9
+            assert rel(imps.rightenv(envs, l), env.rightenv(l)) < 1e-9
+        x = rng.standard_normal((6, 2, 6)) + 1j * rng.standard_normal((6, 2, 6))
+        y = imps.AC_hamiltonian(0, st, H, st, envs)(x)
+        assert np.iscomplexobj(y) and rel(y, cx.hac_apply(env.leftenv(0), W, env.rightenv(0), x)) < 1e-12
+        xc = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+        yc = imps.C_hamiltonian(0, st, H, st, envs)(xc)
+        assert rel(yc, cx.hc_apply(env.leftenv(1), env.rightenv(0), xc)) < 1e-12
+        v = rng.standard_normal(36) + 1j * rng.standard_normal(36)
+        tv = imps.TransferMatrix(envs, "left")(v)
+        assert rel(tv, cx.transfer_left_matrix(psi.AL, psi.AL) @ v) < 1e-12
+        lam, vec, info = imps.fixedpoint(imps.AC_hamiltonian(0, st, H, st, envs), psi.AC[0], "SR",
+                                         imps.Lanczos(tol=1e-12))
+        Hd = cx.dense_operator(lambda z: cx.hac_apply(env.leftenv(0), W, env.rightenv(0), z), (6, 2, 6))
+        w, V = np.linalg.eigh(0.5 * (Hd + Hd.conj().T))
+        assert abs(lam - w[0]) < 1e-9
+        vv = vec.reshape(-1, order="F")
+        assert abs(np.linalg.norm(vv) - 1) < 1e-10 and abs(abs(np.vdot(V[:, 0], vv)) - 1) < 1e-8
+
