@@ -406,6 +406,19 @@ int imps_set_state_complex(imps_ctx* ctx, int nsites, int D, const double* const
  * constructors.jl:13-22): gaugefix!(:LR) from C0 = I as imps_set_state_raw. */
 int imps_set_state_raw_complex(imps_ctx* ctx, int nsites, int D, const double* const* A, double tol, int maxiter);
 
+/* CanonicalIMPS{ComplexF64} with bond dimensions that differ from bond to bond (the complex twin
+ * of imps_set_state_nonuniform: chi[l] complex bond dimensions, D = max chi, tensors at (D, d, D)
+ * with zeros outside their active blocks).  The embedding keeps the profile (complex index a is
+ * rows 2a and 2a + 1, so the active block of chi complex indices is the first 2 chi rows), and
+ * imps_bond_dims reports chi. */
+int imps_set_state_complex_nonuniform(imps_ctx* ctx, int nsites, int D, const int* chi, const double* const* AL,
+                                      const double* const* AR, const double* const* C, const double* const* AC);
+
+/* CanonicalIMPS(As) for complex raw tensors with bond dimensions chi (canonicalmps.jl:151-170):
+ * gaugefix!(:LR) from C0 = I on the active block, as imps_set_state_raw_nonuniform. */
+int imps_set_state_raw_complex_nonuniform(imps_ctx* ctx, int nsites, int D, const int* chi, const double* const* A,
+                                          double tol, int maxiter);
+
 /* The complex state's AL, AR, C, AC (any pointer array or entry may be NULL); IMPS_E_STATE for a
  * real state. */
 int imps_get_state_complex(imps_ctx* ctx, double* const* AL, double* const* AR, double* const* C,

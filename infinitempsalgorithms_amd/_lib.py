@@ -149,6 +149,9 @@ SIGNATURES = [
     ("imps_state_is_complex", c_int, [c_void_p, P_int]),
     ("imps_set_state_nonuniform", c_int, [c_void_p, c_int, c_int, P_int, PP_double, PP_double, PP_double, PP_double]),
     ("imps_set_state_raw_nonuniform", c_int, [c_void_p, c_int, c_int, P_int, PP_double, c_double, c_int]),
+    ("imps_set_state_complex_nonuniform", c_int,
+     [c_void_p, c_int, c_int, P_int, PP_double, PP_double, PP_double, PP_double]),
+    ("imps_set_state_raw_complex_nonuniform", c_int, [c_void_p, c_int, c_int, P_int, PP_double, c_double, c_int]),
     ("imps_bond_dims", c_int, [c_void_p, P_int]),
     ("imps_correlator", c_int, [c_void_p, P_double, P_double, c_int, c_int, c_void_p, P_double]),
     ("imps_dot_state", c_int, [c_void_p, c_int, PP_double, P_double]),

@@ -152,10 +152,10 @@ class Context:
                 raise ValueError(f"{what}: site {l} tensor must be ({D}, {self.dims[l % self.N]}, {D}), got {x.shape}")
 
     def set_state(self, psi: "CanonicalIMPS"):
-        if psi.is_complex():
-            return self._set_state_complex(psi)
         if not psi.uniform_bonds():
             return self._set_state_nonuniform(psi)
+        if psi.is_complex():
+            return self._set_state_complex(psi)
         self.chi = None
         AL, AR, C, AC = ([fortran(x) for x in xs] for xs in (psi.AL, psi.AR, psi.C, psi.AC))
         D = psi.D
@@ -173,8 +173,10 @@ class Context:
 
     def _set_state_nonuniform(self, psi: "CanonicalIMPS"):
         # bond dimensions that differ from bond to bond: every tensor at the largest one with
-        # zeros outside its active block (imps_set_state_nonuniform)
+        # zeros outside its active block (imps_set_state_nonuniform; complex states:
+        # imps_set_state_complex_nonuniform)
         chi = psi.bonddims()
+        cplx = psi.is_complex()
         N = len(psi)
         for l in range(N):
             want = (chi[(l - 1) % N], self.dims[l % self.N] if self.d else psi.AL[l].shape[1], chi[l])
@@ -185,16 +187,18 @@ class Context:
                 raise ValueError(f"set_state: C[{l}] must be ({chi[l]}, {chi[l]})")
         D = max(chi)
         self.D, self.chi = D, list(chi)
-        AL, AR, AC = ([self.pad_site(l, x) for l, x in enumerate(xs)] for xs in (psi.AL, psi.AR, psi.AC))
-        C = [self.pad_bond(l, x) for l, x in enumerate(psi.C)]
-        keep = [ptr_array(x) for x in (AL, AR, C, AC)]
+        conv = zfortran if cplx else fortran
+        AL, AR, AC = ([self.pad_site(l, conv(x)) for l, x in enumerate(xs)] for xs in (psi.AL, psi.AR, psi.AC))
+        C = [self.pad_bond(l, conv(x)) for l, x in enumerate(psi.C)]
+        keep = [(zptr_array if cplx else ptr_array)(x) for x in (AL, AR, C, AC)]
         carr = (ctypes.c_int * N)(*chi)
         try:
-            self.call("imps_set_state_nonuniform", N, D, carr, *(k[0] for k in keep))
+            self.call("imps_set_state_complex_nonuniform" if cplx else "imps_set_state_nonuniform", N, D, carr,
+                      *(k[0] for k in keep))
         except Exception:
             self.chi = None
             raise
-        self.cplx = False
+        self.cplx = cplx
 
     def _set_state_complex(self, psi: "CanonicalIMPS"):
         # CanonicalIMPS{ComplexF64}: the library holds the exact real embedding (imps_set_state_complex)
@@ -218,19 +222,19 @@ class Context:
         if any(As[l].shape[0] != chi[(l - 1) % N] for l in range(N)):
             raise ValueError("set_state_raw: bond dimensions of neighbouring tensors differ")
         if len(set(chi)) > 1:
-            if any(np.iscomplexobj(a) for a in As):
-                raise ValueError("set_state_raw: complex states take one bond dimension")
+            cplx = any(np.iscomplexobj(a) for a in As)
             D = max(chi)
             self.D, self.chi = D, list(chi)
-            Ap = [self.pad_site(l, fortran(a)) for l, a in enumerate(As)]
-            pp, keep = ptr_array(Ap)
+            Ap = [self.pad_site(l, zfortran(a) if cplx else fortran(a)) for l, a in enumerate(As)]
+            pp, keep = (zptr_array if cplx else ptr_array)(Ap)
             carr = (ctypes.c_int * N)(*chi)
             try:
-                self.call("imps_set_state_raw_nonuniform", N, D, carr, pp, float(tol), int(maxiter))
+                self.call("imps_set_state_raw_complex_nonuniform" if cplx else "imps_set_state_raw_nonuniform",
+                          N, D, carr, pp, float(tol), int(maxiter))
             except Exception:
                 self.chi = None
                 raise
-            self.cplx = False
+            self.cplx = cplx
             return
         self.chi = None
         cplx = any(np.iscomplexobj(a) for a in As)

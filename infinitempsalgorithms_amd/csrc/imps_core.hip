@@ -3339,7 +3339,8 @@ int imps_bond_dims(imps_ctx* c, int* chi) {
   return run_guarded(c, [&] {
     need_state(c);
     const int N = (int)c->AL.size();
-    for (int l = 0; l < N; ++l) chi[l] = c->chi.empty() ? (c->cplx ? c->D / 2 : c->D) : c->chi[l];
+    const int f = c->cplx ? 2 : 1;  // a complex state's profile is held on its embedding
+    for (int l = 0; l < N; ++l) chi[l] = (c->chi.empty() ? c->D : c->chi[l]) / f;
   });
 }
 
@@ -3414,33 +3415,96 @@ int imps_set_state_complex(imps_ctx* c, int nsites, int D, const double* const* 
   });
 }
 
+// the embedding of a complex bond profile: complex index a occupies rows 2a and 2a + 1, so a
+// complex active block of chi indices is the real active block of 2 chi
+static void set_profile_c(imps_ctx* c, const int* chi, int nsites) {
+  c->chi.resize(nsites);
+  for (int l = 0; l < nsites; ++l) c->chi[l] = 2 * chi[l];
+}
+
+static void set_state_raw_complex_impl(imps_ctx* c, int nsites, int D, const double* const* A, double tol,
+                                       int maxiter, const int* chi) {
+  need_mpo(c);
+  require(nsites == c->N && D >= 1 && A, IMPS_E_ARG, "imps_set_state_raw_complex: unit cell mismatch");
+  for (int l = 0; l < nsites; ++l) require(A[l] != nullptr, IMPS_E_ARG, "imps_set_state_raw_complex: null tensor");
+  require_empty_sectors_c(c, D, A, "imps_set_state_raw_complex");
+  const int D2 = 2 * D;
+  alloc_state(c, nsites, D2, c->d);
+  c->cplx = true;
+  if (chi) set_profile_c(c, chi, nsites);
+  const size_t nC = (size_t)D2 * D2;
+  std::vector<double*> raw(nsites);
+  for (int l = 0; l < nsites; ++l) {
+    raw[l] = c->pAC[l];
+    embed_from_host(c, A[l], D, c->d, D, 1.0, raw[l]);
+  }
+  double* I = ws(c, "sr_I", nC);
+  HIPCHK(launch_set_identity(D2, I, D2, c->stream));
+  if (chi) HIPCHK(launch_mask_block(D2, D2, c->chi[nsites - 1], c->chi[nsites - 1], I, D2, c->stream));
+  uniform_leftorth(c, raw, I, tol, maxiter);
+  double* C0 = ws(c, "sr_C0", nC);
+  d2d(c, C0, c->C[nsites - 1], nC);
+  uniform_rightorth(c, c->AL, C0, tol, maxiter);
+  for (int l = 0; l < nsites; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
+  project_state_c(c);
+  init_pending(c);
+  c->envs_valid = false;
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 int imps_set_state_raw_complex(imps_ctx* c, int nsites, int D, const double* const* A, double tol, int maxiter) {
   if (!c || !A) return IMPS_E_ARG;
+  return run_guarded(c, [&] { set_state_raw_complex_impl(c, nsites, D, A, tol, maxiter, nullptr); });
+}
+
+// the complex twin of require_outside_zero (interleaved re, im)
+static void require_outside_zero_c(imps_ctx* c, int D, const int* chi, const double* const* A, bool mps,
+                                   const char* what) {
+  if (!A) return;
+  const int N = c->N, m = mps ? c->d : 1;
+  for (int l = 0; l < N; ++l) {
+    if (!A[l]) continue;
+    const int ra = mps ? chi[(l - 1 + N) % N] : chi[l], ca = chi[l];
+    for (int b = 0; b < D; ++b)
+      for (int k = 0; k < m; ++k)
+        for (int a = 0; a < D; ++a)
+          if (a >= ra || b >= ca) {
+            const size_t e = 2 * (a + (size_t)D * (k + (size_t)m * b));
+            require(A[l][e] == 0.0 && A[l][e + 1] == 0.0, IMPS_E_ARG,
+                    std::string(what) + ": site " + std::to_string(l) + " has weight outside its bond dimensions");
+          }
+  }
+}
+
+int imps_set_state_complex_nonuniform(imps_ctx* c, int nsites, int D, const int* chi, const double* const* AL,
+                                      const double* const* AR, const double* const* Cm, const double* const* AC) {
+  if (!c || !chi || !AL || !AR || !Cm) return IMPS_E_ARG;
+  int st = run_guarded(c, [&] {
+    need_mpo(c);
+    check_bond_profile(c, nsites, D, chi, "imps_set_state_complex_nonuniform");
+    for (int l = 0; l < nsites; ++l)
+      require(AL[l] && AR[l] && Cm[l] && (!AC || AC[l]), IMPS_E_ARG, "imps_set_state_complex_nonuniform: null tensor");
+    require_outside_zero_c(c, D, chi, AL, true, "imps_set_state_complex_nonuniform");
+    require_outside_zero_c(c, D, chi, AR, true, "imps_set_state_complex_nonuniform");
+    require_outside_zero_c(c, D, chi, AC, true, "imps_set_state_complex_nonuniform");
+    require_outside_zero_c(c, D, chi, Cm, false, "imps_set_state_complex_nonuniform");
+  });
+  if (st != IMPS_OK) return st;
+  st = imps_set_state_complex(c, nsites, D, AL, AR, Cm, AC);
+  if (st != IMPS_OK) return st;
+  return run_guarded(c, [&] { set_profile_c(c, chi, nsites); });
+}
+
+int imps_set_state_raw_complex_nonuniform(imps_ctx* c, int nsites, int D, const int* chi, const double* const* A,
+                                          double tol, int maxiter) {
+  if (!c || !chi || !A) return IMPS_E_ARG;
   return run_guarded(c, [&] {
     need_mpo(c);
-    require(nsites == c->N && D >= 1, IMPS_E_ARG, "imps_set_state_raw_complex: unit cell mismatch");
-    for (int l = 0; l < nsites; ++l) require(A[l] != nullptr, IMPS_E_ARG, "imps_set_state_raw_complex: null tensor");
-    require_empty_sectors_c(c, D, A, "imps_set_state_raw_complex");
-    const int D2 = 2 * D;
-    alloc_state(c, nsites, D2, c->d);
-    c->cplx = true;
-    const size_t nC = (size_t)D2 * D2;
-    std::vector<double*> raw(nsites);
-    for (int l = 0; l < nsites; ++l) {
-      raw[l] = c->pAC[l];
-      embed_from_host(c, A[l], D, c->d, D, 1.0, raw[l]);
-    }
-    double* I = ws(c, "sr_I", nC);
-    HIPCHK(launch_set_identity(D2, I, D2, c->stream));
-    uniform_leftorth(c, raw, I, tol, maxiter);
-    double* C0 = ws(c, "sr_C0", nC);
-    d2d(c, C0, c->C[nsites - 1], nC);
-    uniform_rightorth(c, c->AL, C0, tol, maxiter);
-    for (int l = 0; l < nsites; ++l) mul_AL_C(c, c->AL[l], c->C[l], c->AC[l]);
-    project_state_c(c);
-    init_pending(c);
-    c->envs_valid = false;
-    HIPCHK(hipStreamSynchronize(c->stream));
+    check_bond_profile(c, nsites, D, chi, "imps_set_state_raw_complex_nonuniform");
+    for (int l = 0; l < nsites; ++l)
+      require(A[l] != nullptr, IMPS_E_ARG, "imps_set_state_raw_complex_nonuniform: null tensor");
+    require_outside_zero_c(c, D, chi, A, true, "imps_set_state_raw_complex_nonuniform");
+    set_state_raw_complex_impl(c, nsites, D, A, tol, maxiter, chi);
   });
 }
 

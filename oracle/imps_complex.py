@@ -12,7 +12,8 @@ the library's embedded path independently:
   (src/algorithms/transfermatrix.jl:5-26: conj on the bra layer) and the slice pushes
   (src/algorithms/groundstates/envs.jl:104-114);
 * regularize! without conjugation (envs.jl:28-32) and DMRGCache's level recursion
-  (envs.jl:321-518, square Schur tensors), each linsolve done as a dense solve of
+  (envs.jl:321-518, square Schur tensors, bond dimensions that may differ from bond to bond),
+  each linsolve done as a dense solve of
   (1 - T) x = b;
 * H_AC / H_C (effective.jl:72-102), the TDVP integrator as the exact exponential
   scipy.linalg.expm(dt H) of the dense effective Hamiltonian (tdvp.jl:14-19: the Lanczos
@@ -269,14 +270,18 @@ def dmrg_cache(psi, Ws):
     levels in Schur order, identity levels by the regularised solve (ρ = C C^H on the left,
     C^H C on the right), other diagonal levels by (1 - T_W) x = b, each solve dense."""
     N = len(psi)
-    D = psi.AL[0].shape[0]
     w = Ws[0].shape[0]
-    lefts = [np.zeros((D, w, D), dtype=complex) for _ in range(N)]
-    rights = [np.zeros((D, w, D), dtype=complex) for _ in range(N)]
-    I = np.eye(D, dtype=complex)
+    # bond dimensions may differ from bond to bond: leftenv(l) lives on the bond left of site l,
+    # rightenv(l) on the bond right of it (envs.jl:327-343)
+    Dl = [psi.AL[l].shape[0] for l in range(N)]
+    Dr = [psi.AR[l].shape[2] for l in range(N)]
+    lefts = [np.zeros((Dl[l], w, Dl[l]), dtype=complex) for l in range(N)]
+    rights = [np.zeros((Dr[l], w, Dr[l]), dtype=complex) for l in range(N)]
+    IL = [np.eye(Dl[l], dtype=complex) for l in range(N)]
+    IR = [np.eye(Dr[l], dtype=complex) for l in range(N)]
     for l in range(N):
-        lefts[l][:, 0, :] = I
-        rights[l][:, w - 1, :] = I
+        lefts[l][:, 0, :] = IL[l]
+        rights[l][:, w - 1, :] = IR[l]
     AL, AR, C = psi.AL, psi.AR, psi.C
     for i in range(1, w):
         _left_cyclethrough(lefts, Ws, AL, i)
@@ -287,13 +292,13 @@ def dmrg_cache(psi, Ws):
             def op(X, rho=rho):
                 for l in range(N):
                     X = push_left2(X, AL[l], AL[l])
-                return regularize(X, rho, I)
+                return regularize(X, rho, IL[0])
 
             lefts[0][:, i, :] = dense_solve(op, rhs)
             if N > 1:
                 _left_cyclethrough(lefts, Ws, AL, i)
             for l in range(N):
-                regularize(lefts[l][:, i, :], C[l - 1] @ C[l - 1].conj().T, I)
+                regularize(lefts[l][:, i, :], C[l - 1] @ C[l - 1].conj().T, IL[l])
         else:
             if not ro.isemptylevel(Ws, i):
                 def op(X, i=i):
@@ -313,13 +318,13 @@ def dmrg_cache(psi, Ws):
             def op(X, rho=rho):
                 for l in range(N - 1, -1, -1):
                     X = push_right2(X, AR[l], AR[l])
-                return regularize(X, rho, I)
+                return regularize(X, rho, IR[N - 1])
 
             rights[N - 1][:, i, :] = dense_solve(op, rhs)
             if N > 1:
                 _right_cyclethrough(rights, Ws, AR, i)
             for l in range(N):
-                regularize(rights[l][:, i, :], C[l].conj().T @ C[l], I)
+                regularize(rights[l][:, i, :], C[l].conj().T @ C[l], IR[l])
         else:
             if not ro.isemptylevel(Ws, i):
                 def op(X, i=i):

@@ -143,3 +143,94 @@ def test_larger_profile_vumps_matches_oracle(imps, oracle):
         k = min(4, len(info.losses), len(info_o.losses))
         for a, b in zip(info.losses[:k], info_o.losses[:k]):
             assert abs(a - b) <= 1e-6 * abs(b)
+
+
+# ---- complex states with a bond profile, against the complex restatement (oracle/imps_complex.py)
+
+
+def _cx():
+    from oracle import imps_complex
+    return imps_complex
+
+
+def _complex_profile_raw(imps, chi, seed):
+    re = imps.random_tensors(chi, [2] * len(chi), seed)
+    im = imps.random_tensors(chi, [2] * len(chi), seed + 1000)
+    return [a + 0.4j * b for a, b in zip(re, im)]
+
+
+def _lib_state_c(imps, p):
+    return imps.CanonicalIMPS([np.array(x) for x in p.AL], [np.array(x) for x in p.AR], [np.array(x) for x in p.C],
+                              [np.array(x) for x in p.AC])
+
+
+def test_complex_nonuniform_canonical_form_and_environments(imps, oracle):
+    cx = _cx()
+    chi = [2, 4, 3]
+    W = oracle.xxz_mpo(1.0, 0.8)
+    Ws = [W] * 3
+    H = imps.SparseIMPO(Ws)
+    raw = _complex_profile_raw(imps, chi, 61)
+    ref = cx.canonical_from_tensors(raw)
+    print("bonds: complex CanonicalIMPS(As) of raw [2, 4, 3] tensors", flush=True)
+    with imps.Context(0) as c:
+        got, _ = imps.canonicalimps(raw, H, ctx=c, tol=1e-14, maxiter=2000)
+        assert got.is_complex() and got.bonddims() == chi
+        for a, b in zip(got.AL + got.AR + got.C + got.AC, ref.AL + ref.AR + ref.C + ref.AC):
+            assert a.shape == b.shape and rel(a, b) < 1e-9
+    renv = cx.dmrg_cache(ref, Ws)
+    print("bonds: environments of a complex [2, 4, 3] state", flush=True)
+    with imps.Context(0) as c:
+        st = _lib_state_c(imps, ref)
+        envs = imps.DMRGCache(st, H, tol=1e-13, ctx=c)
+        for l in range(3):
+            assert rel(imps.leftenv(envs, l), renv.leftenv(l)) < 1e-8
+            assert rel(imps.rightenv(envs, l), renv.rightenv(l)) < 1e-8
+        assert abs(imps.expectationvalue(st, H, envs) - cx.expectation_energy(ref, Ws, renv).real) < 1e-10
+        back = c.get_state()
+        assert back.bonddims() == chi and back.is_complex()
+        for a, b in zip(back.AL + back.C, ref.AL + ref.C):
+            assert rel(a, b) < 1e-13
+
+
+@pytest.mark.parametrize("dt", [-0.04j, -0.03 - 0.02j])
+def test_complex_nonuniform_tdvp_step_matches_oracle(imps, oracle, dt):
+    cx = _cx()
+    chi = [2, 4, 3]
+    W = oracle.tfim_mpo(1.0, 0.9)
+    Ws = [W] * 3
+    H = imps.SparseIMPO(Ws)
+    psi = cx.canonical_from_tensors(_complex_profile_raw(imps, chi, 71))
+    env = cx.dmrg_cache(psi, Ws)
+    p2, env2 = cx.tdvp_timestep(psi, Ws, env, dt)
+    print(f"bonds: complex TDVP step on a [2, 4, 3] state, dt={dt}", flush=True)
+    with imps.Context(0) as c:
+        st = _lib_state_c(imps, psi)
+        envs = imps.DMRGCache(st, H, tol=1e-13, ctx=c)
+        g2, envs = imps.timestep(st, H, dt, imps.TDVP(tol=1e-12, gauge_tol=1e-14, gauge_maxiter=2000), envs)
+        assert g2.is_complex() and g2.bonddims() == chi
+        for a, b in zip(g2.AL + g2.AR + g2.C + g2.AC, p2.AL + p2.AR + p2.C + p2.AC):
+            assert a.shape == b.shape and rel(a, b) < 1e-8
+        assert abs(imps.expectationvalue(g2, H, envs) - cx.expectation_energy(p2, Ws, env2).real) < 1e-10
+
+
+def test_complex_nonuniform_refuses_weight_outside_the_profile(imps, oracle):
+    import ctypes
+
+    from infinitempsalgorithms_amd import _lib
+    chi = [2, 4, 3]
+    H = imps.SparseIMPO([oracle.tfim_mpo(1.0, 0.9)] * 3)
+    raw = _complex_profile_raw(imps, chi, 81)
+    with imps.Context(0) as c:
+        c.set_mpo(H)
+        D = max(chi)
+        padded = []
+        for l, a in enumerate(raw):
+            z = np.zeros((D, 2, D), dtype=complex, order="F")
+            z[:a.shape[0], :, :a.shape[2]] = a
+            padded.append(z)
+        padded[1][3, 0, 3] = 1e-3j  # row 3 lies outside site 1's left bond (chi[0] = 2)
+        pp, keep = _lib.zptr_array(padded)
+        carr = (ctypes.c_int * 3)(*chi)
+        with pytest.raises(_lib.ImpsError, match="outside its bond dimensions"):
+            c.call("imps_set_state_raw_complex_nonuniform", 3, D, carr, pp, 1e-12, 100)
