@@ -6,6 +6,7 @@
 // Reference files cited below are relative to the InfiniteMPSAlgorithms.jl root.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -3954,6 +3955,15 @@ int imps_vumps_default_params(imps_vumps_params* p) {
 
 namespace {
 // updatetol(alg_*, iter, eps) (utility.jl:150-158) for the three solvers of imps_vumps_params
+// A named roctx range over a scope: rocprofv3 --marker-trace records the phases of the solvers'
+// iterations; without a tool attached the calls return at once.
+struct RoctxRange {
+  explicit RoctxRange(const char* name) { roctxRangePushA(name); }
+  ~RoctxRange() { roctxRangePop(); }
+  RoctxRange(const RoctxRange&) = delete;
+  RoctxRange& operator=(const RoctxRange&) = delete;
+};
+
 struct SolverTols {
   const imps_vumps_params* p;
   double eig(int it, double eps) const {
@@ -4013,13 +4023,14 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     EnvStats est;
     double tl = 0, tg = 0, te = 0, tk = 0, tf = 0;
     int sweeps_total = 0;
-    auto lap = [&](double& acc, auto&& fn) {
+    auto lap = [&](const char* phase, double& acc, auto&& fn) {
+      RoctxRange range(phase);
       auto a = std::chrono::steady_clock::now();
       fn();
       HIPCHK(hipStreamSynchronize(c->stream));
       acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     };
-    if (!c->envs_valid) lap(te, [&] { envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->envs_krylovdim, &est); });
+    if (!c->envs_valid) lap("vumps_envs", te, [&] { envs_recalculate(c, 1.0e-10, p->envs_maxiter, p->envs_krylovdim, &est); });
     int prec = p->prec;
     bool refining = false;  // refinement phase with JD corrections in p->prec (refine_eig)
     const long rf0 = g_refine_fallbacks;
@@ -4028,14 +4039,15 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
     double best_eps = std::numeric_limits<double>::infinity();
     bool stalled = false;
     double eps = 0.0;
-    lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
-    lap(te, [&] { envs_recalculate(c, stol.envs(0, eps), p->envs_maxiter, p->envs_krylovdim, &est); });
+    lap("vumps_galerkin", tk, [&] { eps = galerkin(c, PREC_FP64); });
+    lap("vumps_envs", te, [&] { envs_recalculate(c, stol.envs(0, eps), p->envs_maxiter, p->envs_krylovdim, &est); });
     int nl = 0;
     if (losses && nl < nloss) losses[nl++] = eps;
     bool converged = false;
     double f = 0.0, t_reach = -1.0;
     int it = 0;
     for (it = 1; it <= p->maxiter; ++it) {
+      RoctxRange iter_range("vumps_iteration");
       const int siter = it - 1;
       // FP64 refinement: below refine_below, or as soon as the reduced-precision phase
       // stops improving (an iteration after the third that gains less than 10 % on the best
@@ -4055,7 +4067,7 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
       const double floor_rel = (prec == PREC_FP64 || refining) ? 0.0
                                : p->eig_floor > 0.0 ? p->eig_floor
                                                     : (prec == PREC_FP32 ? 1e-6 : 1e-4);
-      lap(tl, [&] {
+      lap("vumps_localupdate", tl, [&] {
         std::vector<double> ta(N), tc(N);
         for (int site = 0; site < N; ++site) {
           ta[site] = std::max(tol_eig, floor_rel * std::max(1.0, std::fabs(lam_ac[site])));
@@ -4064,17 +4076,17 @@ int imps_vumps_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
         local_update(c, ta, tc, p->krylovdim, p->eig_maxiter, p->eig_eager != 0, prec, lam_ac, lam_c, refining);
       });
       int sw = 0;
-      lap(tg, [&] { gauge_step(c, stol.gauge(siter, eps), p->gauge_maxiter, &sw, p->gauge_eig_miniter); });
+      lap("vumps_gauge", tg, [&] { gauge_step(c, stol.gauge(siter, eps), p->gauge_maxiter, &sw, p->gauge_eig_miniter); });
       sweeps_total += sw;
-      lap(te, [&] { envs_recalculate(c, stol.envs(siter, eps), p->envs_maxiter, p->envs_krylovdim, &est); });
+      lap("vumps_envs", te, [&] { envs_recalculate(c, stol.envs(siter, eps), p->envs_maxiter, p->envs_krylovdim, &est); });
       // finalize (vumps.jl:75), between recalculate! and calc_galerkin
       bool fstop = false;
-      lap(te, [&] { fstop = run_finalize(c, p, it, stol.envs(siter, eps), &est); });
-      lap(tk, [&] { eps = galerkin(c, PREC_FP64); });
+      lap("vumps_finalize", te, [&] { fstop = run_finalize(c, p, it, stol.envs(siter, eps), &est); });
+      lap("vumps_galerkin", tk, [&] { eps = galerkin(c, PREC_FP64); });
       stalled = eps > 0.9 * best_eps;
       best_eps = std::min(best_eps, eps);
       if (losses && nl < nloss) losses[nl++] = eps;
-      lap(tf, [&] { f = energy(c); });
+      lap("vumps_energy", tf, [&] { f = energy(c); });
       if (p->verbosity > 0) {
         printf("VUMPS iter %4d : eps = %.3e f = %.8g  [%.2fs: local %.2f gauge %.2f (%d sweeps) envs %.2f]\n", it, eps, f,
                secs(), tl, tg, sw, te);
@@ -4213,6 +4225,7 @@ int imps_idmrg_run(imps_ctx* c, const imps_vumps_params* p, imps_vumps_info* inf
 // tensors are projected back onto the embedded form at the end.
 static void tdvp_step(imps_ctx* c, std::complex<double> dt, double tol, int krylovdim, int maxiter,
                       double gauge_tol, int gauge_maxiter) {
+  RoctxRange range("tdvp_step");
   need_match(c);
   require(std::isfinite(dt.real()) && std::isfinite(dt.imag()), IMPS_E_ARG, "imps_tdvp_timestep: dt must be finite");
   require(dt.imag() == 0.0 || c->cplx, IMPS_E_STATE,
