@@ -383,6 +383,15 @@ void regularize_dev(imps_ctx* c, int n, double* v, long ldv, const double* l, in
   HIPCHK(launch_reduce_add_diag(np, part, -1.0, v, ldv, (nact >= 0 && nact < n) ? nact : n, nullptr, c->stream));
 }
 
+// A named roctx range over a scope: rocprofv3 --marker-trace records the phases of the solvers'
+// iterations; without a tool attached the calls return at once.
+struct RoctxRange {
+  explicit RoctxRange(const char* name) { roctxRangePushA(name); }
+  ~RoctxRange() { roctxRangePop(); }
+  RoctxRange(const RoctxRange&) = delete;
+  RoctxRange& operator=(const RoctxRange&) = delete;
+};
+
 // ------------------------------------------------------------------ GEMM wrappers
 
 // GEMM tile autotuning (process-wide, on by default): the first GEMM of each shape, stride and
@@ -1899,6 +1908,7 @@ int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* 
   double eps = std::numeric_limits<double>::infinity();
   while (true) {
     if (it >= eig_miniter) {
+      RoctxRange range("gauge_eigvec_refresh");
       const double tol_e = updatetol(1.0, 1, eps * eps);
       Op tm = [&](const double* v, double* out) {
         double* t0 = ws(c, "ro_t0", DD);
@@ -1917,6 +1927,7 @@ int uniform_rightorth(imps_ctx* c, const std::vector<double*>& A, const double* 
     // the sweep runs with the Cholesky breakdown checks deferred to its end (one host round trip
     // per sweep), as one captured kernel sequence (run_graphed); on a breakdown it is redone from
     // its input with per-site checks and the Householder fallback
+    RoctxRange sweep_range("gauge_lq_sweep");
     std::vector<uintptr_t> key{1, (uintptr_t)N, (uintptr_t)D, (uintptr_t)d};
     for (int i = 0; i < N; ++i)
       for (const void* q : {(const void*)A[i], (const void*)c->C[i], (const void*)c->AR[i]}) key.push_back((uintptr_t)q);
@@ -3955,15 +3966,6 @@ int imps_vumps_default_params(imps_vumps_params* p) {
 
 namespace {
 // updatetol(alg_*, iter, eps) (utility.jl:150-158) for the three solvers of imps_vumps_params
-// A named roctx range over a scope: rocprofv3 --marker-trace records the phases of the solvers'
-// iterations; without a tool attached the calls return at once.
-struct RoctxRange {
-  explicit RoctxRange(const char* name) { roctxRangePushA(name); }
-  ~RoctxRange() { roctxRangePop(); }
-  RoctxRange(const RoctxRange&) = delete;
-  RoctxRange& operator=(const RoctxRange&) = delete;
-};
-
 struct SolverTols {
   const imps_vumps_params* p;
   double eig(int it, double eps) const {
