@@ -260,6 +260,11 @@ int imps_tangent_hessian_apply(imps_ctx* ctx, double rcond, double tol, const do
 
 /* expectationvalue(ψ, (site,) => O) (expval.jl:33-37), O: (d, d). */
 int imps_local_expectation(imps_ctx* ctx, int site, const double* O, double* val);
+/* expectationvalue(ψ, site => O) (expval.jl:33-37) for a ComplexF64 d x d operator O (column-major,
+ * re and im interleaved), on real and complex states: val[0] = Re <O>, val[1] = Im <O> (the value
+ * is real for a Hermitian O).  On a complex state this is the entry point for Hermitian operators
+ * with an imaginary part (Sy, currents), which imps_local_expectation refuses. */
+int imps_local_expectation_complex(imps_ctx* ctx, int site, const double* O, double* val);
 
 /* ---- the VUMPS driver ------------------------------------------------------------ */
 
@@ -389,7 +394,8 @@ int imps_tdvp_timestep_complex(imps_ctx* ctx, double dt_re, double dt_im, double
  * QR/LQ/Cholesky factors commute with the embedding.  On a context holding a complex state:
  * find_groundstate (imps_vumps_run), the environments (imps_envs_recalculate), imps_energy,
  * imps_galerkin, imps_gaugefix_right, both TDVP steps and imps_local_expectation /
- * imps_correlator with symmetric real operators work as for real states; imps_hac_apply,
+ * imps_correlator with symmetric real operators work as for real states;
+ * imps_local_expectation_complex takes any ComplexF64 one-site operator; imps_hac_apply,
  * imps_hc_apply, imps_transfer_apply, imps_get_env and the vectors of imps_eigsolve_ac/c take and
  * return ComplexF64 arrays of the complex bond dimension (eigenvectors of unit norm);
  * imps_state_dims reports the complex bond dimension; the entry points that pass real

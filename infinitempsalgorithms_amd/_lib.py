@@ -136,6 +136,7 @@ SIGNATURES = [
                                          P_double, ctypes.POINTER(ResponseInfo)]),
     ("imps_tangent_hessian_apply", c_int, [c_void_p, c_double, c_double, P_double, P_double]),
     ("imps_local_expectation", c_int, [c_void_p, c_int, P_double, P_double]),
+    ("imps_local_expectation_complex", c_int, [c_void_p, c_int, P_double, P_double]),
     ("imps_vumps_default_params", c_int, [ctypes.POINTER(VumpsParams)]),
     ("imps_vumps_run", c_int, [c_void_p, ctypes.POINTER(VumpsParams), ctypes.POINTER(VumpsInfo), P_double,
                                c_int, ITER_CB, c_void_p]),

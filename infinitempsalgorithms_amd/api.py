@@ -635,13 +635,19 @@ def expectationvalue(psi: CanonicalIMPS, op, envs: GPUCache):
         envs.ctx.call("imps_energy", ctypes.byref(val))
     else:
         site, O = op
-        if np.iscomplexobj(O) and np.any(np.imag(O) != 0):
-            raise ValueError("expectationvalue: the library takes real operators")
-        O = fortran(np.real(O))
         dl = psi.AL[int(site) % len(psi)].shape[1]
-        if O.shape != (dl, dl):
+        if np.shape(O) != (dl, dl):
             raise ValueError(f"expectationvalue: the operator at site {site} must be {dl} x {dl}")
-        O = envs.ctx.pad_operator(O)
+        if np.iscomplexobj(O) and np.any(np.imag(O) != 0):
+            # a complex operator (Sy, currents): the complex value, as the reference returns
+            d = envs.ctx.d or dl
+            Z = np.zeros((d, d), dtype=np.complex128, order="F")
+            Z[:dl, :dl] = O
+            out = np.zeros(2)
+            envs.ctx.call("imps_local_expectation_complex", int(site), Z.ctypes.data_as(_lib.P_double),
+                          out.ctypes.data_as(_lib.P_double))
+            return complex(out[0], out[1])
+        O = envs.ctx.pad_operator(fortran(np.real(O)))
         envs.ctx.call("imps_local_expectation", int(site), dptr(O), ctypes.byref(val))
     return val.value
 

@@ -3928,6 +3928,41 @@ int imps_local_expectation(imps_ctx* c, int site, const double* O, double* val) 
   });
 }
 
+// <O> for a complex d x d operator O = A + i B (A = Re O, B = Im O, real): <O> = z_A + i z_B with
+// z_R = <psi|R|psi>.  On a real state z_R is real.  On a complex state the real trace of the
+// embedding gives Re z_R, and Im z_R = Re <psi|R|(-i) psi>, with -i psi the row-pair rotation of AC.
+int imps_local_expectation_complex(imps_ctx* c, int site, const double* O, double* val) {
+  if (!c || !O || !val) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_match(c);
+    site = wrap(c, site);
+    const int d = c->d;
+    std::vector<double> Ar((size_t)d * d), Bi((size_t)d * d);
+    for (size_t k = 0; k < (size_t)d * d; ++k) {
+      Ar[k] = O[2 * k];
+      Bi[k] = O[2 * k + 1];
+    }
+    const size_t n = (size_t)c->D * d * c->D;
+    double* t = ws(c, "lx_t", n);
+    auto expect = [&](const std::vector<double>& R, const double* ket) {
+      HIPCHK(launch_mix_mid(c->D, d, c->D, R.data(), ket, t, 0.0, c->stream));
+      return dot(c, n, c->AC[site], t);
+    };
+    const double zAr = expect(Ar, c->AC[site]), zBr = expect(Bi, c->AC[site]);
+    double zAi = 0.0, zBi = 0.0;
+    if (c->cplx) {
+      double* m = ws(c, "lx_mi", n);
+      HIPCHK(launch_fill(n, 0.0, m, c->stream));
+      HIPCHK(launch_cplx_axpy_i(n, -1.0, c->AC[site], m, c->stream));
+      zAi = expect(Ar, m);
+      zBi = expect(Bi, m);
+    }
+    const double np = dot_psi(c);
+    val[0] = (zAr - zBi) / np;
+    val[1] = (zAi + zBr) / np;
+  });
+}
+
 int imps_vumps_default_params(imps_vumps_params* p) {
   if (!p) return IMPS_E_ARG;
   p->tol = 1.0e-10;

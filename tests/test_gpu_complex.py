@@ -111,6 +111,8 @@ def test_product_state_precession_on_device(imps, oracle, cx):
             ex, ey = cx.larmor(h, 0.1 * k)
             assert abs(imps.expectationvalue(st, (0, sx), envs) - ex) < 1e-10
             assert abs(cx.local_expectation1(st, 0, np.array([[0, -1j], [1j, 0]])) - ey) < 1e-10
+            # the same <sy> on the device (a complex Hermitian operator)
+            assert abs(imps.expectationvalue(st, (0, np.array([[0, -1j], [1j, 0]])), envs) - ey) < 1e-10
             assert rel(st.AC[0], ref.AC[0]) < 1e-10
 
 
@@ -267,3 +269,30 @@ def test_complex_primitives_match_oracle(imps, oracle, cx):
         vv = vec.reshape(-1, order="F")
         assert abs(np.linalg.norm(vv) - 1) < 1e-10 and abs(abs(np.vdot(V[:, 0], vv)) - 1) < 1e-8
 
+
+
+def test_complex_operator_expectation_values_match_oracle(imps, oracle, cx):
+    # expectationvalue(ψ, site => O) (expval.jl:33-37) for complex O: Sy, a random Hermitian and a
+    # random non-Hermitian operator, on a complex state and on a real one, against the complex
+    # restatement (the Hermitian values real to rounding)
+    W = oracle.xxz_mpo(1.0, 0.8)
+    Ws = [W, W]
+    H = imps.SparseIMPO(Ws)
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    ops = {"sy": np.array([[0, -0.5j], [0.5j, 0]]), "hermitian": G + G.conj().T, "general": G}
+    psi = cx.canonical_from_tensors(_complex_raw(oracle, 5, [2, 2], 77))
+    rpsi = oracle.randomimps(4, [2, 2], 5)
+    print("complex: one-site expectation values of complex operators", flush=True)
+    for state, ref in ((psi, psi), (rpsi, cx.from_real(rpsi))):
+        with imps.Context(0) as c:
+            st = _as_lib(imps, state)
+            envs = imps.DMRGCache(st, H, ctx=c)
+            for site in (0, 1):
+                for name, O in ops.items():
+                    want = cx.local_expectation1(ref, site, O)
+                    got = imps.expectationvalue(st, (site, O), envs)
+                    assert isinstance(got, complex)
+                    assert abs(got - want) < 1e-12 * max(1.0, abs(want)), (name, site, got, want)
+                    if name != "general":
+                        assert abs(got.imag) < 1e-13
