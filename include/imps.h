@@ -395,7 +395,7 @@ int imps_tdvp_timestep_complex(imps_ctx* ctx, double dt_re, double dt_im, double
  * find_groundstate (imps_vumps_run), the environments (imps_envs_recalculate), imps_energy,
  * imps_galerkin, imps_gaugefix_right, both TDVP steps and imps_local_expectation /
  * imps_correlator with symmetric real operators work as for real states;
- * imps_local_expectation_complex takes any ComplexF64 one-site operator; imps_hac_apply,
+ * imps_local_expectation_complex and imps_correlator_complex take any ComplexF64 operators; imps_hac_apply,
  * imps_hc_apply, imps_transfer_apply, imps_get_env and the vectors of imps_eigsolve_ac/c take and
  * return ComplexF64 arrays of the complex bond dimension (eigenvectors of unit norm);
  * imps_state_dims reports the complex bond dimension; the entry points that pass real
@@ -439,6 +439,11 @@ int imps_state_is_complex(imps_ctx* ctx, int* flag);
  * sites already evaluated are pushed through before the next j. */
 int imps_correlator(imps_ctx* ctx, const double* O1, const double* O2, int i, int nj, const int* js,
                     double* out);
+/* imps_correlator for ComplexF64 operators O1, O2 (d x d, column-major, re and im interleaved), on
+ * real and complex states: out[2k], out[2k+1] = Re, Im <O1_i O2_js[k]> (correlators.jl:13-42 with
+ * complex operators, e.g. Sy Sy). */
+int imps_correlator_complex(imps_ctx* ctx, const double* O1, const double* O2, int i, int nj, const int* js,
+                            double* out);
 /* dot(ψ₁, ψ₂) (src/states/canonicalmps.jl:262-269) with ψ₁ the context's state and ψ₂ given by
  * its AL tensors (host, same unit cell and bond dimension): the dominant eigenvalue of
  * TransferMatrix(ψ₂.AL, ψ₁.AL) from the identity (Arnoldi, tolerance 1e-10).  The eigenvalue of

@@ -661,12 +661,26 @@ def correlator(psi: CanonicalIMPS, O1, O2, i: int, js, envs: GPUCache):
         envs.ket = psi
     scalar = isinstance(js, (int, np.integer))
     jl = [int(js)] if scalar else [int(j) for j in js]
-    O1, O2 = fortran(O1), fortran(O2)
     di = psi.AL[int(i) % len(psi)].shape[1]
-    if O1.shape != (di, di) or any(O2.shape != (psi.AL[j % len(psi)].shape[1],) * 2 for j in jl):
+    if np.shape(O1) != (di, di) or any(np.shape(O2) != (psi.AL[j % len(psi)].shape[1],) * 2 for j in jl):
         raise ValueError("correlator: O1 must match the physical dimension of site i and O2 that of every j")
-    O1, O2 = envs.ctx.pad_operator(O1), envs.ctx.pad_operator(O2)
     jarr = np.array(jl, dtype=np.int32)
+    if any(np.iscomplexobj(O) and np.any(np.imag(O) != 0) for O in (O1, O2)):
+        # complex operators (Sy ⊗ Sy, currents): complex values (imps_correlator_complex)
+        d = envs.ctx.d or di
+
+        def zpad(O):
+            Z = np.zeros((d, d), dtype=np.complex128, order="F")
+            Z[:np.shape(O)[0], :np.shape(O)[1]] = O
+            return Z
+
+        Z1, Z2 = zpad(O1), zpad(O2)
+        out = np.zeros(2 * len(jl))
+        envs.ctx.call("imps_correlator_complex", Z1.ctypes.data_as(_lib.P_double), Z2.ctypes.data_as(_lib.P_double),
+                      int(i), len(jl), jarr.ctypes.data_as(ctypes.c_void_p), dptr(out))
+        vals = out[0::2] + 1j * out[1::2]
+        return complex(vals[0]) if scalar else vals
+    O1, O2 = envs.ctx.pad_operator(fortran(np.real(O1))), envs.ctx.pad_operator(fortran(np.real(O2)))
     out = np.zeros(len(jl))
     envs.ctx.call("imps_correlator", dptr(O1), dptr(O2), int(i), len(jl),
                   jarr.ctypes.data_as(ctypes.c_void_p), dptr(out))

@@ -4307,45 +4307,99 @@ int imps_tdvp_timestep_complex(imps_ctx* c, double dt_re, double dt_im, double t
   });
 }
 
+// sum over the bra of <AC_i| O1 ... O2_j |ket_i AR ...> for real d x d operators O1, O2 (host),
+// each j of js, unnormalised; ket_i is the ket tensor at site i (AC, or a rotation of it)
+static std::vector<double> correlator_terms(imps_ctx* c, const double* O1, const double* O2, int i, int nj,
+                                            const int* js, const double* ket_i) {
+  const int N = (int)c->AL.size(), D = c->D, d = c->d;
+  const size_t DD = (size_t)D * D;
+  double* V = ws(c, "co_V", DD);
+  double* V2 = ws(c, "co_V2", DD);
+  double* Z = ws(c, "co_Z", (size_t)D * d * D);
+  double* E = ws(c, "co_E", DD);
+  double* I = ws(c, "co_I", DD);
+  HIPCHK(launch_set_identity(D, I, D, c->stream));
+  // V[b, c] = AC[a,u,b] O1[u,s] ket[a,s,c] at site i (correlators.jl:28-30)
+  const double* AC = c->AC[wrap(c, i)];
+  HIPCHK(launch_mix_mid(D, d, D, O1, ket_i, Z, 0.0, c->stream));
+  gemm2(c, 'T', 'N', D, D, d * D, 1.0, AC, (long)d * D, Z, (long)d * D, 0.0, V, D);
+  std::vector<double> O2v(O2, O2 + (size_t)d * d);
+  auto push = [&](int l) {
+    const double* AR = c->AR[((l % N) + N) % N];
+    push_left2(c, V, D, AR, AR, V2, D);
+    std::swap(V, V2);
+  };
+  int ctr = i;
+  std::vector<double> vals(nj);
+  for (int k = 0; k < nj; ++k) {
+    const int j = js[k];
+    for (int l = ctr + 1; l < j; ++l) push(l);
+    // E = V[b, c] AR[b,u,e] O2[u,s] AR[c,s,e] = trace of the slice push of V with O2 (31-38)
+    slice_sum_left(c, {SliceTerm{V, (long)D, O2v}}, c->AR[((j % N) + N) % N], E, D);
+    vals[k] = trace_prod(c, D, I, E);
+    push(j);
+    ctr = j;
+  }
+  return vals;
+}
+
+static void require_increasing(int i, int nj, const int* js, const char* what) {
+  for (int k = 0; k < nj; ++k)
+    require(js[k] > (k == 0 ? i : js[k - 1]), IMPS_E_ARG, std::string(what) + ": need i < j1 < j2 < ...");
+}
+
 int imps_correlator(imps_ctx* c, const double* O1, const double* O2, int i, int nj, const int* js, double* out) {
   if (!c || !O1 || !O2 || nj < 0 || (nj > 0 && (!js || !out))) return IMPS_E_ARG;
   return run_guarded(c, [&] {
     need_state(c);
     require_symmetric_if_complex(c, O1, "imps_correlator");
     require_symmetric_if_complex(c, O2, "imps_correlator");
-    const int N = (int)c->AL.size(), D = c->D, d = c->d;
-    for (int k = 0; k < nj; ++k)
-      require(js[k] > (k == 0 ? i : js[k - 1]), IMPS_E_ARG, "imps_correlator: need i < j1 < j2 < ...");
-    const size_t DD = (size_t)D * D;
-    double* V = ws(c, "co_V", DD);
-    double* V2 = ws(c, "co_V2", DD);
-    double* Z = ws(c, "co_Z", (size_t)D * d * D);
-    double* E = ws(c, "co_E", DD);
-    double* I = ws(c, "co_I", DD);
-    HIPCHK(launch_set_identity(D, I, D, c->stream));
-    // V[b, c] = AC[a,u,b] O1[u,s] AC[a,s,c] at site i (correlators.jl:28-30)
-    const double* AC = c->AC[wrap(c, i)];
-    HIPCHK(launch_mix_mid(D, d, D, O1, AC, Z, 0.0, c->stream));
-    gemm2(c, 'T', 'N', D, D, d * D, 1.0, AC, (long)d * D, Z, (long)d * D, 0.0, V, D);
-    std::vector<double> O2v(O2, O2 + (size_t)d * d);
-    auto push = [&](int l) {
-      const double* AR = c->AR[((l % N) + N) % N];
-      push_left2(c, V, D, AR, AR, V2, D);
-      std::swap(V, V2);
-    };
-    int ctr = i;
-    std::vector<double> vals(nj);
-    for (int k = 0; k < nj; ++k) {
-      const int j = js[k];
-      for (int l = ctr + 1; l < j; ++l) push(l);
-      // E = V[b, c] AR[b,u,e] O2[u,s] AR[c,s,e] = trace of the slice push of V with O2 (31-38)
-      slice_sum_left(c, {SliceTerm{V, (long)D, O2v}}, c->AR[((j % N) + N) % N], E, D);
-      vals[k] = trace_prod(c, D, I, E);
-      push(j);
-      ctr = j;
-    }
+    require_increasing(i, nj, js, "imps_correlator");
+    const std::vector<double> vals = correlator_terms(c, O1, O2, i, nj, js, c->AC[wrap(c, i)]);
     const double nrm = dot_psi(c);
     for (int k = 0; k < nj; ++k) out[k] = vals[k] / nrm;
+  });
+}
+
+// <O1_i O2_j> for ComplexF64 operators: with O = A + i B (A, B real) and z(R1, R2) the correlator
+// of real operators, <O1 O2> = z(A1,A2) - z(B1,B2) + i (z(A1,B2) + z(B1,A2)).  On a complex state
+// the embedding's real trace gives Re z, and Im z = Re z evaluated with the ket -i AC at site i.
+int imps_correlator_complex(imps_ctx* c, const double* O1, const double* O2, int i, int nj, const int* js,
+                            double* out) {
+  if (!c || !O1 || !O2 || nj < 0 || (nj > 0 && (!js || !out))) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    require_increasing(i, nj, js, "imps_correlator_complex");
+    const int d = c->d;
+    const size_t dd = (size_t)d * d;
+    std::vector<double> A1(dd), B1(dd), A2(dd), B2(dd);
+    for (size_t k = 0; k < dd; ++k) {
+      A1[k] = O1[2 * k];
+      B1[k] = O1[2 * k + 1];
+      A2[k] = O2[2 * k];
+      B2[k] = O2[2 * k + 1];
+    }
+    const double* AC = c->AC[wrap(c, i)];
+    auto terms = [&](const std::vector<double>& R1, const std::vector<double>& R2, const double* ket) {
+      return correlator_terms(c, R1.data(), R2.data(), i, nj, js, ket);
+    };
+    const auto aa = terms(A1, A2, AC), bb = terms(B1, B2, AC), ab = terms(A1, B2, AC), ba = terms(B1, A2, AC);
+    std::vector<double> aai(nj, 0.0), bbi(nj, 0.0), abi(nj, 0.0), bai(nj, 0.0);
+    if (c->cplx) {
+      const size_t n = (size_t)c->D * d * c->D;
+      double* m = ws(c, "co_mi", n);
+      HIPCHK(launch_fill(n, 0.0, m, c->stream));
+      HIPCHK(launch_cplx_axpy_i(n, -1.0, AC, m, c->stream));
+      aai = terms(A1, A2, m);
+      bbi = terms(B1, B2, m);
+      abi = terms(A1, B2, m);
+      bai = terms(B1, A2, m);
+    }
+    const double nrm = dot_psi(c);
+    for (int k = 0; k < nj; ++k) {
+      out[2 * k] = (aa[k] - bb[k] - abi[k] - bai[k]) / nrm;
+      out[2 * k + 1] = (aai[k] - bbi[k] + ab[k] + ba[k]) / nrm;
+    }
   });
 }
 

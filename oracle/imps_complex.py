@@ -393,6 +393,25 @@ def local_expectation1(psi, site, O):
     return np.einsum("aub,us,asb->", AC.conj(), O, AC) / dot_psi(psi)
 
 
+def correlator(psi, O1, O2, i, js):
+    """correlators.jl:13-42 (complex): <psi| O1[i] O2[j] |psi> / dot(psi, psi) for increasing j > i;
+    V[b, c] = conj(AC[i][a,u,b]) O1[u,s] AC[i][a,s,c], pushed through the AR of the sites between,
+    closed with conj(AR[j][b,u,e]) O2[u,s] AR[j][c,s,e]."""
+    N = len(psi)
+    AC = psi.AC[i % N]
+    V = np.einsum("aub,us,asc->bc", AC.conj(), O1, AC)
+    ctr, out = i, []
+    for j in js:
+        for l in range(ctr + 1, j):
+            A = psi.AR[l % N]
+            V = np.einsum("bc,bud,cue->de", V, A.conj(), A)
+        A = psi.AR[j % N]
+        out.append(np.einsum("bc,bue,us,cse->", V, A.conj(), O2, A))
+        V = np.einsum("bc,bud,cue->de", V, A.conj(), A)
+        ctr = j
+    return np.array(out) / dot_psi(psi)
+
+
 def tdvp_timestep(psi, Ws, envs, dt, gauge_tol=1e-14, gauge_maxiter=2000):
     """timestep(ψ, H, dt, alg::TDVP, envs) (tdvp.jl:33-52) for complex dt: AC -> exp(dt H_AC) AC,
     C -> exp(dt H_C) C per site (H_C at site l between leftenv(l+1) and rightenv(l)), AL from

@@ -296,3 +296,31 @@ def test_complex_operator_expectation_values_match_oracle(imps, oracle, cx):
                     assert abs(got - want) < 1e-12 * max(1.0, abs(want)), (name, site, got, want)
                     if name != "general":
                         assert abs(got.imag) < 1e-13
+
+
+def test_complex_operator_correlators_match_oracle(imps, oracle, cx):
+    # correlator(ψ, O1, O2, i, js) (correlators.jl:13-42) with complex operators (Sy Sy, random
+    # complex ones) on a complex and on a real state, against the complex restatement
+    W = oracle.xxz_mpo(1.0, 0.8)
+    H = imps.SparseIMPO([W, W])
+    rng = np.random.default_rng(11)
+    G1 = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    G2 = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    sy = np.array([[0, -0.5j], [0.5j, 0]])
+    pairs = [(sy, sy), (G1, G2), (sy, oracle.sigma_z())]
+    psi = cx.canonical_from_tensors(_complex_raw(oracle, 5, [2, 2], 91))
+    rpsi = oracle.randomimps(4, [2, 2], 6)
+    print("complex: correlators of complex operators", flush=True)
+    for state, ref in ((psi, psi), (rpsi, cx.from_real(rpsi))):
+        with imps.Context(0) as c:
+            st = _as_lib(imps, state)
+            envs = imps.DMRGCache(st, H, ctx=c)
+            for O1, O2 in pairs:
+                for i, js in ((0, [1, 2, 5]), (1, [2, 3])):
+                    want = cx.correlator(ref, O1, O2, i, js)
+                    got = imps.correlator(st, O1, O2, i, js, envs)
+                    assert np.iscomplexobj(got)
+                    assert np.max(np.abs(got - want)) < 1e-12 * max(1.0, float(np.max(np.abs(want)))), (i, js)
+                single = imps.correlator(st, O1, O2, 0, 2, envs)
+                assert isinstance(single, complex)
+                assert abs(single - cx.correlator(ref, O1, O2, 0, [2])[0]) < 1e-12

@@ -208,3 +208,29 @@ def test_complex_hessenberg_dominant_eigenpair():
         assert abs(complex(lam[0], lam[1]) - w[k]) < 1e-12 * abs(w[k])
         assert np.linalg.norm(Hm @ x - w[k] * x) < 1e-10 and abs(np.linalg.norm(x) - 1) < 1e-12
         assert abs(last.value - abs(x[-1])) < 1e-15
+
+
+def test_complex_correlator_pins(oracle, cx):
+    # (1) real state, real operators: the real restatement; (2) neighbouring sites: the two-site
+    # wavefunction theta = AC_i AR_{i+1} contracted with O1 (x) O2 directly; (3) a product state
+    # (D = 1): <O1_i O2_j> = <O1><O2> for every j > i
+    sx, sz = oracle.sigma_x(), oracle.sigma_z()
+    rpsi = oracle.randomimps(4, [2, 2], 8)
+    got = cx.correlator(cx.from_real(rpsi), sx, sz, 0, [1, 2, 5])
+    want = oracle.correlator(rpsi, sx, sz, 0, [1, 2, 5])
+    assert np.max(np.abs(got - want)) < 1e-13
+    psi = cx.canonical_from_tensors([a + 0.5j * b for a, b in zip(oracle.random_tensors(3, [2, 2], 12),
+                                                                   oracle.random_tensors(3, [2, 2], 13))])
+    rng = np.random.default_rng(2)
+    O1 = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    O2 = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    theta = np.einsum("asb,btc->astc", psi.AC[0], psi.AR[1])
+    brute = np.einsum("auvc,us,vt,astc->", theta.conj(), O1, O2, theta) / np.vdot(theta, theta)
+    assert abs(cx.correlator(psi, O1, O2, 0, [1])[0] - brute) < 1e-12
+    a = [np.array([0.6, 0.8j]).reshape(1, 2, 1), np.array([0.8, -0.6 + 0.0j]).reshape(1, 2, 1)]
+    prod = cx.canonical_from_tensors(a)
+    e1 = [cx.local_expectation1(prod, l, O1) for l in (0, 1)]
+    e2 = [cx.local_expectation1(prod, l, O2) for l in (0, 1)]
+    vals = cx.correlator(prod, O1, O2, 0, [1, 2, 3])
+    for j, v in zip([1, 2, 3], vals):
+        assert abs(v - e1[0] * e2[j % 2]) < 1e-12
