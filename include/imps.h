@@ -400,8 +400,8 @@ int imps_tdvp_timestep_complex(imps_ctx* ctx, double dt_re, double dt_im, double
  * return ComplexF64 arrays of the complex bond dimension (eigenvectors of unit norm);
  * imps_state_dims reports the complex bond dimension; the entry points that pass real
  * state-shaped host arrays or need real arithmetic (imps_get_state, set_env, regauge, the device-
- * pointer applies, IDMRG, dot, dominant_env, dense_expectation, the gradients, checkpointing)
- * return IMPS_E_UNSUPPORTED. */
+ * pointer applies, IDMRG, imps_dot_state (imps_dot_state_complex takes complex states),
+ * dominant_env, dense_expectation, the gradients, checkpointing) return IMPS_E_UNSUPPORTED. */
 
 /* CanonicalIMPS{ComplexF64} from canonical tensors (the complex twin of imps_set_state; AC may
  * be NULL: AL C). */
@@ -451,6 +451,10 @@ int imps_correlator_complex(imps_ctx* ctx, const double* O1, const double* O2, i
  * out[1] its imaginary part (0 for a real dominant eigenvalue).  IMPS_E_SOLVER when the
  * eigensolver does not converge. */
 int imps_dot_state(imps_ctx* ctx, int nsites, const double* const* AL2, double* out);
+/* dot(ψ₁, ψ₂) for complex states (canonicalmps.jl:262-269): ψ₁ the context's complex state, AL2
+ * the ComplexF64 (D, d, D) AL tensors of ψ₂ (re and im interleaved, the complex bond dimension);
+ * out[0], out[1] = Re, Im ⟨ψ₁|ψ₂⟩.  IMPS_E_STATE for a real context state. */
+int imps_dot_state_complex(imps_ctx* ctx, int nsites, const double* const* AL2, double* out);
 /* dominant_env(ψ; side) and dominant_env(W::DenseIMPO, ψ; side) (src/algorithms/groundstates/
  * envs.jl:71-99, transfer maps 34-68) on the context's state: the dominant (:LM) eigenpair of the
  * unit-cell transfer map, from the left over AL (side 0) or from the right over AR (side 1).

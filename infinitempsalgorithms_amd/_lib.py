@@ -157,6 +157,7 @@ SIGNATURES = [
     ("imps_correlator", c_int, [c_void_p, P_double, P_double, c_int, c_int, c_void_p, P_double]),
     ("imps_correlator_complex", c_int, [c_void_p, P_double, P_double, c_int, c_int, c_void_p, P_double]),
     ("imps_dot_state", c_int, [c_void_p, c_int, PP_double, P_double]),
+    ("imps_dot_state_complex", c_int, [c_void_p, c_int, PP_double, P_double]),
     ("imps_dominant_env", c_int, [c_void_p, c_int, c_int, c_int, c_int, PP_double, c_double, c_int, c_int, P_double,
                                   P_double]),
     ("imps_dense_expectation", c_int, [c_void_p, c_int, c_int, c_int, PP_double, c_double, c_int, c_int, P_double,

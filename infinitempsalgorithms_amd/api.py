@@ -691,13 +691,21 @@ def dot(psi1: CanonicalIMPS, psi2: CanonicalIMPS, envs: GPUCache):
     """dot(ψ₁, ψ₂) (canonicalmps.jl:262-269): dominant eigenvalue of the mixed AL transfer
     matrix, on the device (ψ₁ becomes the context's state).  A float, or a complex number when the
     dominant eigenvalue is complex (as the reference returns it)."""
-    if psi1.is_complex() or psi2.is_complex():
-        # on the real embedding the mixed transfer map of psi1 with conj(psi2) is as dominant as
-        # the one asked for, so the overlap of complex states is not computed there
-        raise ValueError("dot: complex states are not supported")
     if len(psi2) != len(psi1) or any(a.shape != b.shape for a, b in zip(psi1.AL, psi2.AL)):
         raise ValueError("dot: the library takes states with the same unit cell, bond and physical "
                          "dimensions")
+    if psi1.is_complex() or psi2.is_complex():
+        # complex states (a real one is promoted, as Julia promotes the scalar type): the complex
+        # dominant eigenvalue on the embedding (imps_dot_state_complex), always a complex number
+        p1 = psi1 if psi1.is_complex() else psi1.complex()
+        if p1 is not envs.ket:
+            envs.ctx.set_state(p1)
+            envs.ket = p1
+        AL2 = [envs.ctx.pad_site(l, np.asarray(a, dtype=np.complex128)) for l, a in enumerate(psi2.AL)]
+        pp, keep = zptr_array(AL2)
+        val = np.zeros(2)
+        envs.ctx.call("imps_dot_state_complex", len(AL2), pp, dptr(val))
+        return complex(val[0], val[1])
     if psi1 is not envs.ket:
         envs.ctx.set_state(psi1)
         envs.ket = psi1

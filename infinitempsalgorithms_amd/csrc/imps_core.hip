@@ -4440,6 +4440,45 @@ int imps_dot_state(imps_ctx* c, int nsites, const double* const* AL2, double* ou
   });
 }
 
+// dot(psi1, psi2) of complex states: psi1 the context's complex state, AL2 ComplexF64 (D, d, D)
+// tensors; the same transfer map on the embedding, whose pushes keep the embedded form, so the
+// complex Arnoldi finds the complex dominant eigenvalue (re, im)
+int imps_dot_state_complex(imps_ctx* c, int nsites, const double* const* AL2, double* out) {
+  if (!c || !AL2 || !out) return IMPS_E_ARG;
+  return run_guarded(c, [&] {
+    need_state(c);
+    require(c->cplx, IMPS_E_STATE, "imps_dot_state_complex: the context holds a real state (imps_dot_state)");
+    const int N = (int)c->AL.size(), D = c->D, d = c->d, Dc = D / 2;
+    require(nsites == N, IMPS_E_ARG, "imps_dot_state_complex: unit cells differ");
+    const size_t nA = (size_t)D * d * D, DD = (size_t)D * D;
+    std::vector<double*> A2(N);
+    for (int l = 0; l < N; ++l) {
+      require(AL2[l] != nullptr, IMPS_E_ARG, "imps_dot_state_complex: null tensor");
+      A2[l] = ws(c, "ds_A2_" + std::to_string(l), nA);
+      embed_from_host(c, AL2[l], Dc, d, Dc, 1.0, A2[l]);
+    }
+    double* v0 = ws(c, "ds_v0", DD);
+    double* ev = ws(c, "ds_ev", DD);
+    HIPCHK(launch_set_identity(D, v0, D, c->stream));
+    Op tm = [&](const double* v, double* o) {
+      double* t0 = ws(c, "ds_t0", DD);
+      double* t1 = ws(c, "ds_t1", DD);
+      const double* cur = v;
+      for (int l = 0; l < N; ++l) {
+        double* dst = (l == N - 1) ? o : (l % 2 == 0 ? t0 : t1);
+        push_left2(c, cur, D, A2[l], c->AL[l], dst, D);
+        cur = dst;
+      }
+    };
+    KInfo ki;
+    double im = 0.0;
+    const double re = arnoldi_dominant(c, tm, DD, v0, ev, 1.0e-10, 30, 300, true, &ki, nullptr, &im);
+    require(ki.converged != 0, IMPS_E_SOLVER, "dot: the transfer-matrix eigensolver did not converge");
+    out[0] = re;
+    out[1] = im;
+  });
+}
+
 // dominant_env (envs.jl:83-99) of the context's state: the unit-cell transfer map of envs.jl:34-68,
 // from the left over AL[0..N-1] (side 0) or from the right over AR[N-1..0] (side 1), identity
 // channel when Ws is empty, else DenseIMPO tensors Ws[l % nw] (w, d, w, d); Arnoldi (:LM, eager)

@@ -179,8 +179,10 @@ def test_complex_state_refuses_real_only_entry_points(imps, oracle, cx):
             imps.regauge_(envs, 0)
         with pytest.raises(_lib.ImpsError, match="status -5"):
             imps.expectationvalue(st, (0, np.array([[0.0, 1.0], [0.0, 0.0]])), envs)
-        with pytest.raises(ValueError, match="complex"):
-            imps.dot(st, st, envs)
+        # the real-state overlap entry refuses a complex context state (imps_dot_state_complex)
+        pp, keep = _lib.ptr_array([np.zeros((8, 2, 8), order="F")])
+        with pytest.raises(_lib.ImpsError, match="status -5"):
+            c.call("imps_dot_state", 1, pp, np.zeros(2).ctypes.data_as(_lib.P_double))
         # symmetric operators are whole on complex states
         sz = oracle.sigma_z()
         assert abs(imps.expectationvalue(st, (0, sz), envs) - cx.local_expectation1(psi, 0, sz).real) < 1e-12
@@ -324,3 +326,34 @@ def test_complex_operator_correlators_match_oracle(imps, oracle, cx):
                 single = imps.correlator(st, O1, O2, 0, 2, envs)
                 assert isinstance(single, complex)
                 assert abs(single - cx.correlator(ref, O1, O2, 0, [2])[0]) < 1e-12
+
+
+def test_complex_dot_and_fidelity(imps, oracle, cx):
+    # dot(ψ₁, ψ₂) = <ψ₁|ψ₂> (canonicalmps.jl:262-269) for complex states: 1 for a normalised state
+    # with itself, e^{iφ} per unit cell against the same state with one tensor times e^{iφ} (fixes
+    # the conjugated layer), the dense transfer matrix's dominant eigenvalue for two random states,
+    # and a real state promoted against its complex copy
+    W = oracle.xxz_mpo(1.0, 0.8)
+    H = imps.SparseIMPO([W, W])
+    psi = cx.canonical_from_tensors(_complex_raw(oracle, 4, [2, 2], 101))
+    psi2 = cx.canonical_from_tensors(_complex_raw(oracle, 4, [2, 2], 202))
+    phi = 0.7
+    rot = imps.CanonicalIMPS([psi.AL[0] * np.exp(1j * phi), psi.AL[1].copy()], [x.copy() for x in psi.AR],
+                             [x.copy() for x in psi.C], [x.copy() for x in psi.AC])
+    w = np.linalg.eigvals(cx.transfer_left_matrix(psi2.AL, psi.AL))
+    want = w[int(np.argmax(np.abs(w)))]
+    rpsi = oracle.randomimps(4, [2, 2], 9)
+    print("complex: dot and fidelity of complex states", flush=True)
+    with imps.Context(0) as c:
+        st = _as_lib(imps, psi)
+        envs = imps.DMRGCache(st, H, ctx=c)
+        d11 = imps.dot(st, st, envs)
+        assert isinstance(d11, complex) and abs(d11 - 1.0) < 1e-10
+        assert abs(imps.dot(st, rot, envs) - np.exp(1j * phi)) < 1e-10
+        d12 = imps.dot(st, _as_lib(imps, psi2), envs)
+        assert abs(d12 - want) < 1e-9 * max(1.0, abs(want)), (d12, want)
+        assert abs(imps.fidelity(st, _as_lib(imps, psi2), envs) - abs(want)) < 1e-9
+    with imps.Context(0) as c:
+        rs = imps.CanonicalIMPS(rpsi.AL, rpsi.AR, rpsi.C, rpsi.AC)
+        envs = imps.DMRGCache(rs, H, ctx=c)
+        assert abs(imps.dot(rs, rs.complex(), envs) - 1.0) < 1e-10
